@@ -1,0 +1,118 @@
+/*
+ * CPU oracle for the evolving-graph hot path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This library is the checker: a plain C++ restatement of the reference's
+ * streaming ingest and incremental vertex-program executors, plus textbook
+ * checkers (queue BFS, Dijkstra, min-reachable labels, FP64 PageRank).
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product (cytom_amd/, include/cytom.h) never links it.
+ *
+ * Reference anchors (paths relative to the reference repository):
+ *   hash / hashDouble ............ src/include/util/util.h:178-200
+ *   RMAT recurrence .............. src/include/edge-inserter/rmat-context.h:19-73
+ *   tile routing ................. src/include/core/tile-store.h:88-96, traversal row_first.cc:6-11
+ *   insert / delete .............. src/include/tile-manager/tile-manager.h:136-190,
+ *                                  src/include/tile-manager/meta-tile-manager.h:112-231
+ *   pull / applyTiles ............ src/include/core/algorithm-executor.h:214-505
+ *   apply ........................ src/include/core/apply-executor.h:40-115
+ *   iteration loop ............... src/include/core/continuous-algorithm-executor.h:49-249
+ *   programs ..................... examples/{bfs,cc,sssp,pagerank-delta}.cc
+ *
+ * Parity pins (what checks each function; see tests/test_oracle.py):
+ *   orc_hash64 / orc_hash_double / orc_rmat_edges: the reference's own
+ *     hash and rMatIterative text compiled by oracle/ref_build.py into
+ *     oracle/_ref (when the reference tree is present), the recursive
+ *     form rMat (rmat-context.h:19-37) restated independently, quadrant
+ *     frequency statistics, and committed fixtures.
+ *   store (insert/delete/degrees): brute-force multiset replay.
+ *   BFS / CC / SSSP executors: textbook checkers below, bit-exact.
+ *   PageRank-delta executor: FP64 restatement (orc_prd_f64) and FP64
+ *     power iteration; hand-computed values on tiny graphs.
+ *   stream weights / deletion sampler: definitional (STREAMS spec in
+ *     DESIGN.md) -- pinned by invariants (range, uniqueness, liveness),
+ *     otherwise "parity unpinned" as they have no reference counterpart.
+ */
+#ifndef CYTOM_ORACLE_H
+#define CYTOM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_BFS = 0, ORC_CC = 1, ORC_SSSP = 2, ORC_PRD = 3 };
+
+/* ---- a1: hash + RMAT ---------------------------------------------------- */
+uint32_t orc_hash32(uint32_t a);
+uint64_t orc_hash64(uint64_t a);
+double orc_hash_double(uint64_t i);
+/* Edges [first, first+n) of the RMAT stream with 2^scale vertices. */
+void orc_rmat_edges(uint32_t scale, uint64_t seed, uint64_t first, uint64_t n,
+                    uint32_t* src, uint32_t* dst);
+/* The recursive form of the same recurrence (rmat-context.h:19-37). */
+void orc_rmat_edges_recursive(uint32_t scale, uint64_t seed, uint64_t first,
+                              uint64_t n, uint32_t* src, uint32_t* dst);
+/* Per-edge weights of the stream: u32 in [1,255] and f32 in (0,1]. */
+void orc_stream_weights_u32(uint64_t seed, uint64_t first, uint64_t n, uint32_t* w);
+void orc_stream_weights_f32(uint64_t seed, uint64_t first, uint64_t n, float* w);
+
+/* ---- stream with deletion sampler -------------------------------------- */
+typedef struct orc_stream orc_stream;
+orc_stream* orc_stream_create(uint64_t seed);
+void orc_stream_destroy(orc_stream* s);
+/* One batch: inserts are stream indices [*ins_first, *ins_first+n_ins);
+ * deletions are n_del live stream indices written to del_idx.
+ * Returns the number of deletions actually drawn (<= live count). */
+int64_t orc_stream_next(orc_stream* s, uint64_t n_ins, uint64_t n_del,
+                        uint64_t* ins_first, uint64_t* del_idx);
+uint64_t orc_stream_live_count(const orc_stream* s);
+
+/* ---- tiled store + executors ------------------------------------------- */
+typedef struct orc_graph orc_graph;
+
+typedef struct {
+  uint32_t iterations;
+  uint32_t converged;
+  uint32_t reexecuted; /* 1 if the last update batch re-initialised the program */
+  uint32_t pad;
+  uint64_t traversed_edges; /* sum over iterations of live out-edges of active vertices */
+  uint64_t active_vertices; /* sum over iterations of |frontier| */
+} orc_stats;
+
+orc_graph* orc_create(uint64_t n_vertices, uint32_t vertices_per_tile, int weighted,
+                      uint32_t max_iterations, double delta, int n_threads);
+void orc_destroy(orc_graph* g);
+uint64_t orc_num_vertices(const orc_graph* g);
+int orc_register(orc_graph* g, int kind, uint32_t source);
+/* One EdgeInserter batch (edge-inserter.h:53-194): inserts, deletions, hooks.
+ * dw may be NULL (match any weight).  Returns number of deletions found. */
+int64_t orc_update(orc_graph* g, const uint32_t* src, const uint32_t* dst,
+                   const uint32_t* w, uint64_t n_ins, const uint32_t* dsrc,
+                   const uint32_t* ddst, const uint32_t* dw, uint64_t n_del);
+int orc_execute(orc_graph* g, int prog, orc_stats* out);
+/* BFS/CC/SSSP: uint32[V]; PRD: float rank[V]. */
+int orc_read(const orc_graph* g, int prog, void* out);
+int orc_last_reexec(const orc_graph* g, int prog);
+uint64_t orc_live_edges(const orc_graph* g, uint32_t* src, uint32_t* dst, uint32_t* w);
+void orc_degrees(const orc_graph* g, uint32_t* outdeg, uint32_t* indeg);
+
+/* ---- textbook checkers on an explicit edge list ------------------------ */
+void orc_check_bfs(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                   uint32_t source, uint32_t* out);
+void orc_check_sssp(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                    const uint32_t* w, uint32_t source, uint32_t* out);
+void orc_check_cc(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                  uint32_t* out);
+/* FP64 restatement of the PageRank-delta program from scratch (same cap and
+ * activation threshold).  Returns the iteration count. */
+uint32_t orc_prd_f64(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                     uint32_t max_iterations, double delta, double* rank);
+/* FP64 power iteration r = (1-a) + a * sum_{u->v} r_u / outdeg(u). */
+uint32_t orc_pagerank_f64(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                          double alpha, double tol, uint32_t max_iterations, double* rank);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
