@@ -1,0 +1,286 @@
+#!/usr/bin/env python3
+"""Streaming-graph benchmark on one MI355X.
+
+Workload: the RMAT-22 update stream (4,194,304 vertices, edge factor 16,
+a=0.57 b=0.19 c=0.19 d=0.05) cut into 64 batches of 1,048,576 updates
+(943,719 inserts from the RMAT stream + 104,857 deletions of uniformly drawn
+live edges).  After every batch the engine runs BFS and SSSP (u32 weights in
+[1,255]) from the highest out-degree vertex, directed min-label CC and FP32
+PageRank-delta (damping 0.85, threshold 1e-7, 100-iteration cap).
+
+One step = one update batch: ingest + all four programs to convergence.
+The first --warmup batches are untimed; the next --steps batches are timed.
+All inputs are generated into HBM before the timed region.
+
+value = traversed edges of all programs over the timed batches / wall time
+(GTEPS); a traversed edge is a live out-edge of a vertex active in an
+iteration, counted by the engine.
+"""
+import argparse
+import concurrent.futures
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GTEPS per update batch, RMAT-22 incremental BFS/CC/PR; batched vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--batches", type=int, default=64)
+    ap.add_argument("--delete-fraction", type=float, default=0.1)
+    ap.add_argument("--vertices-per-tile", type=int, default=0)
+    ap.add_argument("--programs", default="bfs,cc,sssp,pagerank-delta")
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
+    ap.add_argument("--cpu-batches", type=int, default=2,
+                    help="timed batches of the CPU sample (the last ones of the stream)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def stream_plan(args):
+    E = args.edge_factor << args.scale
+    per_batch = E // args.batches
+    n_del = int(per_batch * args.delete_fraction)
+    n_ins = per_batch - n_del
+    return E, per_batch, n_ins, n_del
+
+
+def gpu_run(args):
+    import cytom_amd as C
+
+    E, per_batch, n_ins, n_del = stream_plan(args)
+    V = 1 << args.scale
+    names = [p.strip() for p in args.programs.split(",") if p.strip()]
+    weighted = "sssp" in names
+    t_setup = time.perf_counter()
+    # Inputs: inserts from the counter-based generator, deletions sampled from
+    # the live set on the host and materialised on the device.
+    stream = C.Stream(args.seed)
+    batches = []
+    src_all = []
+    for b in range(args.batches):
+        first, didx = stream.next(n_ins, n_del)
+        s, d, w = C.rmat_generate(args.scale, args.seed, first, n_ins, weights=weighted)
+        di = C.DeviceArray.from_host(didx)
+        ds, dd, dw = C.rmat_gather(args.scale, args.seed, di, len(didx), weights=weighted)
+        batches.append((s, d, w, ds, dd, dw, len(didx)))
+        src_all.append(s.to_host())
+    # BFS/SSSP source: highest out-degree vertex of the insert stream, ties to
+    # the lowest id.
+    outdeg = np.bincount(np.concatenate(src_all), minlength=V)
+    source = int(np.argmax(outdeg))
+    del src_all
+    log("bench: generated %d batches (%d inserts + %d deletions each) in %.1fs; source=%d"
+        % (args.batches, n_ins, n_del, time.perf_counter() - t_setup, source))
+    g = C.Graph(V, vertices_per_tile=args.vertices_per_tile, weighted=weighted,
+                max_iterations=100, delta=1e-7, edge_capacity=int(3 * E))
+    pids = {}
+    for n in names:
+        pids[n] = g.register(n, source if n in ("bfs", "sssp") else 0)
+    warm = min(args.warmup, args.batches)
+    steps = min(args.steps, args.batches - warm)
+
+    def step(b, acc):
+        s, d, w, ds, dd, dw, nd = batches[b]
+        bs = g.update_device(s, d, w, n_ins, ds, dd, dw, nd)
+        acc["ingest_ms"] += bs.t_ingest_ms
+        acc["deleted"] += bs.deleted
+        for n, p in pids.items():
+            st = g.execute(p)
+            acc["te"][n] += st.traversed_edges
+            acc["ms"][n] += st.t_compute_ms
+            acc["iters"][n] += st.iterations
+            acc["scanned"][n] += st.slots_scanned
+        acc["live"] = bs.live_edges
+
+    def new_acc():
+        return {"ingest_ms": 0.0, "deleted": 0, "live": 0, "te": {n: 0 for n in names},
+                "ms": {n: 0.0 for n in names}, "iters": {n: 0 for n in names},
+                "scanned": {n: 0 for n in names}}
+
+    wacc = new_acc()
+    for b in range(warm):
+        step(b, wacc)
+    log("bench: warmup %d batches done" % warm)
+    g.set_profiling(True)
+    acc = new_acc()
+    C.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(warm + k, acc)
+        if (k + 1) % 8 == 0:
+            log("bench: timed batch %d/%d" % (k + 1, steps))
+    C.synchronize()
+    wall = time.perf_counter() - t0
+    te_total = sum(acc["te"].values())
+    kinds = {n: C.PROGRAMS[n] for n in names}
+    prof = {n: g.kernel_profile(kinds[n]) for n in names}
+    dom = max(names, key=lambda n: prof[n][0])
+    ms, launches, alg = prof[dom]
+    achieved = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    detail = {
+        "gteps_by_program": {n: (acc["te"][n] / (acc["ms"][n] * 1e-3) / 1e9
+                                 if acc["ms"][n] else 0.0) for n in names},
+        "traversed_edges_by_program": acc["te"],
+        "iterations_by_program": acc["iters"],
+        "slots_scanned_by_program": acc["scanned"],
+        "compute_ms_by_program": {n: round(acc["ms"][n], 3) for n in names},
+        "ingest_ms_total": round(acc["ingest_ms"], 3),
+        "update_batches_per_s": steps / wall if wall > 0 else 0.0,
+        "live_edges_final": acc["live"],
+        "gather_kernel_by_program": {
+            n: {"ms": round(prof[n][0], 3), "launches": prof[n][1],
+                "alg_gbytes": round(prof[n][2] / 1e9, 3),
+                "achieved_gbs": round(prof[n][2] / (prof[n][0] * 1e-3) / 1e9, 1)
+                if prof[n][0] else 0.0}
+            for n in names},
+    }
+    roofline = {
+        "bound": "hbm",
+        "kernel": "k_gather<%s>" % dom,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_us": round(ms * 1e3 / launches, 2) if launches else None,
+        "alg_bytes_per_launch": round(alg / launches) if launches else None,
+    }
+    res = {
+        "value": te_total / wall / 1e9 if wall > 0 else 0.0,
+        "ms_per_step": wall * 1e3 / steps if steps else 0.0,
+        "steps": steps,
+        "warmup": warm,
+        "source": source,
+        "n_ins": n_ins,
+        "n_del": n_del,
+        "vpt": g.vpt,
+        "roofline": roofline,
+        "detail": detail,
+    }
+    g.close()
+    return res
+
+
+def cpu_baseline(args, source):
+    """The oracle (multi-threaded C++ restatement) on the host cores: the
+    last --cpu-batches batches of the same stream, ingest + all programs."""
+    import oracle as O
+
+    E, per_batch, n_ins, n_del = stream_plan(args)
+    V = 1 << args.scale
+    names = [p.strip() for p in args.programs.split(",") if p.strip()]
+    weighted = "sssp" in names
+    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    t_setup = time.perf_counter()
+    st = O.Stream(args.seed)
+    plan = [st.next(n_ins, n_del) for _ in range(args.batches)]
+
+    def gen(b):
+        first, didx = plan[b]
+        return O.batch_edges(args.scale, args.seed, first, n_ins, didx, weighted)
+
+    g = O.Graph(V, vertices_per_tile=65536, weighted=weighted, max_iterations=100, delta=1e-7,
+                threads=threads)
+    first_timed = max(0, args.batches - args.cpu_batches)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as ex:
+        futs = [ex.submit(gen, b) for b in range(args.batches)]
+        for b in range(first_timed):
+            g.update(*futs[b].result())
+            futs[b] = None
+        timed_inputs = [futs[b].result() for b in range(first_timed, args.batches)]
+    kinds = {"bfs": O.BFS, "cc": O.CC, "sssp": O.SSSP, "pagerank-delta": O.PRD}
+    pids = {n: g.register(kinds[n], source if n in ("bfs", "sssp") else 0) for n in names}
+    log("bench: cpu baseline setup %.1fs (%d threads)" % (time.perf_counter() - t_setup, threads))
+    te = 0
+    t0 = time.perf_counter()
+    for inp in timed_inputs:
+        g.update(*inp)
+        for n, p in pids.items():
+            te += g.execute(p).traversed_edges
+    wall = time.perf_counter() - t0
+    cpu_model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": te / wall / 1e9 if wall > 0 else 0.0,
+        "unit": "GTEPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": "last %d of %d batches of the same RMAT-%d stream (earlier batches ingested "
+                  "untimed, programs from init as after a re-execution), ingest + %s; "
+                  "%.1f s timed; %s" % (len(timed_inputs), args.batches, args.scale,
+                                        "/".join(names), wall, cpu_model),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        return 2
+    E, per_batch, n_ins, n_del = stream_plan(args)
+    res = gpu_run(args)
+    cpu = None
+    if args.cpu_baseline == "auto":
+        cpu = cpu_baseline(args, res["source"])
+    names = [p.strip() for p in args.programs.split(",") if p.strip()]
+    line = {
+        "metric": METRIC,
+        "value": round(res["value"], 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": res["steps"],
+        "warmup": res["warmup"],
+        "ms_per_step": round(res["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32/f32",
+        "data": "synthetic (counter-based RMAT stream, seed %d)" % args.seed,
+        "config": {
+            "workload": "RMAT-%d stream, %d batches x %d updates (%d inserts + %d deletions); "
+                        "%s after every batch" % (args.scale, args.batches, per_batch, n_ins,
+                                                  n_del, ", ".join(names)),
+            "scale": args.scale,
+            "edge_factor": args.edge_factor,
+            "vertices_per_tile": res["vpt"],
+            "source": res["source"],
+            "parallelism": "single GPU",
+        },
+        "roofline": res["roofline"],
+        "cpu_baseline": cpu,
+        "detail": res["detail"],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,354 @@
+"""Python host side of the MI355X-native evolving-graph engine.
+
+Thin ctypes layer over the C-ABI in include/cytom.h (libcytom_hip.so, built
+in-tree by cytom_amd/build.py).  The vertex programs keep the reference's
+algorithm-store names (examples/bfs.cc main_bfs, cc.cc main_cc, sssp.cc
+main_sssp, pagerank-delta.cc main_pagerank_delta).  There is no CPU
+fallback: if the HIP library is missing or no GPU is visible, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcytom_hip.so")
+
+BFS, CC, SSSP, PRD = 0, 1, 2, 3
+PROGRAMS = {"bfs": BFS, "cc": CC, "sssp": SSSP, "pagerank-delta": PRD}
+
+EINVAL, ENOMEM, ERANGE, EDEVICE = -22, -12, -34, -5
+
+
+class CytomError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("cytom error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("n_vertices", ctypes.c_uint64),
+        ("vertices_per_tile", ctypes.c_uint32),
+        ("max_iterations", ctypes.c_uint32),
+        ("delta", ctypes.c_double),
+        ("edge_capacity", ctypes.c_uint64),
+        ("weighted", ctypes.c_uint32),
+        ("device", ctypes.c_uint32),
+    ]
+
+
+class ProgramDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("source", ctypes.c_uint32)]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_uint32),
+        ("converged", ctypes.c_uint32),
+        ("reexecuted", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+        ("traversed_edges", ctypes.c_uint64),
+        ("active_vertices", ctypes.c_uint64),
+        ("slots_scanned", ctypes.c_uint64),
+        ("t_compute_ms", ctypes.c_double),
+    ]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("inserted", ctypes.c_uint64),
+        ("deleted", ctypes.c_uint64),
+        ("missing", ctypes.c_uint64),
+        ("live_edges", ctypes.c_uint64),
+        ("tiles_relocated", ctypes.c_uint32),
+        ("tiles_compacted", ctypes.c_uint32),
+        ("t_ingest_ms", ctypes.c_double),
+    ]
+
+
+# Every function include/cytom.h declares: name -> (restype, argtypes).
+_VP = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_U64 = ctypes.c_uint64
+SIGNATURES = {
+    "cytom_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(_VP)]),
+    "cytom_destroy": (ctypes.c_int, [_VP]),
+    "cytom_last_error": (ctypes.c_char_p, [_VP]),
+    "cytom_geometry": (ctypes.c_int, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U32),
+                                      ctypes.POINTER(_U32)]),
+    "cytom_program_register": (ctypes.c_int, [_VP, ctypes.POINTER(ProgramDesc),
+                                              ctypes.POINTER(ctypes.c_int)]),
+    "cytom_program_init": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cytom_update_batch": (ctypes.c_int, [_VP, _VP, _VP, _VP, _U64, _VP, _VP, _VP, _U64,
+                                          ctypes.POINTER(BatchStats)]),
+    "cytom_update_batch_device": (ctypes.c_int, [_VP, _VP, _VP, _VP, _U64, _VP, _VP, _VP, _U64,
+                                                 ctypes.POINTER(BatchStats)]),
+    "cytom_execute": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(RunStats)]),
+    "cytom_read_values": (ctypes.c_int, [_VP, ctypes.c_int, _VP, ctypes.c_size_t]),
+    "cytom_read_degrees": (ctypes.c_int, [_VP, _VP, _VP]),
+    "cytom_export_edges": (ctypes.c_int64, [_VP, _VP, _VP, _VP, _U64]),
+    "cytom_set_profiling": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cytom_kernel_profile": (ctypes.c_int, [_VP, _U32, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(_U64),
+                                            ctypes.POINTER(ctypes.c_double)]),
+    "cytom_rmat_generate_device": (ctypes.c_int, [_U32, _U64, _U64, _U64, _VP, _VP, _VP]),
+    "cytom_rmat_gather_device": (ctypes.c_int, [_U32, _U64, _VP, _U64, _VP, _VP, _VP]),
+    "cytom_stream_create": (ctypes.c_int, [_U64, ctypes.POINTER(_VP)]),
+    "cytom_stream_destroy": (ctypes.c_int, [_VP]),
+    "cytom_stream_next": (ctypes.c_int64, [_VP, _U64, _U64, ctypes.POINTER(_U64), _VP]),
+    "cytom_device_alloc": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_size_t]),
+    "cytom_device_free": (ctypes.c_int, [_VP]),
+    "cytom_memcpy_h2d": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
+    "cytom_memcpy_d2h": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
+    "cytom_device_synchronize": (ctypes.c_int, []),
+    "cytom_device_count": (ctypes.c_int, []),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libcytom_hip.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libcytom_hip.so is not built: run `python -m cytom_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _u32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def device_count():
+    return lib().cytom_device_count()
+
+
+def synchronize():
+    rc = lib().cytom_device_synchronize()
+    if rc:
+        raise CytomError(rc, "device synchronize failed")
+
+
+class DeviceArray:
+    """A flat device allocation (HBM) with host copy helpers."""
+
+    def __init__(self, n, dtype):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        rc = lib().cytom_device_alloc(ctypes.byref(p), max(self.nbytes, 1))
+        if rc:
+            raise CytomError(rc, "device allocation of %d bytes failed" % self.nbytes)
+        self._ptr = p
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def offset(self, elems):
+        """Raw pointer to element `elems` (for sub-range views)."""
+        return ctypes.c_void_p(self._ptr.value + int(elems) * self.dtype.itemsize)
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        if a.size:
+            rc = lib().cytom_memcpy_h2d(d.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+            if rc:
+                raise CytomError(rc, "H2D copy failed")
+        return d
+
+    def to_host(self, n=None):
+        n = self.n if n is None else int(n)
+        out = np.empty(n, self.dtype)
+        if n:
+            rc = lib().cytom_memcpy_d2h(out.ctypes.data_as(ctypes.c_void_p), self.ptr,
+                                        n * self.dtype.itemsize)
+            if rc:
+                raise CytomError(rc, "D2H copy failed")
+        return out
+
+    def free(self):
+        if getattr(self, "_ptr", None) is not None and self._ptr.value:
+            lib().cytom_device_free(self._ptr)
+            self._ptr = ctypes.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def rmat_generate(scale, seed, first, n, weights=False):
+    """Device RMAT edges [first, first+n): returns DeviceArrays (src, dst, w|None)."""
+    s = DeviceArray(n, np.uint32)
+    d = DeviceArray(n, np.uint32)
+    w = DeviceArray(n, np.uint32) if weights else None
+    rc = lib().cytom_rmat_generate_device(scale, seed, first, n, s.ptr, d.ptr,
+                                          w.ptr if w else None)
+    if rc:
+        raise CytomError(rc, "rmat generation failed")
+    return s, d, w
+
+
+def rmat_gather(scale, seed, index_dev, n, weights=False):
+    s = DeviceArray(n, np.uint32)
+    d = DeviceArray(n, np.uint32)
+    w = DeviceArray(n, np.uint32) if weights else None
+    rc = lib().cytom_rmat_gather_device(scale, seed, index_dev.ptr if n else None, n,
+                                        s.ptr, d.ptr, w.ptr if w else None)
+    if rc:
+        raise CytomError(rc, "rmat gather failed")
+    return s, d, w
+
+
+class Stream:
+    """Synthetic update stream: consecutive RMAT inserts, sampled deletions."""
+
+    def __init__(self, seed):
+        h = ctypes.c_void_p()
+        rc = lib().cytom_stream_create(seed, ctypes.byref(h))
+        if rc:
+            raise CytomError(rc, "stream create failed")
+        self._h = h
+
+    def next(self, n_ins, n_del):
+        first = ctypes.c_uint64(0)
+        idx = np.empty(max(int(n_del), 1), np.uint64)
+        m = lib().cytom_stream_next(self._h, n_ins, n_del, ctypes.byref(first),
+                                    idx.ctypes.data_as(ctypes.c_void_p))
+        if m < 0:
+            raise CytomError(m, "stream next failed")
+        return first.value, idx[:m].copy()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().cytom_stream_destroy(self._h)
+            self._h = None
+
+
+class Graph:
+    """One evolving graph in HBM plus the vertex programs registered on it."""
+
+    def __init__(self, n_vertices, vertices_per_tile=0, weighted=False, max_iterations=100,
+                 delta=1e-7, edge_capacity=0, device=0):
+        cfg = Config(n_vertices, vertices_per_tile, max_iterations, delta, edge_capacity,
+                     1 if weighted else 0, device)
+        h = ctypes.c_void_p()
+        rc = lib().cytom_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc:
+            raise CytomError(rc, "cytom_create failed")
+        self._h = h
+        V = ctypes.c_uint64()
+        vpt = ctypes.c_uint32()
+        T = ctypes.c_uint32()
+        lib().cytom_geometry(h, ctypes.byref(V), ctypes.byref(vpt), ctypes.byref(T))
+        self.V, self.vpt, self.T = V.value, vpt.value, T.value
+        self.weighted = weighted
+        self.kinds = []
+
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = lib().cytom_last_error(self._h)
+            raise CytomError(rc, "%s: %s" % (what, msg.decode() if msg else ""))
+        return rc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cytom_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def register(self, kind, source=0):
+        if isinstance(kind, str):
+            kind = PROGRAMS[kind]
+        desc = ProgramDesc(kind, source)
+        pid = ctypes.c_int()
+        self._check(lib().cytom_program_register(self._h, ctypes.byref(desc), ctypes.byref(pid)),
+                    "register")
+        self.kinds.append(kind)
+        return pid.value
+
+    def init(self, pid):
+        self._check(lib().cytom_program_init(self._h, pid), "program init")
+
+    def update(self, src, dst, w=None, dsrc=None, ddst=None, dw=None):
+        src, dst, w = _u32(src), _u32(dst), _u32(w)
+        dsrc = _u32(dsrc if dsrc is not None else np.empty(0, np.uint32))
+        ddst = _u32(ddst if ddst is not None else np.empty(0, np.uint32))
+        dw = _u32(dw)
+        st = BatchStats()
+        self._check(lib().cytom_update_batch(self._h, _p(src), _p(dst), _p(w), len(src), _p(dsrc),
+                                             _p(ddst), _p(dw), len(dsrc), ctypes.byref(st)),
+                    "update")
+        return st
+
+    def update_device(self, src, dst, w, n_ins, dsrc=None, ddst=None, dw=None, n_del=0):
+        st = BatchStats()
+        self._check(lib().cytom_update_batch_device(
+            self._h, _p(src), _p(dst), _p(w), n_ins, _p(dsrc), _p(ddst), _p(dw), n_del,
+            ctypes.byref(st)), "update_device")
+        return st
+
+    def execute(self, pid):
+        st = RunStats()
+        self._check(lib().cytom_execute(self._h, pid, ctypes.byref(st)), "execute")
+        return st
+
+    def read(self, pid):
+        dt = np.float32 if self.kinds[pid] == PRD else np.uint32
+        out = np.empty(self.V, dt)
+        self._check(lib().cytom_read_values(self._h, pid, _p(out), out.nbytes), "read")
+        return out
+
+    def degrees(self):
+        o = np.empty(self.V, np.uint32)
+        i = np.empty(self.V, np.uint32)
+        self._check(lib().cytom_read_degrees(self._h, _p(o), _p(i)), "degrees")
+        return o, i
+
+    def set_profiling(self, enable=True):
+        self._check(lib().cytom_set_profiling(self._h, 1 if enable else 0), "profiling")
+
+    def kernel_profile(self, kind):
+        """(gather ms, launches, algorithmic bytes) accumulated for a program kind."""
+        ms = ctypes.c_double()
+        n = ctypes.c_uint64()
+        b = ctypes.c_double()
+        self._check(lib().cytom_kernel_profile(self._h, kind, ctypes.byref(ms), ctypes.byref(n),
+                                               ctypes.byref(b)), "kernel_profile")
+        return ms.value, n.value, b.value
+
+    def export_edges(self):
+        n = self._check(lib().cytom_export_edges(self._h, None, None, None, 0), "export")
+        s = np.empty(n, np.uint32)
+        d = np.empty(n, np.uint32)
+        w = np.empty(n, np.uint32)
+        m = self._check(lib().cytom_export_edges(self._h, _p(s), _p(d), _p(w), n), "export")
+        assert m == n
+        return s, d, w

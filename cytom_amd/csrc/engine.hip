@@ -1,0 +1,410 @@
+// C-ABI host engine: context life cycle, algorithm store, batch control.
+//
+// Replaces the job driver execute<VertexType, Algorithm, is_weighted>
+// (src/include/main/execute.h:534-771), the EdgeInserter batch loop
+// (edge-inserter.h:53-194) and the plugin registration performed by the
+// composeJobs pass (src/lib/llvm-pass/ParallizeAlgorithm.cpp:68-154):
+// programs are registered through cytom_program_register instead of LLVM
+// bitcode surgery.
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "internal.h"
+
+namespace cytom {
+
+int set_err(cytom_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_fail(cytom_ctx* ctx, hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (ctx) ctx->err = m;
+  return e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? CYTOM_ENOMEM : CYTOM_EDEVICE;
+}
+
+int ensure(cytom_ctx* ctx, GrowBuf& b, size_t bytes) {
+  if (bytes <= b.bytes) return 0;
+  if (b.p) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = std::max<size_t>(bytes, 256);
+  want = want + want / 4;  // amortise growth
+  if (hipMalloc(&b.p, want) != hipSuccess) {
+    b.p = nullptr;
+    return set_err(ctx, CYTOM_ENOMEM, "scratch allocation failed");
+  }
+  b.bytes = want;
+  return 0;
+}
+
+}  // namespace cytom
+
+using namespace cytom;
+
+namespace {
+
+void free_buf(GrowBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+void free_ctx(cytom_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& P : c->progs) program_free(P);
+  void* ps[] = {c->tile_off, c->tile_cap, c->tile_cnt, c->tile_live, c->pool, c->wpool,
+                c->outdeg,   c->indeg,    c->d_misc,   c->d_iter};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  if (c->h_misc) (void)hipHostFree(c->h_misc);
+  if (c->h_iter) (void)hipHostFree(c->h_iter);
+  GrowBuf* bufs[] = {&c->g_keys,     &c->g_vals,      &c->g_keys2,     &c->g_vals2,
+                     &c->g_flags,    &c->g_scan,      &c->g_seg_start, &c->g_seg_tile,
+                     &c->g_reloc_cap, &c->g_reloc_list, &c->g_ins_slot, &c->g_scan_tmp,
+                     &c->g_hist,     &c->g_in_src,    &c->g_in_dst,    &c->g_in_w,
+                     &c->g_del_src,  &c->g_del_dst,   &c->g_del_w,     &c->g_tab_key,
+                     &c->g_tab_head, &c->g_node_next, &c->g_claimed,   &c->g_tile_flag,
+                     &c->g_del_tiles, &c->g_found,    &c->g_tile_scan, &c->g_nch,
+                     &c->g_cbase,    &c->g_chunks,    &c->g_colj,      &c->g_jbase,
+                     &c->g_jobs};
+  for (GrowBuf* b : bufs) free_buf(*b);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->kev) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+#define CY_CHECK_CTX(ctx)                                   \
+  do {                                                      \
+    if (!(ctx)) return CYTOM_EINVAL;                        \
+    if (hipSetDevice((ctx)->device) != hipSuccess)          \
+      return set_err(ctx, CYTOM_EDEVICE, "hipSetDevice failed"); \
+  } while (0)
+
+}  // namespace
+
+struct cytom_stream {
+  uint64_t hash_del = 0;
+  uint64_t next_index = 0;
+  uint64_t batch = 0;
+  std::vector<uint64_t> live;
+};
+
+extern "C" {
+
+int cytom_create(const cytom_config* cfg, cytom_ctx** out) {
+  if (!cfg || !out) return CYTOM_EINVAL;
+  *out = nullptr;
+  if (cfg->n_vertices == 0 || cfg->n_vertices > MAX_VERTICES) return CYTOM_EINVAL;
+  uint64_t V = 64;
+  while (V < cfg->n_vertices) V <<= 1;
+  uint32_t vpt = cfg->vertices_per_tile;
+  if (vpt == 0) vpt = 16384;
+  if (vpt < 64 || vpt > 32768 || (vpt & (vpt - 1))) return CYTOM_EINVAL;
+  if (vpt > V) vpt = (uint32_t)V;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CYTOM_EDEVICE;
+  if ((int)cfg->device >= ndev) return CYTOM_EINVAL;
+  auto* c = new (std::nothrow) cytom_ctx();
+  if (!c) return CYTOM_ENOMEM;
+  c->device = (int)cfg->device;
+  c->V = V;
+  c->vpt = vpt;
+  c->T = (uint32_t)(V / vpt);
+  c->log2vpt = 0;
+  while ((1u << c->log2vpt) < vpt) ++c->log2vpt;
+  c->n_tiles = (uint64_t)c->T * c->T;
+  c->max_iter = cfg->max_iterations ? cfg->max_iterations : 100;
+  c->delta = cfg->delta;
+  c->weighted = cfg->weighted != 0;
+  auto fail = [&](int code) {
+    free_ctx(c);
+    return code;
+  };
+  if (hipSetDevice(c->device) != hipSuccess) return fail(CYTOM_EDEVICE);
+  if (hipStreamCreate(&c->stream) != hipSuccess) return fail(CYTOM_EDEVICE);
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(CYTOM_EDEVICE);
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess &&
+      cus > 0)
+    c->num_cus = cus;
+  const uint64_t nt = c->n_tiles;
+  bool ok = hipMalloc(&c->tile_off, nt * 8) == hipSuccess &&
+            hipMalloc(&c->tile_cap, nt * 4) == hipSuccess &&
+            hipMalloc(&c->tile_cnt, nt * 4) == hipSuccess &&
+            hipMalloc(&c->tile_live, nt * 4) == hipSuccess &&
+            hipMalloc(&c->outdeg, V * 4) == hipSuccess &&
+            hipMalloc(&c->indeg, V * 4) == hipSuccess &&
+            hipMalloc(&c->d_misc, 64 * 8) == hipSuccess &&
+            hipHostMalloc(&c->h_misc, 64 * 8, hipHostMallocDefault) == hipSuccess &&
+            hipMalloc(&c->d_iter, sizeof(IterStats) * (c->max_iter + 2)) == hipSuccess &&
+            hipHostMalloc(&c->h_iter, sizeof(IterStats) * (c->max_iter + 2),
+                          hipHostMallocDefault) == hipSuccess;
+  if (!ok) return fail(CYTOM_ENOMEM);
+  c->pool_size = cfg->edge_capacity ? cfg->edge_capacity : (1ull << 22);
+  if (hipMalloc(&c->pool, c->pool_size * 8) != hipSuccess) return fail(CYTOM_ENOMEM);
+  if (c->weighted && hipMalloc(&c->wpool, c->pool_size * 4) != hipSuccess)
+    return fail(CYTOM_ENOMEM);
+  ok = hipMemsetAsync(c->tile_off, 0, nt * 8, c->stream) == hipSuccess &&
+       hipMemsetAsync(c->tile_cap, 0, nt * 4, c->stream) == hipSuccess &&
+       hipMemsetAsync(c->tile_cnt, 0, nt * 4, c->stream) == hipSuccess &&
+       hipMemsetAsync(c->tile_live, 0, nt * 4, c->stream) == hipSuccess &&
+       hipMemsetAsync(c->outdeg, 0, V * 4, c->stream) == hipSuccess &&
+       hipMemsetAsync(c->indeg, 0, V * 4, c->stream) == hipSuccess &&
+       hipMemsetAsync(c->d_misc, 0, 64 * 8, c->stream) == hipSuccess &&
+       hipStreamSynchronize(c->stream) == hipSuccess;
+  if (!ok) return fail(CYTOM_EDEVICE);
+  if (configure_gather(c)) return fail(CYTOM_EDEVICE);
+  *out = c;
+  return 0;
+}
+
+int cytom_destroy(cytom_ctx* ctx) {
+  if (!ctx) return CYTOM_EINVAL;
+  free_ctx(ctx);
+  return 0;
+}
+
+const char* cytom_last_error(const cytom_ctx* ctx) {
+  if (!ctx) return "null context";
+  return ctx->err.c_str();
+}
+
+int cytom_geometry(const cytom_ctx* ctx, uint64_t* n_vertices, uint32_t* vertices_per_tile,
+                   uint32_t* tiles_per_dim) {
+  if (!ctx) return CYTOM_EINVAL;
+  if (n_vertices) *n_vertices = ctx->V;
+  if (vertices_per_tile) *vertices_per_tile = ctx->vpt;
+  if (tiles_per_dim) *tiles_per_dim = ctx->T;
+  return 0;
+}
+
+int cytom_program_register(cytom_ctx* ctx, const cytom_program_desc* desc, int* prog_id) {
+  CY_CHECK_CTX(ctx);
+  if (!desc || !prog_id) return set_err(ctx, CYTOM_EINVAL, "null descriptor");
+  if (desc->kind > CYTOM_PRD) return set_err(ctx, CYTOM_EINVAL, "unknown program kind");
+  if (desc->kind == CYTOM_SSSP && !ctx->weighted)
+    return set_err(ctx, CYTOM_EINVAL, "SSSP needs a weighted graph");
+  if ((desc->kind == CYTOM_BFS || desc->kind == CYTOM_SSSP) && desc->source >= ctx->V)
+    return set_err(ctx, CYTOM_EINVAL, "source vertex out of range");
+  if (ctx->progs.size() >= 48) return set_err(ctx, CYTOM_ERANGE, "too many programs");
+  Program P;
+  P.kind = (int)desc->kind;
+  P.source = desc->source;
+  int rc = program_alloc(ctx, P);
+  if (rc) return rc;
+  rc = program_init(ctx, P);
+  if (rc) {
+    program_free(P);
+    return rc;
+  }
+  ctx->progs.push_back(P);
+  *prog_id = (int)ctx->progs.size() - 1;
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+    return set_err(ctx, CYTOM_EDEVICE, "program init failed");
+  return 0;
+}
+
+int cytom_program_init(cytom_ctx* ctx, int prog_id) {
+  CY_CHECK_CTX(ctx);
+  if (prog_id < 0 || prog_id >= (int)ctx->progs.size())
+    return set_err(ctx, CYTOM_EINVAL, "bad program id");
+  int rc = program_init(ctx, ctx->progs[prog_id]);
+  if (rc) return rc;
+  CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int cytom_update_batch_device(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
+                              const uint32_t* d_w, uint64_t n_ins, const uint32_t* d_del_src,
+                              const uint32_t* d_del_dst, const uint32_t* d_del_w,
+                              uint64_t n_del, cytom_batch_stats* stats) {
+  CY_CHECK_CTX(ctx);
+  if ((n_ins && (!d_src || !d_dst)) || (n_del && (!d_del_src || !d_del_dst)))
+    return set_err(ctx, CYTOM_EINVAL, "null update arrays");
+  return ingest_batch(ctx, d_src, d_dst, d_w, n_ins, d_del_src, d_del_dst, d_del_w, n_del, stats);
+}
+
+int cytom_update_batch(cytom_ctx* ctx, const uint32_t* src, const uint32_t* dst,
+                       const uint32_t* w, uint64_t n_ins, const uint32_t* del_src,
+                       const uint32_t* del_dst, const uint32_t* del_w, uint64_t n_del,
+                       cytom_batch_stats* stats) {
+  CY_CHECK_CTX(ctx);
+  if ((n_ins && (!src || !dst)) || (n_del && (!del_src || !del_dst)))
+    return set_err(ctx, CYTOM_EINVAL, "null update arrays");
+  int rc;
+  const uint64_t ni = std::max<uint64_t>(n_ins, 1), nd = std::max<uint64_t>(n_del, 1);
+  if ((rc = ensure(ctx, ctx->g_in_src, ni * 4)) || (rc = ensure(ctx, ctx->g_in_dst, ni * 4)) ||
+      (rc = ensure(ctx, ctx->g_in_w, ni * 4)) || (rc = ensure(ctx, ctx->g_del_src, nd * 4)) ||
+      (rc = ensure(ctx, ctx->g_del_dst, nd * 4)) || (rc = ensure(ctx, ctx->g_del_w, nd * 4)))
+    return rc;
+  hipStream_t st = ctx->stream;
+  uint32_t* ds = static_cast<uint32_t*>(ctx->g_in_src.p);
+  uint32_t* dd = static_cast<uint32_t*>(ctx->g_in_dst.p);
+  uint32_t* dw = w ? static_cast<uint32_t*>(ctx->g_in_w.p) : nullptr;
+  uint32_t* xs = static_cast<uint32_t*>(ctx->g_del_src.p);
+  uint32_t* xd = static_cast<uint32_t*>(ctx->g_del_dst.p);
+  uint32_t* xw = del_w ? static_cast<uint32_t*>(ctx->g_del_w.p) : nullptr;
+  if (n_ins) {
+    CY_HIP(ctx, hipMemcpyAsync(ds, src, n_ins * 4, hipMemcpyHostToDevice, st));
+    CY_HIP(ctx, hipMemcpyAsync(dd, dst, n_ins * 4, hipMemcpyHostToDevice, st));
+    if (w) CY_HIP(ctx, hipMemcpyAsync(dw, w, n_ins * 4, hipMemcpyHostToDevice, st));
+  }
+  if (n_del) {
+    CY_HIP(ctx, hipMemcpyAsync(xs, del_src, n_del * 4, hipMemcpyHostToDevice, st));
+    CY_HIP(ctx, hipMemcpyAsync(xd, del_dst, n_del * 4, hipMemcpyHostToDevice, st));
+    if (del_w) CY_HIP(ctx, hipMemcpyAsync(xw, del_w, n_del * 4, hipMemcpyHostToDevice, st));
+  }
+  return ingest_batch(ctx, n_ins ? ds : nullptr, n_ins ? dd : nullptr, n_ins ? dw : nullptr,
+                      n_ins, n_del ? xs : nullptr, n_del ? xd : nullptr, n_del ? xw : nullptr,
+                      n_del, stats);
+}
+
+int cytom_execute(cytom_ctx* ctx, int prog_id, cytom_run_stats* stats) {
+  CY_CHECK_CTX(ctx);
+  if (prog_id < 0 || prog_id >= (int)ctx->progs.size())
+    return set_err(ctx, CYTOM_EINVAL, "bad program id");
+  return run_program(ctx, ctx->progs[prog_id], stats);
+}
+
+int cytom_read_values(cytom_ctx* ctx, int prog_id, void* out, size_t bytes) {
+  CY_CHECK_CTX(ctx);
+  if (prog_id < 0 || prog_id >= (int)ctx->progs.size())
+    return set_err(ctx, CYTOM_EINVAL, "bad program id");
+  if (!out || bytes < ctx->V * 4) return set_err(ctx, CYTOM_EINVAL, "output buffer too small");
+  const Program& P = ctx->progs[prog_id];
+  const void* src = P.kind == CYTOM_PRD ? (const void*)P.rank : (const void*)P.cur;
+  CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  CY_HIP(ctx, hipMemcpy(out, src, ctx->V * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int cytom_read_degrees(cytom_ctx* ctx, uint32_t* out_degree, uint32_t* in_degree) {
+  CY_CHECK_CTX(ctx);
+  CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (out_degree) CY_HIP(ctx, hipMemcpy(out_degree, ctx->outdeg, ctx->V * 4, hipMemcpyDeviceToHost));
+  if (in_degree) CY_HIP(ctx, hipMemcpy(in_degree, ctx->indeg, ctx->V * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int64_t cytom_export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_t* w,
+                           uint64_t cap) {
+  CY_CHECK_CTX(ctx);
+  if (src && !dst) return set_err(ctx, CYTOM_EINVAL, "dst array missing");
+  return export_edges(ctx, src, dst, w, cap);
+}
+
+int cytom_set_profiling(cytom_ctx* ctx, int enable) {
+  CY_CHECK_CTX(ctx);
+  if (enable && ctx->kev.size() < 2 * (size_t)ctx->max_iter) {
+    while (ctx->kev.size() < 2 * (size_t)ctx->max_iter) {
+      hipEvent_t e;
+      CY_HIP(ctx, hipEventCreate(&e));
+      ctx->kev.push_back(e);
+    }
+  }
+  ctx->profile = enable != 0;
+  for (int k = 0; k < 4; ++k) {
+    ctx->gather_ms[k] = 0;
+    ctx->gather_launches[k] = 0;
+    ctx->gather_alg_bytes[k] = 0;
+  }
+  return 0;
+}
+
+int cytom_kernel_profile(cytom_ctx* ctx, uint32_t kind, double* gather_ms, uint64_t* launches,
+                         double* alg_bytes) {
+  if (!ctx || kind > CYTOM_PRD) return CYTOM_EINVAL;
+  if (gather_ms) *gather_ms = ctx->gather_ms[kind];
+  if (launches) *launches = ctx->gather_launches[kind];
+  if (alg_bytes) *alg_bytes = ctx->gather_alg_bytes[kind];
+  return 0;
+}
+
+int cytom_rmat_generate_device(uint32_t scale, uint64_t seed, uint64_t first, uint64_t n,
+                               uint32_t* d_src, uint32_t* d_dst, uint32_t* d_w) {
+  if (scale > 31 || (n && (!d_src || !d_dst))) return CYTOM_EINVAL;
+  launch_rmat(scale, seed, first, nullptr, n, d_src, d_dst, d_w, nullptr);
+  if (hipStreamSynchronize(nullptr) != hipSuccess) return CYTOM_EDEVICE;
+  return hipGetLastError() == hipSuccess ? 0 : CYTOM_EDEVICE;
+}
+
+int cytom_rmat_gather_device(uint32_t scale, uint64_t seed, const uint64_t* d_index, uint64_t n,
+                             uint32_t* d_src, uint32_t* d_dst, uint32_t* d_w) {
+  if (scale > 31 || (n && (!d_src || !d_dst || !d_index))) return CYTOM_EINVAL;
+  launch_rmat(scale, seed, 0, d_index, n, d_src, d_dst, d_w, nullptr);
+  if (hipStreamSynchronize(nullptr) != hipSuccess) return CYTOM_EDEVICE;
+  return hipGetLastError() == hipSuccess ? 0 : CYTOM_EDEVICE;
+}
+
+int cytom_stream_create(uint64_t seed, cytom_stream** out) {
+  if (!out) return CYTOM_EINVAL;
+  auto* s = new (std::nothrow) cytom_stream();
+  if (!s) return CYTOM_ENOMEM;
+  s->hash_del = hash64(seed ^ DELETE_SALT);
+  *out = s;
+  return 0;
+}
+
+int cytom_stream_destroy(cytom_stream* s) {
+  delete s;
+  return 0;
+}
+
+// Inserts take the next n_ins stream indices; then n_del live indices are
+// removed by sequential swap-remove sampling: draw j picks position
+// hash(hd ^ (batch << 32 | j)) mod |live|.
+int64_t cytom_stream_next(cytom_stream* s, uint64_t n_ins, uint64_t n_del, uint64_t* ins_first,
+                          uint64_t* del_index) {
+  if (!s || !ins_first || (n_del && !del_index)) return CYTOM_EINVAL;
+  *ins_first = s->next_index;
+  try {
+    s->live.reserve(s->live.size() + n_ins);
+  } catch (...) {
+    return CYTOM_ENOMEM;
+  }
+  for (uint64_t j = 0; j < n_ins; ++j) s->live.push_back(s->next_index + j);
+  s->next_index += n_ins;
+  const uint64_t m = std::min<uint64_t>(n_del, s->live.size());
+  for (uint64_t j = 0; j < m; ++j) {
+    const uint64_t n_live = s->live.size();
+    const uint64_t pos = hash64(s->hash_del ^ ((s->batch << 32) | j)) % n_live;
+    del_index[j] = s->live[pos];
+    s->live[pos] = s->live.back();
+    s->live.pop_back();
+  }
+  s->batch += 1;
+  return (int64_t)m;
+}
+
+int cytom_device_alloc(void** ptr, size_t bytes) {
+  if (!ptr) return CYTOM_EINVAL;
+  return hipMalloc(ptr, std::max<size_t>(bytes, 1)) == hipSuccess ? 0 : CYTOM_ENOMEM;
+}
+int cytom_device_free(void* ptr) { return hipFree(ptr) == hipSuccess ? 0 : CYTOM_EDEVICE; }
+int cytom_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : CYTOM_EDEVICE;
+}
+int cytom_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : CYTOM_EDEVICE;
+}
+int cytom_device_synchronize(void) {
+  return hipDeviceSynchronize() == hipSuccess ? 0 : CYTOM_EDEVICE;
+}
+int cytom_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,695 @@
+// Incremental vertex-program executors on the device.
+//
+// Replaces ContinuousAlgorithmExecutor::executeAlgorithm
+// (continuous-algorithm-executor.h:49-249), the pull workers
+// (AlgorithmExecutor, algorithm-executor.h:110-505), the TileDistributor
+// (tile-distributor.h:54-177) and the ApplyExecutor (apply-executor.h:40-115)
+// for the BFS, CC, SSSP and PageRank-delta programs
+// (examples/{bfs,cc,sssp,pagerank-delta}.cc).
+//
+// One iteration = gather + apply:
+//  * gather: a persistent grid takes "column jobs" from a device work queue.
+//    A job is a run of edge chunks of one target column y (tiles (x, y) for
+//    all x); the block keeps the column's vertex values in LDS (the
+//    reference's per-tile-row local array, algorithm-executor.h:160-200),
+//    reads packed edges coalesced, tests the source's bit in the active
+//    bitmap and folds candidates with LDS atomics (min for BFS/CC/SSSP, add
+//    for PageRank-delta).  The window is then folded into the global next
+//    array with one atomic per touched vertex (applyTiles,
+//    algorithm-executor.h:431-505), marking `changed`.
+//  * apply: one thread per vertex applies the program to changed vertices;
+//    the next active bitmap is built a wavefront word at a time by ballot,
+//    and the frontier size, its live out-edge count and row activity are
+//    reduced on the fly.
+#include <algorithm>
+
+#include "devutil.h"
+#include "internal.h"
+
+namespace cytom {
+namespace {
+
+constexpr int GATHER_THREADS = 512;
+constexpr int APPLY_THREADS = 256;
+constexpr int GATHER_UNROLL = 8;  // slots per lane per chunk step
+
+inline uint32_t grid_for(uint64_t n, uint32_t threads, uint32_t cap = 65536) {
+  uint64_t b = (n + threads - 1) / threads;
+  if (b == 0) b = 1;
+  return (uint32_t)std::min<uint64_t>(b, cap);
+}
+
+// ---------------------------------------------------------------- init
+// bfs.cc:170-197, cc.cc:171-189, sssp.cc:171-195 (+ source set to 0).
+__global__ void k_init_min(int kind, uint64_t V, uint32_t source, uint32_t* __restrict__ cur,
+                           uint32_t* __restrict__ nxt, uint64_t* __restrict__ act,
+                           uint8_t* __restrict__ changed) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t val = kind == CYTOM_CC ? (uint32_t)v : (v == source ? 0u : INF);
+    cur[v] = val;
+    nxt[v] = val;
+    changed[v] = 0;
+    if ((v & 63) == 0) {
+      const uint64_t w = v >> 6;
+      uint64_t word = 0;
+      if (kind == CYTOM_CC) word = ~0ull;
+      else if ((source >> 6) == w) word = 1ull << (source & 63);
+      act[w] = word;
+    }
+  }
+}
+
+// pagerank-delta.cc:246-270: rank 0, delta 1, all active.
+__global__ void k_init_prd(uint64_t V, float* __restrict__ rank, float* __restrict__ delta,
+                           float* __restrict__ dsum, uint64_t* __restrict__ act,
+                           uint8_t* __restrict__ changed) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    rank[v] = 0.f;
+    delta[v] = 1.f;
+    dsum[v] = 0.f;
+    changed[v] = 0;
+    if ((v & 63) == 0) act[v >> 6] = ~0ull;
+  }
+}
+
+// ------------------------------------------------------------- frontier
+// Frontier size, live out-edges and row activity of the active bitmap
+// (TileManager::isActive over full words); PRD contributions of active
+// vertices.
+__global__ void __launch_bounds__(APPLY_THREADS) k_frontier(
+    int kind, uint64_t V, uint32_t log2vpt, const uint64_t* __restrict__ act,
+    const uint32_t* __restrict__ outdeg, uint8_t* __restrict__ rowact,
+    const float* __restrict__ delta, float* __restrict__ contrib, IterStats* it) {
+  unsigned long long te = 0, cnt = 0;
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t word = act[v >> 6];
+    const bool a = (word >> (v & 63)) & 1ull;
+    if (a) {
+      const uint32_t od = outdeg[v];
+      te += od;
+      cnt += 1;
+      if (kind == CYTOM_PRD) contrib[v] = od ? delta[v] / (float)od : 0.f;
+    }
+    if ((threadIdx.x & 63) == 0 && word) rowact[v >> log2vpt] = 1;
+  }
+  te = block_sum<APPLY_THREADS, unsigned long long>(te);
+  cnt = block_sum<APPLY_THREADS, unsigned long long>(cnt);
+  if (threadIdx.x == 0) {
+    if (te) atomicAdd(&it->te, te);
+    if (cnt) atomicAdd(&it->count, cnt);
+  }
+}
+
+// ------------------------------------------------------------ work lists
+__global__ void k_nch(const uint32_t* __restrict__ cnt, uint64_t n_tiles, uint32_t CH,
+                      uint32_t* __restrict__ nch) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles;
+       t += (uint64_t)gridDim.x * blockDim.x)
+    nch[t] = (cnt[t] + CH - 1) / CH;
+}
+
+__global__ void k_emit_chunks(const uint32_t* __restrict__ nch, const uint32_t* __restrict__ cbase,
+                              uint64_t n_tiles, uint32_t CH, uint64_t* __restrict__ chunks) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t n = nch[t], b = cbase[t];
+    for (uint32_t c = 0; c < n; ++c) chunks[b + c] = ((uint64_t)t << 32) | (uint64_t)(c * CH);
+  }
+}
+
+__global__ void k_col_jobs(const uint32_t* __restrict__ cbase, uint32_t T, uint32_t JC,
+                           uint32_t* __restrict__ colj) {
+  for (uint32_t y = blockIdx.x * blockDim.x + threadIdx.x; y < T; y += gridDim.x * blockDim.x) {
+    const uint32_t n = cbase[(uint64_t)(y + 1) * T] - cbase[(uint64_t)y * T];
+    colj[y] = (n + JC - 1) / JC;
+  }
+}
+
+__global__ void k_emit_jobs(const uint32_t* __restrict__ cbase, const uint32_t* __restrict__ colj,
+                            const uint32_t* __restrict__ jbase, uint32_t T, uint32_t JC,
+                            uint4* __restrict__ jobs) {
+  for (uint32_t y = blockIdx.x * blockDim.x + threadIdx.x; y < T; y += gridDim.x * blockDim.x) {
+    const uint32_t c_begin = cbase[(uint64_t)y * T], c_end = cbase[(uint64_t)(y + 1) * T];
+    const uint32_t nj = colj[y], b = jbase[y];
+    for (uint32_t j = 0; j < nj; ++j) {
+      const uint32_t c0 = c_begin + j * JC;
+      const uint32_t c1 = min(c0 + JC, c_end);
+      jobs[b + j] = make_uint4(c0, c1, y, 0u);
+    }
+  }
+}
+
+// --------------------------------------------------------------- gather
+template <int KIND>
+struct GatherArgs {
+  const uint4* jobs;
+  const uint32_t* njobs;
+  const uint64_t* chunks;
+  StoreView S;
+  const uint64_t* act;
+  const uint8_t* rowact;
+  uint8_t* rowact_clear;
+  const uint32_t* cur;
+  uint32_t* nxt;
+  const float* contrib;
+  float* dsum;
+  uint8_t* changed;
+  IterStats* it;
+  uint32_t CH;
+};
+
+template <int KIND>
+__device__ inline void gather_chunk(const GatherArgs<KIND>& A, uint32_t* win, float* winf,
+                                    uint64_t off, uint32_t n, uint32_t mask) {
+  const int lane = threadIdx.x & 63;
+  for (uint32_t base = 0; base < n; base += 64 * GATHER_UNROLL) {
+    uint64_t e[GATHER_UNROLL];
+#pragma unroll
+    for (int u = 0; u < GATHER_UNROLL; ++u) {
+      const uint32_t k = base + u * 64 + lane;
+      e[u] = k < n ? A.S.pool[off + k] : TOMB;
+    }
+    uint32_t wt[GATHER_UNROLL];
+    if (KIND == CYTOM_SSSP) {
+#pragma unroll
+      for (int u = 0; u < GATHER_UNROLL; ++u) {
+        const uint32_t k = base + u * 64 + lane;
+        wt[u] = k < n ? A.S.wpool[off + k] : 0u;
+      }
+    }
+    bool on[GATHER_UNROLL];
+#pragma unroll
+    for (int u = 0; u < GATHER_UNROLL; ++u) {
+      const uint32_t s = (uint32_t)e[u];
+      on[u] = e[u] != TOMB && ((A.act[s >> 6] >> (s & 63)) & 1ull);
+    }
+#pragma unroll
+    for (int u = 0; u < GATHER_UNROLL; ++u) {
+      if (!on[u]) continue;
+      const uint32_t s = (uint32_t)e[u];
+      const uint32_t dl = (uint32_t)(e[u] >> 32) & mask;
+      if (KIND == CYTOM_PRD) {
+        atomicAdd(&winf[dl], A.contrib[s]);  // pagerank-delta.cc:127-132
+      } else {
+        const uint32_t val = A.cur[s];
+        if (val == INF) continue;
+        uint32_t cand;
+        if (KIND == CYTOM_BFS) cand = val + 1;             // bfs.cc:83-91
+        else if (KIND == CYTOM_CC) cand = val;             // cc.cc:81-91
+        else cand = sat_add(val, wt[u]);                   // sssp.cc:90-99
+        if (cand < win[dl]) atomicMin(&win[dl], cand);
+      }
+    }
+  }
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
+  extern __shared__ uint32_t win[];
+  float* winf = reinterpret_cast<float*>(win);
+  __shared__ uint32_t s_job;
+  const uint32_t vpt = A.S.vpt, T = A.S.T, mask = vpt - 1;
+  // Clear next iteration's row activity (written by the apply kernel).
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x)
+    A.rowact_clear[i] = 0;
+  if (A.it->count == 0) return;  // empty frontier (uniform)
+  const uint32_t njobs = *A.njobs;
+  const int wave = threadIdx.x >> 6, nwaves = GATHER_THREADS / 64;
+  unsigned long long scanned = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_job = (uint32_t)atomicAdd(&A.it->job_cursor, 1ull);
+    __syncthreads();
+    const uint32_t j = s_job;
+    __syncthreads();
+    if (j >= njobs) break;
+    const uint4 job = A.jobs[j];
+    const uint32_t c0 = job.x, c1 = job.y, y = job.z;
+    int any = 0;
+    for (uint32_t c = c0 + threadIdx.x; c < c1; c += GATHER_THREADS) {
+      const uint32_t t = (uint32_t)(A.chunks[c] >> 32);
+      any |= A.rowact[t - y * T];
+    }
+    if (!__syncthreads_or(any)) continue;
+    const uint32_t neutral = KIND == CYTOM_PRD ? 0u : INF;
+    for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) win[i] = neutral;
+    __syncthreads();
+    for (uint32_t c = c0 + wave; c < c1; c += nwaves) {
+      const uint64_t ch = A.chunks[c];
+      const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
+      if (!A.rowact[t - y * T]) continue;
+      const uint32_t cnt = A.S.tile_cnt[t];
+      const uint32_t n = min(A.CH, cnt - s0);
+      scanned += n;
+      gather_chunk<KIND>(A, win, winf, A.S.tile_off[t] + s0, n, mask);
+    }
+    __syncthreads();
+    const uint64_t base = (uint64_t)y * vpt;
+    for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) {
+      if (KIND == CYTOM_PRD) {
+        const float f = winf[i];
+        if (f != 0.f) {
+          unsafeAtomicAdd(&A.dsum[base + i], f);
+          A.changed[base + i] = 1;
+        }
+      } else {
+        const uint32_t val = win[i];
+        if (val != INF && val < A.nxt[base + i]) {
+          const uint32_t old = atomicMin(&A.nxt[base + i], val);
+          if (val < old) A.changed[base + i] = 1;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // Every lane of a wave holds the wave's count; keep lane 0's.
+  scanned = wave_sum_u64((threadIdx.x & 63) == 0 ? scanned : 0ull);
+  if ((threadIdx.x & 63) == 0 && scanned) atomicAdd(&A.it->scanned, scanned);
+}
+
+// ---------------------------------------------------------------- apply
+// bfs.cc:101-113, cc.cc:101-111, sssp.cc:101-113: copy next into current
+// and activate on change.
+template <int KIND>
+__global__ void __launch_bounds__(APPLY_THREADS) k_apply_min(
+    uint64_t V, uint32_t log2vpt, uint32_t* __restrict__ cur, const uint32_t* __restrict__ nxt,
+    uint8_t* __restrict__ changed, uint64_t* __restrict__ act_next,
+    uint8_t* __restrict__ rowact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
+  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // V % 64 == 0
+  bool a = false;
+  unsigned long long te = 0, upd = 0;
+  if (v < V) {
+    if (changed[v]) {
+      changed[v] = 0;
+      upd = 1;
+      const uint32_t n = nxt[v];
+      if (n != cur[v]) {
+        cur[v] = n;
+        a = true;
+        te = outdeg[v];
+      }
+    }
+  }
+  const uint64_t word = __ballot(a);
+  if (v < V && (threadIdx.x & 63) == 0) {
+    act_next[v >> 6] = word;
+    if (word) rowact_next[v >> log2vpt] = 1;
+  }
+  te = block_sum<APPLY_THREADS, unsigned long long>(te);
+  const unsigned long long cnt = block_sum<APPLY_THREADS, unsigned long long>(a ? 1ull : 0ull);
+  upd = block_sum<APPLY_THREADS, unsigned long long>(upd);
+  if (threadIdx.x == 0) {
+    if (te) atomicAdd(&nit->te, te);
+    if (cnt) atomicAdd(&nit->count, cnt);
+    if (upd) atomicAdd(&(nit - 1)->updated, upd);
+  }
+}
+
+// pagerank-delta.cc:142-174 (FP32 state, double intermediates as in the
+// reference: ALPHA and the threshold are doubles).
+__global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
+    uint64_t V, uint32_t log2vpt, int first, double thr, float* __restrict__ rank,
+    float* __restrict__ delta, float* __restrict__ dsum, float* __restrict__ contrib,
+    uint8_t* __restrict__ changed, uint64_t* __restrict__ act_next,
+    uint8_t* __restrict__ rowact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
+  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool a = false;
+  unsigned long long te = 0, upd = 0;
+  if (v < V) {
+    const bool ch = changed[v] != 0;
+    upd = ch ? 1ull : 0ull;
+    if (first || ch) {
+      if (ch) changed[v] = 0;
+      const float s = dsum[v];
+      float d;
+      float r = rank[v];
+      bool applied = true;
+      if (first) {
+        d = static_cast<float>((1. - 0.85) + (0.85 * (double)s));
+        r = r + d;
+        d = (float)((double)d - 1.0);
+      } else if (s == 0.f) {
+        delta[v] = 0.f;
+        applied = false;
+      } else {
+        d = static_cast<float>(0.85 * (double)s);
+        r = r + d;
+      }
+      if (applied) {
+        rank[v] = r;
+        delta[v] = d;
+        dsum[v] = 0.f;
+        if ((double)fabsf(d) > thr * (double)r) {
+          a = true;
+          const uint32_t od = outdeg[v];
+          te = od;
+          contrib[v] = od ? d / (float)od : 0.f;
+        }
+      }
+    }
+  }
+  const uint64_t word = __ballot(a);
+  if (v < V && (threadIdx.x & 63) == 0) {
+    act_next[v >> 6] = word;
+    if (word) rowact_next[v >> log2vpt] = 1;
+  }
+  te = block_sum<APPLY_THREADS, unsigned long long>(te);
+  const unsigned long long cnt = block_sum<APPLY_THREADS, unsigned long long>(a ? 1ull : 0ull);
+  upd = block_sum<APPLY_THREADS, unsigned long long>(upd);
+  if (threadIdx.x == 0) {
+    if (te) atomicAdd(&nit->te, te);
+    if (cnt) atomicAdd(&nit->count, cnt);
+    if (upd) atomicAdd(&(nit - 1)->updated, upd);
+  }
+}
+
+// ------------------------------------------------------ deletion tests
+__global__ void k_reexec_min(int kind, const uint32_t* __restrict__ found, uint64_t n,
+                             const uint32_t* __restrict__ cur, unsigned long long* flag) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = found[3 * i], d = found[3 * i + 1], w = found[3 * i + 2];
+    const uint32_t u = cur[s], v = cur[d];
+    bool tight;
+    if (kind == CYTOM_BFS) tight = u != INF && v == u + 1;
+    else if (kind == CYTOM_SSSP) tight = u != INF && v == sat_add(u, w);
+    else tight = u == v && v != d;
+    if (tight) *flag = 1ull;
+  }
+}
+
+// pagerank-delta.cc:190-206
+__global__ void k_reexec_prd(const uint32_t* __restrict__ found, uint64_t n,
+                             const float* __restrict__ rank, const uint32_t* __restrict__ outdeg,
+                             double thr, unsigned long long* flag) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = found[3 * i], d = found[3 * i + 1];
+    const float dl = rank[s] / (float)outdeg[s];
+    if ((double)dl > thr * (double)rank[d]) *flag = 1ull;
+  }
+}
+
+// --------------------------------------------------------------- hooks
+// edgeChanged(INSERT): bfs.cc:115-131, cc.cc:113-131, sssp.cc:115-133.
+__global__ void k_hooks_min(int kind, const uint32_t* __restrict__ src,
+                            const uint32_t* __restrict__ dst, const uint32_t* __restrict__ w,
+                            const uint64_t* __restrict__ ins_slot, uint64_t n,
+                            const uint64_t* __restrict__ pool, uint32_t* __restrict__ cur,
+                            uint32_t* __restrict__ nxt, uint64_t* __restrict__ act,
+                            uint8_t* __restrict__ changed) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (pool[ins_slot[i]] == TOMB) continue;  // deleted within the batch
+    const uint32_t s = src[i], d = dst[i];
+    const uint32_t sv = nxt[s];
+    if (sv == INF) continue;
+    uint32_t cand;
+    if (kind == CYTOM_BFS) cand = sv + 1;
+    else if (kind == CYTOM_CC) cand = sv;
+    else cand = sat_add(sv, w[i]);
+    if (cand >= nxt[d]) continue;
+    const uint32_t old = atomicMin(&nxt[d], cand);
+    if (cand < old) {
+      atomicMin(&cur[d], cand);
+      atomicOr(reinterpret_cast<unsigned long long*>(&act[d >> 6]), 1ull << (d & 63));
+      changed[d] = 1;
+    }
+  }
+}
+
+// pagerank-delta.cc:176-189 (INSERT) and 190-196 (DELETE side effect).
+__global__ void k_hooks_prd(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                            const uint64_t* __restrict__ ins_slot, uint64_t n,
+                            const uint32_t* __restrict__ found, uint64_t n_found,
+                            const uint64_t* __restrict__ pool, const float* __restrict__ rank,
+                            const uint32_t* __restrict__ outdeg, float* __restrict__ dsum,
+                            uint64_t* __restrict__ act, uint8_t* __restrict__ changed, double thr) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n + n_found;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i < n) {
+      if (pool[ins_slot[i]] == TOMB) continue;
+      const uint32_t s = src[i], d = dst[i];
+      const float dl = rank[s] / (float)outdeg[s];
+      unsafeAtomicAdd(&dsum[d], dl);
+      if ((double)dl > thr * (double)rank[d]) {
+        atomicOr(reinterpret_cast<unsigned long long*>(&act[s >> 6]), 1ull << (s & 63));
+        changed[s] = 1;
+      }
+    } else {
+      const uint64_t k = i - n;
+      const uint32_t s = found[3 * k], d = found[3 * k + 1];
+      const float dl = rank[s] / (float)outdeg[s];
+      unsafeAtomicAdd(&dsum[d], -dl);
+    }
+  }
+}
+
+template <int KIND>
+void launch_gather(cytom_ctx* ctx, Program& P, int it) {
+  GatherArgs<KIND> A;
+  A.jobs = static_cast<const uint4*>(ctx->g_jobs.p);
+  A.njobs = static_cast<const uint32_t*>(ctx->g_jbase.p) + ctx->T;
+  A.chunks = static_cast<const uint64_t*>(ctx->g_chunks.p);
+  A.S = ctx->view();
+  A.act = P.act[P.cur_act];
+  A.rowact = P.rowact[P.cur_act];
+  A.rowact_clear = P.rowact[P.cur_act ^ 1];
+  A.cur = P.cur;
+  A.nxt = P.nxt;
+  A.contrib = P.contrib;
+  A.dsum = P.dsum;
+  A.changed = P.changed;
+  A.it = ctx->d_iter + it;
+  A.CH = ctx->chunk_slots;
+  const size_t lds = (size_t)ctx->vpt * 4;
+  k_gather<KIND><<<ctx->gather_blocks, GATHER_THREADS, lds, ctx->stream>>>(A);
+}
+
+int build_jobs(cytom_ctx* ctx) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  const uint64_t nt = ctx->n_tiles;
+  if ((rc = ensure(ctx, ctx->g_nch, nt * 4)) || (rc = ensure(ctx, ctx->g_cbase, (nt + 1) * 4)) ||
+      (rc = ensure(ctx, ctx->g_colj, (uint64_t)ctx->T * 4)) ||
+      (rc = ensure(ctx, ctx->g_jbase, ((uint64_t)ctx->T + 1) * 4)) ||
+      (rc = ensure(ctx, ctx->g_scan_tmp, scan_tmp_bytes(nt + 1) + 256)))
+    return rc;
+  auto* nch = static_cast<uint32_t*>(ctx->g_nch.p);
+  auto* cbase = static_cast<uint32_t*>(ctx->g_cbase.p);
+  auto* colj = static_cast<uint32_t*>(ctx->g_colj.p);
+  auto* jbase = static_cast<uint32_t*>(ctx->g_jbase.p);
+  k_nch<<<grid_for(nt, 256), 256, 0, st>>>(ctx->tile_cnt, nt, ctx->chunk_slots, nch);
+  scan_exclusive_u32(nch, cbase, nt, ctx->g_scan_tmp.p, st);
+  CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc + 32, cbase + nt, 4, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipStreamSynchronize(st));
+  const uint64_t n_chunks = (uint32_t)ctx->h_misc[32];
+  if ((rc = ensure(ctx, ctx->g_chunks, std::max<uint64_t>(n_chunks, 1) * 8))) return rc;
+  k_emit_chunks<<<grid_for(nt, 256), 256, 0, st>>>(nch, cbase, nt, ctx->chunk_slots,
+                                                   static_cast<uint64_t*>(ctx->g_chunks.p));
+  k_col_jobs<<<grid_for(ctx->T, 256), 256, 0, st>>>(cbase, ctx->T, ctx->job_chunks, colj);
+  scan_exclusive_u32(colj, jbase, ctx->T, ctx->g_scan_tmp.p, st);
+  CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc + 33, jbase + ctx->T, 4, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipStreamSynchronize(st));
+  const uint64_t n_jobs = (uint32_t)ctx->h_misc[33];
+  if ((rc = ensure(ctx, ctx->g_jobs, std::max<uint64_t>(n_jobs, 1) * 16))) return rc;
+  k_emit_jobs<<<grid_for(ctx->T, 256), 256, 0, st>>>(cbase, colj, jbase, ctx->T, ctx->job_chunks,
+                                                     static_cast<uint4*>(ctx->g_jobs.p));
+  ctx->n_chunks = n_chunks;
+  ctx->n_jobs = n_jobs;
+  ctx->jobs_dirty = false;
+  return 0;
+}
+
+}  // namespace
+
+int program_alloc(cytom_ctx* ctx, Program& P) {
+  const uint64_t V = ctx->V;
+  auto alloc = [&](void** p, size_t b) -> bool { return hipMalloc(p, b) == hipSuccess; };
+  bool ok = true;
+  if (P.kind == CYTOM_PRD) {
+    ok = ok && alloc((void**)&P.rank, V * 4) && alloc((void**)&P.delta, V * 4) &&
+         alloc((void**)&P.dsum, V * 4) && alloc((void**)&P.contrib, V * 4);
+  } else {
+    ok = ok && alloc((void**)&P.cur, V * 4) && alloc((void**)&P.nxt, V * 4);
+  }
+  for (int i = 0; i < 2 && ok; ++i)
+    ok = alloc((void**)&P.act[i], (V / 64) * 8) && alloc((void**)&P.rowact[i], ctx->T);
+  ok = ok && alloc((void**)&P.changed, V);
+  if (!ok) {
+    program_free(P);
+    return set_err(ctx, CYTOM_ENOMEM, "program state allocation failed");
+  }
+  return 0;
+}
+
+void program_free(Program& P) {
+  void* ps[] = {P.cur, P.nxt, P.rank, P.delta, P.dsum, P.contrib, P.act[0], P.act[1],
+                P.rowact[0], P.rowact[1], P.changed};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  P.cur = P.nxt = nullptr;
+  P.rank = P.delta = P.dsum = P.contrib = nullptr;
+  P.act[0] = P.act[1] = nullptr;
+  P.rowact[0] = P.rowact[1] = nullptr;
+  P.changed = nullptr;
+}
+
+int program_init(cytom_ctx* ctx, Program& P) {
+  hipStream_t st = ctx->stream;
+  P.cur_act = 0;
+  P.first_iter = true;
+  P.fixpoint = false;
+  if (P.kind == CYTOM_PRD) {
+    k_init_prd<<<grid_for(ctx->V, 256), 256, 0, st>>>(ctx->V, P.rank, P.delta, P.dsum, P.act[0],
+                                                      P.changed);
+  } else {
+    k_init_min<<<grid_for(ctx->V, 256), 256, 0, st>>>(P.kind, ctx->V, P.source, P.cur, P.nxt,
+                                                      P.act[0], P.changed);
+  }
+  CY_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+void launch_reexec_check(cytom_ctx* ctx, Program& P, const uint32_t* found, uint64_t n_found,
+                         unsigned long long* flag) {
+  if (P.kind == CYTOM_PRD)
+    k_reexec_prd<<<grid_for(n_found, 256), 256, 0, ctx->stream>>>(found, n_found, P.rank,
+                                                                  ctx->outdeg, ctx->delta, flag);
+  else
+    k_reexec_min<<<grid_for(n_found, 256), 256, 0, ctx->stream>>>(P.kind, found, n_found, P.cur,
+                                                                  flag);
+}
+
+void launch_hooks(cytom_ctx* ctx, Program& P, const uint32_t* d_src, const uint32_t* d_dst,
+                  const uint32_t* d_w, const uint64_t* ins_slot, uint64_t n_ins,
+                  const uint32_t* found, uint64_t n_found) {
+  hipStream_t st = ctx->stream;
+  if (P.kind == CYTOM_PRD) {
+    if (n_ins + n_found)
+      k_hooks_prd<<<grid_for(n_ins + n_found, 256), 256, 0, st>>>(
+          d_src, d_dst, ins_slot, n_ins, found, n_found, ctx->pool, P.rank, ctx->outdeg, P.dsum,
+          P.act[P.cur_act], P.changed, ctx->delta);
+  } else if (n_ins) {
+    k_hooks_min<<<grid_for(n_ins, 256), 256, 0, st>>>(P.kind, d_src, d_dst, d_w, ins_slot, n_ins,
+                                                      ctx->pool, P.cur, P.nxt, P.act[P.cur_act],
+                                                      P.changed);
+  }
+  P.fixpoint = false;
+}
+
+int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  if (ctx->jobs_dirty && (rc = build_jobs(ctx))) return rc;
+  const uint32_t maxit = ctx->max_iter;
+  CY_HIP(ctx, hipMemsetAsync(ctx->d_iter, 0, sizeof(IterStats) * (maxit + 2), st));
+  CY_HIP(ctx, hipMemsetAsync(P.rowact[P.cur_act], 0, ctx->T, st));
+  CY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+  k_frontier<<<grid_for(ctx->V, APPLY_THREADS, 8192), APPLY_THREADS, 0, st>>>(
+      P.kind, ctx->V, ctx->log2vpt, P.act[P.cur_act], ctx->outdeg, P.rowact[P.cur_act], P.delta,
+      P.contrib, ctx->d_iter);
+  const uint32_t apply_blocks = (uint32_t)((ctx->V + APPLY_THREADS - 1) / APPLY_THREADS);
+  uint32_t it = 0;
+  bool converged = false;
+  const bool prof = ctx->profile && ctx->kev.size() >= 2 * (size_t)maxit;
+  while (it < maxit) {
+    if (prof) CY_HIP(ctx, hipEventRecord(ctx->kev[2 * it], st));
+    switch (P.kind) {
+      case CYTOM_BFS: launch_gather<CYTOM_BFS>(ctx, P, it); break;
+      case CYTOM_CC: launch_gather<CYTOM_CC>(ctx, P, it); break;
+      case CYTOM_SSSP: launch_gather<CYTOM_SSSP>(ctx, P, it); break;
+      default: launch_gather<CYTOM_PRD>(ctx, P, it); break;
+    }
+    if (prof) CY_HIP(ctx, hipEventRecord(ctx->kev[2 * it + 1], st));
+    const int nx = P.cur_act ^ 1;
+    if (P.kind == CYTOM_PRD) {
+      k_apply_prd<<<apply_blocks, APPLY_THREADS, 0, st>>>(
+          ctx->V, ctx->log2vpt, P.first_iter ? 1 : 0, ctx->delta, P.rank, P.delta, P.dsum,
+          P.contrib, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg, ctx->d_iter + it + 1);
+    } else if (P.kind == CYTOM_BFS) {
+      k_apply_min<CYTOM_BFS><<<apply_blocks, APPLY_THREADS, 0, st>>>(
+          ctx->V, ctx->log2vpt, P.cur, P.nxt, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg,
+          ctx->d_iter + it + 1);
+    } else if (P.kind == CYTOM_CC) {
+      k_apply_min<CYTOM_CC><<<apply_blocks, APPLY_THREADS, 0, st>>>(
+          ctx->V, ctx->log2vpt, P.cur, P.nxt, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg,
+          ctx->d_iter + it + 1);
+    } else {
+      k_apply_min<CYTOM_SSSP><<<apply_blocks, APPLY_THREADS, 0, st>>>(
+          ctx->V, ctx->log2vpt, P.cur, P.nxt, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg,
+          ctx->d_iter + it + 1);
+    }
+    P.cur_act = nx;
+    P.first_iter = false;
+    ++it;
+    CY_HIP(ctx, hipMemcpyAsync(&ctx->h_iter[it], &ctx->d_iter[it], sizeof(IterStats),
+                               hipMemcpyDeviceToHost, st));
+    CY_HIP(ctx, hipStreamSynchronize(st));
+    if (ctx->h_iter[it].count == 0) {
+      converged = true;
+      break;
+    }
+  }
+  CY_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+  CY_HIP(ctx, hipMemcpyAsync(ctx->h_iter, ctx->d_iter, sizeof(IterStats) * (it + 1),
+                             hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipStreamSynchronize(st));
+  CY_HIP(ctx, hipGetLastError());
+  P.fixpoint = converged;
+  if (prof) {
+    // Algorithmic bytes of one gather launch: 8 B per traversed packed edge
+    // (+4 B weight for SSSP), the frontier's vertex state (4 B, or the 12 B
+    // PageRank-delta vertex), a read-modify-write of every updated target and
+    // one pass over the active bitmap.
+    const double eb = P.kind == CYTOM_SSSP ? 12.0 : 8.0;
+    const double sv = P.kind == CYTOM_PRD ? 12.0 : 4.0;
+    for (uint32_t i = 0; i < it; ++i) {
+      float ms = 0.f;
+      CY_HIP(ctx, hipEventElapsedTime(&ms, ctx->kev[2 * i], ctx->kev[2 * i + 1]));
+      const IterStats& s = ctx->h_iter[i];
+      ctx->gather_ms[P.kind] += ms;
+      ctx->gather_launches[P.kind] += 1;
+      ctx->gather_alg_bytes[P.kind] += (double)s.te * eb + (double)s.count * sv +
+                                       (double)s.updated * 2.0 * sv + (double)ctx->V / 8.0;
+    }
+  }
+  if (out) {
+    unsigned long long te = 0, act = 0, sc = 0;
+    for (uint32_t i = 0; i < it; ++i) {
+      te += ctx->h_iter[i].te;
+      act += ctx->h_iter[i].count;
+      sc += ctx->h_iter[i].scanned;
+    }
+    float ms = 0.f;
+    CY_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+    out->iterations = it;
+    out->converged = converged ? 1 : 0;
+    out->reexecuted = (uint32_t)P.last_reexec;
+    out->pad = 0;
+    out->traversed_edges = te;
+    out->active_vertices = act;
+    out->slots_scanned = sc;
+    out->t_compute_ms = ms;
+  }
+  return 0;
+}
+
+int configure_gather(cytom_ctx* ctx) {
+  const size_t lds = (size_t)ctx->vpt * 4;
+  CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_BFS>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_CC>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_SSSP>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_PRD>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const uint32_t per_cu = (uint32_t)std::max<size_t>(1, (160 * 1024) / std::max<size_t>(lds, 1));
+  ctx->gather_blocks = (uint32_t)ctx->num_cus * std::min<uint32_t>(per_cu, 2);
+  return 0;
+}
+
+}  // namespace cytom

@@ -1,0 +1,227 @@
+// Internal declarations shared by the HIP sources of libcytom_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cytom.h"
+
+namespace cytom {
+
+constexpr uint32_t INF = 0xFFFFFFFFu;
+// Packed edge slot: src in the low word, dst in the high word.  A deleted
+// slot holds TOMB (src = dst = 0xFFFFFFFF, never a valid id: V <= 2^31).
+constexpr uint64_t TOMB = ~0ull;
+constexpr uint64_t MAX_VERTICES = 1ull << 31;
+
+__host__ __device__ inline uint64_t pack_edge(uint32_t s, uint32_t d) {
+  return (uint64_t)s | ((uint64_t)d << 32);
+}
+
+// Ligra's 64-bit integer hash (reference util.h:188-196).
+__host__ __device__ inline uint64_t hash64(uint64_t a) {
+  a = (a + 0x7ed55d166bef7a1dull) + (a << 12);
+  a = (a ^ 0xc761c23c510fa2ddull) ^ (a >> 9);
+  a = (a + 0x165667b183a9c0e1ull) + (a << 59);
+  a = (a + 0xd3a2646cab3487e3ull) ^ (a << 49);
+  a = (a + 0xfd7046c5ef9ab54cull) + (a << 3);
+  a = (a ^ 0xb55a4f090dd4a67bull) ^ (a >> 32);
+  return a;
+}
+
+// Stream salts: weights and deletion draws (no reference counterpart).
+constexpr uint64_t WEIGHT_SALT = 0x7765696768747321ull;
+constexpr uint64_t DELETE_SALT = 0x64656c6574657321ull;
+
+__host__ __device__ inline uint32_t sat_add(uint32_t a, uint32_t b) {
+  uint64_t s = (uint64_t)a + b;
+  return s >= INF ? INF : (uint32_t)s;
+}
+
+// ---------------------------------------------------------------- store view
+// Device-resident tiled edge store.  Tile t = y*T + x holds edges with
+// src in [x*vpt, (x+1)*vpt) and dst in [y*vpt, (y+1)*vpt) (reference
+// tile-store.h:88-96 with the row_first.cc:6-11 traversal order).
+struct StoreView {
+  uint64_t V;
+  uint32_t vpt, T, log2vpt, pad;
+  uint64_t n_tiles;
+  uint64_t* tile_off;   // first slot of the tile's block in the pool
+  uint32_t* tile_cap;   // slots reserved for the tile
+  uint32_t* tile_cnt;   // slots used (live + tombstones)
+  uint32_t* tile_live;  // live edges
+  uint64_t* pool;       // packed edges
+  uint32_t* wpool;      // per-slot weights (nullptr when unweighted)
+  uint32_t* outdeg;
+  uint32_t* indeg;
+};
+
+// Per-program device state.
+struct Program {
+  int kind = 0;
+  uint32_t source = 0;
+  uint32_t* cur = nullptr;  // BFS/CC/SSSP current values
+  uint32_t* nxt = nullptr;  // BFS/CC/SSSP next values
+  float* rank = nullptr;    // PRD next.rank
+  float* delta = nullptr;   // PRD current.delta
+  float* dsum = nullptr;    // PRD next.delta_sum
+  float* contrib = nullptr; // PRD delta/out_degree of active vertices
+  uint64_t* act[2] = {nullptr, nullptr};  // active bitmaps (wave-sized words)
+  uint8_t* rowact[2] = {nullptr, nullptr};  // per source-row activity
+  uint8_t* changed = nullptr;               // per-vertex changed flags
+  int cur_act = 0;
+  bool first_iter = true;
+  bool fixpoint = false;
+  bool frontier_stale = true;
+  int last_reexec = 0;
+};
+
+// Per-iteration device counters.
+struct IterStats {
+  unsigned long long te;       // live out-edges of the frontier
+  unsigned long long count;    // frontier size
+  unsigned long long scanned;  // edge slots read by the gather
+  unsigned long long updated;  // targets whose value the gather changed
+  unsigned long long job_cursor;
+  unsigned long long pad[3];
+};
+
+struct GrowBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace cytom
+
+struct cytom_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint64_t V = 0;
+  uint32_t vpt = 0, T = 0, log2vpt = 0;
+  uint64_t n_tiles = 0;
+  uint32_t max_iter = 100;
+  double delta = 0.0;
+  bool weighted = false;
+  // store
+  uint64_t* tile_off = nullptr;
+  uint32_t* tile_cap = nullptr;
+  uint32_t* tile_cnt = nullptr;
+  uint32_t* tile_live = nullptr;
+  uint64_t* pool = nullptr;
+  uint32_t* wpool = nullptr;
+  uint64_t pool_size = 0;
+  uint32_t* outdeg = nullptr;
+  uint32_t* indeg = nullptr;
+  uint64_t live_edges = 0;
+  // device scalars: [0] pool_top, [1..] scratch counters
+  unsigned long long* d_misc = nullptr;
+  unsigned long long* h_misc = nullptr;  // pinned mirror
+  // ingest scratch
+  cytom::GrowBuf g_keys, g_vals, g_keys2, g_vals2, g_flags, g_scan, g_seg_start, g_seg_tile,
+      g_reloc_cap, g_reloc_list, g_ins_slot, g_scan_tmp, g_hist, g_in_src, g_in_dst, g_in_w,
+      g_del_src, g_del_dst, g_del_w, g_tab_key, g_tab_head, g_node_next, g_claimed,
+      g_tile_flag, g_del_tiles, g_found, g_tile_scan;
+  // execution work lists
+  cytom::GrowBuf g_nch, g_cbase, g_chunks, g_colj, g_jbase, g_jobs;
+  bool jobs_dirty = true;
+  uint64_t n_chunks = 0, n_jobs = 0;
+  uint32_t chunk_slots = 512;   // slots per chunk
+  uint32_t job_chunks = 128;    // chunks per column job
+  uint32_t gather_threads = 512;
+  uint32_t gather_blocks = 0;   // persistent grid
+  int num_cus = 256;
+  cytom::IterStats* d_iter = nullptr;  // [max_iter + 2]
+  cytom::IterStats* h_iter = nullptr;  // pinned
+  std::vector<cytom::Program> progs;
+  std::string err;
+  uint64_t batches = 0;
+  // Optional per-kernel timing of the gather launches (HIP events on the
+  // context stream), accumulated per program kind.
+  bool profile = false;
+  std::vector<hipEvent_t> kev;
+  double gather_ms[4] = {0, 0, 0, 0};
+  uint64_t gather_launches[4] = {0, 0, 0, 0};
+  double gather_alg_bytes[4] = {0, 0, 0, 0};
+
+  cytom::StoreView view() const {
+    cytom::StoreView s;
+    s.V = V;
+    s.vpt = vpt;
+    s.T = T;
+    s.log2vpt = log2vpt;
+    s.pad = 0;
+    s.n_tiles = n_tiles;
+    s.tile_off = tile_off;
+    s.tile_cap = tile_cap;
+    s.tile_cnt = tile_cnt;
+    s.tile_live = tile_live;
+    s.pool = pool;
+    s.wpool = wpool;
+    s.outdeg = outdeg;
+    s.indeg = indeg;
+    return s;
+  }
+};
+
+namespace cytom {
+
+int set_err(cytom_ctx* ctx, int code, const std::string& msg);
+int hip_fail(cytom_ctx* ctx, hipError_t e, const char* what);
+// Grow-only device buffer; returns 0 or a negative error.
+int ensure(cytom_ctx* ctx, GrowBuf& b, size_t bytes);
+
+#define CY_HIP(ctx, call)                              \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return cytom::hip_fail(ctx, e_, #call); \
+  } while (0)
+
+// ------------------------------------------------------------ primitives
+// Exclusive scan of n values into out[0..n]; out[n] holds the total.
+// tmp must hold scan_tmp_bytes(n) bytes (valid for both widths).
+size_t scan_tmp_bytes(uint64_t n);
+void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp,
+                        hipStream_t st);
+void scan_exclusive_u64(const uint64_t* in, uint64_t* out, uint64_t n, void* tmp,
+                        hipStream_t st);
+// Stable LSD radix sort of (key, value) pairs on the low key_bits bits.
+// Result lands in (k_out, v_out); (k_tmp, v_tmp) are scratch of n entries;
+// hist must hold radix_hist_bytes(n) bytes.
+size_t radix_hist_bytes(uint64_t n);
+void radix_sort_pairs(const uint32_t* k_in, const uint32_t* v_in, uint32_t* k_out,
+                      uint32_t* v_out, uint32_t* k_tmp, uint32_t* v_tmp, uint64_t n,
+                      uint32_t key_bits, void* hist, void* scan_tmp, hipStream_t st);
+
+// ------------------------------------------------------------ stream gen
+void launch_rmat(uint32_t scale, uint64_t seed, uint64_t first, const uint64_t* index,
+                 uint64_t n, uint32_t* src, uint32_t* dst, uint32_t* w, hipStream_t st);
+
+// ------------------------------------------------------------ store ops
+int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
+                 const uint32_t* d_w, uint64_t n_ins, const uint32_t* d_dsrc,
+                 const uint32_t* d_ddst, const uint32_t* d_dw, uint64_t n_del,
+                 cytom_batch_stats* st);
+int64_t export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_t* w, uint64_t cap);
+
+// ------------------------------------------------------------ executors
+int program_alloc(cytom_ctx* ctx, Program& P);
+void program_free(Program& P);
+int program_init(cytom_ctx* ctx, Program& P);
+int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* st);
+// Sets the gather kernels' LDS limit and persistent grid for the context.
+int configure_gather(cytom_ctx* ctx);
+// Re-execution tests of the found deletions against the program's values;
+// sets *flag (device) to 1 when a re-initialisation is needed.
+void launch_reexec_check(cytom_ctx* ctx, Program& P, const uint32_t* found, uint64_t n_found,
+                         unsigned long long* flag);
+// edgeChanged(INSERT) hooks of the batch's surviving inserts and the
+// PRD edgeChanged(DELETE) side effect.
+void launch_hooks(cytom_ctx* ctx, Program& P, const uint32_t* d_src, const uint32_t* d_dst,
+                  const uint32_t* d_w, const uint64_t* ins_slot, uint64_t n_ins,
+                  const uint32_t* found, uint64_t n_found);
+
+}  // namespace cytom

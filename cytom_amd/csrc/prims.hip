@@ -1,0 +1,196 @@
+// Device primitives: exclusive scan and stable LSD radix sort (gfx950,
+// 64-lane wavefronts).  Used by ingestion (sort of updates by tile key,
+// segment heads) and by the executor work-list builder.
+#include "internal.h"
+#include "devutil.h"
+
+namespace cytom {
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr uint64_t SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 2048
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const T* __restrict__ in,
+                                                              uint64_t n, T* __restrict__ bsum) {
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  T s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    uint64_t i = base + k;
+    if (i < n) s += in[i];
+  }
+  T tot = block_sum<SCAN_THREADS, T>(s);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(const T* __restrict__ in,
+                                                            T* __restrict__ out, uint64_t n,
+                                                            const T* __restrict__ boff) {
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  T v[SCAN_ITEMS];
+  T s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    uint64_t i = base + k;
+    v[k] = i < n ? in[i] : T(0);
+    s += v[k];
+  }
+  T tot;
+  T p = block_scan_excl<SCAN_THREADS, T>(s, &tot);
+  const T off = boff ? boff[blockIdx.x] : T(0);
+  p += off;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    uint64_t i = base + k;
+    if (i < n) out[i] = p;
+    p += v[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = off + tot;
+}
+
+template <typename T>
+void scan_impl(const T* in, T* out, uint64_t n, void* tmp, hipStream_t st) {
+  if (n == 0) {
+    hipMemsetAsync(out, 0, sizeof(T), st);
+    return;
+  }
+  if (n <= SCAN_TILE) {
+    k_scan_down<T><<<1, SCAN_THREADS, 0, st>>>(in, out, n, nullptr);
+    return;
+  }
+  uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  T* bsum = static_cast<T*>(tmp);
+  T* boff = reinterpret_cast<T*>(static_cast<char*>(tmp) + align256(nb * sizeof(T)));
+  void* rest =
+      static_cast<char*>(tmp) + align256(nb * sizeof(T)) + align256((nb + 1) * sizeof(T));
+  k_scan_reduce<T><<<(uint32_t)nb, SCAN_THREADS, 0, st>>>(in, n, bsum);
+  scan_impl<T>(bsum, boff, nb, rest, st);
+  k_scan_down<T><<<(uint32_t)nb, SCAN_THREADS, 0, st>>>(in, out, n, boff);
+}
+
+// ------------------------------------------------------------ radix sort
+constexpr int RS_THREADS = 256;
+constexpr int RS_ROUNDS = 16;
+constexpr uint64_t RS_TILE = RS_THREADS * RS_ROUNDS;  // 4096 keys per block
+
+__global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t* __restrict__ keys,
+                                                           uint64_t n, uint32_t shift,
+                                                           uint32_t* __restrict__ hist,
+                                                           uint32_t nb) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    uint64_t i = base + (uint64_t)r * RS_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: ranks inside a round of 256 keys come from wavefront
+// ballots (lanes with equal digits), ordered by wave then lane.
+__global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t n,
+    uint32_t shift, const uint32_t* __restrict__ hscan, uint32_t nb,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  run[tid] = hscan[(uint64_t)tid * nb + blockIdx.x];
+  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    const uint64_t i = base + (uint64_t)r * RS_THREADS + tid;
+    if (base + (uint64_t)r * RS_THREADS >= n) break;  // uniform across the block
+    const bool valid = i < n;
+    const uint32_t k = valid ? kin[i] : 0u;
+    const uint32_t v = valid ? vin[i] : 0u;
+    const uint32_t d = (k >> shift) & 255u;
+#pragma unroll
+    for (int w = 0; w < RS_THREADS / 64; ++w) wcnt[w][tid] = 0;
+    __syncthreads();
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[d] + rank;
+      for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < RS_THREADS / 64; ++w) add += wcnt[w][tid];
+    run[tid] += add;
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+size_t scan_tmp_bytes(uint64_t n) {
+  if (n <= SCAN_TILE) return 0;
+  uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  return align256(nb * 8) + align256((nb + 1) * 8) + scan_tmp_bytes(nb);
+}
+
+void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp,
+                        hipStream_t st) {
+  scan_impl<uint32_t>(in, out, n, tmp, st);
+}
+
+void scan_exclusive_u64(const uint64_t* in, uint64_t* out, uint64_t n, void* tmp,
+                        hipStream_t st) {
+  scan_impl<uint64_t>(in, out, n, tmp, st);
+}
+
+size_t radix_hist_bytes(uint64_t n) {
+  uint64_t nb = (n + RS_TILE - 1) / RS_TILE;
+  if (nb == 0) nb = 1;
+  // histogram + its scan (with the total slot)
+  return align256(256 * nb * 4) + align256((256 * nb + 1) * 4);
+}
+
+void radix_sort_pairs(const uint32_t* k_in, const uint32_t* v_in, uint32_t* k_out,
+                      uint32_t* v_out, uint32_t* k_tmp, uint32_t* v_tmp, uint64_t n,
+                      uint32_t key_bits, void* hist, void* scan_tmp, hipStream_t st) {
+  if (n == 0) return;
+  const uint32_t passes = (key_bits + 7) / 8;
+  if (passes == 0) {
+    hipMemcpyAsync(k_out, k_in, n * 4, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(v_out, v_in, n * 4, hipMemcpyDeviceToDevice, st);
+    return;
+  }
+  const uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+  uint32_t* h = static_cast<uint32_t*>(hist);
+  uint32_t* hs = reinterpret_cast<uint32_t*>(static_cast<char*>(hist) + align256(256ull * nb * 4));
+  const uint32_t* ks = k_in;
+  const uint32_t* vs = v_in;
+  for (uint32_t p = 0; p < passes; ++p) {
+    const bool last_to_out = ((passes - p) & 1u) == 1u;
+    uint32_t* kd = last_to_out ? k_out : k_tmp;
+    uint32_t* vd = last_to_out ? v_out : v_tmp;
+    k_radix_hist<<<nb, RS_THREADS, 0, st>>>(ks, n, 8 * p, h, nb);
+    scan_exclusive_u32(h, hs, 256ull * nb, scan_tmp, st);
+    k_radix_scatter<<<nb, RS_THREADS, 0, st>>>(ks, vs, n, 8 * p, hs, nb, kd, vd);
+    ks = kd;
+    vs = vd;
+  }
+}
+
+}  // namespace cytom

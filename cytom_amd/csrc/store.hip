@@ -1,0 +1,612 @@
+// Device-resident tiled edge store: batched insertion and deletion.
+//
+// Replaces, for one update batch, the provider routing
+// (i-edge-provider.h:81-155), MetaTileManager::insertEdge / the deletion
+// loop (meta-tile-manager.h:112-231), TileManager::insertEdge /
+// deleteRandomEdge / compactStorage (tile-manager.h:136-291) and
+// TileStore::doGlobalCompaction (tile-store.h:144-172).
+//
+// Layout in HBM: a tile index (offset, capacity, used slots, live edges per
+// tile) and one pool of packed 8-byte edges (src low word, dst high word)
+// plus a parallel u32 weight pool.  Each tile owns a contiguous block of
+// the pool with slack at its end.
+//
+// Insertion: the batch is radix-sorted by tile key, runs of equal keys are
+// the per-tile segments; tiles whose slack is too small are relocated to a
+// fresh block (dropping tombstones on the way); then every segment is
+// scattered into its tile's slack.  Deletion: the requested edges go into
+// an open-addressing table keyed by (src, dst); the tiles that hold a
+// request are scanned once and each live slot claims one matching request
+// (a CAS on the request), the claimed slot becomes a tombstone.  Tiles left
+// with many tombstones are compacted in place.
+#include <algorithm>
+
+#include "devutil.h"
+#include "internal.h"
+
+namespace cytom {
+namespace {
+
+constexpr uint64_t EMPTY64 = ~0ull;
+constexpr uint32_t NONE32 = 0xFFFFFFFFu;
+
+enum Misc : int {
+  M_POOL_TOP = 0,
+  M_ERR = 1,
+  M_NEED = 2,
+  M_NRELOC = 3,
+  M_RELOC_CUR = 4,
+  M_NFOUND = 5,
+  M_NCOMPACT = 6,
+  M_REEXEC = 8,  // one flag per program from here
+};
+
+__host__ __device__ inline uint32_t grow_cap(uint64_t n) {
+  uint64_t c = n + n / 2 + 64;
+  c = (c + 63) & ~63ull;
+  return c > 0xFFFFFFC0ull ? 0xFFFFFFC0u : (uint32_t)c;
+}
+
+__device__ inline uint32_t tile_of(uint32_t s, uint32_t d, uint32_t lv, uint32_t T) {
+  return (d >> lv) * T + (s >> lv);
+}
+
+// ---- validation and keys
+__global__ void k_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                       const uint32_t* __restrict__ w, uint64_t n, uint64_t V, uint32_t lv,
+                       uint32_t T, int need_w, uint32_t* __restrict__ keys,
+                       uint32_t* __restrict__ vals, unsigned long long* misc) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = src[i], d = dst[i];
+    const bool bad = s >= V || d >= V || (need_w && w[i] == 0u);
+    if (bad) atomicOr(&misc[M_ERR], 1ull);
+    keys[i] = bad ? 0u : tile_of(s, d, lv, T);
+    vals[i] = (uint32_t)i;
+  }
+}
+
+__global__ void k_del_validate(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                               uint64_t n, uint64_t V, unsigned long long* misc) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (src[i] >= V || dst[i] >= V) atomicOr(&misc[M_ERR], 2ull);
+  }
+}
+
+// ---- segments of the sorted batch
+__global__ void k_heads(const uint32_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ f) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    f[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+__global__ void k_seg_emit(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ f,
+                           const uint32_t* __restrict__ sx, uint64_t n,
+                           uint32_t* __restrict__ seg_start, uint32_t* __restrict__ seg_tile) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (f[i]) {
+      seg_start[sx[i]] = (uint32_t)i;
+      seg_tile[sx[i]] = keys[i];
+    }
+    if (i == n - 1) seg_start[sx[n]] = (uint32_t)n;
+  }
+}
+
+// ---- slack planning: which tiles must move to a bigger block
+__global__ void k_plan(const uint32_t* __restrict__ nseg_p, const uint32_t* __restrict__ seg_start,
+                       const uint32_t* __restrict__ seg_tile, StoreView S,
+                       uint32_t* __restrict__ reloc_cap, unsigned long long* misc) {
+  const uint32_t nseg = *nseg_p;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x) {
+    const uint32_t len = seg_start[s + 1] - seg_start[s];
+    const uint32_t t = seg_tile[s];
+    uint32_t nc = 0;
+    if ((uint64_t)S.tile_cnt[t] + len > S.tile_cap[t]) {
+      nc = grow_cap((uint64_t)S.tile_live[t] + len);
+      atomicAdd(&misc[M_NEED], (unsigned long long)nc);
+      atomicAdd(&misc[M_NRELOC], 1ull);
+    }
+    reloc_cap[s] = nc;
+  }
+}
+
+__global__ void k_reloc_commit(const uint32_t* __restrict__ nseg_p,
+                               const uint32_t* __restrict__ seg_tile,
+                               const uint32_t* __restrict__ reloc_cap,
+                               unsigned long long* __restrict__ reloc_list,
+                               unsigned long long* misc) {
+  const uint32_t nseg = *nseg_p;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x) {
+    const uint32_t nc = reloc_cap[s];
+    if (!nc) continue;
+    const unsigned long long off = atomicAdd(&misc[M_POOL_TOP], (unsigned long long)nc);
+    const unsigned long long k = atomicAdd(&misc[M_RELOC_CUR], 1ull);
+    reloc_list[3 * k + 0] = seg_tile[s];
+    reloc_list[3 * k + 1] = off;
+    reloc_list[3 * k + 2] = nc;
+  }
+}
+
+// Stable copy of a tile's live slots to dst_off, 1024 slots per step.
+constexpr int CP_THREADS = 256;
+constexpr int CP_ITEMS = 4;
+__device__ inline uint32_t copy_live(const uint64_t* __restrict__ pool_in,
+                                     const uint32_t* __restrict__ w_in, uint64_t src_off,
+                                     uint32_t cnt, uint64_t* __restrict__ pool_out,
+                                     uint32_t* __restrict__ w_out, uint64_t dst_off) {
+  uint32_t written = 0;
+  for (uint32_t base = 0; base < cnt; base += CP_THREADS * CP_ITEMS) {
+    uint64_t e[CP_ITEMS];
+    uint32_t ww[CP_ITEMS];
+    uint32_t nlive = 0;
+#pragma unroll
+    for (int k = 0; k < CP_ITEMS; ++k) {
+      const uint32_t i = base + threadIdx.x * CP_ITEMS + k;
+      e[k] = i < cnt ? pool_in[src_off + i] : TOMB;
+      ww[k] = (w_in && i < cnt) ? w_in[src_off + i] : 0u;
+      nlive += e[k] != TOMB;
+    }
+    uint32_t tot;
+    uint32_t p = block_scan_excl<CP_THREADS, uint32_t>(nlive, &tot);
+#pragma unroll
+    for (int k = 0; k < CP_ITEMS; ++k) {
+      if (e[k] != TOMB) {
+        pool_out[dst_off + written + p] = e[k];
+        if (w_out) w_out[dst_off + written + p] = ww[k];
+        ++p;
+      }
+    }
+    written += tot;
+    __syncthreads();
+  }
+  return written;
+}
+
+__global__ void __launch_bounds__(CP_THREADS) k_relocate(const unsigned long long* __restrict__ rl,
+                                                         uint32_t nreloc, StoreView S) {
+  for (uint32_t b = blockIdx.x; b < nreloc; b += gridDim.x) {
+    const uint32_t t = (uint32_t)rl[3 * b];
+    const uint64_t noff = rl[3 * b + 1];
+    const uint32_t ncap = (uint32_t)rl[3 * b + 2];
+    const uint32_t live = copy_live(S.pool, S.wpool, S.tile_off[t], S.tile_cnt[t], S.pool, S.wpool,
+                                    noff);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      S.tile_off[t] = noff;
+      S.tile_cap[t] = ncap;
+      S.tile_cnt[t] = live;
+    }
+  }
+}
+
+// Global repack into a new pool (TileStore::doGlobalCompaction analogue).
+__global__ void k_repack_caps(StoreView S, uint64_t* __restrict__ ncap) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < S.n_tiles;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t live = S.tile_live[t];
+    ncap[t] = live ? grow_cap(live) : 0u;
+  }
+}
+
+__global__ void __launch_bounds__(CP_THREADS) k_repack(StoreView S, const uint64_t* __restrict__ noff,
+                                                       uint64_t* __restrict__ npool,
+                                                       uint32_t* __restrict__ nwpool) {
+  for (uint64_t t = blockIdx.x; t < S.n_tiles; t += gridDim.x) {
+    const uint32_t cnt = S.tile_cnt[t];
+    uint32_t live = 0;
+    if (cnt) live = copy_live(S.pool, S.wpool, S.tile_off[t], cnt, npool, nwpool, noff[t]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      S.tile_off[t] = noff[t];
+      S.tile_cap[t] = (uint32_t)(noff[t + 1] - noff[t]);
+      S.tile_cnt[t] = live;
+    }
+  }
+}
+
+// ---- segmented scatter into slack
+__global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                          uint64_t n, const uint32_t* __restrict__ f,
+                          const uint32_t* __restrict__ sx, const uint32_t* __restrict__ seg_start,
+                          const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                          const uint32_t* __restrict__ w, StoreView S,
+                          uint64_t* __restrict__ ins_slot) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t seg = sx[i] + f[i] - 1u;
+    const uint32_t pos = (uint32_t)i - seg_start[seg];
+    const uint32_t t = keys[i];
+    const uint32_t v = vals[i];
+    const uint32_t s = src[v], d = dst[v];
+    const uint64_t slot = S.tile_off[t] + S.tile_cnt[t] + pos;
+    S.pool[slot] = pack_edge(s, d);
+    if (S.wpool) S.wpool[slot] = w ? w[v] : 0u;
+    ins_slot[v] = slot;
+    atomicAdd(&S.outdeg[s], 1u);
+    atomicAdd(&S.indeg[d], 1u);
+  }
+}
+
+__global__ void k_seg_commit(const uint32_t* __restrict__ nseg_p,
+                             const uint32_t* __restrict__ seg_start,
+                             const uint32_t* __restrict__ seg_tile, StoreView S) {
+  const uint32_t nseg = *nseg_p;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x) {
+    const uint32_t len = seg_start[s + 1] - seg_start[s];
+    const uint32_t t = seg_tile[s];
+    S.tile_cnt[t] += len;
+    S.tile_live[t] += len;
+  }
+}
+
+// ---- deletions
+__global__ void k_del_build(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                            uint64_t m, unsigned long long* __restrict__ tab_key,
+                            uint32_t* __restrict__ tab_head, uint64_t hmask,
+                            uint32_t* __restrict__ node_next, uint32_t* __restrict__ tile_flag,
+                            uint32_t lv, uint32_t T) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = src[i], d = dst[i];
+    const unsigned long long key = pack_edge(s, d);
+    uint64_t h = hash64(key) & hmask;
+    for (;;) {
+      const unsigned long long prev = atomicCAS(&tab_key[h], (unsigned long long)EMPTY64, key);
+      if (prev == EMPTY64 || prev == key) break;
+      h = (h + 1) & hmask;
+    }
+    node_next[i] = atomicExch(&tab_head[h], (uint32_t)i);
+    tile_flag[tile_of(s, d, lv, T)] = 1u;
+  }
+}
+
+__global__ void k_flag_emit(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ fx,
+                            uint64_t n, uint32_t* __restrict__ list) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    if (flag[i]) list[fx[i]] = (uint32_t)i;
+}
+
+__global__ void __launch_bounds__(256) k_del_match(const uint32_t* __restrict__ tiles,
+                                                   const uint32_t* __restrict__ ntiles_p,
+                                                   StoreView S,
+                                                   const unsigned long long* __restrict__ tab_key,
+                                                   const uint32_t* __restrict__ tab_head,
+                                                   uint64_t hmask,
+                                                   const uint32_t* __restrict__ node_next,
+                                                   const uint32_t* __restrict__ dw,
+                                                   unsigned long long* __restrict__ claimed) {
+  const uint32_t ntiles = *ntiles_p;
+  for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+    const uint32_t t = tiles[b];
+    const uint64_t off = S.tile_off[t];
+    const uint32_t cnt = S.tile_cnt[t];
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const uint64_t slot = off + i;
+      const uint64_t e = S.pool[slot];
+      if (e == TOMB) continue;
+      uint64_t h = hash64(e) & hmask;
+      unsigned long long k;
+      while ((k = tab_key[h]) != EMPTY64 && k != e) h = (h + 1) & hmask;
+      if (k == EMPTY64) continue;
+      const uint32_t sw = S.wpool ? S.wpool[slot] : 0u;
+      for (uint32_t r = tab_head[h]; r != NONE32; r = node_next[r]) {
+        if (dw && dw[r] != sw) continue;
+        if (atomicCAS(&claimed[r], (unsigned long long)EMPTY64, (unsigned long long)slot) ==
+            EMPTY64) {
+          S.pool[slot] = TOMB;
+          break;
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_del_finish(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                             uint64_t m, const unsigned long long* __restrict__ claimed,
+                             StoreView S, uint32_t* __restrict__ found, unsigned long long* misc) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long c = claimed[i];
+    if (c == EMPTY64) continue;
+    const uint32_t s = src[i], d = dst[i];
+    atomicSub(&S.outdeg[s], 1u);
+    atomicSub(&S.indeg[d], 1u);
+    atomicSub(&S.tile_live[tile_of(s, d, S.log2vpt, S.T)], 1u);
+    const unsigned long long k = atomicAdd(&misc[M_NFOUND], 1ull);
+    found[3 * k + 0] = s;
+    found[3 * k + 1] = d;
+    found[3 * k + 2] = S.wpool ? S.wpool[c] : 0u;
+  }
+}
+
+// In-place compaction of tiles with many tombstones (compactStorage
+// analogue that keeps multiplicity and only drops deleted slots).
+__global__ void __launch_bounds__(CP_THREADS) k_compact(const uint32_t* __restrict__ tiles,
+                                                        const uint32_t* __restrict__ ntiles_p,
+                                                        StoreView S, unsigned long long* misc) {
+  const uint32_t ntiles = *ntiles_p;
+  for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+    const uint32_t t = tiles[b];
+    const uint32_t cnt = S.tile_cnt[t], live = S.tile_live[t];
+    const uint32_t dead = cnt - live;
+    if (dead < 64 || (uint64_t)dead * 4 < cnt) continue;  // uniform per block
+    const uint64_t off = S.tile_off[t];
+    const uint32_t kept = copy_live(S.pool, S.wpool, off, cnt, S.pool, S.wpool, off);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      S.tile_cnt[t] = kept;
+      atomicAdd(&misc[M_NCOMPACT], 1ull);
+    }
+  }
+}
+
+inline uint32_t grid_for(uint64_t n, uint32_t threads, uint32_t cap = 65536) {
+  uint64_t b = (n + threads - 1) / threads;
+  if (b == 0) b = 1;
+  return (uint32_t)std::min<uint64_t>(b, cap);
+}
+
+inline size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Repack every tile into a new pool of at least min_slots slots.
+int repack(cytom_ctx* ctx, uint64_t min_slots) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  if ((rc = ensure(ctx, ctx->g_tile_scan, (ctx->n_tiles + 1) * 16 + 256))) return rc;
+  uint64_t* ncap = static_cast<uint64_t*>(ctx->g_tile_scan.p);
+  uint64_t* noff = ncap + ctx->n_tiles + 1;
+  if ((rc = ensure(ctx, ctx->g_scan_tmp, scan_tmp_bytes(ctx->n_tiles) + 256))) return rc;
+  StoreView S = ctx->view();
+  k_repack_caps<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(S, ncap);
+  scan_exclusive_u64(ncap, noff, ctx->n_tiles, ctx->g_scan_tmp.p, st);
+  uint64_t total = 0;
+  CY_HIP(ctx, hipMemcpyAsync(&ctx->h_misc[0], noff + ctx->n_tiles, 8, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipStreamSynchronize(st));
+  total = ctx->h_misc[0];
+  uint64_t new_size = std::max<uint64_t>(min_slots, total + total / 2 + (1u << 20));
+  uint64_t* npool = nullptr;
+  uint32_t* nw = nullptr;
+  if (hipMalloc(&npool, new_size * 8) != hipSuccess)
+    return set_err(ctx, CYTOM_ENOMEM, "edge pool allocation failed");
+  if (ctx->weighted && hipMalloc(&nw, new_size * 4) != hipSuccess) {
+    (void)hipFree(npool);
+    return set_err(ctx, CYTOM_ENOMEM, "weight pool allocation failed");
+  }
+  k_repack<<<grid_for(ctx->n_tiles, 1, 8192), CP_THREADS, 0, st>>>(S, noff, npool, nw);
+  ctx->h_misc[0] = total;
+  CY_HIP(ctx, hipMemcpyAsync(&ctx->d_misc[M_POOL_TOP], &ctx->h_misc[0], 8, hipMemcpyHostToDevice, st));
+  CY_HIP(ctx, hipStreamSynchronize(st));
+  (void)hipFree(ctx->pool);
+  if (ctx->wpool) (void)hipFree(ctx->wpool);
+  ctx->pool = npool;
+  ctx->wpool = nw;
+  ctx->pool_size = new_size;
+  return 0;
+}
+
+}  // namespace
+
+int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
+                 const uint32_t* d_w, uint64_t n_ins, const uint32_t* d_dsrc,
+                 const uint32_t* d_ddst, const uint32_t* d_dw, uint64_t n_del,
+                 cytom_batch_stats* bst) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  if (n_ins > 0xFFFFFFFEull || n_del > 0xFFFFFFFEull)
+    return set_err(ctx, CYTOM_ERANGE, "batch larger than 2^32-2 updates");
+  if (ctx->weighted && n_ins && !d_w)
+    return set_err(ctx, CYTOM_EINVAL, "weighted graph needs insertion weights");
+  const uint32_t lv = ctx->log2vpt, T = ctx->T;
+  const uint32_t key_bits = [&] {
+    uint32_t b = 0;
+    while ((1ull << b) < ctx->n_tiles) ++b;
+    return b;
+  }();
+  const uint64_t nmax = std::max<uint64_t>(n_ins, 1);
+  if ((rc = ensure(ctx, ctx->g_keys, nmax * 4)) || (rc = ensure(ctx, ctx->g_vals, nmax * 4)) ||
+      (rc = ensure(ctx, ctx->g_keys2, nmax * 4)) || (rc = ensure(ctx, ctx->g_vals2, nmax * 4)) ||
+      (rc = ensure(ctx, ctx->g_flags, nmax * 4)) ||
+      (rc = ensure(ctx, ctx->g_scan, (nmax + 1) * 4)) ||
+      (rc = ensure(ctx, ctx->g_seg_start, (nmax + 1) * 4)) ||
+      (rc = ensure(ctx, ctx->g_seg_tile, nmax * 4)) ||
+      (rc = ensure(ctx, ctx->g_reloc_cap, nmax * 4)) ||
+      (rc = ensure(ctx, ctx->g_reloc_list, nmax * 24)) ||
+      (rc = ensure(ctx, ctx->g_ins_slot, nmax * 8)) ||
+      (rc = ensure(ctx, ctx->g_hist, radix_hist_bytes(nmax))) ||
+      (rc = ensure(ctx, ctx->g_scan_tmp,
+                   scan_tmp_bytes(std::max<uint64_t>({nmax, 256 * ((nmax + 4095) / 4096) + 1,
+                                                      ctx->n_tiles})) +
+                       256)))
+    return rc;
+  uint32_t* keys = static_cast<uint32_t*>(ctx->g_keys.p);
+  uint32_t* vals = static_cast<uint32_t*>(ctx->g_vals.p);
+  uint32_t* skeys = static_cast<uint32_t*>(ctx->g_keys2.p);
+  uint32_t* svals = static_cast<uint32_t*>(ctx->g_vals2.p);
+  uint32_t* flags = static_cast<uint32_t*>(ctx->g_flags.p);
+  uint32_t* sx = static_cast<uint32_t*>(ctx->g_scan.p);
+  uint32_t* seg_start = static_cast<uint32_t*>(ctx->g_seg_start.p);
+  uint32_t* seg_tile = static_cast<uint32_t*>(ctx->g_seg_tile.p);
+  uint32_t* reloc_cap = static_cast<uint32_t*>(ctx->g_reloc_cap.p);
+  unsigned long long* reloc_list = static_cast<unsigned long long*>(ctx->g_reloc_list.p);
+  uint64_t* ins_slot = static_cast<uint64_t*>(ctx->g_ins_slot.p);
+  unsigned long long* misc = ctx->d_misc;
+
+  CY_HIP(ctx, hipEventRecord(ctx->ev[0], st));
+  // misc[1..63] := 0 (the pool top in misc[0] persists)
+  CY_HIP(ctx, hipMemsetAsync(misc + 1, 0, 63 * 8, st));
+  // 1. validation (nothing is modified before it passes)
+  if (n_ins)
+    k_keys<<<grid_for(n_ins, 256), 256, 0, st>>>(d_src, d_dst, d_w, n_ins, ctx->V, lv, T,
+                                                 ctx->weighted ? 1 : 0, keys, vals, misc);
+  if (n_del)
+    k_del_validate<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, ctx->V, misc);
+  CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipStreamSynchronize(st));
+  if (ctx->h_misc[M_ERR]) {
+    return set_err(ctx, CYTOM_EINVAL,
+                   (ctx->h_misc[M_ERR] & 1) ? "insertion with out-of-range vertex or zero weight"
+                                            : "deletion with out-of-range vertex");
+  }
+  uint32_t nreloc = 0;
+  // 2. insertions
+  if (n_ins) {
+    // flags / sx are free until the segment pass: they serve as the sort's
+    // ping-pong buffers.
+    radix_sort_pairs(keys, vals, skeys, svals, flags, sx, n_ins, key_bits, ctx->g_hist.p,
+                     ctx->g_scan_tmp.p, st);
+    k_heads<<<grid_for(n_ins, 256), 256, 0, st>>>(skeys, n_ins, flags);
+    scan_exclusive_u32(flags, sx, n_ins, ctx->g_scan_tmp.p, st);
+    k_seg_emit<<<grid_for(n_ins, 256), 256, 0, st>>>(skeys, flags, sx, n_ins, seg_start, seg_tile);
+    const uint32_t* nseg_p = sx + n_ins;
+    for (int attempt = 0;; ++attempt) {
+      CY_HIP(ctx, hipMemsetAsync(misc + M_NEED, 0, 16, st));
+      k_plan<<<grid_for(n_ins, 256), 256, 0, st>>>(nseg_p, seg_start, seg_tile, ctx->view(),
+                                                   reloc_cap, misc);
+      CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st));
+      CY_HIP(ctx, hipStreamSynchronize(st));
+      const uint64_t need = ctx->h_misc[M_NEED], top = ctx->h_misc[M_POOL_TOP];
+      nreloc = (uint32_t)ctx->h_misc[M_NRELOC];
+      if (top + need <= ctx->pool_size) break;
+      if (attempt > 0) return set_err(ctx, CYTOM_ENOMEM, "edge pool exhausted after repack");
+      if ((rc = repack(ctx, std::max<uint64_t>(2 * ctx->pool_size,
+                                               2 * (ctx->live_edges + n_ins) + need))))
+        return rc;
+    }
+    if (nreloc) {
+      k_reloc_commit<<<grid_for(n_ins, 256), 256, 0, st>>>(nseg_p, seg_tile, reloc_cap,
+                                                           reloc_list, misc);
+      k_relocate<<<std::min<uint32_t>(nreloc, 8192), CP_THREADS, 0, st>>>(reloc_list, nreloc,
+                                                                         ctx->view());
+    }
+    k_scatter<<<grid_for(n_ins, 256), 256, 0, st>>>(skeys, svals, n_ins, flags, sx, seg_start,
+                                                    d_src, d_dst, d_w, ctx->view(), ins_slot);
+    k_seg_commit<<<grid_for(n_ins, 256), 256, 0, st>>>(nseg_p, seg_start, seg_tile, ctx->view());
+  }
+  // 3. deletions
+  uint64_t n_found = 0;
+  uint32_t* found = nullptr;
+  uint32_t* del_tiles = nullptr;
+  uint32_t* n_del_tiles_p = nullptr;
+  if (n_del) {
+    uint64_t H = 64;
+    while (H < 2 * n_del) H <<= 1;
+    if ((rc = ensure(ctx, ctx->g_tab_key, H * 8)) || (rc = ensure(ctx, ctx->g_tab_head, H * 4)) ||
+        (rc = ensure(ctx, ctx->g_node_next, n_del * 4)) ||
+        (rc = ensure(ctx, ctx->g_claimed, n_del * 8)) ||
+        (rc = ensure(ctx, ctx->g_tile_flag, ctx->n_tiles * 4)) ||
+        (rc = ensure(ctx, ctx->g_tile_scan, (ctx->n_tiles + 1) * 16 + 256)) ||
+        (rc = ensure(ctx, ctx->g_del_tiles, ctx->n_tiles * 4)) ||
+        (rc = ensure(ctx, ctx->g_found, n_del * 12)) ||
+        (rc = ensure(ctx, ctx->g_scan_tmp, scan_tmp_bytes(ctx->n_tiles) + 256)))
+      return rc;
+    auto* tab_key = static_cast<unsigned long long*>(ctx->g_tab_key.p);
+    auto* tab_head = static_cast<uint32_t*>(ctx->g_tab_head.p);
+    auto* node_next = static_cast<uint32_t*>(ctx->g_node_next.p);
+    auto* claimed = static_cast<unsigned long long*>(ctx->g_claimed.p);
+    auto* tile_flag = static_cast<uint32_t*>(ctx->g_tile_flag.p);
+    auto* tscan = static_cast<uint32_t*>(ctx->g_tile_scan.p);
+    del_tiles = static_cast<uint32_t*>(ctx->g_del_tiles.p);
+    found = static_cast<uint32_t*>(ctx->g_found.p);
+    CY_HIP(ctx, hipMemsetAsync(tab_key, 0xFF, H * 8, st));
+    CY_HIP(ctx, hipMemsetAsync(tab_head, 0xFF, H * 4, st));
+    CY_HIP(ctx, hipMemsetAsync(claimed, 0xFF, n_del * 8, st));
+    CY_HIP(ctx, hipMemsetAsync(tile_flag, 0, ctx->n_tiles * 4, st));
+    k_del_build<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, tab_key, tab_head,
+                                                      H - 1, node_next, tile_flag, lv, T);
+    scan_exclusive_u32(tile_flag, tscan, ctx->n_tiles, ctx->g_scan_tmp.p, st);
+    k_flag_emit<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(tile_flag, tscan, ctx->n_tiles,
+                                                            del_tiles);
+    n_del_tiles_p = tscan + ctx->n_tiles;
+    k_del_match<<<std::min<uint64_t>(std::max<uint64_t>(n_del, 1), 4096), 256, 0, st>>>(
+        del_tiles, n_del_tiles_p, ctx->view(), tab_key, tab_head, H - 1, node_next,
+        ctx->weighted ? d_dw : nullptr, claimed);
+    k_del_finish<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, claimed,
+                                                       ctx->view(), found, misc);
+    CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st));
+    CY_HIP(ctx, hipStreamSynchronize(st));
+    n_found = ctx->h_misc[M_NFOUND];
+  }
+  // 4. per program: re-execution test, then init or hooks
+  const size_t np = ctx->progs.size();
+  if (n_found && np) {
+    CY_HIP(ctx, hipMemsetAsync(misc + M_REEXEC, 0, np * 8, st));
+    for (size_t p = 0; p < np; ++p)
+      launch_reexec_check(ctx, ctx->progs[p], found, n_found, misc + M_REEXEC + p);
+    CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st));
+    CY_HIP(ctx, hipStreamSynchronize(st));
+  }
+  for (size_t p = 0; p < np; ++p) {
+    Program& P = ctx->progs[p];
+    bool reexec = false;
+    if (n_found) {
+      reexec = ctx->h_misc[M_REEXEC + p] != 0;
+      if (P.kind != CYTOM_PRD && !P.fixpoint) reexec = true;
+    }
+    P.last_reexec = reexec ? 1 : 0;
+    if (reexec) {
+      if ((rc = program_init(ctx, P))) return rc;
+    } else if (n_ins || n_found) {
+      launch_hooks(ctx, P, d_src, d_dst, d_w, ins_slot, n_ins, found, n_found);
+    }
+  }
+  // 5. compaction of tiles with many tombstones
+  if (n_del) {
+    k_compact<<<4096, CP_THREADS, 0, st>>>(del_tiles, n_del_tiles_p, ctx->view(), misc);
+  }
+  CY_HIP(ctx, hipEventRecord(ctx->ev[1], st));
+  CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipStreamSynchronize(st));
+  ctx->live_edges += n_ins;
+  ctx->live_edges -= n_found;
+  ctx->jobs_dirty = true;
+  ctx->batches += 1;
+  if (bst) {
+    float ms = 0.f;
+    CY_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    bst->inserted = n_ins;
+    bst->deleted = n_found;
+    bst->missing = n_del - n_found;
+    bst->live_edges = ctx->live_edges;
+    bst->tiles_relocated = nreloc;
+    bst->tiles_compacted = (uint32_t)ctx->h_misc[M_NCOMPACT];
+    bst->t_ingest_ms = ms;
+  }
+  return 0;
+}
+
+int64_t export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_t* w, uint64_t cap) {
+  if (!src) return (int64_t)ctx->live_edges;
+  hipStream_t st = ctx->stream;
+  std::vector<uint64_t> off(ctx->n_tiles);
+  std::vector<uint32_t> cnt(ctx->n_tiles);
+  CY_HIP(ctx, hipMemcpyAsync(off.data(), ctx->tile_off, ctx->n_tiles * 8, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipMemcpyAsync(cnt.data(), ctx->tile_cnt, ctx->n_tiles * 4, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, ctx->d_misc, 8, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipStreamSynchronize(st));
+  const uint64_t top = ctx->h_misc[0];
+  std::vector<uint64_t> pool(top);
+  std::vector<uint32_t> wp(ctx->wpool ? top : 0);
+  if (top) {
+    CY_HIP(ctx, hipMemcpy(pool.data(), ctx->pool, top * 8, hipMemcpyDeviceToHost));
+    if (ctx->wpool) CY_HIP(ctx, hipMemcpy(wp.data(), ctx->wpool, top * 4, hipMemcpyDeviceToHost));
+  }
+  uint64_t n = 0;
+  for (uint64_t t = 0; t < ctx->n_tiles; ++t) {
+    for (uint32_t i = 0; i < cnt[t]; ++i) {
+      const uint64_t e = pool[off[t] + i];
+      if (e == TOMB) continue;
+      if (n < cap) {
+        src[n] = (uint32_t)e;
+        dst[n] = (uint32_t)(e >> 32);
+        if (w) w[n] = ctx->wpool ? wp[off[t] + i] : 0u;
+      }
+      ++n;
+    }
+  }
+  return (int64_t)n;
+}
+
+}  // namespace cytom

@@ -1,0 +1,187 @@
+/*
+ * cytom.h -- C-ABI of the MI355X-native evolving-graph engine.
+ *
+ * One context = one evolving graph held in HBM (tiled edge store, degrees)
+ * plus the vertex programs registered on it.  Every entry point returns 0
+ * (or a non-negative count) on success and a negative errno-style code on
+ * failure; cytom_last_error() then describes the failure and the context
+ * stays usable.  A context must be driven from one host thread; all device
+ * work runs on the context's own HIP stream.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the reference repository).
+ */
+#ifndef CYTOM_H
+#define CYTOM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CYTOM_ABI_VERSION 1
+
+/* Built-in vertex programs (examples/{bfs,cc,sssp,pagerank-delta}.cc). */
+enum cytom_program_kind {
+  CYTOM_BFS = 0,  /* examples/bfs.cc:18-230 */
+  CYTOM_CC = 1,   /* examples/cc.cc:18-214 */
+  CYTOM_SSSP = 2, /* examples/sssp.cc:18-228 (u32 weights and distances) */
+  CYTOM_PRD = 3,  /* examples/pagerank-delta.cc:30-308 */
+};
+
+/* Error codes (negative return values). */
+#define CYTOM_EINVAL (-22)
+#define CYTOM_ENOMEM (-12)
+#define CYTOM_ERANGE (-34)
+#define CYTOM_EDEVICE (-5)
+
+typedef struct cytom_ctx cytom_ctx;
+
+/* Engine configuration: replaces the option set parsed by parseOption and
+ * normalised by Config::normalizeConfig (src/include/main/execute.h:45-362,
+ * src/lib/util/config.cc:10-47). */
+typedef struct {
+  uint64_t n_vertices;        /* rounded up to a power of two (>= 64) */
+  uint32_t vertices_per_tile; /* power of two in [64, 32768]; 0 = default */
+  uint32_t max_iterations;    /* algorithm_iterations; 0 = 100 (run.py:119) */
+  double delta;               /* algorithm_config.delta (PRD threshold) */
+  uint64_t edge_capacity;     /* initial edge-pool slots; 0 = automatic */
+  uint32_t weighted;          /* keep a u32 weight per edge (SSSP) */
+  uint32_t device;            /* HIP device ordinal */
+} cytom_config;
+
+/* Program descriptor: mirrors the statics and constructor arguments of the
+ * Algorithm concept (examples/bfs.cc:20-34). */
+typedef struct {
+  uint32_t kind;   /* enum cytom_program_kind */
+  uint32_t source; /* algo_start_id for BFS/SSSP (execute.h:312) */
+} cytom_program_desc;
+
+/* Per-run statistics of cytom_execute
+ * (continuous-algorithm-executor.h:224-247 counters). */
+typedef struct {
+  uint32_t iterations;
+  uint32_t converged;
+  uint32_t reexecuted;       /* last batch re-initialised the program */
+  uint32_t pad;
+  uint64_t traversed_edges;  /* sum over iterations of live out-edges of active vertices */
+  uint64_t active_vertices;  /* sum over iterations of |frontier| */
+  uint64_t slots_scanned;    /* edge slots read by the gather kernels */
+  double t_compute_ms;       /* device time of the run (HIP events) */
+} cytom_run_stats;
+
+/* Per-batch statistics of cytom_update_batch*. */
+typedef struct {
+  uint64_t inserted;
+  uint64_t deleted;          /* deletions that matched a live edge */
+  uint64_t missing;          /* deletions with no live copy */
+  uint64_t live_edges;
+  uint32_t tiles_relocated;
+  uint32_t tiles_compacted;
+  double t_ingest_ms;        /* device time of the batch (HIP events) */
+} cytom_batch_stats;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Replaces execute<VertexType,Algorithm,is_weighted> set-up: VertexStore::init,
+ * TileStore::init, MetaTileStore::init (execute.h:534-652). */
+int cytom_create(const cytom_config* cfg, cytom_ctx** out);
+/* Replaces the teardown at the end of execute (execute.h:755-771). */
+int cytom_destroy(cytom_ctx* ctx);
+const char* cytom_last_error(const cytom_ctx* ctx);
+/* Effective geometry after normalisation (config.cc:10-47). */
+int cytom_geometry(const cytom_ctx* ctx, uint64_t* n_vertices, uint32_t* vertices_per_tile,
+                   uint32_t* tiles_per_dim);
+
+/* ---- algorithm store ---------------------------------------------------- */
+
+/* Registers a vertex program on the graph and runs its init_vertices
+ * (the main_<name> export + execute<> instantiation, examples/bfs.cc:235-238;
+ * ContinuousAlgorithmExecutor::initAlgorithm, continuous-algorithm-executor.h:342). */
+int cytom_program_register(cytom_ctx* ctx, const cytom_program_desc* desc, int* prog_id);
+/* Re-runs init_vertices (EdgeInserter re-execution, edge-inserter.h:164-167). */
+int cytom_program_init(cytom_ctx* ctx, int prog_id);
+
+/* ---- update ingestion --------------------------------------------------- */
+
+/* One EdgeInserter batch (edge-inserter.h:53-194 with
+ * MetaTileManager::run, meta-tile-manager.h:156-274): n_ins insertions then
+ * n_del deletions, then per program either re-initialisation or the
+ * edgeChanged(INSERT) hooks.  Host pointers; w/del_w may be NULL.  With
+ * del_w NULL a deletion matches any weight. */
+int cytom_update_batch(cytom_ctx* ctx, const uint32_t* src, const uint32_t* dst,
+                       const uint32_t* w, uint64_t n_ins, const uint32_t* del_src,
+                       const uint32_t* del_dst, const uint32_t* del_w, uint64_t n_del,
+                       cytom_batch_stats* stats);
+/* The same with device pointers already resident in HBM. */
+int cytom_update_batch_device(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
+                              const uint32_t* d_w, uint64_t n_ins, const uint32_t* d_del_src,
+                              const uint32_t* d_del_dst, const uint32_t* d_del_w,
+                              uint64_t n_del, cytom_batch_stats* stats);
+
+/* ---- execution ---------------------------------------------------------- */
+
+/* ContinuousAlgorithmExecutor::executeAlgorithm
+ * (continuous-algorithm-executor.h:49-249): runs the program until
+ * convergence or max_iterations. */
+int cytom_execute(cytom_ctx* ctx, int prog_id, cytom_run_stats* stats);
+/* Copies the program's values (the `next` array written by ResultsWriter,
+ * results-writer.h:54-67): BFS/CC/SSSP uint32[V], PRD float rank[V]. */
+int cytom_read_values(cytom_ctx* ctx, int prog_id, void* out, size_t bytes);
+
+/* ---- inspection --------------------------------------------------------- */
+
+/* VertexStore degrees (vertex-store.h:64-70). */
+int cytom_read_degrees(cytom_ctx* ctx, uint32_t* out_degree, uint32_t* in_degree);
+/* Live edges of the store (TileStore::countEdges / edge lists,
+ * tile-store.h:174-185).  Returns the live count; writes at most cap edges
+ * (arrays may be NULL to query the count). */
+int64_t cytom_export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_t* w,
+                           uint64_t cap);
+
+/* ---- measurement ------------------------------------------------------- */
+
+/* Enables (1) or disables (0) HIP-event timing of every gather launch and
+ * resets the accumulators (replaces the PerfEventScoped spans around the
+ * pull phase, continuous-algorithm-executor.h:93-97). */
+int cytom_set_profiling(cytom_ctx* ctx, int enable);
+/* Accumulated gather-kernel time, launch count and algorithmic bytes of one
+ * program kind since profiling was enabled. */
+int cytom_kernel_profile(cytom_ctx* ctx, uint32_t kind, double* gather_ms, uint64_t* launches,
+                         double* alg_bytes);
+
+/* ---- synthetic update streams ------------------------------------------ */
+
+/* RMAT edges [first, first+n) with 2^scale vertices on the device
+ * (RMATContext::getEdge, src/include/edge-inserter/rmat-context.h:39-73);
+ * d_w (nullable) receives the stream's u32 weights in [1,255]. */
+int cytom_rmat_generate_device(uint32_t scale, uint64_t seed, uint64_t first, uint64_t n,
+                               uint32_t* d_src, uint32_t* d_dst, uint32_t* d_w);
+/* The same for an arbitrary list of stream indices (device pointer). */
+int cytom_rmat_gather_device(uint32_t scale, uint64_t seed, const uint64_t* d_index,
+                             uint64_t n, uint32_t* d_src, uint32_t* d_dst, uint32_t* d_w);
+
+/* Deletion sampler of the synthetic stream: inserts are consecutive stream
+ * indices; deletions are drawn uniformly without replacement from the live
+ * indices (the deterministic replacement of deleteRandomEdge,
+ * tile-manager.h:175-190). */
+typedef struct cytom_stream cytom_stream;
+int cytom_stream_create(uint64_t seed, cytom_stream** out);
+int cytom_stream_destroy(cytom_stream* s);
+int64_t cytom_stream_next(cytom_stream* s, uint64_t n_ins, uint64_t n_del, uint64_t* ins_first,
+                          uint64_t* del_index);
+
+/* ---- device memory helpers for callers without a HIP runtime ----------- */
+int cytom_device_alloc(void** ptr, size_t bytes);
+int cytom_device_free(void* ptr);
+int cytom_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int cytom_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int cytom_device_synchronize(void);
+int cytom_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -54,6 +54,11 @@ def lib():
         f.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                       _u32p, _u32p]
         f.restype = None
+    L.orc_rmat_gather.argtypes = [ctypes.c_uint32, ctypes.c_uint64, _u64p, ctypes.c_uint64,
+                                  _u32p, _u32p]
+    L.orc_rmat_gather.restype = None
+    L.orc_stream_weights_u32_at.argtypes = [ctypes.c_uint64, _u64p, ctypes.c_uint64, _u32p]
+    L.orc_stream_weights_u32_at.restype = None
     L.orc_stream_weights_u32.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, _u32p]
     L.orc_stream_weights_f32.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, _f32p]
     L.orc_stream_create.argtypes = [ctypes.c_uint64]
@@ -163,14 +168,14 @@ def batch_edges(scale, seed, first, n_ins, del_idx, weighted):
     """Materialise one batch's (src, dst, w) arrays from stream indices."""
     s, d = rmat_edges(scale, seed, first, n_ins)
     w = weights_u32(seed, first, n_ins) if weighted else None
-    ds = np.empty(len(del_idx), np.uint32)
-    dd = np.empty(len(del_idx), np.uint32)
-    dw = np.empty(len(del_idx), np.uint32) if weighted else None
-    for i, ix in enumerate(del_idx):
-        a, b = rmat_edges(scale, seed, int(ix), 1)
-        ds[i], dd[i] = a[0], b[0]
-        if weighted:
-            dw[i] = weights_u32(seed, int(ix), 1)[0]
+    idx = np.ascontiguousarray(del_idx, dtype=np.uint64)
+    ds = np.empty(len(idx), np.uint32)
+    dd = np.empty(len(idx), np.uint32)
+    lib().orc_rmat_gather(scale, seed, idx, len(idx), ds, dd)
+    dw = None
+    if weighted:
+        dw = np.empty(len(idx), np.uint32)
+        lib().orc_stream_weights_u32_at(seed, idx, len(idx), dw)
     return s, d, w, ds, dd, dw
 
 

@@ -578,6 +578,20 @@ void orc_rmat_edges_recursive(uint32_t scale, uint64_t seed, uint64_t first, uin
   }
 }
 
+void orc_rmat_gather(uint32_t scale, uint64_t seed, const uint64_t* index, uint64_t n,
+                     uint32_t* src, uint32_t* dst) {
+  const uint64_t h = hash64(seed);
+  for (uint64_t j = 0; j < n; ++j) {
+    const uint64_t i = index[j];
+    rmat_iterative(scale, hash64(2ull * i * h), hash64((2ull * i + 1) * h), &src[j], &dst[j]);
+  }
+}
+
+void orc_stream_weights_u32_at(uint64_t seed, const uint64_t* index, uint64_t n, uint32_t* w) {
+  const uint64_t hw = hash64(seed ^ WEIGHT_SALT);
+  for (uint64_t j = 0; j < n; ++j) w[j] = 1u + (uint32_t)(hash64(hw ^ index[j]) % 255u);
+}
+
 void orc_stream_weights_u32(uint64_t seed, uint64_t first, uint64_t n, uint32_t* w) {
   const uint64_t hw = hash64(seed ^ WEIGHT_SALT);
   for (uint64_t j = 0; j < n; ++j) w[j] = 1u + (uint32_t)(hash64(hw ^ (first + j)) % 255u);

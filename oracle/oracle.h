@@ -53,6 +53,10 @@ void orc_rmat_edges(uint32_t scale, uint64_t seed, uint64_t first, uint64_t n,
 /* The recursive form of the same recurrence (rmat-context.h:19-37). */
 void orc_rmat_edges_recursive(uint32_t scale, uint64_t seed, uint64_t first,
                               uint64_t n, uint32_t* src, uint32_t* dst);
+/* The same edges for an arbitrary list of stream indices. */
+void orc_rmat_gather(uint32_t scale, uint64_t seed, const uint64_t* index, uint64_t n,
+                     uint32_t* src, uint32_t* dst);
+void orc_stream_weights_u32_at(uint64_t seed, const uint64_t* index, uint64_t n, uint32_t* w);
 /* Per-edge weights of the stream: u32 in [1,255] and f32 in (0,1]. */
 void orc_stream_weights_u32(uint64_t seed, uint64_t first, uint64_t n, uint32_t* w);
 void orc_stream_weights_f32(uint64_t seed, uint64_t first, uint64_t n, float* w);

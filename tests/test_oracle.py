@@ -1,0 +1,267 @@
+"""The CPU oracle, pinned against the reference's own code, closed forms and
+textbook checkers (no GPU needed)."""
+import collections
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import ref_build
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KAT = json.load(open(os.path.join(HERE, "golden", "rmat_kat.json")))
+
+
+# ------------------------------------------------------------------ a1: hash/RMAT
+def test_hash_known_answers():
+    for x, h in KAT["hash64"]:
+        assert O.lib().orc_hash64(x) == h
+    for x, h in KAT["hash32"]:
+        assert O.lib().orc_hash32(x) == h
+    for x, hd in KAT["hash_double"]:
+        assert O.hash_double(x) == float.fromhex(hd)
+
+
+def test_rmat_known_answers():
+    for case in KAT["rmat"]:
+        n = len(case["src"])
+        s, d = O.rmat_edges(case["scale"], case["seed"], case["first"], n)
+        assert s.tolist() == case["src"], case["scale"]
+        assert d.tolist() == case["dst"], case["scale"]
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference"), reason="reference tree absent")
+def test_rmat_against_reference_code():
+    if not os.path.exists(ref_build.LIB):
+        assert ref_build.build(quiet=True)
+    R = ctypes.CDLL(ref_build.LIB)
+    u32p = np.ctypeslib.ndpointer(np.uint32)
+    R.ref_rmat.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                           ctypes.c_int, u32p, u32p]
+    for scale in (3, 12, 22, 27):
+        for rec in (0, 1):
+            s = np.empty(20000, np.uint32)
+            d = np.empty(20000, np.uint32)
+            R.ref_rmat(scale, 2, 777, 20000, rec, s, d)
+            s2, d2 = O.rmat_edges(scale, 2, 777, 20000)
+            assert (s == s2).all() and (d == d2).all()
+
+
+def test_rmat_iterative_equals_recursive():
+    # Two structurally different restatements of the same recurrence: a wrong
+    # level/stride order in either breaks equality.
+    for scale in (1, 2, 7, 16, 22, 27):
+        a = O.rmat_edges(scale, 2, 5, 4000)
+        b = O.rmat_edges(scale, 2, 5, 4000, recursive=True)
+        assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
+
+
+def test_rmat_quadrant_frequencies():
+    # Every level picks quadrant (src bit, dst bit) = (0,0) with p=a, (0,1) b,
+    # (1,0) c, (1,1) d.
+    n = 200000
+    s, d = O.rmat_edges(16, 2, 0, n)
+    for bit in (0, 7, 15):
+        sb = (s >> bit) & 1
+        db = (d >> bit) & 1
+        p00 = np.mean((sb == 0) & (db == 0))
+        p01 = np.mean((sb == 0) & (db == 1))
+        p10 = np.mean((sb == 1) & (db == 0))
+        p11 = np.mean((sb == 1) & (db == 1))
+        tol = 4 * np.sqrt(0.25 / n)
+        assert abs(p00 - 0.57) < tol and abs(p01 - 0.19) < tol
+        assert abs(p10 - 0.19) < tol and abs(p11 - 0.05) < tol
+
+
+def test_weights_range():
+    w = O.weights_u32(2, 0, 100000)
+    assert w.min() >= 1 and w.max() <= 255
+    counts = np.bincount(w, minlength=256)[1:]
+    assert counts.min() > 0.7 * counts.mean()
+    wf = O.weights_f32(2, 0, 100000)
+    assert wf.min() > 0.0 and wf.max() <= 1.0
+
+
+# ------------------------------------------------------------------ stream sampler
+def test_stream_deletions_are_live_and_distinct():
+    st = O.Stream(9)
+    live = collections.Counter()
+    for b in range(6):
+        first, dels = st.next(1000, 150)
+        for i in range(first, first + 1000):
+            live[i] += 1
+        assert len(dels) == 150
+        assert len(set(dels.tolist())) == 150
+        for i in dels.tolist():
+            assert live[i] == 1
+            live[i] -= 1
+        assert st.live == sum(live.values())
+    # More deletions than live indices: clamps.
+    first, dels = st.next(0, 10**6)
+    assert len(dels) == sum(live.values()) and st.live == 0
+
+
+# ------------------------------------------------------------------ store
+def _multiset(s, d, w=None):
+    if w is None:
+        return collections.Counter(zip(s.tolist(), d.tolist()))
+    return collections.Counter(zip(s.tolist(), d.tolist(), w.tolist()))
+
+
+def test_store_insert_delete_matches_replay():
+    rng = np.random.default_rng(3)
+    V = 256
+    g = O.Graph(V, vertices_per_tile=64, weighted=True)
+    ref = collections.Counter()
+    for b in range(5):
+        s = rng.integers(0, V, 400).astype(np.uint32)
+        d = rng.integers(0, V, 400).astype(np.uint32)
+        w = rng.integers(1, 4, 400).astype(np.uint32)
+        for e in zip(s.tolist(), d.tolist(), w.tolist()):
+            ref[e] += 1
+        live = list(ref.elements())
+        pick = rng.choice(len(live), 120, replace=False)
+        ds = np.array([live[i][0] for i in pick], np.uint32)
+        dd = np.array([live[i][1] for i in pick], np.uint32)
+        dw = np.array([live[i][2] for i in pick], np.uint32)
+        # plus deletions that match nothing
+        ds = np.concatenate([ds, np.array([1, 2], np.uint32)])
+        dd = np.concatenate([dd, np.array([3, 4], np.uint32)])
+        dw = np.concatenate([dw, np.array([999, 999], np.uint32)])
+        found = g.update(s, d, w, ds, dd, dw)
+        assert found == 120
+        for i in pick:
+            ref[live[i]] -= 1
+        ls, ld, lw = g.live_edges()
+        assert _multiset(ls, ld, lw) == +ref
+        od, idg = g.degrees()
+        exp_o = np.zeros(V, np.int64)
+        exp_i = np.zeros(V, np.int64)
+        for (a, c, _), k in ref.items():
+            exp_o[a] += k
+            exp_i[c] += k
+        assert (od == exp_o).all() and (idg == exp_i).all()
+
+
+def test_store_rejects_out_of_range():
+    g = O.Graph(64, vertices_per_tile=64)
+    with pytest.raises(ValueError):
+        g.update(np.array([1], np.uint32), np.array([64], np.uint32))
+    assert len(g.live_edges()[0]) == 0
+
+
+# ------------------------------------------------------------------ executors
+def _stream_run(scale, vpt, batches, n_ins, n_del, threads=1, seed=2, cap=100):
+    V = 1 << scale
+    st = O.Stream(seed)
+    g = O.Graph(V, vertices_per_tile=vpt, weighted=True, threads=threads, max_iterations=cap)
+    pids = dict(bfs=g.register(O.BFS, 0), cc=g.register(O.CC), sssp=g.register(O.SSSP, 0),
+                prd=g.register(O.PRD))
+    for b in range(batches):
+        first, didx = st.next(n_ins, n_del if b else 0)
+        s, d, w, ds, dd, dw = O.batch_edges(scale, seed, first, n_ins, didx, True)
+        g.update(s, d, w, ds, dd, dw)
+        stats = {k: g.execute(p) for k, p in pids.items()}
+        yield b, g, pids, stats
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_executors_match_textbook_checkers(threads):
+    scale = 11
+    V = 1 << scale
+    for b, g, pids, stats in _stream_run(scale, 512, 5, 3000, 300, threads=threads):
+        ls, ld, lw = g.live_edges()
+        assert (g.read(pids["bfs"]) == O.check_bfs(V, ls, ld, 0)).all()
+        assert (g.read(pids["cc"]) == O.check_cc(V, ls, ld)).all()
+        assert (g.read(pids["sssp"]) == O.check_sssp(V, ls, ld, lw, 0)).all()
+        r = g.read(pids["prd"]).astype(np.float64)
+        r64, _ = O.prd_f64(V, ls, ld)
+        l1 = np.abs(r / r.sum() - r64 / r64.sum()).sum()
+        assert l1 < 1e-6, l1
+        for k in ("bfs", "cc", "sssp", "prd"):
+            assert stats[k].converged == 1
+
+
+def test_insert_only_incremental_cc_matches_checker():
+    # Insert-only CC on an RMAT stream: hooks plus incremental runs, never a
+    # re-execution.
+    scale = 12
+    V = 1 << scale
+    st = O.Stream(2)
+    g = O.Graph(V, vertices_per_tile=4096)
+    p = g.register(O.CC)
+    for b in range(8):
+        first, _ = st.next(2048, 0)
+        s, d = O.rmat_edges(scale, 2, first, 2048)
+        g.update(s, d)
+        g.execute(p)
+        assert g.last_reexec(p) == 0
+        ls, ld, _ = g.live_edges()
+        assert (g.read(p) == O.check_cc(V, ls, ld)).all()
+
+
+def test_deletion_reexecution_is_exact():
+    # The counterexample to critical-neighbour re-execution: a hook lowers a
+    # label through an edge that a later deletion removes.
+    g = O.Graph(64, vertices_per_tile=64)
+    p = g.register(O.CC)
+    g.update(np.array([5], np.uint32), np.array([7], np.uint32))
+    g.execute(p)
+    g.update(np.array([2], np.uint32), np.array([5], np.uint32),
+             dsrc=np.array([5], np.uint32), ddst=np.array([7], np.uint32))
+    g.execute(p)
+    lab = g.read(p)
+    assert lab[7] == 7 and lab[5] == 2
+
+
+def test_prd_closed_form_tiny():
+    # 0 -> 1: r0 = 0.15, r1 = 0.15 + 0.85 * 0.15 = 0.2775
+    g = O.Graph(64, vertices_per_tile=64)
+    p = g.register(O.PRD)
+    g.update(np.array([0], np.uint32), np.array([1], np.uint32))
+    st = g.execute(p)
+    r = g.read(p)
+    assert abs(r[0] - 0.15) < 1e-7 and abs(r[1] - 0.2775) < 1e-7
+    assert st.converged == 1
+    # 2-cycle: r = 1 for both; isolated vertices 0.15
+    g2 = O.Graph(64, vertices_per_tile=64)
+    p2 = g2.register(O.PRD)
+    g2.update(np.array([3, 4], np.uint32), np.array([4, 3], np.uint32))
+    g2.execute(p2)
+    r2 = g2.read(p2)
+    assert abs(r2[3] - 1.0) < 1e-6 and abs(r2[4] - 1.0) < 1e-6
+    assert abs(r2[10] - 0.15) < 1e-7
+
+
+def test_prd_converges_to_power_iteration():
+    scale = 10
+    V = 1 << scale
+    s, d = O.rmat_edges(scale, 2, 0, 8000)
+    g = O.Graph(V, vertices_per_tile=256, max_iterations=1000, delta=1e-12)
+    p = g.register(O.PRD)
+    g.update(s, d)
+    g.execute(p)
+    r = g.read(p).astype(np.float64)
+    pr, _ = O.pagerank_f64(V, s, d)
+    assert np.abs(r - pr).max() < 1e-4
+    rel = np.abs(r / r.sum() - pr / pr.sum()).sum()
+    assert rel < 1e-5
+
+
+def test_iteration_cap_leaves_leftovers_for_next_run():
+    # A path 0->1->...->9 inserted back to front: the hooks only relax 0->1,
+    # the run needs 9 iterations; with a cap of 3 it stops unconverged and
+    # the next run continues from the leftover frontier.
+    g = O.Graph(64, vertices_per_tile=64, max_iterations=3)
+    p = g.register(O.BFS, 0)
+    s = np.arange(8, -1, -1).astype(np.uint32)
+    g.update(s, s + 1)
+    st = g.execute(p)
+    assert st.converged == 0 and st.iterations == 3
+    for _ in range(3):
+        g.execute(p)
+    lv = g.read(p)
+    assert lv[:10].tolist() == list(range(10))
