@@ -82,18 +82,22 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_frontier(
     int kind, uint64_t V, uint32_t log2vpt, const uint64_t* __restrict__ act,
     const uint32_t* __restrict__ outdeg, uint8_t* __restrict__ rowact,
     const float* __restrict__ delta, float* __restrict__ contrib, IterStats* it) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nwords = V >> 6;
+  const uint64_t nw_total = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   unsigned long long te = 0, cnt = 0;
-  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
-       v += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t word = act[v >> 6];
-    const bool a = (word >> (v & 63)) & 1ull;
-    if (a) {
+  for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwords;
+       w += nw_total) {
+    const uint64_t word = act[w];
+    if (!word) continue;  // uniform across the wave
+    const uint64_t v = (w << 6) | lane;
+    if ((word >> lane) & 1ull) {
       const uint32_t od = outdeg[v];
       te += od;
-      cnt += 1;
+      ++cnt;
       if (kind == CYTOM_PRD) contrib[v] = od ? delta[v] / (float)od : 0.f;
     }
-    if ((threadIdx.x & 63) == 0 && word) rowact[v >> log2vpt] = 1;
+    if (lane == 0) rowact[v >> log2vpt] = 1;
   }
   te = block_sum<APPLY_THREADS, unsigned long long>(te);
   cnt = block_sum<APPLY_THREADS, unsigned long long>(cnt);
@@ -164,43 +168,55 @@ struct GatherArgs {
 template <int KIND>
 __device__ inline void gather_chunk(const GatherArgs<KIND>& A, uint32_t* win, float* winf,
                                     uint64_t off, uint32_t n, uint32_t mask) {
+  // Branch-free loads: indices past the chunk are clamped to its last slot
+  // and tombstones to vertex 0, so each of the three dependent load waves
+  // (edges, active words, source values) is issued in full before one wait.
   const int lane = threadIdx.x & 63;
   for (uint32_t base = 0; base < n; base += 64 * GATHER_UNROLL) {
     uint64_t e[GATHER_UNROLL];
 #pragma unroll
     for (int u = 0; u < GATHER_UNROLL; ++u) {
-      const uint32_t k = base + u * 64 + lane;
-      e[u] = k < n ? A.S.pool[off + k] : TOMB;
+      const uint32_t k = min(base + u * 64 + lane, n - 1);
+      e[u] = A.S.pool[off + k];
     }
     uint32_t wt[GATHER_UNROLL];
     if (KIND == CYTOM_SSSP) {
 #pragma unroll
       for (int u = 0; u < GATHER_UNROLL; ++u) {
-        const uint32_t k = base + u * 64 + lane;
-        wt[u] = k < n ? A.S.wpool[off + k] : 0u;
+        const uint32_t k = min(base + u * 64 + lane, n - 1);
+        wt[u] = A.S.wpool[off + k];
       }
     }
-    bool on[GATHER_UNROLL];
+    uint32_t s[GATHER_UNROLL];
+    bool ok[GATHER_UNROLL];
 #pragma unroll
     for (int u = 0; u < GATHER_UNROLL; ++u) {
-      const uint32_t s = (uint32_t)e[u];
-      on[u] = e[u] != TOMB && ((A.act[s >> 6] >> (s & 63)) & 1ull);
+      ok[u] = (base + u * 64 + lane < n) && e[u] != TOMB;
+      s[u] = ok[u] ? (uint32_t)e[u] : 0u;
     }
+    uint64_t aw[GATHER_UNROLL];
 #pragma unroll
-    for (int u = 0; u < GATHER_UNROLL; ++u) {
-      if (!on[u]) continue;
-      const uint32_t s = (uint32_t)e[u];
-      const uint32_t dl = (uint32_t)(e[u] >> 32) & mask;
-      if (KIND == CYTOM_PRD) {
-        atomicAdd(&winf[dl], A.contrib[s]);  // pagerank-delta.cc:127-132
-      } else {
-        const uint32_t val = A.cur[s];
-        if (val == INF) continue;
+    for (int u = 0; u < GATHER_UNROLL; ++u) aw[u] = A.act[s[u] >> 6];
+#pragma unroll
+    for (int u = 0; u < GATHER_UNROLL; ++u) ok[u] = ok[u] && ((aw[u] >> (s[u] & 63)) & 1ull);
+    if (KIND == CYTOM_PRD) {
+      float c[GATHER_UNROLL];
+#pragma unroll
+      for (int u = 0; u < GATHER_UNROLL; ++u) c[u] = A.contrib[s[u]];
+#pragma unroll
+      for (int u = 0; u < GATHER_UNROLL; ++u)
+        if (ok[u]) atomicAdd(&winf[(uint32_t)(e[u] >> 32) & mask], c[u]);  // pagerank-delta.cc:127-132
+    } else {
+      uint32_t val[GATHER_UNROLL];
+#pragma unroll
+      for (int u = 0; u < GATHER_UNROLL; ++u) val[u] = A.cur[s[u]];
+#pragma unroll
+      for (int u = 0; u < GATHER_UNROLL; ++u) {
         uint32_t cand;
-        if (KIND == CYTOM_BFS) cand = val + 1;             // bfs.cc:83-91
-        else if (KIND == CYTOM_CC) cand = val;             // cc.cc:81-91
-        else cand = sat_add(val, wt[u]);                   // sssp.cc:90-99
-        if (cand < win[dl]) atomicMin(&win[dl], cand);
+        if (KIND == CYTOM_BFS) cand = val[u] == INF ? INF : val[u] + 1;  // bfs.cc:83-91
+        else if (KIND == CYTOM_CC) cand = val[u];                         // cc.cc:81-91
+        else cand = sat_add(val[u], wt[u]);                               // sssp.cc:90-99
+        if (ok[u] && cand != INF) atomicMin(&win[(uint32_t)(e[u] >> 32) & mask], cand);
       }
     }
   }
@@ -270,6 +286,21 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
 }
 
 // ---------------------------------------------------------------- apply
+// Persistent grid: each wavefront walks bitmap words (64 consecutive
+// vertices, one per lane), builds the next active word with a ballot and
+// keeps per-lane counters; one reduction and three atomics per block.
+__device__ inline void apply_tail(unsigned long long te, unsigned long long cnt,
+                                  unsigned long long upd, IterStats* nit) {
+  te = block_sum<APPLY_THREADS, unsigned long long>(te);
+  cnt = block_sum<APPLY_THREADS, unsigned long long>(cnt);
+  upd = block_sum<APPLY_THREADS, unsigned long long>(upd);
+  if (threadIdx.x == 0) {
+    if (te) atomicAdd(&nit->te, te);
+    if (cnt) atomicAdd(&nit->count, cnt);
+    if (upd) atomicAdd(&(nit - 1)->updated, upd);
+  }
+}
+
 // bfs.cc:101-113, cc.cc:101-111, sssp.cc:101-113: copy next into current
 // and activate on change.
 template <int KIND>
@@ -277,34 +308,32 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_min(
     uint64_t V, uint32_t log2vpt, uint32_t* __restrict__ cur, const uint32_t* __restrict__ nxt,
     uint8_t* __restrict__ changed, uint64_t* __restrict__ act_next,
     uint8_t* __restrict__ rowact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
-  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // V % 64 == 0
-  bool a = false;
-  unsigned long long te = 0, upd = 0;
-  if (v < V) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nwords = V >> 6;
+  const uint64_t nw_total = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long te = 0, cnt = 0, upd = 0;
+  for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwords;
+       w += nw_total) {
+    const uint64_t v = (w << 6) | lane;
+    bool a = false;
     if (changed[v]) {
       changed[v] = 0;
-      upd = 1;
+      ++upd;
       const uint32_t n = nxt[v];
       if (n != cur[v]) {
         cur[v] = n;
         a = true;
-        te = outdeg[v];
+        te += outdeg[v];
+        ++cnt;
       }
     }
+    const uint64_t word = __ballot(a);
+    if (lane == 0) {
+      act_next[w] = word;
+      if (word) rowact_next[v >> log2vpt] = 1;
+    }
   }
-  const uint64_t word = __ballot(a);
-  if (v < V && (threadIdx.x & 63) == 0) {
-    act_next[v >> 6] = word;
-    if (word) rowact_next[v >> log2vpt] = 1;
-  }
-  te = block_sum<APPLY_THREADS, unsigned long long>(te);
-  const unsigned long long cnt = block_sum<APPLY_THREADS, unsigned long long>(a ? 1ull : 0ull);
-  upd = block_sum<APPLY_THREADS, unsigned long long>(upd);
-  if (threadIdx.x == 0) {
-    if (te) atomicAdd(&nit->te, te);
-    if (cnt) atomicAdd(&nit->count, cnt);
-    if (upd) atomicAdd(&(nit - 1)->updated, upd);
-  }
+  apply_tail(te, cnt, upd, nit);
 }
 
 // pagerank-delta.cc:142-174 (FP32 state, double intermediates as in the
@@ -314,16 +343,22 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
     float* __restrict__ delta, float* __restrict__ dsum, float* __restrict__ contrib,
     uint8_t* __restrict__ changed, uint64_t* __restrict__ act_next,
     uint8_t* __restrict__ rowact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
-  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool a = false;
-  unsigned long long te = 0, upd = 0;
-  if (v < V) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nwords = V >> 6;
+  const uint64_t nw_total = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long te = 0, cnt = 0, upd = 0;
+  for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwords;
+       w += nw_total) {
+    const uint64_t v = (w << 6) | lane;
+    bool a = false;
     const bool ch = changed[v] != 0;
-    upd = ch ? 1ull : 0ull;
     if (first || ch) {
-      if (ch) changed[v] = 0;
+      if (ch) {
+        changed[v] = 0;
+        ++upd;
+      }
       const float s = dsum[v];
-      float d;
+      float d = 0.f;
       float r = rank[v];
       bool applied = true;
       if (first) {
@@ -344,25 +379,19 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
         if ((double)fabsf(d) > thr * (double)r) {
           a = true;
           const uint32_t od = outdeg[v];
-          te = od;
+          te += od;
+          ++cnt;
           contrib[v] = od ? d / (float)od : 0.f;
         }
       }
     }
+    const uint64_t word = __ballot(a);
+    if (lane == 0) {
+      act_next[w] = word;
+      if (word) rowact_next[v >> log2vpt] = 1;
+    }
   }
-  const uint64_t word = __ballot(a);
-  if (v < V && (threadIdx.x & 63) == 0) {
-    act_next[v >> 6] = word;
-    if (word) rowact_next[v >> log2vpt] = 1;
-  }
-  te = block_sum<APPLY_THREADS, unsigned long long>(te);
-  const unsigned long long cnt = block_sum<APPLY_THREADS, unsigned long long>(a ? 1ull : 0ull);
-  upd = block_sum<APPLY_THREADS, unsigned long long>(upd);
-  if (threadIdx.x == 0) {
-    if (te) atomicAdd(&nit->te, te);
-    if (cnt) atomicAdd(&nit->count, cnt);
-    if (upd) atomicAdd(&(nit - 1)->updated, upd);
-  }
+  apply_tail(te, cnt, upd, nit);
 }
 
 // ------------------------------------------------------ deletion tests
@@ -588,50 +617,65 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
   CY_HIP(ctx, hipMemsetAsync(ctx->d_iter, 0, sizeof(IterStats) * (maxit + 2), st));
   CY_HIP(ctx, hipMemsetAsync(P.rowact[P.cur_act], 0, ctx->T, st));
   CY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-  k_frontier<<<grid_for(ctx->V, APPLY_THREADS, 8192), APPLY_THREADS, 0, st>>>(
-      P.kind, ctx->V, ctx->log2vpt, P.act[P.cur_act], ctx->outdeg, P.rowact[P.cur_act], P.delta,
-      P.contrib, ctx->d_iter);
-  const uint32_t apply_blocks = (uint32_t)((ctx->V + APPLY_THREADS - 1) / APPLY_THREADS);
-  uint32_t it = 0;
+  const uint64_t words = ctx->V >> 6;
+  const uint32_t apply_blocks = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>((uint64_t)ctx->num_cus * 4, (words + 3) / 4));
+  k_frontier<<<apply_blocks, APPLY_THREADS, 0, st>>>(P.kind, ctx->V, ctx->log2vpt,
+                                                     P.act[P.cur_act], ctx->outdeg,
+                                                     P.rowact[P.cur_act], P.delta, P.contrib,
+                                                     ctx->d_iter);
+  // Iterations are launched in growing groups with one host check per
+  // group; once the frontier is empty the remaining launches of a group are
+  // no-ops (the gather returns at once, apply finds nothing changed).
+  uint32_t it = 0, launched = 0, group = 2;
   bool converged = false;
   const bool prof = ctx->profile && ctx->kev.size() >= 2 * (size_t)maxit;
-  while (it < maxit) {
-    if (prof) CY_HIP(ctx, hipEventRecord(ctx->kev[2 * it], st));
-    switch (P.kind) {
-      case CYTOM_BFS: launch_gather<CYTOM_BFS>(ctx, P, it); break;
-      case CYTOM_CC: launch_gather<CYTOM_CC>(ctx, P, it); break;
-      case CYTOM_SSSP: launch_gather<CYTOM_SSSP>(ctx, P, it); break;
-      default: launch_gather<CYTOM_PRD>(ctx, P, it); break;
+  while (!converged && launched < maxit) {
+    const uint32_t n = std::min(group, maxit - launched);
+    for (uint32_t k = 0; k < n; ++k, ++launched) {
+      const uint32_t i = launched;
+      if (prof) CY_HIP(ctx, hipEventRecord(ctx->kev[2 * i], st));
+      switch (P.kind) {
+        case CYTOM_BFS: launch_gather<CYTOM_BFS>(ctx, P, i); break;
+        case CYTOM_CC: launch_gather<CYTOM_CC>(ctx, P, i); break;
+        case CYTOM_SSSP: launch_gather<CYTOM_SSSP>(ctx, P, i); break;
+        default: launch_gather<CYTOM_PRD>(ctx, P, i); break;
+      }
+      if (prof) CY_HIP(ctx, hipEventRecord(ctx->kev[2 * i + 1], st));
+      const int nx = P.cur_act ^ 1;
+      if (P.kind == CYTOM_PRD) {
+        k_apply_prd<<<apply_blocks, APPLY_THREADS, 0, st>>>(
+            ctx->V, ctx->log2vpt, P.first_iter ? 1 : 0, ctx->delta, P.rank, P.delta, P.dsum,
+            P.contrib, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg, ctx->d_iter + i + 1);
+      } else if (P.kind == CYTOM_BFS) {
+        k_apply_min<CYTOM_BFS><<<apply_blocks, APPLY_THREADS, 0, st>>>(
+            ctx->V, ctx->log2vpt, P.cur, P.nxt, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg,
+            ctx->d_iter + i + 1);
+      } else if (P.kind == CYTOM_CC) {
+        k_apply_min<CYTOM_CC><<<apply_blocks, APPLY_THREADS, 0, st>>>(
+            ctx->V, ctx->log2vpt, P.cur, P.nxt, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg,
+            ctx->d_iter + i + 1);
+      } else {
+        k_apply_min<CYTOM_SSSP><<<apply_blocks, APPLY_THREADS, 0, st>>>(
+            ctx->V, ctx->log2vpt, P.cur, P.nxt, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg,
+            ctx->d_iter + i + 1);
+      }
+      P.cur_act = nx;
+      P.first_iter = false;
     }
-    if (prof) CY_HIP(ctx, hipEventRecord(ctx->kev[2 * it + 1], st));
-    const int nx = P.cur_act ^ 1;
-    if (P.kind == CYTOM_PRD) {
-      k_apply_prd<<<apply_blocks, APPLY_THREADS, 0, st>>>(
-          ctx->V, ctx->log2vpt, P.first_iter ? 1 : 0, ctx->delta, P.rank, P.delta, P.dsum,
-          P.contrib, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg, ctx->d_iter + it + 1);
-    } else if (P.kind == CYTOM_BFS) {
-      k_apply_min<CYTOM_BFS><<<apply_blocks, APPLY_THREADS, 0, st>>>(
-          ctx->V, ctx->log2vpt, P.cur, P.nxt, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg,
-          ctx->d_iter + it + 1);
-    } else if (P.kind == CYTOM_CC) {
-      k_apply_min<CYTOM_CC><<<apply_blocks, APPLY_THREADS, 0, st>>>(
-          ctx->V, ctx->log2vpt, P.cur, P.nxt, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg,
-          ctx->d_iter + it + 1);
-    } else {
-      k_apply_min<CYTOM_SSSP><<<apply_blocks, APPLY_THREADS, 0, st>>>(
-          ctx->V, ctx->log2vpt, P.cur, P.nxt, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg,
-          ctx->d_iter + it + 1);
-    }
-    P.cur_act = nx;
-    P.first_iter = false;
-    ++it;
-    CY_HIP(ctx, hipMemcpyAsync(&ctx->h_iter[it], &ctx->d_iter[it], sizeof(IterStats),
+    const uint32_t lo = launched - n + 1;
+    CY_HIP(ctx, hipMemcpyAsync(&ctx->h_iter[lo], &ctx->d_iter[lo], sizeof(IterStats) * n,
                                hipMemcpyDeviceToHost, st));
     CY_HIP(ctx, hipStreamSynchronize(st));
-    if (ctx->h_iter[it].count == 0) {
-      converged = true;
-      break;
+    it = launched;
+    for (uint32_t k = lo; k <= launched; ++k) {
+      if (ctx->h_iter[k].count == 0) {
+        converged = true;
+        it = k;
+        break;
+      }
     }
+    group = std::min<uint32_t>(group * 2, 16);
   }
   CY_HIP(ctx, hipEventRecord(ctx->ev[3], st));
   CY_HIP(ctx, hipMemcpyAsync(ctx->h_iter, ctx->d_iter, sizeof(IterStats) * (it + 1),

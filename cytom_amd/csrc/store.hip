@@ -242,16 +242,23 @@ __global__ void k_seg_commit(const uint32_t* __restrict__ nseg_p,
 }
 
 // ---- deletions
+// Open-addressing table over (src, dst) keys; a multiplicative hash keeps
+// the per-slot probe cheap.
+__device__ inline uint64_t tab_hash(uint64_t key, uint32_t log_h) {
+  return (key * 0x9E3779B97F4A7C15ull) >> (64 - log_h);
+}
+
 __global__ void k_del_build(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                             uint64_t m, unsigned long long* __restrict__ tab_key,
-                            uint32_t* __restrict__ tab_head, uint64_t hmask,
+                            uint32_t* __restrict__ tab_head, uint32_t log_h,
                             uint32_t* __restrict__ node_next, uint32_t* __restrict__ tile_flag,
                             uint32_t lv, uint32_t T) {
+  const uint64_t hmask = (1ull << log_h) - 1;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t s = src[i], d = dst[i];
     const unsigned long long key = pack_edge(s, d);
-    uint64_t h = hash64(key) & hmask;
+    uint64_t h = tab_hash(key, log_h);
     for (;;) {
       const unsigned long long prev = atomicCAS(&tab_key[h], (unsigned long long)EMPTY64, key);
       if (prev == EMPTY64 || prev == key) break;
@@ -269,35 +276,80 @@ __global__ void k_flag_emit(const uint32_t* __restrict__ flag, const uint32_t* _
     if (flag[i]) list[fx[i]] = (uint32_t)i;
 }
 
-__global__ void __launch_bounds__(256) k_del_match(const uint32_t* __restrict__ tiles,
-                                                   const uint32_t* __restrict__ ntiles_p,
+// Chunks of the flagged tiles: DEL_CH slots each, so that one heavy tile is
+// shared by many wavefronts.
+constexpr uint32_t DEL_CH = 2048;
+
+__global__ void k_del_nch(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ cnt,
+                          uint64_t n_tiles, uint32_t* __restrict__ nch) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles;
+       t += (uint64_t)gridDim.x * blockDim.x)
+    nch[t] = flag[t] ? (cnt[t] + DEL_CH - 1) / DEL_CH : 0u;
+}
+
+__global__ void k_del_chunks(const uint32_t* __restrict__ nch, const uint32_t* __restrict__ cbase,
+                             uint64_t n_tiles, uint64_t* __restrict__ chunks) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t n = nch[t], b = cbase[t];
+    for (uint32_t c = 0; c < n; ++c) chunks[b + c] = ((uint64_t)t << 32) | (uint64_t)(c * DEL_CH);
+  }
+}
+
+// Each live slot of a chunk probes the table; a hit walks the request list
+// of its (src, dst) and claims the first unclaimed request with a matching
+// weight (any weight when dw is null).  The claimed slot becomes a tombstone.
+constexpr int DM_UNROLL = 8;
+__global__ void __launch_bounds__(256) k_del_match(const uint64_t* __restrict__ chunks,
+                                                   const uint32_t* __restrict__ nchunks_p,
                                                    StoreView S,
                                                    const unsigned long long* __restrict__ tab_key,
                                                    const uint32_t* __restrict__ tab_head,
-                                                   uint64_t hmask,
+                                                   uint32_t log_h,
                                                    const uint32_t* __restrict__ node_next,
                                                    const uint32_t* __restrict__ dw,
                                                    unsigned long long* __restrict__ claimed) {
-  const uint32_t ntiles = *ntiles_p;
-  for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
-    const uint32_t t = tiles[b];
-    const uint64_t off = S.tile_off[t];
-    const uint32_t cnt = S.tile_cnt[t];
-    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-      const uint64_t slot = off + i;
-      const uint64_t e = S.pool[slot];
-      if (e == TOMB) continue;
-      uint64_t h = hash64(e) & hmask;
-      unsigned long long k;
-      while ((k = tab_key[h]) != EMPTY64 && k != e) h = (h + 1) & hmask;
-      if (k == EMPTY64) continue;
-      const uint32_t sw = S.wpool ? S.wpool[slot] : 0u;
-      for (uint32_t r = tab_head[h]; r != NONE32; r = node_next[r]) {
-        if (dw && dw[r] != sw) continue;
-        if (atomicCAS(&claimed[r], (unsigned long long)EMPTY64, (unsigned long long)slot) ==
-            EMPTY64) {
-          S.pool[slot] = TOMB;
-          break;
+  const uint64_t hmask = (1ull << log_h) - 1;
+  const uint32_t nchunks = *nchunks_p;
+  const int lane = threadIdx.x & 63;
+  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < nchunks;
+       c += nwaves) {
+    const uint64_t ch = chunks[c];
+    const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
+    const uint64_t off = S.tile_off[t] + s0;
+    const uint32_t n = min(DEL_CH, S.tile_cnt[t] - s0);
+    for (uint32_t base = 0; base < n; base += 64 * DM_UNROLL) {
+      uint64_t e[DM_UNROLL];
+#pragma unroll
+      for (int u = 0; u < DM_UNROLL; ++u) e[u] = S.pool[off + min(base + u * 64 + lane, n - 1)];
+      uint64_t h[DM_UNROLL];
+      unsigned long long k[DM_UNROLL];
+#pragma unroll
+      for (int u = 0; u < DM_UNROLL; ++u) {
+        h[u] = tab_hash(e[u], log_h);
+        k[u] = tab_key[h[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < DM_UNROLL; ++u) {
+        const uint32_t i = base + u * 64 + lane;
+        if (i >= n || e[u] == TOMB || k[u] == EMPTY64) continue;
+        uint64_t hh = h[u];
+        unsigned long long kk = k[u];
+        while (kk != e[u] && kk != EMPTY64) {
+          hh = (hh + 1) & hmask;
+          kk = tab_key[hh];
+        }
+        if (kk == EMPTY64) continue;
+        const uint64_t slot = off + i;
+        const uint32_t sw = S.wpool ? S.wpool[slot] : 0u;
+        for (uint32_t r = tab_head[hh]; r != NONE32; r = node_next[r]) {
+          if (dw && dw[r] != sw) continue;
+          if (atomicCAS(&claimed[r], (unsigned long long)EMPTY64, (unsigned long long)slot) ==
+              EMPTY64) {
+            S.pool[slot] = TOMB;
+            break;
+          }
         }
       }
     }
@@ -492,7 +544,12 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
   uint32_t* n_del_tiles_p = nullptr;
   if (n_del) {
     uint64_t H = 64;
-    while (H < 2 * n_del) H <<= 1;
+    uint32_t log_h = 6;
+    while (H < 2 * n_del) {
+      H <<= 1;
+      ++log_h;
+    }
+    const uint64_t max_chunks = ctx->pool_size / DEL_CH + ctx->n_tiles + 1;
     if ((rc = ensure(ctx, ctx->g_tab_key, H * 8)) || (rc = ensure(ctx, ctx->g_tab_head, H * 4)) ||
         (rc = ensure(ctx, ctx->g_node_next, n_del * 4)) ||
         (rc = ensure(ctx, ctx->g_claimed, n_del * 8)) ||
@@ -500,7 +557,10 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
         (rc = ensure(ctx, ctx->g_tile_scan, (ctx->n_tiles + 1) * 16 + 256)) ||
         (rc = ensure(ctx, ctx->g_del_tiles, ctx->n_tiles * 4)) ||
         (rc = ensure(ctx, ctx->g_found, n_del * 12)) ||
-        (rc = ensure(ctx, ctx->g_scan_tmp, scan_tmp_bytes(ctx->n_tiles) + 256)))
+        (rc = ensure(ctx, ctx->g_nch, ctx->n_tiles * 4)) ||
+        (rc = ensure(ctx, ctx->g_cbase, (ctx->n_tiles + 1) * 4)) ||
+        (rc = ensure(ctx, ctx->g_chunks, max_chunks * 8)) ||
+        (rc = ensure(ctx, ctx->g_scan_tmp, scan_tmp_bytes(ctx->n_tiles + 1) + 256)))
       return rc;
     auto* tab_key = static_cast<unsigned long long*>(ctx->g_tab_key.p);
     auto* tab_head = static_cast<uint32_t*>(ctx->g_tab_head.p);
@@ -508,20 +568,28 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
     auto* claimed = static_cast<unsigned long long*>(ctx->g_claimed.p);
     auto* tile_flag = static_cast<uint32_t*>(ctx->g_tile_flag.p);
     auto* tscan = static_cast<uint32_t*>(ctx->g_tile_scan.p);
+    auto* dnch = static_cast<uint32_t*>(ctx->g_nch.p);
+    auto* dcbase = static_cast<uint32_t*>(ctx->g_cbase.p);
+    auto* dchunks = static_cast<uint64_t*>(ctx->g_chunks.p);
     del_tiles = static_cast<uint32_t*>(ctx->g_del_tiles.p);
     found = static_cast<uint32_t*>(ctx->g_found.p);
+    ctx->jobs_dirty = true;  // the gather work lists share these buffers
     CY_HIP(ctx, hipMemsetAsync(tab_key, 0xFF, H * 8, st));
     CY_HIP(ctx, hipMemsetAsync(tab_head, 0xFF, H * 4, st));
     CY_HIP(ctx, hipMemsetAsync(claimed, 0xFF, n_del * 8, st));
     CY_HIP(ctx, hipMemsetAsync(tile_flag, 0, ctx->n_tiles * 4, st));
     k_del_build<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, tab_key, tab_head,
-                                                      H - 1, node_next, tile_flag, lv, T);
+                                                      log_h, node_next, tile_flag, lv, T);
     scan_exclusive_u32(tile_flag, tscan, ctx->n_tiles, ctx->g_scan_tmp.p, st);
     k_flag_emit<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(tile_flag, tscan, ctx->n_tiles,
                                                             del_tiles);
     n_del_tiles_p = tscan + ctx->n_tiles;
-    k_del_match<<<std::min<uint64_t>(std::max<uint64_t>(n_del, 1), 4096), 256, 0, st>>>(
-        del_tiles, n_del_tiles_p, ctx->view(), tab_key, tab_head, H - 1, node_next,
+    k_del_nch<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(tile_flag, ctx->tile_cnt, ctx->n_tiles,
+                                                           dnch);
+    scan_exclusive_u32(dnch, dcbase, ctx->n_tiles, ctx->g_scan_tmp.p, st);
+    k_del_chunks<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(dnch, dcbase, ctx->n_tiles, dchunks);
+    k_del_match<<<(uint32_t)ctx->num_cus * 4, 256, 0, st>>>(
+        dchunks, dcbase + ctx->n_tiles, ctx->view(), tab_key, tab_head, log_h, node_next,
         ctx->weighted ? d_dw : nullptr, claimed);
     k_del_finish<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, claimed,
                                                        ctx->view(), found, misc);
