@@ -47,13 +47,32 @@ def parse():
     ap.add_argument("--batches", type=int, default=64)
     ap.add_argument("--delete-fraction", type=float, default=0.1)
     ap.add_argument("--vertices-per-tile", type=int, default=0)
+    ap.add_argument("--chunk-slots", type=int, default=0)
+    ap.add_argument("--job-chunks", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--programs", default="bfs,cc,sssp,pagerank-delta")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--cpu-batches", type=int, default=2,
                     help="timed batches of the CPU sample (the last ones of the stream)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this size (the reference "
+                         "kernel for HBM traffic counters)")
     return ap.parse_args()
+
+
+def calibration_copy(mib):
+    import cytom_amd as C
+
+    n = mib << 20
+    a = C.DeviceArray(n, np.uint8)
+    b = C.DeviceArray(n, np.uint8)
+    rc = C.lib().cytom_memcpy_d2d(b.ptr, a.ptr, n)
+    if rc:
+        raise C.CytomError(rc, "calibration copy failed")
+    C.synchronize()
+    log("bench: calibration copy of %d MiB done" % mib)
 
 
 def stream_plan(args):
@@ -93,6 +112,7 @@ def gpu_run(args):
         % (args.batches, n_ins, n_del, time.perf_counter() - t_setup, source))
     g = C.Graph(V, vertices_per_tile=args.vertices_per_tile, weighted=weighted,
                 max_iterations=100, delta=1e-7, edge_capacity=int(3 * E))
+    g.set_tuning(args.chunk_slots, args.job_chunks, args.blocks_per_cu)
     pids = {}
     for n in names:
         pids[n] = g.register(n, source if n in ("bfs", "sssp") else 0)
@@ -174,6 +194,7 @@ def gpu_run(args):
         "n_ins": n_ins,
         "n_del": n_del,
         "vpt": g.vpt,
+        "tuning": g.get_tuning(),
         "roofline": roofline,
         "detail": detail,
     }
@@ -246,6 +267,8 @@ def main():
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         return 2
     E, per_batch, n_ins, n_del = stream_plan(args)
+    if args.calibrate_copy_mib:
+        calibration_copy(args.calibrate_copy_mib)
     res = gpu_run(args)
     cpu = None
     if args.cpu_baseline == "auto":
@@ -271,6 +294,7 @@ def main():
             "scale": args.scale,
             "edge_factor": args.edge_factor,
             "vertices_per_tile": res["vpt"],
+            "tuning": res["tuning"],
             "source": res["source"],
             "parallelism": "single GPU",
         },

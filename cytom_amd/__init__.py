@@ -55,6 +55,15 @@ class RunStats(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("chunk_slots", ctypes.c_uint32),
+        ("job_chunks", ctypes.c_uint32),
+        ("blocks_per_cu", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+    ]
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [
         ("inserted", ctypes.c_uint64),
@@ -88,6 +97,8 @@ SIGNATURES = {
     "cytom_read_values": (ctypes.c_int, [_VP, ctypes.c_int, _VP, ctypes.c_size_t]),
     "cytom_read_degrees": (ctypes.c_int, [_VP, _VP, _VP]),
     "cytom_export_edges": (ctypes.c_int64, [_VP, _VP, _VP, _VP, _U64]),
+    "cytom_set_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
+    "cytom_get_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
     "cytom_set_profiling": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cytom_kernel_profile": (ctypes.c_int, [_VP, _U32, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(_U64),
@@ -101,6 +112,7 @@ SIGNATURES = {
     "cytom_device_free": (ctypes.c_int, [_VP]),
     "cytom_memcpy_h2d": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
     "cytom_memcpy_d2h": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
+    "cytom_memcpy_d2d": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
     "cytom_device_synchronize": (ctypes.c_int, []),
     "cytom_device_count": (ctypes.c_int, []),
 }
@@ -331,6 +343,16 @@ class Graph:
         i = np.empty(self.V, np.uint32)
         self._check(lib().cytom_read_degrees(self._h, _p(o), _p(i)), "degrees")
         return o, i
+
+    def set_tuning(self, chunk_slots=0, job_chunks=0, blocks_per_cu=0):
+        t = Tuning(chunk_slots, job_chunks, blocks_per_cu, 0)
+        self._check(lib().cytom_set_tuning(self._h, ctypes.byref(t)), "tuning")
+
+    def get_tuning(self):
+        t = Tuning()
+        self._check(lib().cytom_get_tuning(self._h, ctypes.byref(t)), "tuning")
+        return {"chunk_slots": t.chunk_slots, "job_chunks": t.job_chunks,
+                "blocks_per_cu": t.blocks_per_cu}
 
     def set_profiling(self, enable=True):
         self._check(lib().cytom_set_profiling(self._h, 1 if enable else 0), "profiling")

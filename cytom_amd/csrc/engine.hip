@@ -109,8 +109,9 @@ int cytom_create(const cytom_config* cfg, cytom_ctx** out) {
   while (V < cfg->n_vertices) V <<= 1;
   uint32_t vpt = cfg->vertices_per_tile;
   if (vpt == 0) vpt = 16384;
-  if (vpt < 64 || vpt > 32768 || (vpt & (vpt - 1))) return CYTOM_EINVAL;
+  if (vpt < 64 || vpt > 16384 || (vpt & (vpt - 1))) return CYTOM_EINVAL;  // 8-byte LDS window
   if (vpt > V) vpt = (uint32_t)V;
+  if ((V / vpt) * (V / vpt) > (1ull << 30)) return CYTOM_EINVAL;  // tile ids stay 32-bit keys
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CYTOM_EDEVICE;
   if ((int)cfg->device >= ndev) return CYTOM_EINVAL;
@@ -123,6 +124,11 @@ int cytom_create(const cytom_config* cfg, cytom_ctx** out) {
   c->log2vpt = 0;
   while ((1u << c->log2vpt) < vpt) ++c->log2vpt;
   c->n_tiles = (uint64_t)c->T * c->T;
+  {
+    int lg = 0;
+    while ((1ull << lg) < V) ++lg;
+    c->prd_shift = 60 - (lg + 1);
+  }
   c->max_iter = cfg->max_iterations ? cfg->max_iterations : 100;
   c->delta = cfg->delta;
   c->weighted = cfg->weighted != 0;
@@ -304,6 +310,29 @@ int64_t cytom_export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_
   return export_edges(ctx, src, dst, w, cap);
 }
 
+int cytom_set_tuning(cytom_ctx* ctx, const cytom_tuning* t) {
+  CY_CHECK_CTX(ctx);
+  if (!t) return set_err(ctx, CYTOM_EINVAL, "null tuning");
+  if (t->chunk_slots && (t->chunk_slots % 64 || t->chunk_slots > 512))
+    return set_err(ctx, CYTOM_EINVAL, "chunk_slots must be a multiple of 64 up to 512");
+  if (t->job_chunks > 512) return set_err(ctx, CYTOM_EINVAL, "job_chunks above 512");
+  if (t->blocks_per_cu > 16) return set_err(ctx, CYTOM_EINVAL, "blocks_per_cu above 16");
+  if (t->chunk_slots) ctx->chunk_slots = t->chunk_slots;
+  if (t->job_chunks) ctx->job_chunks = t->job_chunks;
+  if (t->blocks_per_cu) ctx->gather_blocks = (uint32_t)ctx->num_cus * t->blocks_per_cu;
+  ctx->jobs_dirty = true;
+  return 0;
+}
+
+int cytom_get_tuning(const cytom_ctx* ctx, cytom_tuning* t) {
+  if (!ctx || !t) return CYTOM_EINVAL;
+  t->chunk_slots = ctx->chunk_slots;
+  t->job_chunks = ctx->job_chunks;
+  t->blocks_per_cu = ctx->gather_blocks / (uint32_t)ctx->num_cus;
+  t->pad = 0;
+  return 0;
+}
+
 int cytom_set_profiling(cytom_ctx* ctx, int enable) {
   CY_CHECK_CTX(ctx);
   if (enable && ctx->kev.size() < 2 * (size_t)ctx->max_iter) {
@@ -397,6 +426,9 @@ int cytom_memcpy_h2d(void* dst, const void* src, size_t bytes) {
 }
 int cytom_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : CYTOM_EDEVICE;
+}
+int cytom_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? 0 : CYTOM_EDEVICE;
 }
 int cytom_device_synchronize(void) {
   return hipDeviceSynchronize() == hipSuccess ? 0 : CYTOM_EDEVICE;

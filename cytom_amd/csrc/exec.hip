@@ -12,9 +12,10 @@
 //    A job is a run of edge chunks of one target column y (tiles (x, y) for
 //    all x); the block keeps the column's vertex values in LDS (the
 //    reference's per-tile-row local array, algorithm-executor.h:160-200),
-//    reads packed edges coalesced, tests the source's bit in the active
-//    bitmap and folds candidates with LDS atomics (min for BFS/CC/SSSP, add
-//    for PageRank-delta).  The window is then folded into the global next
+//    reads packed edges coalesced, loads the source's frontier value (its
+//    value while active, else the neutral element, so the active-bitmap
+//    test costs no extra load) and folds candidates with LDS atomics (min
+//    for BFS/CC/SSSP, add for PageRank-delta).  The window is then folded into the global next
 //    array with one atomic per touched vertex (applyTiles,
 //    algorithm-executor.h:431-505), marking `changed`.
 //  * apply: one thread per vertex applies the program to changed vertices;
@@ -62,13 +63,13 @@ __global__ void k_init_min(int kind, uint64_t V, uint32_t source, uint32_t* __re
 
 // pagerank-delta.cc:246-270: rank 0, delta 1, all active.
 __global__ void k_init_prd(uint64_t V, float* __restrict__ rank, float* __restrict__ delta,
-                           float* __restrict__ dsum, uint64_t* __restrict__ act,
+                           long long* __restrict__ dsum, uint64_t* __restrict__ act,
                            uint8_t* __restrict__ changed) {
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
     rank[v] = 0.f;
     delta[v] = 1.f;
-    dsum[v] = 0.f;
+    dsum[v] = 0;
     changed[v] = 0;
     if ((v & 63) == 0) act[v >> 6] = ~0ull;
   }
@@ -81,7 +82,8 @@ __global__ void k_init_prd(uint64_t V, float* __restrict__ rank, float* __restri
 __global__ void __launch_bounds__(APPLY_THREADS) k_frontier(
     int kind, uint64_t V, uint32_t log2vpt, const uint64_t* __restrict__ act,
     const uint32_t* __restrict__ outdeg, uint8_t* __restrict__ rowact,
-    const float* __restrict__ delta, float* __restrict__ contrib, IterStats* it) {
+    const float* __restrict__ delta, float* __restrict__ contrib,
+    const uint32_t* __restrict__ cur, uint32_t* __restrict__ fval, IterStats* it) {
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
   const uint64_t nw_total = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -89,15 +91,18 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_frontier(
   for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwords;
        w += nw_total) {
     const uint64_t word = act[w];
-    if (!word) continue;  // uniform across the wave
     const uint64_t v = (w << 6) | lane;
-    if ((word >> lane) & 1ull) {
-      const uint32_t od = outdeg[v];
+    const bool a = (word >> lane) & 1ull;
+    uint32_t od = 0;
+    if (a) {
+      od = outdeg[v];
       te += od;
       ++cnt;
-      if (kind == CYTOM_PRD) contrib[v] = od ? delta[v] / (float)od : 0.f;
     }
-    if (lane == 0) rowact[v >> log2vpt] = 1;
+    // Frontier values: what the gather reads per edge instead of the bitmap.
+    if (kind == CYTOM_PRD) contrib[v] = (a && od) ? delta[v] / (float)od : 0.f;
+    else fval[v] = a ? cur[v] : INF;
+    if (lane == 0 && word) rowact[v >> log2vpt] = 1;
   }
   te = block_sum<APPLY_THREADS, unsigned long long>(te);
   cnt = block_sum<APPLY_THREADS, unsigned long long>(cnt);
@@ -153,63 +158,87 @@ struct GatherArgs {
   const uint32_t* njobs;
   const uint64_t* chunks;
   StoreView S;
-  const uint64_t* act;
   const uint8_t* rowact;
   uint8_t* rowact_clear;
-  const uint32_t* cur;
+  const uint32_t* fval;
   uint32_t* nxt;
   const float* contrib;
-  float* dsum;
+  long long* dsum;
   uint8_t* changed;
   IterStats* it;
   uint32_t CH;
+  float qscale;  // 2^prd_shift
 };
 
+constexpr uint32_t MAX_JOB_CHUNKS = GATHER_THREADS;  // one metadata pass per job
+
+// Edge slots of one chunk into registers: slot (base + u*64 + lane), clamped
+// to the chunk's last slot so every load is issued without a branch.
 template <int KIND>
-__device__ inline void gather_chunk(const GatherArgs<KIND>& A, uint32_t* win, float* winf,
-                                    uint64_t off, uint32_t n, uint32_t mask) {
-  // Branch-free loads: indices past the chunk are clamped to its last slot
-  // and tombstones to vertex 0, so each of the three dependent load waves
-  // (edges, active words, source values) is issued in full before one wait.
+__device__ inline void load_chunk(const GatherArgs<KIND>& A, uint64_t off, uint32_t n,
+                                  uint64_t (&e)[GATHER_UNROLL], uint32_t (&wt)[GATHER_UNROLL]) {
   const int lane = threadIdx.x & 63;
-  for (uint32_t base = 0; base < n; base += 64 * GATHER_UNROLL) {
-    uint64_t e[GATHER_UNROLL];
 #pragma unroll
-    for (int u = 0; u < GATHER_UNROLL; ++u) {
-      const uint32_t k = min(base + u * 64 + lane, n - 1);
-      e[u] = A.S.pool[off + k];
-    }
-    uint32_t wt[GATHER_UNROLL];
-    if (KIND == CYTOM_SSSP) {
+  for (int u = 0; u < GATHER_UNROLL; ++u) e[u] = A.S.pool[off + min((uint32_t)(u * 64 + lane), n - 1)];
+  if (KIND == CYTOM_SSSP) {
 #pragma unroll
-      for (int u = 0; u < GATHER_UNROLL; ++u) {
-        const uint32_t k = min(base + u * 64 + lane, n - 1);
-        wt[u] = A.S.wpool[off + k];
-      }
-    }
+    for (int u = 0; u < GATHER_UNROLL; ++u) wt[u] = A.S.wpool[off + min((uint32_t)(u * 64 + lane), n - 1)];
+  }
+}
+
+// One wavefront's share of a column job, software-pipelined: while the
+// source frontier values and LDS atomics of chunk k are in flight, the edge
+// loads of its next chunk are already issued.  A source's frontier value is
+// INF (0 for PageRank-delta) while it is inactive, so the active-bitmap test
+// costs no extra load.
+template <int KIND>
+__device__ inline unsigned long long gather_wave(const GatherArgs<KIND>& A, uint32_t* win,
+                                                 unsigned long long* winq, const uint64_t* s_off,
+                                                 const uint32_t* s_n, uint32_t nact,
+                                                 uint32_t mask) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t nwaves = GATHER_THREADS / 64;
+  uint32_t k = threadIdx.x >> 6;
+  unsigned long long scanned = 0;
+  if (k >= nact) return 0;
+  uint64_t e[GATHER_UNROLL];
+  uint32_t wt[GATHER_UNROLL];
+  uint32_t n = s_n[k];
+  load_chunk<KIND>(A, s_off[k], n, e, wt);
+  for (;;) {
+    scanned += n;
+    const uint32_t kn = k + nwaves;
+    const bool more = kn < nact;
     uint32_t s[GATHER_UNROLL];
     bool ok[GATHER_UNROLL];
 #pragma unroll
     for (int u = 0; u < GATHER_UNROLL; ++u) {
-      ok[u] = (base + u * 64 + lane < n) && e[u] != TOMB;
+      ok[u] = (uint32_t)(u * 64 + lane) < n && e[u] != TOMB;
       s[u] = ok[u] ? (uint32_t)e[u] : 0u;
     }
-    uint64_t aw[GATHER_UNROLL];
-#pragma unroll
-    for (int u = 0; u < GATHER_UNROLL; ++u) aw[u] = A.act[s[u] >> 6];
-#pragma unroll
-    for (int u = 0; u < GATHER_UNROLL; ++u) ok[u] = ok[u] && ((aw[u] >> (s[u] & 63)) & 1ull);
+    float c[GATHER_UNROLL];
+    uint32_t val[GATHER_UNROLL];
     if (KIND == CYTOM_PRD) {
-      float c[GATHER_UNROLL];
 #pragma unroll
       for (int u = 0; u < GATHER_UNROLL; ++u) c[u] = A.contrib[s[u]];
+    } else {
+#pragma unroll
+      for (int u = 0; u < GATHER_UNROLL; ++u) val[u] = A.fval[s[u]];
+    }
+    uint64_t en[GATHER_UNROLL];
+    uint32_t wtn[GATHER_UNROLL];
+    uint32_t nn = 0;
+    if (more) {
+      nn = s_n[kn];
+      load_chunk<KIND>(A, s_off[kn], nn, en, wtn);
+    }
+    if (KIND == CYTOM_PRD) {
 #pragma unroll
       for (int u = 0; u < GATHER_UNROLL; ++u)
-        if (ok[u]) atomicAdd(&winf[(uint32_t)(e[u] >> 32) & mask], c[u]);  // pagerank-delta.cc:127-132
+        if (ok[u] && c[u] != 0.f)  // pagerank-delta.cc:127-132, as exact fixed point
+          atomicAdd(&winq[(uint32_t)(e[u] >> 32) & mask],
+                    (unsigned long long)__float2ll_rn(c[u] * A.qscale));
     } else {
-      uint32_t val[GATHER_UNROLL];
-#pragma unroll
-      for (int u = 0; u < GATHER_UNROLL; ++u) val[u] = A.cur[s[u]];
 #pragma unroll
       for (int u = 0; u < GATHER_UNROLL; ++u) {
         uint32_t cand;
@@ -219,13 +248,28 @@ __device__ inline void gather_chunk(const GatherArgs<KIND>& A, uint32_t* win, fl
         if (ok[u] && cand != INF) atomicMin(&win[(uint32_t)(e[u] >> 32) & mask], cand);
       }
     }
+    if (!more) break;
+#pragma unroll
+    for (int u = 0; u < GATHER_UNROLL; ++u) {
+      e[u] = en[u];
+      wt[u] = wtn[u];
+    }
+    n = nn;
+    k = kn;
   }
+  return scanned;
 }
 
 template <int KIND>
 __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
-  extern __shared__ uint32_t win[];
-  float* winf = reinterpret_cast<float*>(win);
+  // The column window: u32 values for BFS/CC/SSSP, int64 fixed-point sums
+  // for PageRank-delta (LDS integer atomics run an order of magnitude faster
+  // than LDS float atomics on gfx950, and integer sums are exact).
+  extern __shared__ unsigned long long lds_window[];
+  uint32_t* win = reinterpret_cast<uint32_t*>(lds_window);
+  unsigned long long* winq = lds_window;
+  __shared__ uint64_t s_off[MAX_JOB_CHUNKS];
+  __shared__ uint32_t s_n[MAX_JOB_CHUNKS];
   __shared__ uint32_t s_job;
   const uint32_t vpt = A.S.vpt, T = A.S.T, mask = vpt - 1;
   // Clear next iteration's row activity (written by the apply kernel).
@@ -233,41 +277,47 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
     A.rowact_clear[i] = 0;
   if (A.it->count == 0) return;  // empty frontier (uniform)
   const uint32_t njobs = *A.njobs;
-  const int wave = threadIdx.x >> 6, nwaves = GATHER_THREADS / 64;
   unsigned long long scanned = 0;
   for (;;) {
     if (threadIdx.x == 0) s_job = (uint32_t)atomicAdd(&A.it->job_cursor, 1ull);
     __syncthreads();
     const uint32_t j = s_job;
-    __syncthreads();
-    if (j >= njobs) break;
+    if (j >= njobs) break;  // uniform: every thread read s_job after the barrier
     const uint4 job = A.jobs[j];
-    const uint32_t c0 = job.x, c1 = job.y, y = job.z;
-    int any = 0;
-    for (uint32_t c = c0 + threadIdx.x; c < c1; c += GATHER_THREADS) {
-      const uint32_t t = (uint32_t)(A.chunks[c] >> 32);
-      any |= A.rowact[t - y * T];
-    }
-    if (!__syncthreads_or(any)) continue;
-    const uint32_t neutral = KIND == CYTOM_PRD ? 0u : INF;
-    for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) win[i] = neutral;
-    __syncthreads();
-    for (uint32_t c = c0 + wave; c < c1; c += nwaves) {
-      const uint64_t ch = A.chunks[c];
+    const uint32_t c0 = job.x, nc = job.y - job.x, y = job.z;
+    // Chunk table of the job, compacted to the chunks whose source row has an
+    // active vertex (TileManager::isActive, tile-manager.h:403-455).
+    uint32_t flag = 0;
+    uint64_t off = 0;
+    uint32_t n = 0;
+    if (threadIdx.x < nc) {
+      const uint64_t ch = A.chunks[c0 + threadIdx.x];
       const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
-      if (!A.rowact[t - y * T]) continue;
-      const uint32_t cnt = A.S.tile_cnt[t];
-      const uint32_t n = min(A.CH, cnt - s0);
-      scanned += n;
-      gather_chunk<KIND>(A, win, winf, A.S.tile_off[t] + s0, n, mask);
+      flag = A.rowact[t - y * T];
+      off = A.S.tile_off[t] + s0;
+      n = min(A.CH, A.S.tile_cnt[t] - s0);
     }
+    uint32_t nact;
+    const uint32_t pos = block_scan_excl<GATHER_THREADS, uint32_t>(flag, &nact);
+    if (flag) {
+      s_off[pos] = off;
+      s_n[pos] = n;
+    }
+    if (nact == 0) continue;  // uniform; the scan's barriers ordered s_job reuse
+    if (KIND == CYTOM_PRD) {
+      for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) winq[i] = 0ull;
+    } else {
+      for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) win[i] = INF;
+    }
+    __syncthreads();
+    scanned += gather_wave<KIND>(A, win, winq, s_off, s_n, nact, mask);
     __syncthreads();
     const uint64_t base = (uint64_t)y * vpt;
     for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) {
       if (KIND == CYTOM_PRD) {
-        const float f = winf[i];
-        if (f != 0.f) {
-          unsafeAtomicAdd(&A.dsum[base + i], f);
+        const unsigned long long q = winq[i];
+        if (q != 0ull) {
+          atomicAdd(reinterpret_cast<unsigned long long*>(&A.dsum[base + i]), q);
           A.changed[base + i] = 1;
         }
       } else {
@@ -306,7 +356,8 @@ __device__ inline void apply_tail(unsigned long long te, unsigned long long cnt,
 template <int KIND>
 __global__ void __launch_bounds__(APPLY_THREADS) k_apply_min(
     uint64_t V, uint32_t log2vpt, uint32_t* __restrict__ cur, const uint32_t* __restrict__ nxt,
-    uint8_t* __restrict__ changed, uint64_t* __restrict__ act_next,
+    uint32_t* __restrict__ fval, uint8_t* __restrict__ changed,
+    const uint64_t* __restrict__ act_cur, uint64_t* __restrict__ act_next,
     uint8_t* __restrict__ rowact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
@@ -315,6 +366,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_min(
   for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwords;
        w += nw_total) {
     const uint64_t v = (w << 6) | lane;
+    const bool was = (act_cur[w] >> lane) & 1ull;
     bool a = false;
     if (changed[v]) {
       changed[v] = 0;
@@ -322,11 +374,13 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_min(
       const uint32_t n = nxt[v];
       if (n != cur[v]) {
         cur[v] = n;
+        fval[v] = n;
         a = true;
         te += outdeg[v];
         ++cnt;
       }
     }
+    if (was && !a) fval[v] = INF;
     const uint64_t word = __ballot(a);
     if (lane == 0) {
       act_next[w] = word;
@@ -339,10 +393,11 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_min(
 // pagerank-delta.cc:142-174 (FP32 state, double intermediates as in the
 // reference: ALPHA and the threshold are doubles).
 __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
-    uint64_t V, uint32_t log2vpt, int first, double thr, float* __restrict__ rank,
-    float* __restrict__ delta, float* __restrict__ dsum, float* __restrict__ contrib,
-    uint8_t* __restrict__ changed, uint64_t* __restrict__ act_next,
-    uint8_t* __restrict__ rowact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
+    uint64_t V, uint32_t log2vpt, int first, double thr, double qinv, float* __restrict__ rank,
+    float* __restrict__ delta, long long* __restrict__ dsum, float* __restrict__ contrib,
+    uint8_t* __restrict__ changed, const uint64_t* __restrict__ act_cur,
+    uint64_t* __restrict__ act_next, uint8_t* __restrict__ rowact_next,
+    const uint32_t* __restrict__ outdeg, IterStats* nit) {
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
   const uint64_t nw_total = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -350,6 +405,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
   for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwords;
        w += nw_total) {
     const uint64_t v = (w << 6) | lane;
+    const bool was = (act_cur[w] >> lane) & 1ull;
     bool a = false;
     const bool ch = changed[v] != 0;
     if (first || ch) {
@@ -357,7 +413,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
         changed[v] = 0;
         ++upd;
       }
-      const float s = dsum[v];
+      const float s = (float)((double)dsum[v] * qinv);
       float d = 0.f;
       float r = rank[v];
       bool applied = true;
@@ -375,7 +431,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
       if (applied) {
         rank[v] = r;
         delta[v] = d;
-        dsum[v] = 0.f;
+        dsum[v] = 0;
         if ((double)fabsf(d) > thr * (double)r) {
           a = true;
           const uint32_t od = outdeg[v];
@@ -385,6 +441,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
         }
       }
     }
+    if (was && !a) contrib[v] = 0.f;
     const uint64_t word = __ballot(a);
     if (lane == 0) {
       act_next[w] = word;
@@ -454,15 +511,17 @@ __global__ void k_hooks_prd(const uint32_t* __restrict__ src, const uint32_t* __
                             const uint64_t* __restrict__ ins_slot, uint64_t n,
                             const uint32_t* __restrict__ found, uint64_t n_found,
                             const uint64_t* __restrict__ pool, const float* __restrict__ rank,
-                            const uint32_t* __restrict__ outdeg, float* __restrict__ dsum,
-                            uint64_t* __restrict__ act, uint8_t* __restrict__ changed, double thr) {
+                            const uint32_t* __restrict__ outdeg, long long* __restrict__ dsum,
+                            uint64_t* __restrict__ act, uint8_t* __restrict__ changed, double thr,
+                            float qscale) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n + n_found;
        i += (uint64_t)gridDim.x * blockDim.x) {
     if (i < n) {
       if (pool[ins_slot[i]] == TOMB) continue;
       const uint32_t s = src[i], d = dst[i];
       const float dl = rank[s] / (float)outdeg[s];
-      unsafeAtomicAdd(&dsum[d], dl);
+      atomicAdd(reinterpret_cast<unsigned long long*>(&dsum[d]),
+                (unsigned long long)__float2ll_rn(dl * qscale));
       if ((double)dl > thr * (double)rank[d]) {
         atomicOr(reinterpret_cast<unsigned long long*>(&act[s >> 6]), 1ull << (s & 63));
         changed[s] = 1;
@@ -471,9 +530,19 @@ __global__ void k_hooks_prd(const uint32_t* __restrict__ src, const uint32_t* __
       const uint64_t k = i - n;
       const uint32_t s = found[3 * k], d = found[3 * k + 1];
       const float dl = rank[s] / (float)outdeg[s];
-      unsafeAtomicAdd(&dsum[d], -dl);
+      atomicAdd(reinterpret_cast<unsigned long long*>(&dsum[d]),
+                (unsigned long long)__float2ll_rn(-dl * qscale));
     }
   }
+}
+
+// Persistent grid: as many blocks per CU as the LDS window allows (at most
+// two), unless tuned.
+uint32_t gather_grid(const cytom_ctx* ctx, size_t lds) {
+  if (ctx->gather_blocks) return ctx->gather_blocks;
+  const size_t per_block = lds + 16 * 1024;  // window + chunk table and scan scratch
+  const uint32_t per_cu = (uint32_t)std::max<size_t>(1, (160 * 1024) / per_block);
+  return (uint32_t)ctx->num_cus * std::min<uint32_t>(per_cu, 2);
 }
 
 template <int KIND>
@@ -483,18 +552,18 @@ void launch_gather(cytom_ctx* ctx, Program& P, int it) {
   A.njobs = static_cast<const uint32_t*>(ctx->g_jbase.p) + ctx->T;
   A.chunks = static_cast<const uint64_t*>(ctx->g_chunks.p);
   A.S = ctx->view();
-  A.act = P.act[P.cur_act];
   A.rowact = P.rowact[P.cur_act];
   A.rowact_clear = P.rowact[P.cur_act ^ 1];
-  A.cur = P.cur;
+  A.fval = P.fval;
   A.nxt = P.nxt;
   A.contrib = P.contrib;
   A.dsum = P.dsum;
+  A.qscale = ldexpf(1.f, ctx->prd_shift);
   A.changed = P.changed;
   A.it = ctx->d_iter + it;
   A.CH = ctx->chunk_slots;
-  const size_t lds = (size_t)ctx->vpt * 4;
-  k_gather<KIND><<<ctx->gather_blocks, GATHER_THREADS, lds, ctx->stream>>>(A);
+  const size_t lds = (size_t)ctx->vpt * (KIND == CYTOM_PRD ? 8 : 4);
+  k_gather<KIND><<<gather_grid(ctx, lds), GATHER_THREADS, lds, ctx->stream>>>(A);
 }
 
 int build_jobs(cytom_ctx* ctx) {
@@ -540,9 +609,10 @@ int program_alloc(cytom_ctx* ctx, Program& P) {
   bool ok = true;
   if (P.kind == CYTOM_PRD) {
     ok = ok && alloc((void**)&P.rank, V * 4) && alloc((void**)&P.delta, V * 4) &&
-         alloc((void**)&P.dsum, V * 4) && alloc((void**)&P.contrib, V * 4);
+         alloc((void**)&P.dsum, V * 8) && alloc((void**)&P.contrib, V * 4);
   } else {
-    ok = ok && alloc((void**)&P.cur, V * 4) && alloc((void**)&P.nxt, V * 4);
+    ok = ok && alloc((void**)&P.cur, V * 4) && alloc((void**)&P.nxt, V * 4) &&
+         alloc((void**)&P.fval, V * 4);
   }
   for (int i = 0; i < 2 && ok; ++i)
     ok = alloc((void**)&P.act[i], (V / 64) * 8) && alloc((void**)&P.rowact[i], ctx->T);
@@ -555,12 +625,13 @@ int program_alloc(cytom_ctx* ctx, Program& P) {
 }
 
 void program_free(Program& P) {
-  void* ps[] = {P.cur, P.nxt, P.rank, P.delta, P.dsum, P.contrib, P.act[0], P.act[1],
+  void* ps[] = {P.cur, P.nxt, P.fval, P.rank, P.delta, P.dsum, P.contrib, P.act[0], P.act[1],
                 P.rowact[0], P.rowact[1], P.changed};
   for (void* p : ps)
     if (p) (void)hipFree(p);
-  P.cur = P.nxt = nullptr;
-  P.rank = P.delta = P.dsum = P.contrib = nullptr;
+  P.cur = P.nxt = P.fval = nullptr;
+  P.rank = P.delta = P.contrib = nullptr;
+  P.dsum = nullptr;
   P.act[0] = P.act[1] = nullptr;
   P.rowact[0] = P.rowact[1] = nullptr;
   P.changed = nullptr;
@@ -600,7 +671,7 @@ void launch_hooks(cytom_ctx* ctx, Program& P, const uint32_t* d_src, const uint3
     if (n_ins + n_found)
       k_hooks_prd<<<grid_for(n_ins + n_found, 256), 256, 0, st>>>(
           d_src, d_dst, ins_slot, n_ins, found, n_found, ctx->pool, P.rank, ctx->outdeg, P.dsum,
-          P.act[P.cur_act], P.changed, ctx->delta);
+          P.act[P.cur_act], P.changed, ctx->delta, ldexpf(1.f, ctx->prd_shift));
   } else if (n_ins) {
     k_hooks_min<<<grid_for(n_ins, 256), 256, 0, st>>>(P.kind, d_src, d_dst, d_w, ins_slot, n_ins,
                                                       ctx->pool, P.cur, P.nxt, P.act[P.cur_act],
@@ -623,7 +694,7 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
   k_frontier<<<apply_blocks, APPLY_THREADS, 0, st>>>(P.kind, ctx->V, ctx->log2vpt,
                                                      P.act[P.cur_act], ctx->outdeg,
                                                      P.rowact[P.cur_act], P.delta, P.contrib,
-                                                     ctx->d_iter);
+                                                     P.cur, P.fval, ctx->d_iter);
   // Iterations are launched in growing groups with one host check per
   // group; once the frontier is empty the remaining launches of a group are
   // no-ops (the gather returns at once, apply finds nothing changed).
@@ -645,20 +716,22 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
       const int nx = P.cur_act ^ 1;
       if (P.kind == CYTOM_PRD) {
         k_apply_prd<<<apply_blocks, APPLY_THREADS, 0, st>>>(
-            ctx->V, ctx->log2vpt, P.first_iter ? 1 : 0, ctx->delta, P.rank, P.delta, P.dsum,
-            P.contrib, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg, ctx->d_iter + i + 1);
+            ctx->V, ctx->log2vpt, P.first_iter ? 1 : 0, ctx->delta, ldexp(1.0, -ctx->prd_shift),
+            P.rank, P.delta, P.dsum,
+            P.contrib, P.changed, P.act[P.cur_act], P.act[nx], P.rowact[nx], ctx->outdeg,
+            ctx->d_iter + i + 1);
       } else if (P.kind == CYTOM_BFS) {
         k_apply_min<CYTOM_BFS><<<apply_blocks, APPLY_THREADS, 0, st>>>(
-            ctx->V, ctx->log2vpt, P.cur, P.nxt, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg,
-            ctx->d_iter + i + 1);
+            ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
+            P.rowact[nx], ctx->outdeg, ctx->d_iter + i + 1);
       } else if (P.kind == CYTOM_CC) {
         k_apply_min<CYTOM_CC><<<apply_blocks, APPLY_THREADS, 0, st>>>(
-            ctx->V, ctx->log2vpt, P.cur, P.nxt, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg,
-            ctx->d_iter + i + 1);
+            ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
+            P.rowact[nx], ctx->outdeg, ctx->d_iter + i + 1);
       } else {
         k_apply_min<CYTOM_SSSP><<<apply_blocks, APPLY_THREADS, 0, st>>>(
-            ctx->V, ctx->log2vpt, P.cur, P.nxt, P.changed, P.act[nx], P.rowact[nx], ctx->outdeg,
-            ctx->d_iter + i + 1);
+            ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
+            P.rowact[nx], ctx->outdeg, ctx->d_iter + i + 1);
       }
       P.cur_act = nx;
       P.first_iter = false;
@@ -722,17 +795,16 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
 }
 
 int configure_gather(cytom_ctx* ctx) {
-  const size_t lds = (size_t)ctx->vpt * 4;
+  const int l4 = (int)ctx->vpt * 4, l8 = (int)ctx->vpt * 8;
   CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_BFS>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, l4));
   CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_CC>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, l4));
   CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_SSSP>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, l4));
   CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_PRD>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const uint32_t per_cu = (uint32_t)std::max<size_t>(1, (160 * 1024) / std::max<size_t>(lds, 1));
-  ctx->gather_blocks = (uint32_t)ctx->num_cus * std::min<uint32_t>(per_cu, 2);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, l8));
+  ctx->gather_blocks = 0;
   return 0;
 }
 

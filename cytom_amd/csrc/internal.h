@@ -65,10 +65,11 @@ struct Program {
   uint32_t source = 0;
   uint32_t* cur = nullptr;  // BFS/CC/SSSP current values
   uint32_t* nxt = nullptr;  // BFS/CC/SSSP next values
+  uint32_t* fval = nullptr; // BFS/CC/SSSP: cur[v] while v is active, else INF
   float* rank = nullptr;    // PRD next.rank
   float* delta = nullptr;   // PRD current.delta
-  float* dsum = nullptr;    // PRD next.delta_sum
-  float* contrib = nullptr; // PRD delta/out_degree of active vertices
+  long long* dsum = nullptr;  // PRD next.delta_sum, fixed point scaled by 2^prd_shift
+  float* contrib = nullptr; // PRD delta/out_degree while v is active, else 0
   uint64_t* act[2] = {nullptr, nullptr};  // active bitmaps (wave-sized words)
   uint8_t* rowact[2] = {nullptr, nullptr};  // per source-row activity
   uint8_t* changed = nullptr;               // per-vertex changed flags
@@ -132,7 +133,10 @@ struct cytom_ctx {
   uint32_t chunk_slots = 512;   // slots per chunk
   uint32_t job_chunks = 128;    // chunks per column job
   uint32_t gather_threads = 512;
-  uint32_t gather_blocks = 0;   // persistent grid
+  uint32_t gather_blocks = 0;   // persistent grid (0 = from the LDS budget per program)
+  // PageRank-delta sums are exact int64 fixed point with this many fraction
+  // bits: 60 - ceil(log2(2V)) keeps any vertex's sum (bounded by 2V) in range.
+  int prd_shift = 36;
   int num_cus = 256;
   cytom::IterStats* d_iter = nullptr;  // [max_iter + 2]
   cytom::IterStats* h_iter = nullptr;  // pinned

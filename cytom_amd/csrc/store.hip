@@ -52,16 +52,20 @@ __device__ inline uint32_t tile_of(uint32_t s, uint32_t d, uint32_t lv, uint32_t
 }
 
 // ---- validation and keys
+// Sort key of an insert: its tile id in the high bits and the top `sb` bits
+// of its tile-local source below, so every tile's segment of the batch lands
+// ordered by source (neighbouring gather lanes then share source lines).
 __global__ void k_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                        const uint32_t* __restrict__ w, uint64_t n, uint64_t V, uint32_t lv,
-                       uint32_t T, int need_w, uint32_t* __restrict__ keys,
+                       uint32_t T, uint32_t sb, int need_w, uint32_t* __restrict__ keys,
                        uint32_t* __restrict__ vals, unsigned long long* misc) {
+  const uint32_t smask = (1u << lv) - 1u;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t s = src[i], d = dst[i];
     const bool bad = s >= V || d >= V || (need_w && w[i] == 0u);
     if (bad) atomicOr(&misc[M_ERR], 1ull);
-    keys[i] = bad ? 0u : tile_of(s, d, lv, T);
+    keys[i] = bad ? 0u : ((tile_of(s, d, lv, T) << sb) | ((s & smask) >> (lv - sb)));
     vals[i] = (uint32_t)i;
   }
 }
@@ -75,20 +79,21 @@ __global__ void k_del_validate(const uint32_t* __restrict__ src, const uint32_t*
 }
 
 // ---- segments of the sorted batch
-__global__ void k_heads(const uint32_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ f) {
+__global__ void k_heads(const uint32_t* __restrict__ keys, uint64_t n, uint32_t sb,
+                        uint32_t* __restrict__ f) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
-    f[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+    f[i] = (i == 0 || (keys[i] >> sb) != (keys[i - 1] >> sb)) ? 1u : 0u;
 }
 
 __global__ void k_seg_emit(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ f,
-                           const uint32_t* __restrict__ sx, uint64_t n,
+                           const uint32_t* __restrict__ sx, uint64_t n, uint32_t sb,
                            uint32_t* __restrict__ seg_start, uint32_t* __restrict__ seg_tile) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     if (f[i]) {
       seg_start[sx[i]] = (uint32_t)i;
-      seg_tile[sx[i]] = keys[i];
+      seg_tile[sx[i]] = keys[i] >> sb;
     }
     if (i == n - 1) seg_start[sx[n]] = (uint32_t)n;
   }
@@ -208,7 +213,7 @@ __global__ void __launch_bounds__(CP_THREADS) k_repack(StoreView S, const uint64
 
 // ---- segmented scatter into slack
 __global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                          uint64_t n, const uint32_t* __restrict__ f,
+                          uint64_t n, uint32_t sb, const uint32_t* __restrict__ f,
                           const uint32_t* __restrict__ sx, const uint32_t* __restrict__ seg_start,
                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                           const uint32_t* __restrict__ w, StoreView S,
@@ -217,7 +222,7 @@ __global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __r
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t seg = sx[i] + f[i] - 1u;
     const uint32_t pos = (uint32_t)i - seg_start[seg];
-    const uint32_t t = keys[i];
+    const uint32_t t = keys[i] >> sb;
     const uint32_t v = vals[i];
     const uint32_t s = src[v], d = dst[v];
     const uint64_t slot = S.tile_off[t] + S.tile_cnt[t] + pos;
@@ -452,11 +457,14 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
   if (ctx->weighted && n_ins && !d_w)
     return set_err(ctx, CYTOM_EINVAL, "weighted graph needs insertion weights");
   const uint32_t lv = ctx->log2vpt, T = ctx->T;
-  const uint32_t key_bits = [&] {
+  const uint32_t tile_bits = [&] {
     uint32_t b = 0;
     while ((1ull << b) < ctx->n_tiles) ++b;
     return b;
   }();
+  // Source bits that fit beside the tile id in a 32-bit key.
+  const uint32_t sb = std::min<uint32_t>(lv, 32 - tile_bits);
+  const uint32_t key_bits = tile_bits + sb;
   const uint64_t nmax = std::max<uint64_t>(n_ins, 1);
   if ((rc = ensure(ctx, ctx->g_keys, nmax * 4)) || (rc = ensure(ctx, ctx->g_vals, nmax * 4)) ||
       (rc = ensure(ctx, ctx->g_keys2, nmax * 4)) || (rc = ensure(ctx, ctx->g_vals2, nmax * 4)) ||
@@ -491,7 +499,7 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
   CY_HIP(ctx, hipMemsetAsync(misc + 1, 0, 63 * 8, st));
   // 1. validation (nothing is modified before it passes)
   if (n_ins)
-    k_keys<<<grid_for(n_ins, 256), 256, 0, st>>>(d_src, d_dst, d_w, n_ins, ctx->V, lv, T,
+    k_keys<<<grid_for(n_ins, 256), 256, 0, st>>>(d_src, d_dst, d_w, n_ins, ctx->V, lv, T, sb,
                                                  ctx->weighted ? 1 : 0, keys, vals, misc);
   if (n_del)
     k_del_validate<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, ctx->V, misc);
@@ -509,9 +517,10 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
     // ping-pong buffers.
     radix_sort_pairs(keys, vals, skeys, svals, flags, sx, n_ins, key_bits, ctx->g_hist.p,
                      ctx->g_scan_tmp.p, st);
-    k_heads<<<grid_for(n_ins, 256), 256, 0, st>>>(skeys, n_ins, flags);
+    k_heads<<<grid_for(n_ins, 256), 256, 0, st>>>(skeys, n_ins, sb, flags);
     scan_exclusive_u32(flags, sx, n_ins, ctx->g_scan_tmp.p, st);
-    k_seg_emit<<<grid_for(n_ins, 256), 256, 0, st>>>(skeys, flags, sx, n_ins, seg_start, seg_tile);
+    k_seg_emit<<<grid_for(n_ins, 256), 256, 0, st>>>(skeys, flags, sx, n_ins, sb, seg_start,
+                                                     seg_tile);
     const uint32_t* nseg_p = sx + n_ins;
     for (int attempt = 0;; ++attempt) {
       CY_HIP(ctx, hipMemsetAsync(misc + M_NEED, 0, 16, st));
@@ -533,7 +542,7 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
       k_relocate<<<std::min<uint32_t>(nreloc, 8192), CP_THREADS, 0, st>>>(reloc_list, nreloc,
                                                                          ctx->view());
     }
-    k_scatter<<<grid_for(n_ins, 256), 256, 0, st>>>(skeys, svals, n_ins, flags, sx, seg_start,
+    k_scatter<<<grid_for(n_ins, 256), 256, 0, st>>>(skeys, svals, n_ins, sb, flags, sx, seg_start,
                                                     d_src, d_dst, d_w, ctx->view(), ins_slot);
     k_seg_commit<<<grid_for(n_ins, 256), 256, 0, st>>>(nseg_p, seg_start, seg_tile, ctx->view());
   }

@@ -44,7 +44,7 @@ typedef struct cytom_ctx cytom_ctx;
  * src/lib/util/config.cc:10-47). */
 typedef struct {
   uint64_t n_vertices;        /* rounded up to a power of two (>= 64) */
-  uint32_t vertices_per_tile; /* power of two in [64, 32768]; 0 = default */
+  uint32_t vertices_per_tile; /* power of two in [64, 16384]; 0 = default */
   uint32_t max_iterations;    /* algorithm_iterations; 0 = 100 (run.py:119) */
   double delta;               /* algorithm_config.delta (PRD threshold) */
   uint64_t edge_capacity;     /* initial edge-pool slots; 0 = automatic */
@@ -141,6 +141,20 @@ int cytom_read_degrees(cytom_ctx* ctx, uint32_t* out_degree, uint32_t* in_degree
 int64_t cytom_export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_t* w,
                            uint64_t cap);
 
+/* ---- tuning ------------------------------------------------------------- */
+
+/* Launch shape of the gather (the device analogue of the TileDistributor's
+ * tile_batching and executor count, tile-distributor.h:113-177,
+ * run.py:115-132).  Zero fields keep the current value. */
+typedef struct {
+  uint32_t chunk_slots;   /* edge slots per work chunk (multiple of 64, <= 512) */
+  uint32_t job_chunks;    /* chunks per column job (<= 512) */
+  uint32_t blocks_per_cu; /* persistent gather blocks per compute unit */
+  uint32_t pad;
+} cytom_tuning;
+int cytom_set_tuning(cytom_ctx* ctx, const cytom_tuning* t);
+int cytom_get_tuning(const cytom_ctx* ctx, cytom_tuning* t);
+
 /* ---- measurement ------------------------------------------------------- */
 
 /* Enables (1) or disables (0) HIP-event timing of every gather launch and
@@ -178,6 +192,9 @@ int cytom_device_alloc(void** ptr, size_t bytes);
 int cytom_device_free(void* ptr);
 int cytom_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int cytom_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* Device-to-device copy (a plain copy kernel: the reference point for HBM
+ * traffic counters). */
+int cytom_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int cytom_device_synchronize(void);
 int cytom_device_count(void);
 
