@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--chunk-slots", type=int, default=0)
     ap.add_argument("--job-chunks", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--source-bands", type=int, default=0)
+    ap.add_argument("--resort-pct", type=int, default=0,
+                    help="sort-compaction threshold in percent (-1 disables)")
     ap.add_argument("--programs", default="bfs,cc,sssp,pagerank-delta")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
@@ -112,7 +115,8 @@ def gpu_run(args):
         % (args.batches, n_ins, n_del, time.perf_counter() - t_setup, source))
     g = C.Graph(V, vertices_per_tile=args.vertices_per_tile, weighted=weighted,
                 max_iterations=100, delta=1e-7, edge_capacity=int(3 * E))
-    g.set_tuning(args.chunk_slots, args.job_chunks, args.blocks_per_cu)
+    g.set_tuning(args.chunk_slots, args.job_chunks, args.blocks_per_cu, args.source_bands,
+                 C.RESORT_OFF if args.resort_pct < 0 else args.resort_pct)
     pids = {}
     for n in names:
         pids[n] = g.register(n, source if n in ("bfs", "sssp") else 0)

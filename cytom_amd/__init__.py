@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libcytom_hip.so")
 
 BFS, CC, SSSP, PRD = 0, 1, 2, 3
 PROGRAMS = {"bfs": BFS, "cc": CC, "sssp": SSSP, "pagerank-delta": PRD}
+RESORT_OFF = 0xFFFFFFFF  # cytom_tuning.resort_pct value that disables sort-compaction
 
 EINVAL, ENOMEM, ERANGE, EDEVICE = -22, -12, -34, -5
 
@@ -60,7 +61,9 @@ class Tuning(ctypes.Structure):
         ("chunk_slots", ctypes.c_uint32),
         ("job_chunks", ctypes.c_uint32),
         ("blocks_per_cu", ctypes.c_uint32),
-        ("pad", ctypes.c_uint32),
+        ("source_bands", ctypes.c_uint32),
+        ("resort_pct", ctypes.c_uint32),
+        ("resort_max_slots", ctypes.c_uint32),
     ]
 
 
@@ -72,6 +75,8 @@ class BatchStats(ctypes.Structure):
         ("live_edges", ctypes.c_uint64),
         ("tiles_relocated", ctypes.c_uint32),
         ("tiles_compacted", ctypes.c_uint32),
+        ("tiles_sorted", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
         ("t_ingest_ms", ctypes.c_double),
     ]
 
@@ -344,15 +349,19 @@ class Graph:
         self._check(lib().cytom_read_degrees(self._h, _p(o), _p(i)), "degrees")
         return o, i
 
-    def set_tuning(self, chunk_slots=0, job_chunks=0, blocks_per_cu=0):
-        t = Tuning(chunk_slots, job_chunks, blocks_per_cu, 0)
+    def set_tuning(self, chunk_slots=0, job_chunks=0, blocks_per_cu=0, source_bands=0,
+                   resort_pct=0, resort_max_slots=0):
+        """Zero keeps a field; resort_pct=RESORT_OFF disables sort-compaction."""
+        t = Tuning(chunk_slots, job_chunks, blocks_per_cu, source_bands, resort_pct,
+                   resort_max_slots)
         self._check(lib().cytom_set_tuning(self._h, ctypes.byref(t)), "tuning")
 
     def get_tuning(self):
         t = Tuning()
         self._check(lib().cytom_get_tuning(self._h, ctypes.byref(t)), "tuning")
         return {"chunk_slots": t.chunk_slots, "job_chunks": t.job_chunks,
-                "blocks_per_cu": t.blocks_per_cu}
+                "blocks_per_cu": t.blocks_per_cu, "source_bands": t.source_bands,
+                "resort_pct": t.resort_pct, "resort_max_slots": t.resort_max_slots}
 
     def set_profiling(self, enable=True):
         self._check(lib().cytom_set_profiling(self._h, 1 if enable else 0), "profiling")

@@ -60,8 +60,9 @@ void free_ctx(cytom_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& P : c->progs) program_free(P);
-  void* ps[] = {c->tile_off, c->tile_cap, c->tile_cnt, c->tile_live, c->pool, c->wpool,
-                c->outdeg,   c->indeg,    c->d_misc,   c->d_iter};
+  void* ps[] = {c->tile_off, c->tile_cap, c->tile_cnt, c->tile_live, c->tile_sorted,
+                c->pool,     c->wpool,    c->outdeg,   c->indeg,    c->d_misc,
+                c->d_iter};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->h_misc) (void)hipHostFree(c->h_misc);
@@ -74,7 +75,7 @@ void free_ctx(cytom_ctx* c) {
                      &c->g_tab_head, &c->g_node_next, &c->g_claimed,   &c->g_tile_flag,
                      &c->g_del_tiles, &c->g_found,    &c->g_tile_scan, &c->g_nch,
                      &c->g_cbase,    &c->g_chunks,    &c->g_colj,      &c->g_jbase,
-                     &c->g_jobs};
+                     &c->g_jobs,     &c->g_rs_e,      &c->g_rs_w,      &c->g_rs_off};
   for (GrowBuf* b : bufs) free_buf(*b);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -109,7 +110,8 @@ int cytom_create(const cytom_config* cfg, cytom_ctx** out) {
   while (V < cfg->n_vertices) V <<= 1;
   uint32_t vpt = cfg->vertices_per_tile;
   if (vpt == 0) vpt = 16384;
-  if (vpt < 64 || vpt > 16384 || (vpt & (vpt - 1))) return CYTOM_EINVAL;  // 8-byte LDS window
+  // 16-bit local ids in the edge slots and an 8-byte LDS column window.
+  if (vpt < 64 || vpt > 16384 || (vpt & (vpt - 1))) return CYTOM_EINVAL;
   if (vpt > V) vpt = (uint32_t)V;
   if ((V / vpt) * (V / vpt) > (1ull << 30)) return CYTOM_EINVAL;  // tile ids stay 32-bit keys
   int ndev = 0;
@@ -149,6 +151,7 @@ int cytom_create(const cytom_config* cfg, cytom_ctx** out) {
             hipMalloc(&c->tile_cap, nt * 4) == hipSuccess &&
             hipMalloc(&c->tile_cnt, nt * 4) == hipSuccess &&
             hipMalloc(&c->tile_live, nt * 4) == hipSuccess &&
+            hipMalloc(&c->tile_sorted, nt * 4) == hipSuccess &&
             hipMalloc(&c->outdeg, V * 4) == hipSuccess &&
             hipMalloc(&c->indeg, V * 4) == hipSuccess &&
             hipMalloc(&c->d_misc, 64 * 8) == hipSuccess &&
@@ -158,13 +161,14 @@ int cytom_create(const cytom_config* cfg, cytom_ctx** out) {
                           hipHostMallocDefault) == hipSuccess;
   if (!ok) return fail(CYTOM_ENOMEM);
   c->pool_size = cfg->edge_capacity ? cfg->edge_capacity : (1ull << 22);
-  if (hipMalloc(&c->pool, c->pool_size * 8) != hipSuccess) return fail(CYTOM_ENOMEM);
+  if (hipMalloc(&c->pool, c->pool_size * 4) != hipSuccess) return fail(CYTOM_ENOMEM);
   if (c->weighted && hipMalloc(&c->wpool, c->pool_size * 4) != hipSuccess)
     return fail(CYTOM_ENOMEM);
   ok = hipMemsetAsync(c->tile_off, 0, nt * 8, c->stream) == hipSuccess &&
        hipMemsetAsync(c->tile_cap, 0, nt * 4, c->stream) == hipSuccess &&
        hipMemsetAsync(c->tile_cnt, 0, nt * 4, c->stream) == hipSuccess &&
        hipMemsetAsync(c->tile_live, 0, nt * 4, c->stream) == hipSuccess &&
+       hipMemsetAsync(c->tile_sorted, 0, nt * 4, c->stream) == hipSuccess &&
        hipMemsetAsync(c->outdeg, 0, V * 4, c->stream) == hipSuccess &&
        hipMemsetAsync(c->indeg, 0, V * 4, c->stream) == hipSuccess &&
        hipMemsetAsync(c->d_misc, 0, 64 * 8, c->stream) == hipSuccess &&
@@ -313,13 +317,22 @@ int64_t cytom_export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_
 int cytom_set_tuning(cytom_ctx* ctx, const cytom_tuning* t) {
   CY_CHECK_CTX(ctx);
   if (!t) return set_err(ctx, CYTOM_EINVAL, "null tuning");
-  if (t->chunk_slots && (t->chunk_slots % 64 || t->chunk_slots > 512))
-    return set_err(ctx, CYTOM_EINVAL, "chunk_slots must be a multiple of 64 up to 512");
+  if (t->chunk_slots && (t->chunk_slots % 64 || t->chunk_slots > 1024))
+    return set_err(ctx, CYTOM_EINVAL, "chunk_slots must be a multiple of 64 up to 1024");
   if (t->job_chunks > 512) return set_err(ctx, CYTOM_EINVAL, "job_chunks above 512");
   if (t->blocks_per_cu > 16) return set_err(ctx, CYTOM_EINVAL, "blocks_per_cu above 16");
-  if (t->chunk_slots) ctx->chunk_slots = t->chunk_slots;
-  if (t->job_chunks) ctx->job_chunks = t->job_chunks;
+  if (t->source_bands && (t->source_bands & (t->source_bands - 1)))
+    return set_err(ctx, CYTOM_EINVAL, "source_bands must be a power of two");
+  const uint32_t cs = t->chunk_slots ? t->chunk_slots : ctx->chunk_slots;
+  const uint32_t jc = t->job_chunks ? t->job_chunks : ctx->job_chunks;
+  if ((uint64_t)cs * jc > (1u << 17))
+    return set_err(ctx, CYTOM_EINVAL, "job_chunks * chunk_slots above 131072 slots per job");
+  ctx->chunk_slots = cs;
+  ctx->job_chunks = jc;
   if (t->blocks_per_cu) ctx->gather_blocks = (uint32_t)ctx->num_cus * t->blocks_per_cu;
+  if (t->source_bands) ctx->source_bands = t->source_bands;
+  if (t->resort_pct) ctx->resort_pct = t->resort_pct;
+  if (t->resort_max_slots) ctx->resort_max_slots = t->resort_max_slots;
   ctx->jobs_dirty = true;
   return 0;
 }
@@ -329,7 +342,9 @@ int cytom_get_tuning(const cytom_ctx* ctx, cytom_tuning* t) {
   t->chunk_slots = ctx->chunk_slots;
   t->job_chunks = ctx->job_chunks;
   t->blocks_per_cu = ctx->gather_blocks / (uint32_t)ctx->num_cus;
-  t->pad = 0;
+  t->source_bands = std::min(ctx->source_bands, ctx->T);
+  t->resort_pct = ctx->resort_pct;
+  t->resort_max_slots = ctx->resort_max_slots;
   return 0;
 }
 

@@ -30,9 +30,9 @@
 namespace cytom {
 namespace {
 
-constexpr int GATHER_THREADS = 512;
+constexpr int GATHER_THREADS = 1024;
 constexpr int APPLY_THREADS = 256;
-constexpr int GATHER_UNROLL = 8;  // slots per lane per chunk step
+constexpr int GATHER_ROWS = 8;  // 64-slot rows per wavefront step
 
 inline uint32_t grid_for(uint64_t n, uint32_t threads, uint32_t cap = 65536) {
   uint64_t b = (n + threads - 1) / threads;
@@ -129,20 +129,37 @@ __global__ void k_emit_chunks(const uint32_t* __restrict__ nch, const uint32_t* 
   }
 }
 
-__global__ void k_col_jobs(const uint32_t* __restrict__ cbase, uint32_t T, uint32_t JC,
-                           uint32_t* __restrict__ colj) {
-  for (uint32_t y = blockIdx.x * blockDim.x + threadIdx.x; y < T; y += gridDim.x * blockDim.x) {
-    const uint32_t n = cbase[(uint64_t)(y + 1) * T] - cbase[(uint64_t)y * T];
-    colj[y] = (n + JC - 1) / JC;
+// Jobs are cut from cells: cell (b, y) holds the chunks of column y whose
+// source rows fall in band b (T/NB consecutive rows).  Cells are numbered
+// band-major, so the persistent grid sweeps the source bands one after the
+// other and every XCD's L2 holds only one band's source values at a time.
+__device__ inline void cell_range(const uint32_t* cbase, uint32_t T, uint32_t NB, uint32_t cell,
+                                  uint32_t* y, uint32_t* c_begin, uint32_t* c_end) {
+  const uint32_t rows = T / NB, b = cell / T;
+  *y = cell % T;
+  const uint64_t t0 = (uint64_t)*y * T + (uint64_t)b * rows;
+  *c_begin = cbase[t0];
+  *c_end = cbase[t0 + rows];
+}
+
+__global__ void k_col_jobs(const uint32_t* __restrict__ cbase, uint32_t T, uint32_t NB,
+                           uint32_t JC, uint32_t* __restrict__ colj) {
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < T * NB;
+       c += gridDim.x * blockDim.x) {
+    uint32_t y, c0, c1;
+    cell_range(cbase, T, NB, c, &y, &c0, &c1);
+    colj[c] = (c1 - c0 + JC - 1) / JC;
   }
 }
 
 __global__ void k_emit_jobs(const uint32_t* __restrict__ cbase, const uint32_t* __restrict__ colj,
-                            const uint32_t* __restrict__ jbase, uint32_t T, uint32_t JC,
-                            uint4* __restrict__ jobs) {
-  for (uint32_t y = blockIdx.x * blockDim.x + threadIdx.x; y < T; y += gridDim.x * blockDim.x) {
-    const uint32_t c_begin = cbase[(uint64_t)y * T], c_end = cbase[(uint64_t)(y + 1) * T];
-    const uint32_t nj = colj[y], b = jbase[y];
+                            const uint32_t* __restrict__ jbase, uint32_t T, uint32_t NB,
+                            uint32_t JC, uint4* __restrict__ jobs) {
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < T * NB;
+       c += gridDim.x * blockDim.x) {
+    uint32_t y, c_begin, c_end;
+    cell_range(cbase, T, NB, c, &y, &c_begin, &c_end);
+    const uint32_t nj = colj[c], b = jbase[c];
     for (uint32_t j = 0; j < nj; ++j) {
       const uint32_t c0 = c_begin + j * JC;
       const uint32_t c1 = min(c0 + JC, c_end);
@@ -171,93 +188,102 @@ struct GatherArgs {
 };
 
 constexpr uint32_t MAX_JOB_CHUNKS = GATHER_THREADS;  // one metadata pass per job
+// Row descriptors of one job held in LDS (job_chunks * chunk_slots / 64).
+constexpr uint32_t MAX_JOB_ROWS = 2048;
 
-// Edge slots of one chunk into registers: slot (base + u*64 + lane), clamped
-// to the chunk's last slot so every load is issued without a branch.
+// The active chunks of a job form one list of 64-slot rows: a chunk of n
+// slots owns ceil(n/64) rows, so a tile with a handful of edges costs one
+// row, not a whole chunk.  A row descriptor packs the pool offset of its
+// first slot (bits 22-61), the tile's source column x (bits 6-21) and the
+// number of valid slots minus one (bits 0-5).  A wavefront reads the
+// descriptors of GATHER_ROWS consecutive rows at once (LDS broadcasts), then
+// issues their edge loads back to back; lanes past a row's end load the
+// row's first slot and are masked off in vmask (bit u = row u is valid for
+// this lane).
+__device__ inline uint64_t row_desc(uint64_t off, uint32_t x, uint32_t nvalid) {
+  return (off << 22) | ((uint64_t)x << 6) | (uint64_t)(nvalid - 1);
+}
+
 template <int KIND>
-__device__ inline void load_chunk(const GatherArgs<KIND>& A, uint64_t off, uint32_t n,
-                                  uint64_t (&e)[GATHER_UNROLL], uint32_t (&wt)[GATHER_UNROLL]) {
-  const int lane = threadIdx.x & 63;
+__device__ inline void load_rows(const GatherArgs<KIND>& A, const uint64_t* rows, uint32_t R,
+                                 uint32_t r0, uint32_t (&e)[GATHER_ROWS],
+                                 uint32_t (&wt)[GATHER_ROWS], uint32_t (&xb)[GATHER_ROWS],
+                                 uint32_t& vmask) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint64_t d[GATHER_ROWS];
 #pragma unroll
-  for (int u = 0; u < GATHER_UNROLL; ++u) e[u] = A.S.pool[off + min((uint32_t)(u * 64 + lane), n - 1)];
-  if (KIND == CYTOM_SSSP) {
+  for (int u = 0; u < GATHER_ROWS; ++u) d[u] = rows[min(r0 + u, R - 1)];
+  vmask = 0;
 #pragma unroll
-    for (int u = 0; u < GATHER_UNROLL; ++u) wt[u] = A.S.wpool[off + min((uint32_t)(u * 64 + lane), n - 1)];
+  for (int u = 0; u < GATHER_ROWS; ++u) {
+    const bool v = r0 + u < R && lane <= (uint32_t)(d[u] & 63u);
+    const uint64_t idx = (d[u] >> 22) + (v ? lane : 0u);
+    e[u] = A.S.pool[idx];
+    if (KIND == CYTOM_SSSP) wt[u] = A.S.wpool[idx];
+    xb[u] = ((uint32_t)(d[u] >> 6) & 0xFFFFu) << A.S.log2vpt;
+    vmask |= (v ? 1u : 0u) << u;
   }
 }
 
 // One wavefront's share of a column job, software-pipelined: while the
-// source frontier values and LDS atomics of chunk k are in flight, the edge
-// loads of its next chunk are already issued.  A source's frontier value is
-// INF (0 for PageRank-delta) while it is inactive, so the active-bitmap test
-// costs no extra load.
+// source frontier values and LDS atomics of one row block are in flight,
+// the edge loads of its next block are already issued.  A source's frontier
+// value is INF (0 for PageRank-delta) while it is inactive, so the
+// active-bitmap test costs no extra load.
 template <int KIND>
-__device__ inline unsigned long long gather_wave(const GatherArgs<KIND>& A, uint32_t* win,
-                                                 unsigned long long* winq, const uint64_t* s_off,
-                                                 const uint32_t* s_n, uint32_t nact,
-                                                 uint32_t mask) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t nwaves = GATHER_THREADS / 64;
-  uint32_t k = threadIdx.x >> 6;
-  unsigned long long scanned = 0;
-  if (k >= nact) return 0;
-  uint64_t e[GATHER_UNROLL];
-  uint32_t wt[GATHER_UNROLL];
-  uint32_t n = s_n[k];
-  load_chunk<KIND>(A, s_off[k], n, e, wt);
+__device__ inline void gather_rows(const GatherArgs<KIND>& A, uint32_t* win,
+                                   unsigned long long* winq, const uint64_t* rows, uint32_t R) {
+  const uint32_t step = (GATHER_THREADS / 64) * GATHER_ROWS;
+  uint32_t r0 = (threadIdx.x >> 6) * GATHER_ROWS;
+  if (r0 >= R) return;
+  uint32_t e[GATHER_ROWS], wt[GATHER_ROWS], xb[GATHER_ROWS], vm;
+  load_rows<KIND>(A, rows, R, r0, e, wt, xb, vm);
   for (;;) {
-    scanned += n;
-    const uint32_t kn = k + nwaves;
-    const bool more = kn < nact;
-    uint32_t s[GATHER_UNROLL];
-    bool ok[GATHER_UNROLL];
+    const uint32_t rn = r0 + step;
+    const bool more = rn < R;
+    uint32_t s[GATHER_ROWS];
+    bool ok[GATHER_ROWS];
 #pragma unroll
-    for (int u = 0; u < GATHER_UNROLL; ++u) {
-      ok[u] = (uint32_t)(u * 64 + lane) < n && e[u] != TOMB;
-      s[u] = ok[u] ? (uint32_t)e[u] : 0u;
+    for (int u = 0; u < GATHER_ROWS; ++u) {
+      ok[u] = ((vm >> u) & 1u) && e[u] != TOMB;
+      s[u] = ok[u] ? xb[u] + (e[u] & 0xFFFFu) : 0u;
     }
-    float c[GATHER_UNROLL];
-    uint32_t val[GATHER_UNROLL];
+    float c[GATHER_ROWS];
+    uint32_t val[GATHER_ROWS];
     if (KIND == CYTOM_PRD) {
 #pragma unroll
-      for (int u = 0; u < GATHER_UNROLL; ++u) c[u] = A.contrib[s[u]];
+      for (int u = 0; u < GATHER_ROWS; ++u) c[u] = A.contrib[s[u]];
     } else {
 #pragma unroll
-      for (int u = 0; u < GATHER_UNROLL; ++u) val[u] = A.fval[s[u]];
+      for (int u = 0; u < GATHER_ROWS; ++u) val[u] = A.fval[s[u]];
     }
-    uint64_t en[GATHER_UNROLL];
-    uint32_t wtn[GATHER_UNROLL];
-    uint32_t nn = 0;
-    if (more) {
-      nn = s_n[kn];
-      load_chunk<KIND>(A, s_off[kn], nn, en, wtn);
-    }
+    uint32_t en[GATHER_ROWS], wtn[GATHER_ROWS], xbn[GATHER_ROWS], vmn = 0;
+    if (more) load_rows<KIND>(A, rows, R, rn, en, wtn, xbn, vmn);
     if (KIND == CYTOM_PRD) {
 #pragma unroll
-      for (int u = 0; u < GATHER_UNROLL; ++u)
+      for (int u = 0; u < GATHER_ROWS; ++u)
         if (ok[u] && c[u] != 0.f)  // pagerank-delta.cc:127-132, as exact fixed point
-          atomicAdd(&winq[(uint32_t)(e[u] >> 32) & mask],
-                    (unsigned long long)__float2ll_rn(c[u] * A.qscale));
+          atomicAdd(&winq[e[u] >> 16], (unsigned long long)__float2ll_rn(c[u] * A.qscale));
     } else {
 #pragma unroll
-      for (int u = 0; u < GATHER_UNROLL; ++u) {
+      for (int u = 0; u < GATHER_ROWS; ++u) {
         uint32_t cand;
         if (KIND == CYTOM_BFS) cand = val[u] == INF ? INF : val[u] + 1;  // bfs.cc:83-91
         else if (KIND == CYTOM_CC) cand = val[u];                         // cc.cc:81-91
         else cand = sat_add(val[u], wt[u]);                               // sssp.cc:90-99
-        if (ok[u] && cand != INF) atomicMin(&win[(uint32_t)(e[u] >> 32) & mask], cand);
+        if (ok[u] && cand != INF) atomicMin(&win[e[u] >> 16], cand);
       }
     }
     if (!more) break;
 #pragma unroll
-    for (int u = 0; u < GATHER_UNROLL; ++u) {
+    for (int u = 0; u < GATHER_ROWS; ++u) {
       e[u] = en[u];
       wt[u] = wtn[u];
+      xb[u] = xbn[u];
     }
-    n = nn;
-    k = kn;
+    vm = vmn;
+    r0 = rn;
   }
-  return scanned;
 }
 
 template <int KIND>
@@ -268,10 +294,9 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
   extern __shared__ unsigned long long lds_window[];
   uint32_t* win = reinterpret_cast<uint32_t*>(lds_window);
   unsigned long long* winq = lds_window;
-  __shared__ uint64_t s_off[MAX_JOB_CHUNKS];
-  __shared__ uint32_t s_n[MAX_JOB_CHUNKS];
+  __shared__ uint64_t s_rows[MAX_JOB_ROWS];
   __shared__ uint32_t s_job;
-  const uint32_t vpt = A.S.vpt, T = A.S.T, mask = vpt - 1;
+  const uint32_t vpt = A.S.vpt, T = A.S.T;
   // Clear next iteration's row activity (written by the apply kernel).
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x)
     A.rowact_clear[i] = 0;
@@ -285,32 +310,35 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
     if (j >= njobs) break;  // uniform: every thread read s_job after the barrier
     const uint4 job = A.jobs[j];
     const uint32_t c0 = job.x, nc = job.y - job.x, y = job.z;
-    // Chunk table of the job, compacted to the chunks whose source row has an
-    // active vertex (TileManager::isActive, tile-manager.h:403-455).
-    uint32_t flag = 0;
+    // Row table of the job's chunks whose source row has an active vertex
+    // (TileManager::isActive, tile-manager.h:403-455).  One scan packs the
+    // rows (bits 24+) and slots (bits 0-23) before each chunk.
+    uint32_t flag = 0, n = 0, x = 0;
     uint64_t off = 0;
-    uint32_t n = 0;
     if (threadIdx.x < nc) {
       const uint64_t ch = A.chunks[c0 + threadIdx.x];
       const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
-      flag = A.rowact[t - y * T];
+      x = t - y * T;
+      flag = A.rowact[x];
       off = A.S.tile_off[t] + s0;
       n = min(A.CH, A.S.tile_cnt[t] - s0);
     }
-    uint32_t nact;
-    const uint32_t pos = block_scan_excl<GATHER_THREADS, uint32_t>(flag, &nact);
-    if (flag) {
-      s_off[pos] = off;
-      s_n[pos] = n;
-    }
-    if (nact == 0) continue;  // uniform; the scan's barriers ordered s_job reuse
+    const uint32_t nrows = flag ? (n + 63) >> 6 : 0u;
+    const uint64_t packed = ((uint64_t)nrows << 24) | (uint64_t)(flag ? n : 0u);
+    uint64_t tot;
+    const uint64_t ex = block_scan_excl<GATHER_THREADS, uint64_t>(packed, &tot);
+    const uint32_t R = (uint32_t)(tot >> 24);
+    if (R == 0) continue;  // uniform; the scan's barriers ordered s_job reuse
+    for (uint32_t j = 0, r = (uint32_t)(ex >> 24); j < nrows; ++j, ++r)
+      s_rows[r] = row_desc(off + 64 * j, x, min(64u, n - 64 * j));
+    scanned += tot & 0xFFFFFFull;
     if (KIND == CYTOM_PRD) {
       for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) winq[i] = 0ull;
     } else {
       for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) win[i] = INF;
     }
     __syncthreads();
-    scanned += gather_wave<KIND>(A, win, winq, s_off, s_n, nact, mask);
+    gather_rows<KIND>(A, win, winq, s_rows, R);
     __syncthreads();
     const uint64_t base = (uint64_t)y * vpt;
     for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) {
@@ -330,9 +358,8 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
     }
     __syncthreads();
   }
-  // Every lane of a wave holds the wave's count; keep lane 0's.
-  scanned = wave_sum_u64((threadIdx.x & 63) == 0 ? scanned : 0ull);
-  if ((threadIdx.x & 63) == 0 && scanned) atomicAdd(&A.it->scanned, scanned);
+  // Every thread holds the block's count.
+  if (threadIdx.x == 0 && scanned) atomicAdd(&A.it->scanned, scanned);
 }
 
 // ---------------------------------------------------------------- apply
@@ -483,7 +510,7 @@ __global__ void k_reexec_prd(const uint32_t* __restrict__ found, uint64_t n,
 __global__ void k_hooks_min(int kind, const uint32_t* __restrict__ src,
                             const uint32_t* __restrict__ dst, const uint32_t* __restrict__ w,
                             const uint64_t* __restrict__ ins_slot, uint64_t n,
-                            const uint64_t* __restrict__ pool, uint32_t* __restrict__ cur,
+                            const uint32_t* __restrict__ pool, uint32_t* __restrict__ cur,
                             uint32_t* __restrict__ nxt, uint64_t* __restrict__ act,
                             uint8_t* __restrict__ changed) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -510,7 +537,7 @@ __global__ void k_hooks_min(int kind, const uint32_t* __restrict__ src,
 __global__ void k_hooks_prd(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                             const uint64_t* __restrict__ ins_slot, uint64_t n,
                             const uint32_t* __restrict__ found, uint64_t n_found,
-                            const uint64_t* __restrict__ pool, const float* __restrict__ rank,
+                            const uint32_t* __restrict__ pool, const float* __restrict__ rank,
                             const uint32_t* __restrict__ outdeg, long long* __restrict__ dsum,
                             uint64_t* __restrict__ act, uint8_t* __restrict__ changed, double thr,
                             float qscale) {
@@ -549,7 +576,7 @@ template <int KIND>
 void launch_gather(cytom_ctx* ctx, Program& P, int it) {
   GatherArgs<KIND> A;
   A.jobs = static_cast<const uint4*>(ctx->g_jobs.p);
-  A.njobs = static_cast<const uint32_t*>(ctx->g_jbase.p) + ctx->T;
+  A.njobs = static_cast<const uint32_t*>(ctx->g_jbase.p) + ctx->n_cells;  // total job count
   A.chunks = static_cast<const uint64_t*>(ctx->g_chunks.p);
   A.S = ctx->view();
   A.rowact = P.rowact[P.cur_act];
@@ -570,9 +597,11 @@ int build_jobs(cytom_ctx* ctx) {
   hipStream_t st = ctx->stream;
   int rc;
   const uint64_t nt = ctx->n_tiles;
+  const uint32_t NB = std::min(ctx->source_bands, ctx->T);
+  const uint64_t cells = (uint64_t)ctx->T * NB;
   if ((rc = ensure(ctx, ctx->g_nch, nt * 4)) || (rc = ensure(ctx, ctx->g_cbase, (nt + 1) * 4)) ||
-      (rc = ensure(ctx, ctx->g_colj, (uint64_t)ctx->T * 4)) ||
-      (rc = ensure(ctx, ctx->g_jbase, ((uint64_t)ctx->T + 1) * 4)) ||
+      (rc = ensure(ctx, ctx->g_colj, cells * 4)) ||
+      (rc = ensure(ctx, ctx->g_jbase, (cells + 1) * 4)) ||
       (rc = ensure(ctx, ctx->g_scan_tmp, scan_tmp_bytes(nt + 1) + 256)))
     return rc;
   auto* nch = static_cast<uint32_t*>(ctx->g_nch.p);
@@ -587,15 +616,17 @@ int build_jobs(cytom_ctx* ctx) {
   if ((rc = ensure(ctx, ctx->g_chunks, std::max<uint64_t>(n_chunks, 1) * 8))) return rc;
   k_emit_chunks<<<grid_for(nt, 256), 256, 0, st>>>(nch, cbase, nt, ctx->chunk_slots,
                                                    static_cast<uint64_t*>(ctx->g_chunks.p));
-  k_col_jobs<<<grid_for(ctx->T, 256), 256, 0, st>>>(cbase, ctx->T, ctx->job_chunks, colj);
-  scan_exclusive_u32(colj, jbase, ctx->T, ctx->g_scan_tmp.p, st);
-  CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc + 33, jbase + ctx->T, 4, hipMemcpyDeviceToHost, st));
+  k_col_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, ctx->T, NB, ctx->job_chunks, colj);
+  scan_exclusive_u32(colj, jbase, cells, ctx->g_scan_tmp.p, st);
+  CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc + 33, jbase + cells, 4, hipMemcpyDeviceToHost, st));
   CY_HIP(ctx, hipStreamSynchronize(st));
   const uint64_t n_jobs = (uint32_t)ctx->h_misc[33];
   if ((rc = ensure(ctx, ctx->g_jobs, std::max<uint64_t>(n_jobs, 1) * 16))) return rc;
-  k_emit_jobs<<<grid_for(ctx->T, 256), 256, 0, st>>>(cbase, colj, jbase, ctx->T, ctx->job_chunks,
-                                                     static_cast<uint4*>(ctx->g_jobs.p));
+  k_emit_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, colj, jbase, ctx->T, NB,
+                                                    ctx->job_chunks,
+                                                    static_cast<uint4*>(ctx->g_jobs.p));
   ctx->n_chunks = n_chunks;
+  ctx->n_cells = cells;
   ctx->n_jobs = n_jobs;
   ctx->jobs_dirty = false;
   return 0;

@@ -12,11 +12,18 @@
 namespace cytom {
 
 constexpr uint32_t INF = 0xFFFFFFFFu;
-// Packed edge slot: src in the low word, dst in the high word.  A deleted
-// slot holds TOMB (src = dst = 0xFFFFFFFF, never a valid id: V <= 2^31).
-constexpr uint64_t TOMB = ~0ull;
+// Edge slot: 4 bytes, the tile-local source in the low half and the
+// tile-local target in the high half (the tile supplies the high bits; the
+// reference's local_edge_t, datatypes.h:77-92).  Tiles are at most 16,384
+// vertices wide, so a deleted slot can hold TOMB (0xFFFF, 0xFFFF).
+constexpr uint32_t TOMB = 0xFFFFFFFFu;
 constexpr uint64_t MAX_VERTICES = 1ull << 31;
 
+__host__ __device__ inline uint32_t pack_local(uint32_t s, uint32_t d, uint32_t mask) {
+  return (s & mask) | ((d & mask) << 16);
+}
+
+// Global (src, dst) key of the deletion table.
 __host__ __device__ inline uint64_t pack_edge(uint32_t s, uint32_t d) {
   return (uint64_t)s | ((uint64_t)d << 32);
 }
@@ -53,7 +60,8 @@ struct StoreView {
   uint32_t* tile_cap;   // slots reserved for the tile
   uint32_t* tile_cnt;   // slots used (live + tombstones)
   uint32_t* tile_live;  // live edges
-  uint64_t* pool;       // packed edges
+  uint32_t* tile_sorted;  // leading slots in source order (sort-compacted prefix)
+  uint32_t* pool;       // local edge pairs
   uint32_t* wpool;      // per-slot weights (nullptr when unweighted)
   uint32_t* outdeg;
   uint32_t* indeg;
@@ -112,7 +120,8 @@ struct cytom_ctx {
   uint32_t* tile_cap = nullptr;
   uint32_t* tile_cnt = nullptr;
   uint32_t* tile_live = nullptr;
-  uint64_t* pool = nullptr;
+  uint32_t* tile_sorted = nullptr;
+  uint32_t* pool = nullptr;
   uint32_t* wpool = nullptr;
   uint64_t pool_size = 0;
   uint32_t* outdeg = nullptr;
@@ -125,13 +134,19 @@ struct cytom_ctx {
   cytom::GrowBuf g_keys, g_vals, g_keys2, g_vals2, g_flags, g_scan, g_seg_start, g_seg_tile,
       g_reloc_cap, g_reloc_list, g_ins_slot, g_scan_tmp, g_hist, g_in_src, g_in_dst, g_in_w,
       g_del_src, g_del_dst, g_del_w, g_tab_key, g_tab_head, g_node_next, g_claimed,
-      g_tile_flag, g_del_tiles, g_found, g_tile_scan;
+      g_tile_flag, g_del_tiles, g_found, g_tile_scan, g_rs_e, g_rs_w, g_rs_off;
   // execution work lists
   cytom::GrowBuf g_nch, g_cbase, g_chunks, g_colj, g_jbase, g_jobs;
   bool jobs_dirty = true;
-  uint64_t n_chunks = 0, n_jobs = 0;
+  uint64_t n_chunks = 0, n_cells = 0, n_jobs = 0;
   uint32_t chunk_slots = 512;   // slots per chunk
-  uint32_t job_chunks = 128;    // chunks per column job
+  uint32_t job_chunks = 256;    // chunks per column job (job_chunks * chunk_slots <= 2^17)
+  uint32_t source_bands = 1;    // source-row bands swept one after the other
+  // A tile is sort-compacted once its unsorted tail reaches 256 slots and
+  // resort_pct percent of its sorted prefix (CYTOM_RESORT_OFF = never).
+  uint32_t resort_pct = 25;
+  uint32_t resort_max_slots = 1u << 26;  // staged slots per pass (plus one tile)
+  uint64_t tiles_resorted = 0, slots_resorted = 0;
   uint32_t gather_threads = 512;
   uint32_t gather_blocks = 0;   // persistent grid (0 = from the LDS budget per program)
   // PageRank-delta sums are exact int64 fixed point with this many fraction
@@ -163,6 +178,7 @@ struct cytom_ctx {
     s.tile_cap = tile_cap;
     s.tile_cnt = tile_cnt;
     s.tile_live = tile_live;
+    s.tile_sorted = tile_sorted;
     s.pool = pool;
     s.wpool = wpool;
     s.outdeg = outdeg;

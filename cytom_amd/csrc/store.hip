@@ -7,9 +7,9 @@
 // TileStore::doGlobalCompaction (tile-store.h:144-172).
 //
 // Layout in HBM: a tile index (offset, capacity, used slots, live edges per
-// tile) and one pool of packed 8-byte edges (src low word, dst high word)
-// plus a parallel u32 weight pool.  Each tile owns a contiguous block of
-// the pool with slack at its end.
+// tile) and one pool of 4-byte edges (tile-local src in the low half,
+// tile-local dst in the high half) plus a parallel u32 weight pool.  Each
+// tile owns a contiguous block of the pool with slack at its end.
 //
 // Insertion: the batch is radix-sorted by tile key, runs of equal keys are
 // the per-tile segments; tiles whose slack is too small are relocated to a
@@ -38,6 +38,7 @@ enum Misc : int {
   M_RELOC_CUR = 4,
   M_NFOUND = 5,
   M_NCOMPACT = 6,
+  M_RS_CUT = 7,  // first selected tile (by rank) left for a later sort-compaction
   M_REEXEC = 8,  // one flag per program from here
 };
 
@@ -135,15 +136,18 @@ __global__ void k_reloc_commit(const uint32_t* __restrict__ nseg_p,
 }
 
 // Stable copy of a tile's live slots to dst_off, 1024 slots per step.
+// *live_sorted receives the live slots among the first `sorted` ones: the
+// copy keeps their order, so they stay the tile's sorted prefix.
 constexpr int CP_THREADS = 256;
 constexpr int CP_ITEMS = 4;
-__device__ inline uint32_t copy_live(const uint64_t* __restrict__ pool_in,
+__device__ inline uint32_t copy_live(const uint32_t* __restrict__ pool_in,
                                      const uint32_t* __restrict__ w_in, uint64_t src_off,
-                                     uint32_t cnt, uint64_t* __restrict__ pool_out,
-                                     uint32_t* __restrict__ w_out, uint64_t dst_off) {
-  uint32_t written = 0;
+                                     uint32_t cnt, uint32_t* __restrict__ pool_out,
+                                     uint32_t* __restrict__ w_out, uint64_t dst_off,
+                                     uint32_t sorted, uint32_t* live_sorted) {
+  uint32_t written = 0, nls = 0;
   for (uint32_t base = 0; base < cnt; base += CP_THREADS * CP_ITEMS) {
-    uint64_t e[CP_ITEMS];
+    uint32_t e[CP_ITEMS];
     uint32_t ww[CP_ITEMS];
     uint32_t nlive = 0;
 #pragma unroll
@@ -152,6 +156,7 @@ __device__ inline uint32_t copy_live(const uint64_t* __restrict__ pool_in,
       e[k] = i < cnt ? pool_in[src_off + i] : TOMB;
       ww[k] = (w_in && i < cnt) ? w_in[src_off + i] : 0u;
       nlive += e[k] != TOMB;
+      nls += (e[k] != TOMB && i < sorted) ? 1u : 0u;
     }
     uint32_t tot;
     uint32_t p = block_scan_excl<CP_THREADS, uint32_t>(nlive, &tot);
@@ -166,6 +171,7 @@ __device__ inline uint32_t copy_live(const uint64_t* __restrict__ pool_in,
     written += tot;
     __syncthreads();
   }
+  *live_sorted = block_sum<CP_THREADS, uint32_t>(nls);
   return written;
 }
 
@@ -175,13 +181,15 @@ __global__ void __launch_bounds__(CP_THREADS) k_relocate(const unsigned long lon
     const uint32_t t = (uint32_t)rl[3 * b];
     const uint64_t noff = rl[3 * b + 1];
     const uint32_t ncap = (uint32_t)rl[3 * b + 2];
+    uint32_t ls;
     const uint32_t live = copy_live(S.pool, S.wpool, S.tile_off[t], S.tile_cnt[t], S.pool, S.wpool,
-                                    noff);
+                                    noff, S.tile_sorted[t], &ls);
     __syncthreads();
     if (threadIdx.x == 0) {
       S.tile_off[t] = noff;
       S.tile_cap[t] = ncap;
       S.tile_cnt[t] = live;
+      S.tile_sorted[t] = ls;
     }
   }
 }
@@ -196,17 +204,20 @@ __global__ void k_repack_caps(StoreView S, uint64_t* __restrict__ ncap) {
 }
 
 __global__ void __launch_bounds__(CP_THREADS) k_repack(StoreView S, const uint64_t* __restrict__ noff,
-                                                       uint64_t* __restrict__ npool,
+                                                       uint32_t* __restrict__ npool,
                                                        uint32_t* __restrict__ nwpool) {
   for (uint64_t t = blockIdx.x; t < S.n_tiles; t += gridDim.x) {
     const uint32_t cnt = S.tile_cnt[t];
-    uint32_t live = 0;
-    if (cnt) live = copy_live(S.pool, S.wpool, S.tile_off[t], cnt, npool, nwpool, noff[t]);
+    uint32_t live = 0, ls = 0;
+    if (cnt)
+      live = copy_live(S.pool, S.wpool, S.tile_off[t], cnt, npool, nwpool, noff[t],
+                       S.tile_sorted[t], &ls);
     __syncthreads();
     if (threadIdx.x == 0) {
       S.tile_off[t] = noff[t];
       S.tile_cap[t] = (uint32_t)(noff[t + 1] - noff[t]);
       S.tile_cnt[t] = live;
+      S.tile_sorted[t] = ls;
     }
   }
 }
@@ -226,7 +237,7 @@ __global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __r
     const uint32_t v = vals[i];
     const uint32_t s = src[v], d = dst[v];
     const uint64_t slot = S.tile_off[t] + S.tile_cnt[t] + pos;
-    S.pool[slot] = pack_edge(s, d);
+    S.pool[slot] = pack_local(s, d, S.vpt - 1);
     if (S.wpool) S.wpool[slot] = w ? w[v] : 0u;
     ins_slot[v] = slot;
     atomicAdd(&S.outdeg[s], 1u);
@@ -322,23 +333,25 @@ __global__ void __launch_bounds__(256) k_del_match(const uint64_t* __restrict__ 
        c += nwaves) {
     const uint64_t ch = chunks[c];
     const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
+    const uint32_t xb = (t % S.T) * S.vpt, yb = (t / S.T) * S.vpt;
     const uint64_t off = S.tile_off[t] + s0;
     const uint32_t n = min(DEL_CH, S.tile_cnt[t] - s0);
     for (uint32_t base = 0; base < n; base += 64 * DM_UNROLL) {
-      uint64_t e[DM_UNROLL];
+      uint32_t le[DM_UNROLL];
 #pragma unroll
-      for (int u = 0; u < DM_UNROLL; ++u) e[u] = S.pool[off + min(base + u * 64 + lane, n - 1)];
-      uint64_t h[DM_UNROLL];
+      for (int u = 0; u < DM_UNROLL; ++u) le[u] = S.pool[off + min(base + u * 64 + lane, n - 1)];
+      uint64_t e[DM_UNROLL], h[DM_UNROLL];
       unsigned long long k[DM_UNROLL];
 #pragma unroll
       for (int u = 0; u < DM_UNROLL; ++u) {
+        e[u] = pack_edge(xb + (le[u] & 0xFFFFu), yb + (le[u] >> 16));
         h[u] = tab_hash(e[u], log_h);
         k[u] = tab_key[h[u]];
       }
 #pragma unroll
       for (int u = 0; u < DM_UNROLL; ++u) {
         const uint32_t i = base + u * 64 + lane;
-        if (i >= n || e[u] == TOMB || k[u] == EMPTY64) continue;
+        if (i >= n || le[u] == TOMB || k[u] == EMPTY64) continue;
         uint64_t hh = h[u];
         unsigned long long kk = k[u];
         while (kk != e[u] && kk != EMPTY64) {
@@ -391,12 +404,98 @@ __global__ void __launch_bounds__(CP_THREADS) k_compact(const uint32_t* __restri
     const uint32_t dead = cnt - live;
     if (dead < 64 || (uint64_t)dead * 4 < cnt) continue;  // uniform per block
     const uint64_t off = S.tile_off[t];
-    const uint32_t kept = copy_live(S.pool, S.wpool, off, cnt, S.pool, S.wpool, off);
+    uint32_t ls;
+    const uint32_t kept =
+        copy_live(S.pool, S.wpool, off, cnt, S.pool, S.wpool, off, S.tile_sorted[t], &ls);
     __syncthreads();
     if (threadIdx.x == 0) {
       S.tile_cnt[t] = kept;
+      S.tile_sorted[t] = ls;
       atomicAdd(&misc[M_NCOMPACT], 1ull);
     }
+  }
+}
+
+// ---- sort-compaction: rewrite a tile's live edges in source order
+// (the layout of a freshly bulk-loaded tile), so the gather's source-value
+// loads of one wavefront fall on few cache lines.  Appends leave an unsorted
+// tail behind the sorted prefix; a tile is rewritten once the tail is long
+// enough, which bounds the rewrite work per inserted edge by a geometric sum.
+constexpr uint32_t RESORT_MIN_TAIL = 256;
+
+__global__ void k_resort_flag(StoreView S, uint32_t pct, uint32_t* __restrict__ flag,
+                              uint64_t* __restrict__ size) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < S.n_tiles;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t cnt = S.tile_cnt[t], s = S.tile_sorted[t], tail = cnt - s;
+    const bool f = tail >= RESORT_MIN_TAIL && (uint64_t)tail * 100 >= (uint64_t)s * pct;
+    flag[t] = f ? 1u : 0u;
+    size[t] = f ? cnt : 0u;
+  }
+}
+
+// Selected tiles past the rank or slot budget of this pass: the smallest
+// such rank is the cut (ranks and offsets grow together).
+__global__ void k_resort_cut(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rank,
+                             const uint64_t* __restrict__ off, uint64_t n_tiles, uint32_t rmax,
+                             uint64_t cap, unsigned long long* misc) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles;
+       t += (uint64_t)gridDim.x * blockDim.x)
+    if (flag[t] && (rank[t] >= rmax || off[t] >= cap))
+      atomicMin(&misc[M_RS_CUT], (unsigned long long)rank[t]);
+}
+
+// One block per selected tile: keys (rank, tombstone bit, source) and the
+// slot's position in the staging copy.
+__global__ void __launch_bounds__(256) k_resort_gather(StoreView S, const uint32_t* __restrict__ list,
+                                                       uint32_t nproc, const uint64_t* __restrict__ off,
+                                                       uint32_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals,
+                                                       uint32_t* __restrict__ e_st,
+                                                       uint32_t* __restrict__ w_st) {
+  const uint32_t lv = S.log2vpt;
+  for (uint32_t r = blockIdx.x; r < nproc; r += gridDim.x) {
+    const uint32_t t = list[r];
+    const uint32_t cnt = S.tile_cnt[t];
+    const uint64_t src = S.tile_off[t];
+    const uint32_t base = (uint32_t)off[t];
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
+      const uint32_t e = S.pool[src + i];
+      const uint32_t low = e == TOMB ? (1u << lv) : (e & 0xFFFFu);
+      keys[base + i] = (r << (lv + 1)) | low;
+      vals[base + i] = base + i;
+      e_st[base + i] = e;
+      if (S.wpool) w_st[base + i] = S.wpool[src + i];
+    }
+  }
+}
+
+// Sorted position k of a tile lands at slot (k - first position of the
+// tile); tombstones sort behind the live edges and are dropped.
+__global__ void k_resort_scatter(StoreView S, const uint32_t* __restrict__ list,
+                                 const uint64_t* __restrict__ off,
+                                 const uint32_t* __restrict__ skeys,
+                                 const uint32_t* __restrict__ svals, uint32_t n,
+                                 const uint32_t* __restrict__ e_st,
+                                 const uint32_t* __restrict__ w_st) {
+  const uint32_t lv = S.log2vpt;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint32_t key = skeys[k];
+    if (key & (1u << lv)) continue;
+    const uint32_t t = list[key >> (lv + 1)];
+    const uint64_t slot = S.tile_off[t] + (k - (uint32_t)off[t]);
+    const uint32_t j = svals[k];
+    S.pool[slot] = e_st[j];
+    if (S.wpool) S.wpool[slot] = w_st[j];
+  }
+}
+
+__global__ void k_resort_commit(StoreView S, const uint32_t* __restrict__ list, uint32_t nproc) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nproc; r += gridDim.x * blockDim.x) {
+    const uint32_t t = list[r];
+    const uint32_t live = S.tile_live[t];
+    S.tile_cnt[t] = live;
+    S.tile_sorted[t] = live;
   }
 }
 
@@ -424,9 +523,9 @@ int repack(cytom_ctx* ctx, uint64_t min_slots) {
   CY_HIP(ctx, hipStreamSynchronize(st));
   total = ctx->h_misc[0];
   uint64_t new_size = std::max<uint64_t>(min_slots, total + total / 2 + (1u << 20));
-  uint64_t* npool = nullptr;
+  uint32_t* npool = nullptr;
   uint32_t* nw = nullptr;
-  if (hipMalloc(&npool, new_size * 8) != hipSuccess)
+  if (hipMalloc(&npool, new_size * 4) != hipSuccess)
     return set_err(ctx, CYTOM_ENOMEM, "edge pool allocation failed");
   if (ctx->weighted && hipMalloc(&nw, new_size * 4) != hipSuccess) {
     (void)hipFree(npool);
@@ -441,6 +540,91 @@ int repack(cytom_ctx* ctx, uint64_t min_slots) {
   ctx->pool = npool;
   ctx->wpool = nw;
   ctx->pool_size = new_size;
+  return 0;
+}
+
+// Sort-compaction of the tiles k_resort_flag selects, at most
+// resort_max_slots staged slots (plus one tile) per pass; tiles past the
+// budget wait for the next batch.
+
+int resort_tiles(cytom_ctx* ctx, uint32_t* n_sorted) {
+  *n_sorted = 0;
+  if (ctx->resort_pct == CYTOM_RESORT_OFF) return 0;
+  hipStream_t st = ctx->stream;
+  const uint64_t nt = ctx->n_tiles;
+  const uint32_t lv = ctx->log2vpt;
+  int rc;
+  if ((rc = ensure(ctx, ctx->g_tile_flag, nt * 4)) ||
+      (rc = ensure(ctx, ctx->g_tile_scan, (nt + 1) * 16 + 256)) ||
+      (rc = ensure(ctx, ctx->g_rs_off, (2 * nt + 1) * 8)) ||
+      (rc = ensure(ctx, ctx->g_del_tiles, nt * 4)) ||
+      (rc = ensure(ctx, ctx->g_scan_tmp, scan_tmp_bytes(nt + 1) + 256)))
+    return rc;
+  auto* flag = static_cast<uint32_t*>(ctx->g_tile_flag.p);
+  auto* rank = static_cast<uint32_t*>(ctx->g_tile_scan.p);
+  auto* size = static_cast<uint64_t*>(ctx->g_rs_off.p);
+  uint64_t* off = size + nt;
+  auto* list = static_cast<uint32_t*>(ctx->g_del_tiles.p);
+  k_resort_flag<<<grid_for(nt, 256), 256, 0, st>>>(ctx->view(), ctx->resort_pct, flag, size);
+  scan_exclusive_u32(flag, rank, nt, ctx->g_scan_tmp.p, st);
+  scan_exclusive_u64(size, off, nt, ctx->g_scan_tmp.p, st);
+  k_flag_emit<<<grid_for(nt, 256), 256, 0, st>>>(flag, rank, nt, list);
+  uint32_t n_sel = 0;
+  uint64_t total = 0;
+  CY_HIP(ctx, hipMemcpyAsync(&n_sel, rank + nt, 4, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipMemcpyAsync(&total, off + nt, 8, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipStreamSynchronize(st));
+  if (n_sel == 0) return 0;
+  // Keys are (rank, tombstone bit, lv source bits) in 32 bits.
+  const uint32_t rmax = 1u << (31 - lv);
+  uint32_t nproc = n_sel;
+  uint64_t nslots = total;
+  if (n_sel > rmax || total > ctx->resort_max_slots) {
+    CY_HIP(ctx, hipMemsetAsync(ctx->d_misc + M_RS_CUT, 0xFF, 8, st));
+    k_resort_cut<<<grid_for(nt, 256), 256, 0, st>>>(flag, rank, off, nt, rmax,
+                                                    ctx->resort_max_slots, ctx->d_misc);
+    unsigned long long cut = 0;
+    CY_HIP(ctx, hipMemcpyAsync(&cut, ctx->d_misc + M_RS_CUT, 8, hipMemcpyDeviceToHost, st));
+    CY_HIP(ctx, hipStreamSynchronize(st));
+    nproc = (uint32_t)std::min<unsigned long long>(cut, n_sel);
+    if (nproc < n_sel) {
+      uint32_t t_cut = 0;
+      CY_HIP(ctx, hipMemcpy(&t_cut, list + nproc, 4, hipMemcpyDeviceToHost));
+      CY_HIP(ctx, hipMemcpy(&nslots, off + t_cut, 8, hipMemcpyDeviceToHost));
+    }
+  }
+  if (nslots > 0xFFFFFFFFull) return set_err(ctx, CYTOM_ERANGE, "sort-compaction pass too large");
+  if ((rc = ensure(ctx, ctx->g_keys, nslots * 4)) || (rc = ensure(ctx, ctx->g_vals, nslots * 4)) ||
+      (rc = ensure(ctx, ctx->g_keys2, nslots * 4)) ||
+      (rc = ensure(ctx, ctx->g_vals2, nslots * 4)) ||
+      (rc = ensure(ctx, ctx->g_flags, nslots * 4)) || (rc = ensure(ctx, ctx->g_scan, nslots * 4)) ||
+      (rc = ensure(ctx, ctx->g_rs_e, nslots * 4)) ||
+      (rc = ensure(ctx, ctx->g_rs_w, ctx->wpool ? nslots * 4 : 4)) ||
+      (rc = ensure(ctx, ctx->g_hist, radix_hist_bytes(nslots))) ||
+      (rc = ensure(ctx, ctx->g_scan_tmp,
+                   scan_tmp_bytes(std::max<uint64_t>({nslots, 256 * ((nslots + 4095) / 4096) + 1,
+                                                      nt + 1})) +
+                       256)))
+    return rc;
+  auto* keys = static_cast<uint32_t*>(ctx->g_keys.p);
+  auto* vals = static_cast<uint32_t*>(ctx->g_vals.p);
+  auto* skeys = static_cast<uint32_t*>(ctx->g_keys2.p);
+  auto* svals = static_cast<uint32_t*>(ctx->g_vals2.p);
+  auto* e_st = static_cast<uint32_t*>(ctx->g_rs_e.p);
+  auto* w_st = static_cast<uint32_t*>(ctx->g_rs_w.p);
+  k_resort_gather<<<std::min<uint32_t>(nproc, 16384), 256, 0, st>>>(ctx->view(), list, nproc, off,
+                                                                    keys, vals, e_st, w_st);
+  uint32_t rank_bits = 0;
+  while ((1ull << rank_bits) < nproc) ++rank_bits;
+  radix_sort_pairs(keys, vals, skeys, svals, static_cast<uint32_t*>(ctx->g_flags.p),
+                   static_cast<uint32_t*>(ctx->g_scan.p), nslots, rank_bits + lv + 1,
+                   ctx->g_hist.p, ctx->g_scan_tmp.p, st);
+  k_resort_scatter<<<grid_for(nslots, 256), 256, 0, st>>>(ctx->view(), list, off, skeys, svals,
+                                                          (uint32_t)nslots, e_st, w_st);
+  k_resort_commit<<<grid_for(nproc, 256), 256, 0, st>>>(ctx->view(), list, nproc);
+  ctx->tiles_resorted += nproc;
+  ctx->slots_resorted += nslots;
+  *n_sorted = nproc;
   return 0;
 }
 
@@ -629,10 +813,13 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
       launch_hooks(ctx, P, d_src, d_dst, d_w, ins_slot, n_ins, found, n_found);
     }
   }
-  // 5. compaction of tiles with many tombstones
+  // 5. compaction of tiles with many tombstones, then sort-compaction of
+  // tiles with long unsorted tails
   if (n_del) {
     k_compact<<<4096, CP_THREADS, 0, st>>>(del_tiles, n_del_tiles_p, ctx->view(), misc);
   }
+  uint32_t n_sorted = 0;
+  if ((rc = resort_tiles(ctx, &n_sorted))) return rc;
   CY_HIP(ctx, hipEventRecord(ctx->ev[1], st));
   CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st));
   CY_HIP(ctx, hipStreamSynchronize(st));
@@ -649,6 +836,8 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
     bst->live_edges = ctx->live_edges;
     bst->tiles_relocated = nreloc;
     bst->tiles_compacted = (uint32_t)ctx->h_misc[M_NCOMPACT];
+    bst->tiles_sorted = n_sorted;
+    bst->pad = 0;
     bst->t_ingest_ms = ms;
   }
   return 0;
@@ -664,20 +853,20 @@ int64_t export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_t* w, 
   CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, ctx->d_misc, 8, hipMemcpyDeviceToHost, st));
   CY_HIP(ctx, hipStreamSynchronize(st));
   const uint64_t top = ctx->h_misc[0];
-  std::vector<uint64_t> pool(top);
+  std::vector<uint32_t> pool(top);
   std::vector<uint32_t> wp(ctx->wpool ? top : 0);
   if (top) {
-    CY_HIP(ctx, hipMemcpy(pool.data(), ctx->pool, top * 8, hipMemcpyDeviceToHost));
+    CY_HIP(ctx, hipMemcpy(pool.data(), ctx->pool, top * 4, hipMemcpyDeviceToHost));
     if (ctx->wpool) CY_HIP(ctx, hipMemcpy(wp.data(), ctx->wpool, top * 4, hipMemcpyDeviceToHost));
   }
   uint64_t n = 0;
   for (uint64_t t = 0; t < ctx->n_tiles; ++t) {
     for (uint32_t i = 0; i < cnt[t]; ++i) {
-      const uint64_t e = pool[off[t] + i];
+      const uint32_t e = pool[off[t] + i];
       if (e == TOMB) continue;
       if (n < cap) {
-        src[n] = (uint32_t)e;
-        dst[n] = (uint32_t)(e >> 32);
+        src[n] = (uint32_t)((t % ctx->T) * ctx->vpt) + (e & 0xFFFFu);
+        dst[n] = (uint32_t)((t / ctx->T) * ctx->vpt) + (e >> 16);
         if (w) w[n] = ctx->wpool ? wp[off[t] + i] : 0u;
       }
       ++n;

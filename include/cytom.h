@@ -80,6 +80,8 @@ typedef struct {
   uint64_t live_edges;
   uint32_t tiles_relocated;
   uint32_t tiles_compacted;
+  uint32_t tiles_sorted;     /* tiles rewritten in source order (sort-compaction) */
+  uint32_t pad;
   double t_ingest_ms;        /* device time of the batch (HIP events) */
 } cytom_batch_stats;
 
@@ -147,11 +149,15 @@ int64_t cytom_export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_
  * tile_batching and executor count, tile-distributor.h:113-177,
  * run.py:115-132).  Zero fields keep the current value. */
 typedef struct {
-  uint32_t chunk_slots;   /* edge slots per work chunk (multiple of 64, <= 512) */
-  uint32_t job_chunks;    /* chunks per column job (<= 512) */
+  uint32_t chunk_slots;   /* edge slots per work chunk (multiple of 64, <= 1024) */
+  uint32_t job_chunks;    /* chunks per column job (<= 512; job_chunks * chunk_slots <= 131072) */
   uint32_t blocks_per_cu; /* persistent gather blocks per compute unit */
-  uint32_t pad;
+  uint32_t source_bands;  /* source-row bands swept in turn (power of two) */
+  uint32_t resort_pct;    /* sort-compact a tile once its unsorted tail reaches this
+                             percentage of its sorted prefix; CYTOM_RESORT_OFF disables */
+  uint32_t resort_max_slots; /* slots staged per sort-compaction pass (more tiles wait) */
 } cytom_tuning;
+#define CYTOM_RESORT_OFF 0xFFFFFFFFu
 int cytom_set_tuning(cytom_ctx* ctx, const cytom_tuning* t);
 int cytom_get_tuning(const cytom_ctx* ctx, cytom_tuning* t);
 

@@ -43,7 +43,8 @@ def test_python_bindings_cover_header():
 def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(C.Config) == 8 + 4 + 4 + 8 + 8 + 4 + 4
     assert ctypes.sizeof(C.RunStats) == 4 * 4 + 8 * 3 + 8
-    assert ctypes.sizeof(C.BatchStats) == 8 * 4 + 4 * 2 + 8
+    assert ctypes.sizeof(C.BatchStats) == 8 * 4 + 4 * 4 + 8
+    assert ctypes.sizeof(C.Tuning) == 4 * 6
 
 
 def test_invalid_arguments_fail_without_a_device():

@@ -56,13 +56,15 @@ def test_rmat_gather_by_index_matches_oracle():
 
 # ------------------------------------------------------------------ streams
 def run_stream_parity(scale, vpt, batches, n_ins, n_del, seed=2, edge_capacity=0,
-                      max_iterations=100):
+                      max_iterations=100, tuning=None):
     V = 1 << scale
     st = O.Stream(seed)
     og = O.Graph(V, vertices_per_tile=min(vpt, 65536), weighted=True,
                  max_iterations=max_iterations)
     gg = C.Graph(V, vertices_per_tile=vpt, weighted=True, edge_capacity=edge_capacity,
                  max_iterations=max_iterations)
+    if tuning:
+        gg.set_tuning(**tuning)
     progs = [("bfs", O.BFS, C.BFS, 0), ("cc", O.CC, C.CC, 0), ("sssp", O.SSSP, C.SSSP, 0),
              ("prd", O.PRD, C.PRD, 0)]
     oid = {n: og.register(k, s) for n, k, _, s in progs}
@@ -106,6 +108,56 @@ def test_stream_parity_wide_tiles():
 def test_stream_parity_with_pool_growth():
     # A tiny initial pool forces repacks (global compaction) mid-stream.
     run_stream_parity(scale=11, vpt=256, batches=5, n_ins=4000, n_del=1000, edge_capacity=1024)
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(source_bands=1), dict(source_bands=4), dict(source_bands=1024),
+    dict(chunk_slots=64, job_chunks=1, source_bands=2),
+    dict(chunk_slots=1024, job_chunks=128, blocks_per_cu=1),
+    dict(resort_pct=C.RESORT_OFF), dict(resort_pct=1), dict(resort_pct=1, resort_max_slots=700)])
+def test_stream_parity_launch_shapes(tuning):
+    # Work-list shapes (chunk size, chunks per job, source bands, grid) and
+    # the sort-compaction policy (off, after every batch, a small staging
+    # budget that defers tiles to later batches) change only the slot order
+    # and the order of the exact reductions, never the values.  Tiles of
+    # 2048 vertices collect ~500 inserts per batch, past the 256-slot tail
+    # that makes a tile eligible for sort-compaction.
+    run_stream_parity(scale=13, vpt=2048, batches=5, n_ins=8000, n_del=1600, tuning=tuning)
+
+
+def test_launch_shapes_agree_at_rmat20():
+    # Size-independent property at a large graph: every launch shape gives
+    # bit-identical values for all four programs, batch after batch.  The
+    # PageRank-delta sums are exact fixed point, so its whole trajectory
+    # (iterations, traversed edges) is identical too; the BFS/CC/SSSP insert
+    # hooks relax concurrently, so only their fixpoint values are compared.
+    scale, V = 20, 1 << 20
+    st = O.Stream(2)
+    shapes = [dict(source_bands=1), dict(source_bands=8, resort_pct=C.RESORT_OFF),
+              dict(source_bands=32, chunk_slots=256, job_chunks=16, resort_pct=5)]
+    graphs = []
+    for t in shapes:
+        g = C.Graph(V, weighted=True)
+        g.set_tuning(**t)
+        graphs.append((g, [g.register(C.BFS, 0), g.register(C.CC), g.register(C.SSSP, 0),
+                           g.register(C.PRD)]))
+    for b in range(4):
+        first, didx = st.next(1 << 20, (1 << 17) if b else 0)
+        s, d, w = C.rmat_generate(scale, 2, first, 1 << 20, weights=True)
+        di = C.DeviceArray.from_host(didx)
+        ds, dd, dw = C.rmat_gather(scale, 2, di, len(didx), weights=True)
+        vals = []
+        for g, ids in graphs:
+            g.update_device(s, d, w, 1 << 20, ds, dd, dw, len(didx))
+            row = []
+            for p in ids:
+                rs = g.execute(p)
+                row.append((rs.iterations, rs.traversed_edges, g.read(p)))
+            vals.append(row)
+        for row in vals[1:]:
+            for k in range(4):
+                assert np.array_equal(row[k][2], vals[0][k][2]), (b, k)
+            assert row[3][0] == vals[0][3][0] and row[3][1] == vals[0][3][1], b
 
 
 def test_insert_only_cc_rmat16():
