@@ -378,6 +378,12 @@ __device__ inline void apply_tail(unsigned long long te, unsigned long long cnt,
   }
 }
 
+// Apply kernels: each wavefront takes APPLY_WORDS bitmap words (64
+// consecutive vertices each, one per lane) per step and issues all of their
+// loads before using any, so a wave keeps several memory round trips in
+// flight; the next active words are built with ballots.
+constexpr int APPLY_WORDS = 4;
+
 // bfs.cc:101-113, cc.cc:101-111, sssp.cc:101-113: copy next into current
 // and activate on change.
 template <int KIND>
@@ -388,37 +394,61 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_min(
     uint8_t* __restrict__ rowact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
-  const uint64_t nw_total = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t stride = (((uint64_t)gridDim.x * blockDim.x) >> 6) * APPLY_WORDS;
   unsigned long long te = 0, cnt = 0, upd = 0;
-  for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwords;
-       w += nw_total) {
-    const uint64_t v = (w << 6) | lane;
-    const bool was = (act_cur[w] >> lane) & 1ull;
-    bool a = false;
-    if (changed[v]) {
-      changed[v] = 0;
-      ++upd;
-      const uint32_t n = nxt[v];
-      if (n != cur[v]) {
-        cur[v] = n;
-        fval[v] = n;
-        a = true;
-        te += outdeg[v];
-        ++cnt;
+  for (uint64_t w0 = wave * APPLY_WORDS; w0 < nwords; w0 += stride) {
+    uint64_t aw[APPLY_WORDS];
+    uint8_t ch[APPLY_WORDS];
+#pragma unroll
+    for (int k = 0; k < APPLY_WORDS; ++k) {
+      const uint64_t w = min(w0 + k, nwords - 1);
+      aw[k] = act_cur[w];
+      ch[k] = changed[(w << 6) | lane];
+    }
+    uint32_t n[APPLY_WORDS], c[APPLY_WORDS], od[APPLY_WORDS];
+#pragma unroll
+    for (int k = 0; k < APPLY_WORDS; ++k) {
+      const uint64_t v = ((w0 + k) << 6) | lane;
+      n[k] = c[k] = od[k] = 0;
+      if (ch[k] && w0 + k < nwords) {
+        n[k] = nxt[v];
+        c[k] = cur[v];
+        od[k] = outdeg[v];
       }
     }
-    if (was && !a) fval[v] = INF;
-    const uint64_t word = __ballot(a);
-    if (lane == 0) {
-      act_next[w] = word;
-      if (word) rowact_next[v >> log2vpt] = 1;
+#pragma unroll
+    for (int k = 0; k < APPLY_WORDS; ++k) {
+      const uint64_t w = w0 + k;
+      if (w >= nwords) break;  // uniform
+      const uint64_t v = (w << 6) | lane;
+      const bool was = (aw[k] >> lane) & 1ull;
+      bool a = false;
+      if (ch[k]) {
+        changed[v] = 0;
+        ++upd;
+        if (n[k] != c[k]) {
+          cur[v] = n[k];
+          fval[v] = n[k];
+          a = true;
+          te += od[k];
+          ++cnt;
+        }
+      }
+      if (was && !a) fval[v] = INF;
+      const uint64_t word = __ballot(a);
+      if (lane == 0) {
+        act_next[w] = word;
+        if (word) rowact_next[v >> log2vpt] = 1;
+      }
     }
   }
   apply_tail(te, cnt, upd, nit);
 }
 
 // pagerank-delta.cc:142-174 (FP32 state, double intermediates as in the
-// reference: ALPHA and the threshold are doubles).
+// reference: ALPHA and the threshold are doubles).  Loads are unconditional:
+// most vertices carry a sum while PageRank-delta runs.
 __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
     uint64_t V, uint32_t log2vpt, int first, double thr, double qinv, float* __restrict__ rank,
     float* __restrict__ delta, long long* __restrict__ dsum, float* __restrict__ contrib,
@@ -427,52 +457,70 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
     const uint32_t* __restrict__ outdeg, IterStats* nit) {
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
-  const uint64_t nw_total = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t stride = (((uint64_t)gridDim.x * blockDim.x) >> 6) * APPLY_WORDS;
   unsigned long long te = 0, cnt = 0, upd = 0;
-  for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwords;
-       w += nw_total) {
-    const uint64_t v = (w << 6) | lane;
-    const bool was = (act_cur[w] >> lane) & 1ull;
-    bool a = false;
-    const bool ch = changed[v] != 0;
-    if (first || ch) {
-      if (ch) {
-        changed[v] = 0;
-        ++upd;
-      }
-      const float s = (float)((double)dsum[v] * qinv);
-      float d = 0.f;
-      float r = rank[v];
-      bool applied = true;
-      if (first) {
-        d = static_cast<float>((1. - 0.85) + (0.85 * (double)s));
-        r = r + d;
-        d = (float)((double)d - 1.0);
-      } else if (s == 0.f) {
-        delta[v] = 0.f;
-        applied = false;
-      } else {
-        d = static_cast<float>(0.85 * (double)s);
-        r = r + d;
-      }
-      if (applied) {
-        rank[v] = r;
-        delta[v] = d;
-        dsum[v] = 0;
-        if ((double)fabsf(d) > thr * (double)r) {
-          a = true;
-          const uint32_t od = outdeg[v];
-          te += od;
-          ++cnt;
-          contrib[v] = od ? d / (float)od : 0.f;
+  for (uint64_t w0 = wave * APPLY_WORDS; w0 < nwords; w0 += stride) {
+    uint64_t aw[APPLY_WORDS];
+    uint8_t ch[APPLY_WORDS];
+    long long ds[APPLY_WORDS];
+    float rk[APPLY_WORDS];
+    uint32_t od[APPLY_WORDS];
+#pragma unroll
+    for (int k = 0; k < APPLY_WORDS; ++k) {
+      const uint64_t w = min(w0 + k, nwords - 1);
+      const uint64_t v = (w << 6) | lane;
+      aw[k] = act_cur[w];
+      ch[k] = changed[v];
+      ds[k] = dsum[v];
+      rk[k] = rank[v];
+      od[k] = outdeg[v];
+    }
+#pragma unroll
+    for (int k = 0; k < APPLY_WORDS; ++k) {
+      const uint64_t w = w0 + k;
+      if (w >= nwords) break;  // uniform
+      const uint64_t v = (w << 6) | lane;
+      const bool was = (aw[k] >> lane) & 1ull;
+      bool a = false;
+      if (first || ch[k]) {
+        if (ch[k]) {
+          changed[v] = 0;
+          ++upd;
+        }
+        const float s = (float)((double)ds[k] * qinv);
+        float d = 0.f;
+        float r = rk[k];
+        bool applied = true;
+        if (first) {
+          d = static_cast<float>((1. - 0.85) + (0.85 * (double)s));
+          r = r + d;
+          d = (float)((double)d - 1.0);
+        } else if (s == 0.f) {
+          delta[v] = 0.f;
+          applied = false;
+        } else {
+          d = static_cast<float>(0.85 * (double)s);
+          r = r + d;
+        }
+        if (applied) {
+          rank[v] = r;
+          delta[v] = d;
+          dsum[v] = 0;
+          if ((double)fabsf(d) > thr * (double)r) {
+            a = true;
+            te += od[k];
+            ++cnt;
+            contrib[v] = od[k] ? d / (float)od[k] : 0.f;
+          }
         }
       }
-    }
-    if (was && !a) contrib[v] = 0.f;
-    const uint64_t word = __ballot(a);
-    if (lane == 0) {
-      act_next[w] = word;
-      if (word) rowact_next[v >> log2vpt] = 1;
+      if (was && !a) contrib[v] = 0.f;
+      const uint64_t word = __ballot(a);
+      if (lane == 0) {
+        act_next[w] = word;
+        if (word) rowact_next[v >> log2vpt] = 1;
+      }
     }
   }
   apply_tail(te, cnt, upd, nit);
