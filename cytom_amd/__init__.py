@@ -14,8 +14,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcytom_hip.so")
 
-BFS, CC, SSSP, PRD = 0, 1, 2, 3
-PROGRAMS = {"bfs": BFS, "cc": CC, "sssp": SSSP, "pagerank-delta": PRD}
+BFS, CC, SSSP, PRD, PPR = 0, 1, 2, 3, 4
+PROGRAMS = {"bfs": BFS, "cc": CC, "sssp": SSSP, "pagerank-delta": PRD, "ppr": PPR}
 RESORT_OFF = 0xFFFFFFFF  # cytom_tuning.resort_pct value that disables sort-compaction
 
 EINVAL, ENOMEM, ERANGE, EDEVICE = -22, -12, -34, -5
@@ -53,6 +53,14 @@ class RunStats(ctypes.Structure):
         ("active_vertices", ctypes.c_uint64),
         ("slots_scanned", ctypes.c_uint64),
         ("t_compute_ms", ctypes.c_double),
+    ]
+
+
+class Query(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_uint32),
+        ("n_sources", ctypes.c_uint32),
+        ("sources", ctypes.c_void_p),
     ]
 
 
@@ -94,6 +102,9 @@ SIGNATURES = {
     "cytom_program_register": (ctypes.c_int, [_VP, ctypes.POINTER(ProgramDesc),
                                               ctypes.POINTER(ctypes.c_int)]),
     "cytom_program_init": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cytom_program_set_sources": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _U32]),
+    "cytom_batched_execute": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
+    "cytom_read_query": (ctypes.c_int, [_VP, _U32, _VP, ctypes.c_size_t]),
     "cytom_update_batch": (ctypes.c_int, [_VP, _VP, _VP, _VP, _U64, _VP, _VP, _VP, _U64,
                                           ctypes.POINTER(BatchStats)]),
     "cytom_update_batch_device": (ctypes.c_int, [_VP, _VP, _VP, _VP, _U64, _VP, _VP, _VP, _U64,
@@ -314,6 +325,12 @@ class Graph:
     def init(self, pid):
         self._check(lib().cytom_program_init(self._h, pid), "program init")
 
+    def set_sources(self, pid, sources):
+        """New BFS/SSSP source (one vertex) or PPR teleport set; re-inits."""
+        s = np.ascontiguousarray(np.atleast_1d(sources), dtype=np.uint32)
+        self._check(lib().cytom_program_set_sources(self._h, pid, s.ctypes.data, len(s)),
+                    "set_sources")
+
     def update(self, src, dst, w=None, dsrc=None, ddst=None, dw=None):
         src, dst, w = _u32(src), _u32(dst), _u32(w)
         dsrc = _u32(dsrc if dsrc is not None else np.empty(0, np.uint32))
@@ -337,8 +354,32 @@ class Graph:
         self._check(lib().cytom_execute(self._h, pid, ctypes.byref(st)), "execute")
         return st
 
+    def batched_execute(self, queries):
+        """queries: list of (kind, sources); kind BFS/SSSP/PPR or its name.
+        Returns the aggregate RunStats; read results with read_query(i)."""
+        keep = []
+        arr = (Query * max(1, len(queries)))()
+        self._qkinds = []
+        for i, (kind, sources) in enumerate(queries):
+            if isinstance(kind, str):
+                kind = PROGRAMS[kind]
+            s = np.ascontiguousarray(np.atleast_1d(sources), dtype=np.uint32)
+            keep.append(s)
+            arr[i] = Query(kind, len(s), s.ctypes.data)
+            self._qkinds.append(kind)
+        st = RunStats()
+        self._check(lib().cytom_batched_execute(self._h, arr, len(queries), ctypes.byref(st)),
+                    "batched_execute")
+        return st
+
+    def read_query(self, i):
+        dt = np.float32 if self._qkinds[i] == PPR else np.uint32
+        out = np.empty(self.V, dt)
+        self._check(lib().cytom_read_query(self._h, i, _p(out), out.nbytes), "read_query")
+        return out
+
     def read(self, pid):
-        dt = np.float32 if self.kinds[pid] == PRD else np.uint32
+        dt = np.float32 if self.kinds[pid] in (PRD, PPR) else np.uint32
         out = np.empty(self.V, dt)
         self._check(lib().cytom_read_values(self._h, pid, _p(out), out.nbytes), "read")
         return out

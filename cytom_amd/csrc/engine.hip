@@ -60,6 +60,7 @@ void free_ctx(cytom_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& P : c->progs) program_free(P);
+  query_free(c);
   void* ps[] = {c->tile_off, c->tile_cap, c->tile_cnt, c->tile_live, c->tile_sorted,
                 c->pool,     c->wpool,    c->outdeg,   c->indeg,    c->d_misc,
                 c->d_iter};
@@ -202,10 +203,11 @@ int cytom_geometry(const cytom_ctx* ctx, uint64_t* n_vertices, uint32_t* vertice
 int cytom_program_register(cytom_ctx* ctx, const cytom_program_desc* desc, int* prog_id) {
   CY_CHECK_CTX(ctx);
   if (!desc || !prog_id) return set_err(ctx, CYTOM_EINVAL, "null descriptor");
-  if (desc->kind > CYTOM_PRD) return set_err(ctx, CYTOM_EINVAL, "unknown program kind");
+  if (desc->kind > CYTOM_PPR) return set_err(ctx, CYTOM_EINVAL, "unknown program kind");
   if (desc->kind == CYTOM_SSSP && !ctx->weighted)
     return set_err(ctx, CYTOM_EINVAL, "SSSP needs a weighted graph");
-  if ((desc->kind == CYTOM_BFS || desc->kind == CYTOM_SSSP) && desc->source >= ctx->V)
+  if ((desc->kind == CYTOM_BFS || desc->kind == CYTOM_SSSP || desc->kind == CYTOM_PPR) &&
+      desc->source >= ctx->V)
     return set_err(ctx, CYTOM_EINVAL, "source vertex out of range");
   if (ctx->progs.size() >= 48) return set_err(ctx, CYTOM_ERANGE, "too many programs");
   Program P;
@@ -213,7 +215,7 @@ int cytom_program_register(cytom_ctx* ctx, const cytom_program_desc* desc, int* 
   P.source = desc->source;
   int rc = program_alloc(ctx, P);
   if (rc) return rc;
-  rc = program_init(ctx, P);
+  rc = P.kind == CYTOM_PPR ? program_set_sources(ctx, P, &desc->source, 1) : program_init(ctx, P);
   if (rc) {
     program_free(P);
     return rc;
@@ -230,6 +232,16 @@ int cytom_program_init(cytom_ctx* ctx, int prog_id) {
   if (prog_id < 0 || prog_id >= (int)ctx->progs.size())
     return set_err(ctx, CYTOM_EINVAL, "bad program id");
   int rc = program_init(ctx, ctx->progs[prog_id]);
+  if (rc) return rc;
+  CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int cytom_program_set_sources(cytom_ctx* ctx, int prog_id, const uint32_t* sources, uint32_t n) {
+  CY_CHECK_CTX(ctx);
+  if (prog_id < 0 || prog_id >= (int)ctx->progs.size())
+    return set_err(ctx, CYTOM_EINVAL, "bad program id");
+  int rc = program_set_sources(ctx, ctx->progs[prog_id], sources, n);
   if (rc) return rc;
   CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return 0;
@@ -287,13 +299,25 @@ int cytom_execute(cytom_ctx* ctx, int prog_id, cytom_run_stats* stats) {
   return run_program(ctx, ctx->progs[prog_id], stats);
 }
 
+int cytom_batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq,
+                          cytom_run_stats* stats) {
+  CY_CHECK_CTX(ctx);
+  if (nq && !q) return set_err(ctx, CYTOM_EINVAL, "null query array");
+  return batched_execute(ctx, q, nq, stats);
+}
+
+int cytom_read_query(cytom_ctx* ctx, uint32_t query, void* out, size_t bytes) {
+  CY_CHECK_CTX(ctx);
+  return read_query(ctx, query, out, bytes);
+}
+
 int cytom_read_values(cytom_ctx* ctx, int prog_id, void* out, size_t bytes) {
   CY_CHECK_CTX(ctx);
   if (prog_id < 0 || prog_id >= (int)ctx->progs.size())
     return set_err(ctx, CYTOM_EINVAL, "bad program id");
   if (!out || bytes < ctx->V * 4) return set_err(ctx, CYTOM_EINVAL, "output buffer too small");
   const Program& P = ctx->progs[prog_id];
-  const void* src = P.kind == CYTOM_PRD ? (const void*)P.rank : (const void*)P.cur;
+  const void* src = prd_like(P.kind) ? (const void*)P.rank : (const void*)P.cur;
   CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   CY_HIP(ctx, hipMemcpy(out, src, ctx->V * 4, hipMemcpyDeviceToHost));
   return 0;
@@ -368,7 +392,7 @@ int cytom_set_profiling(cytom_ctx* ctx, int enable) {
 
 int cytom_kernel_profile(cytom_ctx* ctx, uint32_t kind, double* gather_ms, uint64_t* launches,
                          double* alg_bytes) {
-  if (!ctx || kind > CYTOM_PRD) return CYTOM_EINVAL;
+  if (!ctx || kind > CYTOM_PPR) return CYTOM_EINVAL;
   if (gather_ms) *gather_ms = ctx->gather_ms[kind];
   if (launches) *launches = ctx->gather_launches[kind];
   if (alg_bytes) *alg_bytes = ctx->gather_alg_bytes[kind];

@@ -62,16 +62,30 @@ __global__ void k_init_min(int kind, uint64_t V, uint32_t source, uint32_t* __re
 }
 
 // pagerank-delta.cc:246-270: rank 0, delta 1, all active.
-__global__ void k_init_prd(uint64_t V, float* __restrict__ rank, float* __restrict__ delta,
-                           long long* __restrict__ dsum, uint64_t* __restrict__ act,
-                           uint8_t* __restrict__ changed) {
+// pagerank-delta.cc:246-270 (rank 0, delta 1, all active); for PPR every
+// vertex starts at rank = delta = 0, inactive, before k_init_ppr_set.
+__global__ void k_init_prd(uint64_t V, int personalized, float* __restrict__ rank,
+                           float* __restrict__ delta, long long* __restrict__ dsum,
+                           uint64_t* __restrict__ act, uint8_t* __restrict__ changed) {
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
     rank[v] = 0.f;
-    delta[v] = 1.f;
+    delta[v] = personalized ? 0.f : 1.f;
     dsum[v] = 0;
     changed[v] = 0;
-    if ((v & 63) == 0) act[v >> 6] = ~0ull;
+    if ((v & 63) == 0) act[v >> 6] = personalized ? 0ull : ~0ull;
+  }
+}
+
+// PPR teleport set: rank = delta = (1-alpha)/|T|, active.
+__global__ void k_init_ppr_set(const uint32_t* __restrict__ set, uint32_t n, float* __restrict__ rank,
+                               float* __restrict__ delta, uint64_t* __restrict__ act) {
+  const float r0 = (float)((1. - 0.85) * (1.0 / (double)n));
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t v = set[i];
+    rank[v] = r0;
+    delta[v] = r0;
+    atomicOr(reinterpret_cast<unsigned long long*>(&act[v >> 6]), 1ull << (v & 63));
   }
 }
 
@@ -100,7 +114,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_frontier(
       ++cnt;
     }
     // Frontier values: what the gather reads per edge instead of the bitmap.
-    if (kind == CYTOM_PRD) contrib[v] = (a && od) ? delta[v] / (float)od : 0.f;
+    if (prd_like(kind)) contrib[v] = (a && od) ? delta[v] / (float)od : 0.f;
     else fval[v] = a ? cur[v] : INF;
     if (lane == 0 && word) rowact[v >> log2vpt] = 1;
   }
@@ -686,7 +700,7 @@ int program_alloc(cytom_ctx* ctx, Program& P) {
   const uint64_t V = ctx->V;
   auto alloc = [&](void** p, size_t b) -> bool { return hipMalloc(p, b) == hipSuccess; };
   bool ok = true;
-  if (P.kind == CYTOM_PRD) {
+  if (prd_like(P.kind)) {
     ok = ok && alloc((void**)&P.rank, V * 4) && alloc((void**)&P.delta, V * 4) &&
          alloc((void**)&P.dsum, V * 8) && alloc((void**)&P.contrib, V * 4);
   } else {
@@ -704,8 +718,9 @@ int program_alloc(cytom_ctx* ctx, Program& P) {
 }
 
 void program_free(Program& P) {
-  void* ps[] = {P.cur, P.nxt, P.fval, P.rank, P.delta, P.dsum, P.contrib, P.act[0], P.act[1],
-                P.rowact[0], P.rowact[1], P.changed};
+  void* ps[] = {P.cur,     P.nxt,     P.fval,      P.rank,      P.delta,   P.dsum,
+                P.contrib, P.act[0],  P.act[1],    P.rowact[0], P.rowact[1], P.changed,
+                P.teleport};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   P.cur = P.nxt = P.fval = nullptr;
@@ -714,16 +729,22 @@ void program_free(Program& P) {
   P.act[0] = P.act[1] = nullptr;
   P.rowact[0] = P.rowact[1] = nullptr;
   P.changed = nullptr;
+  P.teleport = nullptr;
+  P.n_teleport = 0;
 }
 
 int program_init(cytom_ctx* ctx, Program& P) {
   hipStream_t st = ctx->stream;
   P.cur_act = 0;
-  P.first_iter = true;
+  P.first_iter = P.kind != CYTOM_PPR;  // the PRD first-iteration apply rule
   P.fixpoint = false;
-  if (P.kind == CYTOM_PRD) {
-    k_init_prd<<<grid_for(ctx->V, 256), 256, 0, st>>>(ctx->V, P.rank, P.delta, P.dsum, P.act[0],
-                                                      P.changed);
+  if (prd_like(P.kind)) {
+    const int ppr = P.kind == CYTOM_PPR ? 1 : 0;
+    k_init_prd<<<grid_for(ctx->V, 256), 256, 0, st>>>(ctx->V, ppr, P.rank, P.delta, P.dsum,
+                                                      P.act[0], P.changed);
+    if (ppr && P.n_teleport)
+      k_init_ppr_set<<<grid_for(P.n_teleport, 256), 256, 0, st>>>(P.teleport, P.n_teleport,
+                                                                  P.rank, P.delta, P.act[0]);
   } else {
     k_init_min<<<grid_for(ctx->V, 256), 256, 0, st>>>(P.kind, ctx->V, P.source, P.cur, P.nxt,
                                                       P.act[0], P.changed);
@@ -732,9 +753,36 @@ int program_init(cytom_ctx* ctx, Program& P) {
   return 0;
 }
 
+int program_set_sources(cytom_ctx* ctx, Program& P, const uint32_t* sources, uint32_t n) {
+  if (!sources || n == 0) return set_err(ctx, CYTOM_EINVAL, "no source vertices");
+  for (uint32_t i = 0; i < n; ++i)
+    if (sources[i] >= ctx->V) return set_err(ctx, CYTOM_EINVAL, "source vertex out of range");
+  if (P.kind == CYTOM_BFS || P.kind == CYTOM_SSSP) {
+    if (n != 1) return set_err(ctx, CYTOM_EINVAL, "BFS and SSSP take one source");
+    P.source = sources[0];
+  } else if (P.kind == CYTOM_PPR) {
+    std::vector<uint32_t> set(sources, sources + n);
+    std::sort(set.begin(), set.end());
+    set.erase(std::unique(set.begin(), set.end()), set.end());
+    uint32_t* d = nullptr;
+    CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CY_HIP(ctx, hipMalloc(&d, set.size() * 4));
+    if (hipMemcpy(d, set.data(), set.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(d);
+      return set_err(ctx, CYTOM_EDEVICE, "teleport set copy failed");
+    }
+    if (P.teleport) (void)hipFree(P.teleport);
+    P.teleport = d;
+    P.n_teleport = (uint32_t)set.size();
+  } else {
+    return set_err(ctx, CYTOM_EINVAL, "CC and PageRank-delta take no sources");
+  }
+  return program_init(ctx, P);
+}
+
 void launch_reexec_check(cytom_ctx* ctx, Program& P, const uint32_t* found, uint64_t n_found,
                          unsigned long long* flag) {
-  if (P.kind == CYTOM_PRD)
+  if (prd_like(P.kind))
     k_reexec_prd<<<grid_for(n_found, 256), 256, 0, ctx->stream>>>(found, n_found, P.rank,
                                                                   ctx->outdeg, ctx->delta, flag);
   else
@@ -746,7 +794,7 @@ void launch_hooks(cytom_ctx* ctx, Program& P, const uint32_t* d_src, const uint3
                   const uint32_t* d_w, const uint64_t* ins_slot, uint64_t n_ins,
                   const uint32_t* found, uint64_t n_found) {
   hipStream_t st = ctx->stream;
-  if (P.kind == CYTOM_PRD) {
+  if (prd_like(P.kind)) {
     if (n_ins + n_found)
       k_hooks_prd<<<grid_for(n_ins + n_found, 256), 256, 0, st>>>(
           d_src, d_dst, ins_slot, n_ins, found, n_found, ctx->pool, P.rank, ctx->outdeg, P.dsum,
@@ -793,7 +841,7 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
       }
       if (prof) CY_HIP(ctx, hipEventRecord(ctx->kev[2 * i + 1], st));
       const int nx = P.cur_act ^ 1;
-      if (P.kind == CYTOM_PRD) {
+      if (prd_like(P.kind)) {
         k_apply_prd<<<apply_blocks, APPLY_THREADS, 0, st>>>(
             ctx->V, ctx->log2vpt, P.first_iter ? 1 : 0, ctx->delta, ldexp(1.0, -ctx->prd_shift),
             P.rank, P.delta, P.dsum,
@@ -841,7 +889,7 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
     // PageRank-delta vertex), a read-modify-write of every updated target and
     // one pass over the active bitmap.
     const double eb = P.kind == CYTOM_SSSP ? 12.0 : 8.0;
-    const double sv = P.kind == CYTOM_PRD ? 12.0 : 4.0;
+    const double sv = prd_like(P.kind) ? 12.0 : 4.0;
     for (uint32_t i = 0; i < it; ++i) {
       float ms = 0.f;
       CY_HIP(ctx, hipEventElapsedTime(&ms, ctx->kev[2 * i], ctx->kev[2 * i + 1]));

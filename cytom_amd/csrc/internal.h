@@ -67,10 +67,15 @@ struct StoreView {
   uint32_t* indeg;
 };
 
+// PageRank-delta and its personalized form share the gather and apply.
+__host__ __device__ inline bool prd_like(int kind) { return kind == CYTOM_PRD || kind == CYTOM_PPR; }
+
 // Per-program device state.
 struct Program {
   int kind = 0;
   uint32_t source = 0;
+  uint32_t* teleport = nullptr;  // PPR teleport set (device, distinct ids)
+  uint32_t n_teleport = 0;
   uint32_t* cur = nullptr;  // BFS/CC/SSSP current values
   uint32_t* nxt = nullptr;  // BFS/CC/SSSP next values
   uint32_t* fval = nullptr; // BFS/CC/SSSP: cur[v] while v is active, else INF
@@ -102,6 +107,8 @@ struct GrowBuf {
   void* p = nullptr;
   size_t bytes = 0;
 };
+
+struct QueryEngine;  // batched queries (query.hip)
 
 }  // namespace cytom
 
@@ -156,15 +163,16 @@ struct cytom_ctx {
   cytom::IterStats* d_iter = nullptr;  // [max_iter + 2]
   cytom::IterStats* h_iter = nullptr;  // pinned
   std::vector<cytom::Program> progs;
+  cytom::QueryEngine* qe = nullptr;
   std::string err;
   uint64_t batches = 0;
   // Optional per-kernel timing of the gather launches (HIP events on the
   // context stream), accumulated per program kind.
   bool profile = false;
   std::vector<hipEvent_t> kev;
-  double gather_ms[4] = {0, 0, 0, 0};
-  uint64_t gather_launches[4] = {0, 0, 0, 0};
-  double gather_alg_bytes[4] = {0, 0, 0, 0};
+  double gather_ms[5] = {0, 0, 0, 0, 0};
+  uint64_t gather_launches[5] = {0, 0, 0, 0, 0};
+  double gather_alg_bytes[5] = {0, 0, 0, 0, 0};
 
   cytom::StoreView view() const {
     cytom::StoreView s;
@@ -231,9 +239,16 @@ int64_t export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_t* w, 
 int program_alloc(cytom_ctx* ctx, Program& P);
 void program_free(Program& P);
 int program_init(cytom_ctx* ctx, Program& P);
+// Replaces a BFS/SSSP source or the PPR teleport set (host ids, validated).
+int program_set_sources(cytom_ctx* ctx, Program& P, const uint32_t* sources, uint32_t n);
 int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* st);
 // Sets the gather kernels' LDS limit and persistent grid for the context.
 int configure_gather(cytom_ctx* ctx);
+
+// ------------------------------------------------------------ batched queries
+int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run_stats* stats);
+int read_query(cytom_ctx* ctx, uint32_t query, void* out, size_t bytes);
+void query_free(cytom_ctx* ctx);
 // Re-execution tests of the found deletions against the program's values;
 // sets *flag (device) to 1 when a re-initialisation is needed.
 void launch_reexec_check(cytom_ctx* ctx, Program& P, const uint32_t* found, uint64_t n_found,

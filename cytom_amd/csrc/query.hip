@@ -1,0 +1,721 @@
+// Batched queries: many BFS / SSSP / personalized-PageRank queries against
+// one snapshot of the live graph, run as a few query groups instead of one
+// program run per query.
+//
+// The reference runs several jobs side by side through its multi-job
+// composition (src/include/main/jobs.h:33-140) and a fused two-instance
+// PageRank-delta (examples/pagerank-delta-merge.cc:151-252); it has no
+// multi-query kernel.  Here a batch works on a CSR snapshot (out-edges by
+// source, built once from the tile store) with frontier-driven push kernels
+// whose per-vertex state holds a whole query group:
+//   * BFS: 64 queries per group as bits of one u64 per vertex (visited,
+//     frontier, next), levels kept as u16 per (vertex, query);
+//   * SSSP: 16 queries per group, u32 distances stored vertex-major
+//     (64 B per vertex), relaxed with atomicMin (any relaxation order
+//     reaches the same fixpoint, so the result is exact);
+//   * PPR: 16 queries per group, the PageRank-delta iteration of the
+//     personalized program with the same FP32 per-vertex arithmetic and
+//     exact fixed-point sums as the single-query program, so each query's
+//     ranks equal its single-query run.
+// The frontier's out-edges are split evenly over the wavefronts (a lane
+// finds its first edge by binary search over the frontier's degree prefix
+// sum), so hub vertices do not serialise the expansion.
+#include <algorithm>
+#include <vector>
+
+#include "devutil.h"
+#include "internal.h"
+
+namespace cytom {
+
+struct QueryEngine {
+  // CSR snapshot
+  uint64_t snap_batch = ~0ull;  // ctx->batches when the snapshot was taken
+  uint64_t E = 0;
+  uint64_t* row_off = nullptr;  // V + 1
+  uint32_t* col = nullptr;      // E
+  uint32_t* wt = nullptr;       // E (weighted graphs)
+  // frontier work space
+  uint32_t* flist = nullptr;  // V
+  uint32_t* nlist = nullptr;  // V
+  uint64_t* fdeg = nullptr;   // V + 1
+  uint64_t* fpre = nullptr;   // V + 1
+  uint32_t* mark = nullptr;   // V
+  unsigned int* counter = nullptr;  // [0] next-list length
+  void* scan_tmp = nullptr;
+  // results: per query (kind, group, lane)
+  struct Q {
+    int kind;
+    uint32_t group, lane;
+  };
+  std::vector<Q> queries;
+  std::vector<void*> group_buf;  // BFS: u16 levels [V][64]; SSSP: u32 [V][16]; PPR: f32 rank [V][16]
+  std::vector<int> group_kind;
+};
+
+namespace {
+
+constexpr int QT = 256;          // threads per block
+constexpr uint32_t EPL = 8;      // edges per lane (scalar) or per lane group (vector)
+constexpr uint32_t BFS_W = 64;   // queries per BFS group (bits of a u64)
+constexpr uint32_t VEC_W = 16;   // queries per SSSP / PPR group
+constexpr uint16_t LEVEL_NONE = 0xFFFF;
+
+inline uint32_t blocks_for(uint64_t n, uint32_t per_block, uint32_t cap = 65536) {
+  uint64_t b = (n + per_block - 1) / per_block;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, cap));
+}
+
+// --------------------------------------------------------------- snapshot
+__global__ void k_widen(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
+// Live edges of every tile into their source's CSR row (the order inside a
+// row is immaterial: every kernel below reduces exactly).
+__global__ void __launch_bounds__(QT) k_csr_fill(StoreView S, unsigned long long* __restrict__ cursor,
+                                                 uint32_t* __restrict__ col,
+                                                 uint32_t* __restrict__ wt) {
+  for (uint64_t t = blockIdx.x; t < S.n_tiles; t += gridDim.x) {
+    const uint32_t cnt = S.tile_cnt[t];
+    if (cnt == 0) continue;
+    const uint64_t off = S.tile_off[t];
+    const uint32_t xb = (uint32_t)(t % S.T) << S.log2vpt, yb = (uint32_t)(t / S.T) << S.log2vpt;
+    for (uint32_t i = threadIdx.x; i < cnt; i += QT) {
+      const uint32_t e = S.pool[off + i];
+      if (e == TOMB) continue;
+      const uint32_t s = xb + (e & 0xFFFFu);
+      const unsigned long long p = atomicAdd(&cursor[s], 1ull);
+      col[p] = yb + (e >> 16);
+      if (wt) wt[p] = S.wpool[off + i];
+    }
+  }
+}
+
+// ------------------------------------------------------------- frontier
+__global__ void k_frontier_deg(const uint32_t* __restrict__ flist, uint32_t n,
+                               const uint32_t* __restrict__ outdeg, uint64_t* __restrict__ fdeg) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    fdeg[i] = outdeg[flist[i]];
+}
+
+// Frontier entry holding edge e: fpre[i] <= e < fpre[i + 1].
+__device__ inline uint32_t find_entry(const uint64_t* fpre, uint32_t n, uint64_t e) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (fpre[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Edge layout of the expansion: wavefront w owns edges
+// [w*64*EPL, (w+1)*64*EPL) of the frontier's concatenated out-edges, lane l
+// takes edges l, l+64, ... of that range (coalesced column reads).
+__global__ void __launch_bounds__(QT) k_bfs_expand(
+    const uint32_t* __restrict__ flist, const uint64_t* __restrict__ fpre, uint32_t n,
+    const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ col,
+    const uint64_t* __restrict__ front, const uint64_t* __restrict__ vis,
+    unsigned long long* __restrict__ nxt, uint32_t* __restrict__ nlist,
+    unsigned int* __restrict__ counter) {
+  const uint64_t TE = fpre[n];
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * QT + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * QT) >> 6;
+  for (uint64_t b = wave * 64 * EPL; b < TE; b += nwaves * 64 * EPL) {
+    uint64_t e = b + lane;
+    if (e >= TE) continue;
+    uint32_t i = find_entry(fpre, n, e);
+    const uint64_t end = min(b + 64 * EPL, TE);
+    for (; e < end; e += 64) {
+      while (fpre[i + 1] <= e) ++i;
+      const uint32_t u = flist[i];
+      const uint32_t v = col[row_off[u] + (e - fpre[i])];
+      const uint64_t m = front[u] & ~vis[v];
+      if (m) {
+        const unsigned long long old = atomicOr(&nxt[v], (unsigned long long)m);
+        if (old == 0ull) nlist[atomicAdd(counter, 1u)] = v;
+      }
+    }
+  }
+}
+
+// New levels of the vertices reached this round; the reached set becomes
+// the next frontier.  front[] of the old frontier was cleared before.
+__global__ void k_bfs_apply(const uint32_t* __restrict__ nlist, uint32_t n, uint16_t level,
+                            uint64_t* __restrict__ nxt, uint64_t* __restrict__ vis,
+                            uint64_t* __restrict__ front, uint16_t* __restrict__ lev) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t v = nlist[i];
+    const uint64_t nb = nxt[v] & ~vis[v];
+    nxt[v] = 0;
+    vis[v] |= nb;
+    front[v] = nb;
+    uint64_t m = nb;
+    while (m) {
+      const int q = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      lev[(uint64_t)v * BFS_W + q] = level;
+    }
+  }
+}
+
+__global__ void k_clear_front(const uint32_t* __restrict__ flist, uint32_t n,
+                              uint64_t* __restrict__ front) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    front[flist[i]] = 0;
+}
+
+__global__ void k_bfs_seed(const uint32_t* __restrict__ src, uint32_t nq, uint64_t* __restrict__ vis,
+                           uint64_t* __restrict__ front, uint16_t* __restrict__ lev) {
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    const uint32_t s = src[q];
+    atomicOr(reinterpret_cast<unsigned long long*>(&vis[s]), 1ull << q);
+    atomicOr(reinterpret_cast<unsigned long long*>(&front[s]), 1ull << q);
+    lev[(uint64_t)s * BFS_W + q] = 0;
+  }
+}
+
+// ------------------------------------------------------------------ SSSP
+// 16 lanes per edge (one per query of the group), 4 edges per wavefront
+// step.  sssp.cc:90-99 relaxation with u32 saturating distances.
+__global__ void __launch_bounds__(QT) k_sssp_expand(
+    const uint32_t* __restrict__ flist, const uint64_t* __restrict__ fpre, uint32_t n,
+    const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ col,
+    const uint32_t* __restrict__ wt, uint32_t* __restrict__ dist, uint32_t* __restrict__ mark,
+    uint32_t* __restrict__ nlist, unsigned int* __restrict__ counter) {
+  const uint64_t TE = fpre[n];
+  const uint32_t lane = threadIdx.x & 63, q = lane & 15, slot = lane >> 4;
+  const uint64_t wave = ((uint64_t)blockIdx.x * QT + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * QT) >> 6;
+  for (uint64_t b = wave * 4 * EPL; b < TE; b += nwaves * 4 * EPL) {
+    uint64_t e = b + slot;
+    const uint64_t end = min(b + 4 * EPL, TE);
+    if (e >= end) continue;
+    uint32_t i = find_entry(fpre, n, e);
+    for (; e < end; e += 4) {
+      while (fpre[i + 1] <= e) ++i;
+      const uint32_t u = flist[i];
+      const uint64_t p = row_off[u] + (e - fpre[i]);
+      const uint32_t v = col[p];
+      const uint32_t du = dist[(uint64_t)u * VEC_W + q];
+      bool improved = false;
+      if (du != INF) {
+        const uint32_t c = sat_add(du, wt[p]);
+        uint32_t* dv = &dist[(uint64_t)v * VEC_W + q];
+        if (c < *dv) improved = c < atomicMin(dv, c);
+      }
+      // One lane of the edge's 16 queues the target when any query improved.
+      const uint64_t any = __ballot(improved);
+      if (q == 0 && ((any >> (slot * 16)) & 0xFFFFull)) {
+        if (atomicCAS(&mark[v], 0u, 1u) == 0u) nlist[atomicAdd(counter, 1u)] = v;
+      }
+    }
+  }
+}
+
+__global__ void k_unmark(const uint32_t* __restrict__ list, uint32_t n, uint32_t* __restrict__ mark) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    mark[list[i]] = 0;
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_sssp_seed(const uint32_t* __restrict__ src, uint32_t nq,
+                            uint32_t* __restrict__ dist) {
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x)
+    dist[(uint64_t)src[q] * VEC_W + q] = 0;
+}
+
+// ------------------------------------------------------------------- PPR
+// pagerank-delta.cc:127-132 push form: every active (vertex, query) adds
+// round(delta / out_degree * 2^S) to each out-neighbour's fixed-point sum.
+__global__ void __launch_bounds__(QT) k_ppr_expand(
+    const uint32_t* __restrict__ flist, const uint64_t* __restrict__ fpre, uint32_t n,
+    const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ col,
+    const uint32_t* __restrict__ outdeg, const uint16_t* __restrict__ act,
+    const float* __restrict__ delta, long long* __restrict__ dsum, uint32_t* __restrict__ mark,
+    uint32_t* __restrict__ nlist, unsigned int* __restrict__ counter, float qscale) {
+  const uint64_t TE = fpre[n];
+  const uint32_t lane = threadIdx.x & 63, q = lane & 15, slot = lane >> 4;
+  const uint64_t wave = ((uint64_t)blockIdx.x * QT + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * QT) >> 6;
+  for (uint64_t b = wave * 4 * EPL; b < TE; b += nwaves * 4 * EPL) {
+    uint64_t e = b + slot;
+    const uint64_t end = min(b + 4 * EPL, TE);
+    if (e >= end) continue;
+    uint32_t i = find_entry(fpre, n, e);
+    for (; e < end; e += 4) {
+      while (fpre[i + 1] <= e) ++i;
+      const uint32_t u = flist[i];
+      const uint32_t v = col[row_off[u] + (e - fpre[i])];
+      bool pushed = false;
+      if ((act[u] >> q) & 1u) {
+        const long long c =
+            __float2ll_rn(delta[(uint64_t)u * VEC_W + q] / (float)outdeg[u] * qscale);
+        if (c != 0) {
+          atomicAdd(reinterpret_cast<unsigned long long*>(&dsum[(uint64_t)v * VEC_W + q]),
+                    (unsigned long long)c);
+          pushed = true;
+        }
+      }
+      const uint64_t any = __ballot(pushed);
+      if (q == 0 && ((any >> (slot * 16)) & 0xFFFFull)) {
+        if (atomicCAS(&mark[v], 0u, 1u) == 0u) nlist[atomicAdd(counter, 1u)] = v;
+      }
+    }
+  }
+}
+
+// pagerank-delta.cc:142-174 (not the first-iteration rule): applied only
+// where a sum arrived, active iff |delta| > threshold * rank.
+__global__ void k_ppr_apply(const uint32_t* __restrict__ nlist, uint32_t n, double thr, double qinv,
+                            float* __restrict__ rank, float* __restrict__ delta,
+                            long long* __restrict__ dsum, uint16_t* __restrict__ act,
+                            uint32_t* __restrict__ mark) {
+  const uint32_t q = threadIdx.x & 15;
+  const uint64_t nthr = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)n * VEC_W;
+       t += nthr) {
+    const uint32_t v = nlist[t / VEC_W];
+    const uint64_t k = (uint64_t)v * VEC_W + q;
+    const long long s_fx = dsum[k];
+    bool a = false;
+    if (s_fx != 0) {
+      const float s = (float)((double)s_fx * qinv);
+      const float d = static_cast<float>(0.85 * (double)s);
+      const float r = rank[k] + d;
+      rank[k] = r;
+      delta[k] = d;
+      dsum[k] = 0;
+      a = (double)fabsf(d) > thr * (double)r;
+    }
+    // 16 consecutive lanes hold one vertex: gather its active bits.
+    const uint64_t bits = __ballot(a);
+    if (q == 0) {
+      const uint32_t sh = threadIdx.x & 48;
+      act[v] = (uint16_t)((bits >> sh) & 0xFFFFull);
+      mark[v] = 0;
+    }
+  }
+}
+
+__global__ void k_clear_act(const uint32_t* __restrict__ flist, uint32_t n, uint16_t* __restrict__ act) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    act[flist[i]] = 0;
+}
+
+// Keep the reached vertices that have an active query.
+__global__ void k_keep_active(const uint32_t* __restrict__ nlist, uint32_t n,
+                              const uint16_t* __restrict__ act, uint32_t* __restrict__ flist,
+                              unsigned int* __restrict__ counter) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t v = nlist[i];
+    if (act[v]) flist[atomicAdd(counter, 1u)] = v;
+  }
+}
+
+// Teleport sets: rank = delta = (1-alpha)/|T| (|T| distinct), active.
+__global__ void k_ppr_seed(const uint32_t* __restrict__ members, const uint32_t* __restrict__ qof,
+                           const float* __restrict__ r0, uint32_t n, float* __restrict__ rank,
+                           float* __restrict__ delta, uint16_t* __restrict__ act) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t v = members[i], q = qof[i];
+    rank[(uint64_t)v * VEC_W + q] = r0[q];
+    delta[(uint64_t)v * VEC_W + q] = r0[q];
+    atomicOr(reinterpret_cast<unsigned int*>(act + (v & ~1u)), (1u << q) << ((v & 1u) * 16));
+  }
+}
+
+// ----------------------------------------------------------- extraction
+__global__ void k_extract_lev(const uint16_t* __restrict__ lev, uint32_t q, uint64_t V,
+                              uint32_t* __restrict__ out) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint16_t l = lev[v * BFS_W + q];
+    out[v] = l == LEVEL_NONE ? INF : (uint32_t)l;
+  }
+}
+
+__global__ void k_extract_vec(const uint32_t* __restrict__ vec, uint32_t q, uint64_t V,
+                              uint32_t* __restrict__ out) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    out[v] = vec[v * VEC_W + q];
+}
+
+}  // namespace
+
+// ================================================================= host
+namespace {
+
+template <typename T>
+int dev_alloc(cytom_ctx* ctx, T** p, size_t bytes) {
+  if (hipMalloc((void**)p, std::max<size_t>(bytes, 4)) != hipSuccess) {
+    *p = nullptr;
+    return set_err(ctx, CYTOM_ENOMEM, "query engine allocation failed");
+  }
+  return 0;
+}
+
+void free_results(QueryEngine* qe) {
+  for (void* p : qe->group_buf)
+    if (p) (void)hipFree(p);
+  qe->group_buf.clear();
+  qe->group_kind.clear();
+  qe->queries.clear();
+}
+
+void free_snapshot(QueryEngine* qe) {
+  void* ps[] = {qe->row_off, qe->col, qe->wt, qe->flist, qe->nlist, qe->fdeg,
+                qe->fpre,    qe->mark, qe->counter, qe->scan_tmp};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  qe->row_off = nullptr;
+  qe->col = qe->wt = qe->flist = qe->nlist = nullptr;
+  qe->fdeg = qe->fpre = nullptr;
+  qe->mark = nullptr;
+  qe->counter = nullptr;
+  qe->scan_tmp = nullptr;
+  qe->snap_batch = ~0ull;
+}
+
+// CSR snapshot of the live graph (rebuilt after every update batch).
+int snapshot(cytom_ctx* ctx, QueryEngine* qe) {
+  if (qe->snap_batch == ctx->batches && qe->row_off) return 0;
+  hipStream_t st = ctx->stream;
+  const uint64_t V = ctx->V, E = ctx->live_edges;
+  int rc;
+  free_snapshot(qe);
+  if ((rc = dev_alloc(ctx, &qe->row_off, (V + 1) * 8)) || (rc = dev_alloc(ctx, &qe->col, E * 4)) ||
+      (ctx->wpool && (rc = dev_alloc(ctx, &qe->wt, E * 4))) ||
+      (rc = dev_alloc(ctx, &qe->flist, V * 4)) || (rc = dev_alloc(ctx, &qe->nlist, V * 4)) ||
+      (rc = dev_alloc(ctx, &qe->fdeg, (V + 1) * 8)) ||
+      (rc = dev_alloc(ctx, &qe->fpre, (V + 1) * 8)) || (rc = dev_alloc(ctx, &qe->mark, V * 4)) ||
+      (rc = dev_alloc(ctx, &qe->counter, 256)) ||
+      (rc = dev_alloc(ctx, (char**)&qe->scan_tmp, scan_tmp_bytes(V + 1) + 256)))
+    return rc;
+  // row_off = exclusive scan of out-degrees; fdeg doubles as the cursor.
+  k_widen<<<blocks_for(V, 256), 256, 0, st>>>(ctx->outdeg, qe->fdeg, V);
+  scan_exclusive_u64(qe->fdeg, qe->row_off, V, qe->scan_tmp, st);
+  CY_HIP(ctx, hipMemcpyAsync(qe->fdeg, qe->row_off, V * 8, hipMemcpyDeviceToDevice, st));
+  k_csr_fill<<<blocks_for(ctx->n_tiles, 1, 16384), QT, 0, st>>>(
+      ctx->view(), reinterpret_cast<unsigned long long*>(qe->fdeg), qe->col, qe->wt);
+  uint64_t total = 0;
+  CY_HIP(ctx, hipMemcpyAsync(&total, qe->row_off + V, 8, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipMemsetAsync(qe->mark, 0, V * 4, st));
+  CY_HIP(ctx, hipStreamSynchronize(st));
+  if (total != E) return set_err(ctx, CYTOM_EDEVICE, "degree sum differs from live edges");
+  qe->E = E;
+  qe->snap_batch = ctx->batches;
+  return 0;
+}
+
+// Degree prefix of the current frontier; returns its out-edge count.
+int frontier_prefix(cytom_ctx* ctx, QueryEngine* qe, uint32_t n, uint64_t* te) {
+  hipStream_t st = ctx->stream;
+  k_frontier_deg<<<blocks_for(n, 256), 256, 0, st>>>(qe->flist, n, ctx->outdeg, qe->fdeg);
+  scan_exclusive_u64(qe->fdeg, qe->fpre, n, qe->scan_tmp, st);
+  CY_HIP(ctx, hipMemcpyAsync(te, qe->fpre + n, 8, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipStreamSynchronize(st));
+  return 0;
+}
+
+uint32_t expand_blocks(cytom_ctx* ctx, uint64_t te, uint32_t edges_per_wave) {
+  const uint64_t waves = (te + edges_per_wave - 1) / edges_per_wave;
+  const uint64_t cap = (uint64_t)ctx->num_cus * 16;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, cap));
+}
+
+int read_counter(cytom_ctx* ctx, QueryEngine* qe, uint32_t* n) {
+  CY_HIP(ctx, hipMemcpyAsync(n, qe->counter, 4, hipMemcpyDeviceToHost, ctx->stream));
+  CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// Unique sources of a group as the first frontier.
+int seed_frontier(cytom_ctx* ctx, QueryEngine* qe, std::vector<uint32_t> v, uint32_t* n) {
+  std::sort(v.begin(), v.end());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+  CY_HIP(ctx, hipMemcpyAsync(qe->flist, v.data(), v.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *n = (uint32_t)v.size();
+  return 0;
+}
+
+int run_bfs_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<uint32_t>& src,
+                  uint16_t* lev, cytom_run_stats* acc) {
+  hipStream_t st = ctx->stream;
+  const uint64_t V = ctx->V;
+  int rc;
+  uint64_t *vis = nullptr, *front = nullptr, *nxt = nullptr;
+  uint32_t* dsrc = nullptr;
+  if ((rc = dev_alloc(ctx, &vis, V * 8)) || (rc = dev_alloc(ctx, &front, V * 8)) ||
+      (rc = dev_alloc(ctx, &nxt, V * 8)) || (rc = dev_alloc(ctx, &dsrc, src.size() * 4))) {
+    (void)hipFree(vis), (void)hipFree(front), (void)hipFree(nxt), (void)hipFree(dsrc);
+    return rc;
+  }
+  auto done = [&](int r) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(vis), (void)hipFree(front), (void)hipFree(nxt), (void)hipFree(dsrc);
+    return r;
+  };
+  if (hipMemsetAsync(vis, 0, V * 8, st) || hipMemsetAsync(front, 0, V * 8, st) ||
+      hipMemsetAsync(nxt, 0, V * 8, st) || hipMemsetAsync(lev, 0xFF, V * BFS_W * 2, st) ||
+      hipMemcpyAsync(dsrc, src.data(), src.size() * 4, hipMemcpyHostToDevice, st))
+    return done(set_err(ctx, CYTOM_EDEVICE, "BFS group set-up failed"));
+  k_bfs_seed<<<1, 64, 0, st>>>(dsrc, (uint32_t)src.size(), vis, front, lev);
+  uint32_t n = 0;
+  if ((rc = seed_frontier(ctx, qe, src, &n))) return done(rc);
+  uint32_t level = 0;
+  while (n > 0) {
+    if (level + 1 >= LEVEL_NONE) return done(set_err(ctx, CYTOM_ERANGE, "BFS depth above 65534"));
+    ++level;
+    uint64_t te = 0;
+    if ((rc = frontier_prefix(ctx, qe, n, &te))) return done(rc);
+    acc->traversed_edges += te;
+    acc->active_vertices += n;
+    CY_HIP(ctx, hipMemsetAsync(qe->counter, 0, 4, st));
+    if (te)
+      k_bfs_expand<<<expand_blocks(ctx, te, 64 * EPL), QT, 0, st>>>(
+          qe->flist, qe->fpre, n, qe->row_off, qe->col, front, vis,
+          reinterpret_cast<unsigned long long*>(nxt), qe->nlist, qe->counter);
+    k_clear_front<<<blocks_for(n, 256), 256, 0, st>>>(qe->flist, n, front);
+    uint32_t m = 0;
+    if ((rc = read_counter(ctx, qe, &m))) return done(rc);
+    if (m)
+      k_bfs_apply<<<blocks_for(m, 256), 256, 0, st>>>(qe->nlist, m, (uint16_t)level, nxt, vis,
+                                                      front, lev);
+    std::swap(qe->flist, qe->nlist);
+    n = m;
+    acc->iterations += 1;
+  }
+  return done(0);
+}
+
+int run_sssp_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<uint32_t>& src,
+                   uint32_t* dist, cytom_run_stats* acc) {
+  hipStream_t st = ctx->stream;
+  const uint64_t V = ctx->V;
+  int rc;
+  uint32_t* dsrc = nullptr;
+  if ((rc = dev_alloc(ctx, &dsrc, src.size() * 4))) return rc;
+  auto done = [&](int r) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(dsrc);
+    return r;
+  };
+  k_fill_u32<<<blocks_for(V * VEC_W, 256, 16384), 256, 0, st>>>(dist, V * VEC_W, INF);
+  if (hipMemcpyAsync(dsrc, src.data(), src.size() * 4, hipMemcpyHostToDevice, st))
+    return done(set_err(ctx, CYTOM_EDEVICE, "SSSP group set-up failed"));
+  k_sssp_seed<<<1, 64, 0, st>>>(dsrc, (uint32_t)src.size(), dist);
+  uint32_t n = 0;
+  if ((rc = seed_frontier(ctx, qe, src, &n))) return done(rc);
+  while (n > 0) {
+    uint64_t te = 0;
+    if ((rc = frontier_prefix(ctx, qe, n, &te))) return done(rc);
+    acc->traversed_edges += te;
+    acc->active_vertices += n;
+    CY_HIP(ctx, hipMemsetAsync(qe->counter, 0, 4, st));
+    if (te)
+      k_sssp_expand<<<expand_blocks(ctx, te, 4 * EPL), QT, 0, st>>>(
+          qe->flist, qe->fpre, n, qe->row_off, qe->col, qe->wt, dist, qe->mark, qe->nlist,
+          qe->counter);
+    uint32_t m = 0;
+    if ((rc = read_counter(ctx, qe, &m))) return done(rc);
+    if (m) k_unmark<<<blocks_for(m, 256), 256, 0, st>>>(qe->nlist, m, qe->mark);
+    std::swap(qe->flist, qe->nlist);
+    n = m;
+    acc->iterations += 1;
+  }
+  return done(0);
+}
+
+int run_ppr_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<std::vector<uint32_t>>& sets,
+                  float* rank, cytom_run_stats* acc) {
+  hipStream_t st = ctx->stream;
+  const uint64_t V = ctx->V;
+  int rc;
+  float* delta = nullptr;
+  long long* dsum = nullptr;
+  uint16_t* act = nullptr;
+  uint32_t *members = nullptr, *qof = nullptr;
+  float* r0 = nullptr;
+  std::vector<uint32_t> hm, hq, all;
+  std::vector<float> hr0(VEC_W, 0.f);
+  for (uint32_t q = 0; q < sets.size(); ++q) {
+    std::vector<uint32_t> s = sets[q];
+    std::sort(s.begin(), s.end());
+    s.erase(std::unique(s.begin(), s.end()), s.end());
+    hr0[q] = (float)((1. - 0.85) * (1.0 / (double)s.size()));
+    for (uint32_t v : s) {
+      hm.push_back(v);
+      hq.push_back(q);
+      all.push_back(v);
+    }
+  }
+  if ((rc = dev_alloc(ctx, &delta, V * VEC_W * 4)) || (rc = dev_alloc(ctx, &dsum, V * VEC_W * 8)) ||
+      (rc = dev_alloc(ctx, &act, V * 2 + 4)) || (rc = dev_alloc(ctx, &members, hm.size() * 4)) ||
+      (rc = dev_alloc(ctx, &qof, hq.size() * 4)) || (rc = dev_alloc(ctx, &r0, VEC_W * 4))) {
+    (void)hipFree(delta), (void)hipFree(dsum), (void)hipFree(act);
+    (void)hipFree(members), (void)hipFree(qof), (void)hipFree(r0);
+    return rc;
+  }
+  auto done = [&](int r) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(delta), (void)hipFree(dsum), (void)hipFree(act);
+    (void)hipFree(members), (void)hipFree(qof), (void)hipFree(r0);
+    return r;
+  };
+  if (hipMemsetAsync(rank, 0, V * VEC_W * 4, st) || hipMemsetAsync(delta, 0, V * VEC_W * 4, st) ||
+      hipMemsetAsync(dsum, 0, V * VEC_W * 8, st) || hipMemsetAsync(act, 0, V * 2 + 4, st) ||
+      hipMemcpyAsync(members, hm.data(), hm.size() * 4, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(qof, hq.data(), hq.size() * 4, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(r0, hr0.data(), VEC_W * 4, hipMemcpyHostToDevice, st))
+    return done(set_err(ctx, CYTOM_EDEVICE, "PPR group set-up failed"));
+  k_ppr_seed<<<blocks_for(hm.size(), 256), 256, 0, st>>>(members, qof, r0, (uint32_t)hm.size(),
+                                                         rank, delta, act);
+  uint32_t n = 0;
+  if ((rc = seed_frontier(ctx, qe, all, &n))) return done(rc);
+  uint32_t* mark = qe->mark;
+  const float qscale = ldexpf(1.f, ctx->prd_shift);
+  const double qinv = ldexp(1.0, -ctx->prd_shift);
+  uint32_t it = 0;
+  while (n > 0 && it < ctx->max_iter) {
+    uint64_t te = 0;
+    if ((rc = frontier_prefix(ctx, qe, n, &te))) return done(rc);
+    acc->traversed_edges += te;
+    acc->active_vertices += n;
+    CY_HIP(ctx, hipMemsetAsync(qe->counter, 0, 4, st));
+    if (te)
+      k_ppr_expand<<<expand_blocks(ctx, te, 4 * EPL), QT, 0, st>>>(
+          qe->flist, qe->fpre, n, qe->row_off, qe->col, ctx->outdeg, act, delta, dsum, mark,
+          qe->nlist, qe->counter, qscale);
+    k_clear_act<<<blocks_for(n, 256), 256, 0, st>>>(qe->flist, n, act);
+    uint32_t m = 0;
+    if ((rc = read_counter(ctx, qe, &m))) return done(rc);
+    CY_HIP(ctx, hipMemsetAsync(qe->counter, 0, 4, st));
+    if (m) {
+      k_ppr_apply<<<blocks_for((uint64_t)m * VEC_W, 256), 256, 0, st>>>(
+          qe->nlist, m, ctx->delta, qinv, rank, delta, dsum, act, mark);
+      k_keep_active<<<blocks_for(m, 256), 256, 0, st>>>(qe->nlist, m, act, qe->flist,
+                                                        qe->counter);
+    }
+    if ((rc = read_counter(ctx, qe, &n))) return done(rc);
+    ++it;
+    acc->iterations += 1;
+  }
+  if (n > 0) acc->converged = 0;  // stopped at the iteration cap
+  return done(0);
+}
+
+}  // namespace
+
+void query_free(cytom_ctx* ctx) {
+  if (!ctx->qe) return;
+  free_results(ctx->qe);
+  free_snapshot(ctx->qe);
+  delete ctx->qe;
+  ctx->qe = nullptr;
+}
+
+int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run_stats* stats) {
+  for (uint32_t i = 0; i < nq; ++i) {
+    const cytom_query& x = q[i];
+    if (x.kind != CYTOM_BFS && x.kind != CYTOM_SSSP && x.kind != CYTOM_PPR)
+      return set_err(ctx, CYTOM_EINVAL, "batched queries are BFS, SSSP or PPR");
+    if (x.kind == CYTOM_SSSP && !ctx->weighted)
+      return set_err(ctx, CYTOM_EINVAL, "SSSP needs a weighted graph");
+    if (!x.sources || x.n_sources == 0) return set_err(ctx, CYTOM_EINVAL, "query without sources");
+    if (x.kind != CYTOM_PPR && x.n_sources != 1)
+      return set_err(ctx, CYTOM_EINVAL, "BFS and SSSP queries take one source");
+    for (uint32_t k = 0; k < x.n_sources; ++k)
+      if (x.sources[k] >= ctx->V) return set_err(ctx, CYTOM_EINVAL, "source vertex out of range");
+  }
+  if (!ctx->qe) ctx->qe = new (std::nothrow) QueryEngine();
+  if (!ctx->qe) return set_err(ctx, CYTOM_ENOMEM, "query engine allocation failed");
+  QueryEngine* qe = ctx->qe;
+  free_results(qe);
+  int rc;
+  if ((rc = snapshot(ctx, qe))) return rc;
+  hipStream_t st = ctx->stream;
+  CY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+  cytom_run_stats acc{};
+  acc.converged = 1;
+  qe->queries.assign(nq, QueryEngine::Q{0, 0, 0});
+  const uint64_t V = ctx->V;
+  // Group queries of one kind in submission order.
+  for (int kind : {CYTOM_BFS, CYTOM_SSSP, CYTOM_PPR}) {
+    const uint32_t width = kind == CYTOM_BFS ? BFS_W : VEC_W;
+    std::vector<uint32_t> idx;
+    for (uint32_t i = 0; i < nq; ++i)
+      if ((int)q[i].kind == kind) idx.push_back(i);
+    for (size_t g0 = 0; g0 < idx.size(); g0 += width) {
+      const size_t g1 = std::min(idx.size(), g0 + width);
+      const uint32_t gid = (uint32_t)qe->group_buf.size();
+      const size_t bytes = kind == CYTOM_BFS ? V * BFS_W * 2 : V * VEC_W * 4;
+      void* buf = nullptr;
+      if (hipMalloc(&buf, bytes) != hipSuccess)
+        return set_err(ctx, CYTOM_ENOMEM, "query result allocation failed");
+      qe->group_buf.push_back(buf);
+      qe->group_kind.push_back(kind);
+      for (size_t k = g0; k < g1; ++k) qe->queries[idx[k]] = QueryEngine::Q{kind, gid, (uint32_t)(k - g0)};
+      if (kind == CYTOM_BFS) {
+        std::vector<uint32_t> src;
+        for (size_t k = g0; k < g1; ++k) src.push_back(q[idx[k]].sources[0]);
+        rc = run_bfs_group(ctx, qe, src, static_cast<uint16_t*>(buf), &acc);
+      } else if (kind == CYTOM_SSSP) {
+        std::vector<uint32_t> src;
+        for (size_t k = g0; k < g1; ++k) src.push_back(q[idx[k]].sources[0]);
+        rc = run_sssp_group(ctx, qe, src, static_cast<uint32_t*>(buf), &acc);
+      } else {
+        std::vector<std::vector<uint32_t>> sets;
+        for (size_t k = g0; k < g1; ++k)
+          sets.emplace_back(q[idx[k]].sources, q[idx[k]].sources + q[idx[k]].n_sources);
+        rc = run_ppr_group(ctx, qe, sets, static_cast<float*>(buf), &acc);
+      }
+      if (rc) return rc;
+    }
+  }
+  CY_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+  CY_HIP(ctx, hipEventSynchronize(ctx->ev[3]));
+  float ms = 0.f;
+  CY_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+  acc.t_compute_ms = ms;
+  acc.reexecuted = 0;
+  if (stats) *stats = acc;
+  return 0;
+}
+
+int read_query(cytom_ctx* ctx, uint32_t query, void* out, size_t bytes) {
+  QueryEngine* qe = ctx->qe;
+  if (!qe || query >= qe->queries.size())
+    return set_err(ctx, CYTOM_EINVAL, "no such query in the last batch");
+  const uint64_t V = ctx->V;
+  if (!out || bytes < V * 4) return set_err(ctx, CYTOM_EINVAL, "output buffer too small");
+  const QueryEngine::Q& x = qe->queries[query];
+  hipStream_t st = ctx->stream;
+  uint32_t* tmp = nullptr;
+  CY_HIP(ctx, hipMalloc(&tmp, V * 4));
+  const void* buf = qe->group_buf[x.group];
+  if (x.kind == CYTOM_BFS)
+    k_extract_lev<<<blocks_for(V, 256, 16384), 256, 0, st>>>(static_cast<const uint16_t*>(buf),
+                                                             x.lane, V, tmp);
+  else
+    k_extract_vec<<<blocks_for(V, 256, 16384), 256, 0, st>>>(static_cast<const uint32_t*>(buf),
+                                                             x.lane, V, tmp);
+  hipError_t e = hipMemcpyAsync(out, tmp, V * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return hip_fail(ctx, e, "read_query");
+  return 0;
+}
+
+}  // namespace cytom

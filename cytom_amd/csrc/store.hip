@@ -804,8 +804,9 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
     bool reexec = false;
     if (n_found) {
       reexec = ctx->h_misc[M_REEXEC + p] != 0;
-      if (P.kind != CYTOM_PRD && !P.fixpoint) reexec = true;
+      if (!prd_like(P.kind) && !P.fixpoint) reexec = true;
     }
+    if (P.kind == CYTOM_PPR && (n_ins || n_found)) reexec = true;  // from scratch per batch
     P.last_reexec = reexec ? 1 : 0;
     if (reexec) {
       if ((rc = program_init(ctx, P))) return rc;

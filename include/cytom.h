@@ -29,6 +29,11 @@ enum cytom_program_kind {
   CYTOM_CC = 1,   /* examples/cc.cc:18-214 */
   CYTOM_SSSP = 2, /* examples/sssp.cc:18-228 (u32 weights and distances) */
   CYTOM_PRD = 3,  /* examples/pagerank-delta.cc:30-308 */
+  /* Personalized PageRank-delta (no reference counterpart): the PRD program
+   * with the teleport term (1-alpha) replaced by (1-alpha)*t_v, t uniform
+   * over the teleport set; starts from rank = delta = (1-alpha)*t with the
+   * teleport set active and re-initialises after every update batch. */
+  CYTOM_PPR = 4,
 };
 
 /* Error codes (negative return values). */
@@ -105,6 +110,10 @@ int cytom_geometry(const cytom_ctx* ctx, uint64_t* n_vertices, uint32_t* vertice
 int cytom_program_register(cytom_ctx* ctx, const cytom_program_desc* desc, int* prog_id);
 /* Re-runs init_vertices (EdgeInserter re-execution, edge-inserter.h:164-167). */
 int cytom_program_init(cytom_ctx* ctx, int prog_id);
+/* Replaces the program's source (BFS/SSSP: n == 1, algo_start_id,
+ * execute.h:312) or teleport set (PPR: n >= 1 vertices, duplicates ignored)
+ * and re-runs init_vertices. */
+int cytom_program_set_sources(cytom_ctx* ctx, int prog_id, const uint32_t* sources, uint32_t n);
 
 /* ---- update ingestion --------------------------------------------------- */
 
@@ -130,8 +139,31 @@ int cytom_update_batch_device(cytom_ctx* ctx, const uint32_t* d_src, const uint3
  * convergence or max_iterations. */
 int cytom_execute(cytom_ctx* ctx, int prog_id, cytom_run_stats* stats);
 /* Copies the program's values (the `next` array written by ResultsWriter,
- * results-writer.h:54-67): BFS/CC/SSSP uint32[V], PRD float rank[V]. */
+ * results-writer.h:54-67): BFS/CC/SSSP uint32[V], PRD/PPR float rank[V]. */
 int cytom_read_values(cytom_ctx* ctx, int prog_id, void* out, size_t bytes);
+
+/* ---- batched queries ---------------------------------------------------- */
+
+/* One query of a batch: BFS or SSSP from one source, or personalized
+ * PageRank-delta from a teleport set (duplicates ignored). */
+typedef struct {
+  uint32_t kind;            /* CYTOM_BFS, CYTOM_SSSP or CYTOM_PPR */
+  uint32_t n_sources;       /* 1 for BFS/SSSP; teleport set size for PPR */
+  const uint32_t* sources;  /* host array */
+} cytom_query;
+
+/* Runs nq queries against the current live graph as one batch (the
+ * reference composes several jobs per process, jobs.h:33-140, and fuses two
+ * PageRank-delta instances, pagerank-delta-merge.cc:151-252).  Each query's
+ * values equal a single-query run of the same program: BFS/SSSP exactly,
+ * PPR ranks exactly (same arithmetic and iteration cap).  Results stay in the
+ * context until the next batch; stats aggregate the batch (traversed_edges
+ * counts each shared frontier edge once per query group). */
+int cytom_batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq,
+                          cytom_run_stats* stats);
+/* Values of query `query` of the last batch: BFS/SSSP uint32[V], PPR
+ * float[V] (ResultsWriter layout, results-writer.h:54-67). */
+int cytom_read_query(cytom_ctx* ctx, uint32_t query, void* out, size_t bytes);
 
 /* ---- inspection --------------------------------------------------------- */
 

@@ -13,8 +13,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 REF_DIR = os.path.join(_HERE, "_ref")
 
-BFS, CC, SSSP, PRD = 0, 1, 2, 3
-KIND_NAMES = {"bfs": BFS, "cc": CC, "sssp": SSSP, "pagerank-delta": PRD, "prd": PRD}
+BFS, CC, SSSP, PRD, PPR = 0, 1, 2, 3, 4
+KIND_NAMES = {"bfs": BFS, "cc": CC, "sssp": SSSP, "pagerank-delta": PRD, "prd": PRD, "ppr": PPR}
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
@@ -99,6 +99,12 @@ def lib():
     L.orc_prd_f64.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _u32p, _u32p, ctypes.c_uint32,
                               ctypes.c_double, _f64p]
     L.orc_prd_f64.restype = ctypes.c_uint32
+    L.orc_set_sources.argtypes = [ctypes.c_void_p, ctypes.c_int, _u32p, ctypes.c_uint32]
+    L.orc_set_sources.restype = ctypes.c_int
+    L.orc_ppr_f64.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _u32p, _u32p, _u32p,
+                              ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
+                              _f64p]
+    L.orc_ppr_f64.restype = ctypes.c_uint32
     L.orc_pagerank_f64.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _u32p, _u32p,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_uint32, _f64p]
     L.orc_pagerank_f64.restype = ctypes.c_uint32
@@ -222,8 +228,13 @@ class Graph:
             raise ValueError("execute failed")
         return st
 
+    def set_sources(self, pid, sources):
+        s = _u32(np.asarray(sources, np.uint32))
+        if lib().orc_set_sources(self._h, pid, s, len(s)) < 0:
+            raise ValueError("set_sources failed")
+
     def read(self, pid):
-        dt = np.float32 if self.kinds[pid] == PRD else np.uint32
+        dt = np.float32 if self.kinds[pid] in (PRD, PPR) else np.uint32
         out = np.empty(self.V, dt)
         lib().orc_read(self._h, pid, _ptr(out))
         return out
@@ -267,6 +278,14 @@ def check_cc(V, s, d):
 def prd_f64(V, s, d, max_iterations=100, delta=1e-7):
     out = np.empty(V, np.float64)
     it = lib().orc_prd_f64(V, len(s), _u32(s), _u32(d), max_iterations, delta, out)
+    return out, it
+
+
+def ppr_f64(V, s, d, teleport, alpha=0.85, tol=1e-13, max_iterations=10000):
+    t = _u32(np.asarray(teleport, np.uint32))
+    out = np.empty(V, np.float64)
+    it = lib().orc_ppr_f64(V, len(s), _u32(s), _u32(d), t, len(t), alpha, tol, max_iterations,
+                           out)
     return out, it
 
 

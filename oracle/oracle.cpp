@@ -234,6 +234,7 @@ struct Tile {
 struct Prog {
   int kind = 0;
   uint32_t source = 0;
+  std::vector<uint32_t> teleport;  // PPR: distinct teleport vertices
   std::vector<uint32_t> cur, nxt;
   std::vector<PRDV> pcur, pnxt;
   std::vector<uint8_t> active, active_next, changed;
@@ -276,6 +277,7 @@ namespace {
 // ------------------------------------------------------------- programs
 inline bool is_min_prog(int k) { return k == ORC_BFS || k == ORC_CC || k == ORC_SSSP; }
 inline bool uses_changed(int k) { return k != ORC_SSSP; }  // USE_CHANGED statics
+inline bool prd_kind(int k) { return k == ORC_PRD || k == ORC_PPR; }
 
 // init_vertices: bfs.cc:170-197, cc.cc:171-189, sssp.cc:171-195 (+D2),
 // pagerank-delta.cc:246-270.
@@ -285,12 +287,23 @@ void init_program(orc_graph* g, Prog& P) {
   P.active.assign(nb, 0);
   P.active_next.assign(nb, 0);
   P.changed.assign(nb, 0);
-  P.first_iter = true;
+  P.first_iter = P.kind != ORC_PPR;
   P.fixpoint = false;
   if (P.kind == ORC_PRD) {
     P.pcur.assign(V, PRDV{0.f, 0.f, 1.f});
     P.pnxt.assign(V, PRDV{0.f, 0.f, 1.f});
     std::fill(P.active.begin(), P.active.end(), 0xFF);
+  } else if (P.kind == ORC_PPR) {
+    // Personalized form: rank = delta = (1-alpha)/|T| on the teleport set,
+    // which starts active; everything else starts at zero.
+    P.pcur.assign(V, PRDV{0.f, 0.f, 0.f});
+    P.pnxt.assign(V, PRDV{0.f, 0.f, 0.f});
+    const float r0 = (float)((1. - ALPHA) * (1.0 / (double)P.teleport.size()));
+    for (uint32_t v : P.teleport) {
+      P.pcur[v].rank = P.pnxt[v].rank = r0;
+      P.pcur[v].delta = P.pnxt[v].delta = r0;
+      bit_set(P.active, v);
+    }
   } else if (P.kind == ORC_CC) {
     P.cur.resize(V);
     P.nxt.resize(V);
@@ -352,7 +365,7 @@ uint64_t pull_batch(orc_graph* g, Prog& P, int tid, uint64_t t0, uint64_t t1,
       if (ys[i] == y) return i;
     size_t i = ys.size();
     ys.push_back(y);
-    if (k == ORC_PRD) {
+    if (prd_kind(k)) {
       if (lf.size() <= i) lf.emplace_back();
       lf[i].assign(vpt, 0.f);  // init_vertices_local: memset 0
     } else {
@@ -369,7 +382,7 @@ uint64_t pull_batch(orc_graph* g, Prog& P, int tid, uint64_t t0, uint64_t t1,
     const size_t li = local_index(y);
     const uint64_t xb = (uint64_t)x * vpt;
     const size_t n = tl.s.size();
-    if (k == ORC_PRD) {
+    if (prd_kind(k)) {
       float* loc = lf[li].data();
       for (size_t i = 0; i < n; ++i) {
         if (tl.del[i]) continue;
@@ -403,7 +416,7 @@ uint64_t pull_batch(orc_graph* g, Prog& P, int tid, uint64_t t0, uint64_t t1,
   // applyTiles: fold the local arrays into next (algorithm-executor.h:431-505).
   for (size_t i = 0; i < ys.size(); ++i) {
     const uint64_t base = (uint64_t)ys[i] * vpt;
-    if (k == ORC_PRD) {
+    if (prd_kind(k)) {
       const float* loc = lf[i].data();
       for (uint32_t j = 0; j < vpt; ++j) {
         if (loc[j] == 0.f) continue;  // == neutral_element
@@ -424,7 +437,7 @@ uint64_t pull_batch(orc_graph* g, Prog& P, int tid, uint64_t t0, uint64_t t1,
 // apply(): bfs.cc:101-113, cc.cc:101-111, sssp.cc:101-113,
 // pagerank-delta.cc:142-174.  Returns true when the vertex converged.
 inline bool apply_vertex(orc_graph* g, Prog& P, uint64_t id) {
-  if (P.kind == ORC_PRD) {
+  if (prd_kind(P.kind)) {
     PRDV& n = P.pnxt[id];
     const float delta_sum = n.delta_sum;
     if (P.first_iter) {
@@ -657,12 +670,13 @@ void orc_destroy(orc_graph* g) { delete g; }
 uint64_t orc_num_vertices(const orc_graph* g) { return g->V; }
 
 int orc_register(orc_graph* g, int kind, uint32_t source) {
-  if (kind < ORC_BFS || kind > ORC_PRD) return -22;
+  if (kind < ORC_BFS || kind > ORC_PPR) return -22;
   if (kind == ORC_SSSP && !g->weighted) return -22;
   if (source >= g->V) return -22;
   Prog P;
   P.kind = kind;
   P.source = source;
+  if (kind == ORC_PPR) P.teleport.assign(1, source);
   init_program(g, P);
   g->progs.push_back(std::move(P));
   return (int)g->progs.size() - 1;
@@ -747,6 +761,13 @@ int64_t orc_update(orc_graph* g, const uint32_t* src, const uint32_t* dst, const
   // 3. Per program: re-execution decision, then hooks (edgeChanged).
   for (Prog& P : g->progs) {
     bool reexec = false;
+    if (P.kind == ORC_PPR) {
+      // Personalized PageRank-delta restarts from its teleport set after
+      // every update batch.
+      P.last_reexec = (n_ins || !found.empty()) ? 1 : 0;
+      if (P.last_reexec) init_program(g, P);
+      continue;
+    }
     if (!found.empty()) {
       if (is_min_prog(P.kind)) {
         if (!P.fixpoint) {
@@ -835,12 +856,31 @@ int orc_execute(orc_graph* g, int prog, orc_stats* out) {
 int orc_read(const orc_graph* g, int prog, void* out) {
   if (prog < 0 || prog >= (int)g->progs.size()) return -22;
   const Prog& P = g->progs[prog];
-  if (P.kind == ORC_PRD) {
+  if (prd_kind(P.kind)) {
     float* r = static_cast<float*>(out);
     for (uint64_t i = 0; i < g->V; ++i) r[i] = P.pnxt[i].rank;
   } else {
     std::memcpy(out, P.cur.data(), g->V * 4);
   }
+  return 0;
+}
+
+int orc_set_sources(orc_graph* g, int prog, const uint32_t* sources, uint32_t n) {
+  if (prog < 0 || prog >= (int)g->progs.size() || !sources || n == 0) return -22;
+  for (uint32_t i = 0; i < n; ++i)
+    if (sources[i] >= g->V) return -22;
+  Prog& P = g->progs[prog];
+  if (P.kind == ORC_BFS || P.kind == ORC_SSSP) {
+    if (n != 1) return -22;
+    P.source = sources[0];
+  } else if (P.kind == ORC_PPR) {
+    P.teleport.assign(sources, sources + n);
+    std::sort(P.teleport.begin(), P.teleport.end());
+    P.teleport.erase(std::unique(P.teleport.begin(), P.teleport.end()), P.teleport.end());
+  } else {
+    return -22;
+  }
+  init_program(g, P);
   return 0;
 }
 
@@ -1009,6 +1049,37 @@ uint32_t orc_prd_f64(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t
     act.swap(nact);
     if (conv) break;
   }
+  return it;
+}
+
+uint32_t orc_ppr_f64(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                     const uint32_t* teleport, uint32_t n_teleport, double alpha, double tol,
+                     uint32_t max_iterations, double* rank) {
+  CSR in = build_csr(V, E, dst, src, nullptr);
+  std::vector<uint32_t> outdeg(V, 0);
+  for (uint64_t i = 0; i < E; ++i) outdeg[src[i]]++;
+  std::vector<uint32_t> set(teleport, teleport + n_teleport);
+  std::sort(set.begin(), set.end());
+  set.erase(std::unique(set.begin(), set.end()), set.end());
+  std::vector<double> t(V, 0.0), r(V, 0.0), nr(V);
+  for (uint32_t v : set) t[v] = 1.0 / (double)set.size();
+  uint32_t it = 0;
+  while (it < max_iterations) {
+    double diff = 0.0;
+    for (uint64_t v = 0; v < V; ++v) {
+      double acc = 0.0;
+      for (uint64_t p = in.off[v]; p < in.off[v + 1]; ++p) {
+        const uint32_t u = in.adj[p];
+        acc += r[u] / (double)outdeg[u];
+      }
+      nr[v] = (1.0 - alpha) * t[v] + alpha * acc;
+      diff = std::max(diff, std::abs(nr[v] - r[v]));
+    }
+    r.swap(nr);
+    ++it;
+    if (diff < tol) break;
+  }
+  std::copy(r.begin(), r.end(), rank);
   return it;
 }
 

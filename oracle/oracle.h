@@ -28,9 +28,11 @@
  *   BFS / CC / SSSP executors: textbook checkers below, bit-exact.
  *   PageRank-delta executor: FP64 restatement (orc_prd_f64) and FP64
  *     power iteration; hand-computed values on tiny graphs.
- *   stream weights / deletion sampler: definitional (STREAMS spec in
- *     DESIGN.md) -- pinned by invariants (range, uniqueness, liveness),
- *     otherwise "parity unpinned" as they have no reference counterpart.
+ *   stream weights / deletion sampler: definitional (no reference
+ *     counterpart) -- pinned by invariants (range, uniqueness, liveness),
+ *     otherwise "parity unpinned".
+ *   Personalized PageRank-delta (ORC_PPR, no reference counterpart): the
+ *     FP64 power iteration orc_ppr_f64 and hand-computed tiny cases.
  */
 #ifndef CYTOM_ORACLE_H
 #define CYTOM_ORACLE_H
@@ -41,7 +43,7 @@
 extern "C" {
 #endif
 
-enum { ORC_BFS = 0, ORC_CC = 1, ORC_SSSP = 2, ORC_PRD = 3 };
+enum { ORC_BFS = 0, ORC_CC = 1, ORC_SSSP = 2, ORC_PRD = 3, ORC_PPR = 4 };
 
 /* ---- a1: hash + RMAT ---------------------------------------------------- */
 uint32_t orc_hash32(uint32_t a);
@@ -95,7 +97,10 @@ int64_t orc_update(orc_graph* g, const uint32_t* src, const uint32_t* dst,
                    const uint32_t* w, uint64_t n_ins, const uint32_t* dsrc,
                    const uint32_t* ddst, const uint32_t* dw, uint64_t n_del);
 int orc_execute(orc_graph* g, int prog, orc_stats* out);
-/* BFS/CC/SSSP: uint32[V]; PRD: float rank[V]. */
+/* Replaces a BFS/SSSP source (n == 1) or the PPR teleport set (duplicates
+ * ignored) and re-runs init.  Returns 0 or -22. */
+int orc_set_sources(orc_graph* g, int prog, const uint32_t* sources, uint32_t n);
+/* BFS/CC/SSSP: uint32[V]; PRD/PPR: float rank[V]. */
 int orc_read(const orc_graph* g, int prog, void* out);
 int orc_last_reexec(const orc_graph* g, int prog);
 uint64_t orc_live_edges(const orc_graph* g, uint32_t* src, uint32_t* dst, uint32_t* w);
@@ -112,6 +117,12 @@ void orc_check_cc(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* d
  * activation threshold).  Returns the iteration count. */
 uint32_t orc_prd_f64(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                      uint32_t max_iterations, double delta, double* rank);
+/* FP64 power iteration of personalized PageRank:
+ * r = (1-a) t + a * sum_{u->v} r_u / outdeg(u), t uniform over the distinct
+ * teleport vertices; mass at vertices without out-edges is dropped. */
+uint32_t orc_ppr_f64(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                     const uint32_t* teleport, uint32_t n_teleport, double alpha, double tol,
+                     uint32_t max_iterations, double* rank);
 /* FP64 power iteration r = (1-a) + a * sum_{u->v} r_u / outdeg(u). */
 uint32_t orc_pagerank_f64(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                           double alpha, double tol, uint32_t max_iterations, double* rank);

@@ -265,3 +265,97 @@ def test_prd_insert_only_hooks_path():
         og.execute(op)
         gg.execute(gp)
         assert l1n(gg.read(gp), og.read(op)) <= PRD_L1_TOL, b
+
+
+# ------------------------------------------------------- personalized PageRank
+def test_ppr_stream_parity():
+    # Single-query PPR program vs the oracle after every batch of a stream
+    # with deletions (from scratch per batch), several teleport sets.
+    scale, V = 13, 1 << 13
+    st = O.Stream(4)
+    og = O.Graph(V, vertices_per_tile=1024)
+    gg = C.Graph(V, vertices_per_tile=1024)
+    op, gp = og.register(O.PPR, 3), gg.register(C.PPR, 3)
+    sets = [[3], [1, 1, 2, 900, 8000], list(range(5, V, 101))]
+    for b in range(4):
+        first, didx = st.next(20000, 2000 if b else 0)
+        s, d, w, ds, dd, dw = O.batch_edges(scale, 4, first, 20000, didx, False)
+        og.update(s, d, None, ds, dd, None)
+        gg.update(s, d, None, ds, dd, None)
+        for tel in sets:
+            og.set_sources(op, tel)
+            gg.set_sources(gp, tel)
+            ost, gst = og.execute(op), gg.execute(gp)
+            assert gst.converged == ost.converged
+            err = l1n(gg.read(gp), og.read(op))
+            assert err <= PRD_L1_TOL, (b, tel[:2], err)
+
+
+# ------------------------------------------------------------ batched queries
+def _stream_graph(scale, batches, n_ins, n_del, weighted=True, seed=6, **kw):
+    V = 1 << scale
+    st = O.Stream(seed)
+    g = C.Graph(V, weighted=weighted, **kw)
+    for b in range(batches):
+        first, didx = st.next(n_ins, n_del if b else 0)
+        s, d, w, ds, dd, dw = O.batch_edges(scale, seed, first, n_ins, didx, weighted)
+        g.update(s, d, w if weighted else None, ds, dd, dw if weighted else None)
+    return g
+
+
+def test_batched_queries_equal_single_runs():
+    scale = 14
+    V = 1 << scale
+    g = _stream_graph(scale, 3, 60000, 6000)
+    rng = np.random.default_rng(3)
+    deg_o, deg_i = g.degrees()
+    live = np.nonzero(deg_o + deg_i)[0]
+    bfs_src = rng.choice(live, 70)
+    bfs_src[5] = bfs_src[6]  # the same source twice in one group
+    bfs_src[7] = np.nonzero(deg_o + deg_i == 0)[0][0]  # an isolated vertex
+    sssp_src = rng.choice(live, 20)
+    ppr_sets = [rng.choice(live, int(rng.integers(1, 60))) for _ in range(18)]
+    ppr_sets[0] = np.array([live[0], live[0], live[1]])
+    queries = ([("bfs", s) for s in bfs_src] + [("sssp", s) for s in sssp_src] +
+               [("ppr", t) for t in ppr_sets])
+    # interleave kinds so grouping by kind is exercised
+    order = rng.permutation(len(queries))
+    queries = [queries[i] for i in order]
+    st = g.batched_execute(queries)
+    assert st.traversed_edges > 0
+    pid = {k: g.register(k, int(live[0])) for k in ("bfs", "sssp", "ppr")}
+    ls, ld, lw = g.export_edges()
+    for i, (kind, src) in enumerate(queries):
+        got = g.read_query(i)
+        g.set_sources(pid[kind], src)
+        g.execute(pid[kind])
+        ref = g.read(pid[kind])
+        assert np.array_equal(got, ref), (i, kind)
+        if kind == "bfs" and i % 7 == 0:
+            assert np.array_equal(got, O.check_bfs(V, ls, ld, int(src)))
+        if kind == "sssp" and i % 5 == 0:
+            assert np.array_equal(got, O.check_sssp(V, ls, ld, lw, int(src)))
+
+
+def test_batched_ppr_iteration_cap_and_errors():
+    g = _stream_graph(12, 2, 20000, 2000, max_iterations=3)
+    sets = [[1, 2, 3], [7], list(range(0, 4096, 9))]
+    g.batched_execute([("ppr", t) for t in sets])
+    p = g.register("ppr", 0)
+    for i, t in enumerate(sets):
+        g.set_sources(p, t)
+        rs = g.execute(p)
+        assert rs.iterations == 3
+        assert np.array_equal(g.read_query(i), g.read(p))
+    with pytest.raises(RuntimeError):
+        g.batched_execute([("bfs", [1, 2])])
+    with pytest.raises(RuntimeError):
+        g.batched_execute([("cc", [1])])
+    with pytest.raises(RuntimeError):
+        g.batched_execute([("bfs", [1 << 20])])
+    gu = C.Graph(256, vertices_per_tile=64)
+    gu.update(np.array([0, 1], np.uint32), np.array([1, 2], np.uint32))
+    with pytest.raises(RuntimeError):
+        gu.batched_execute([("sssp", [0])])
+    r = gu.batched_execute([("bfs", [0]), ("ppr", [0])])
+    assert list(gu.read_query(0)[:4]) == [0, 1, 2, 0xFFFFFFFF]

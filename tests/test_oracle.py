@@ -265,3 +265,50 @@ def test_iteration_cap_leaves_leftovers_for_next_run():
         g.execute(p)
     lv = g.read(p)
     assert lv[:10].tolist() == list(range(10))
+
+
+# ------------------------------------------------------------ personalized
+def test_ppr_f64_closed_form_cycle():
+    # 3-cycle 0->1->2->0, teleport {0}: r_k = a^k (1-a) / (1 - a^3).
+    a = 0.85
+    s = np.array([0, 1, 2], np.uint32)
+    d = np.array([1, 2, 0], np.uint32)
+    r, _ = O.ppr_f64(3, s, d, [0], alpha=a)
+    z = (1 - a) / (1 - a ** 3)
+    assert np.allclose(r, [z, a * z, a * a * z], rtol=1e-12, atol=0)
+
+
+def test_ppr_f64_uniform_teleport_is_pagerank():
+    # t uniform over every vertex makes PPR the global PageRank / V.
+    V = 1 << 10
+    s, d = O.rmat_edges(10, 5, 0, 6000)
+    pr, _ = O.pagerank_f64(V, s, d)
+    ppr, _ = O.ppr_f64(V, s, d, np.arange(V, dtype=np.uint32))
+    assert np.allclose(ppr * V, pr, rtol=1e-9, atol=1e-12)
+
+
+def test_ppr_delta_program_tiny_and_power_iteration():
+    a = 0.85
+    g = O.Graph(64, vertices_per_tile=64)
+    p = g.register(O.PPR, 0)
+    g.update(np.array([0, 1, 2], np.uint32), np.array([1, 2, 0], np.uint32))
+    g.execute(p)
+    r = g.read(p)
+    z = (1 - a) / (1 - a ** 3)
+    assert np.allclose(r[:3], [z, a * z, a * a * z], rtol=2e-6)
+    assert np.all(r[3:] == 0)
+    # Random graph, several teleport sets (duplicates ignored): the FP32
+    # delta program stays within its threshold of the FP64 fixpoint.
+    V = 1 << 11
+    s, d = O.rmat_edges(11, 9, 0, 16000)
+    g = O.Graph(V, vertices_per_tile=512)
+    p = g.register(O.PPR, 0)
+    g.update(s, d)
+    for tel in ([7], [1, 2, 3, 3, 900], list(range(0, V, 17))):
+        g.set_sources(p, tel)
+        st = g.execute(p)
+        assert st.converged == 1
+        r = g.read(p).astype(np.float64)
+        ref, _ = O.ppr_f64(V, s, d, tel)
+        err = np.abs(r - ref).sum() / ref.sum()
+        assert err < 2e-5, (tel[:3], err)
