@@ -15,6 +15,13 @@ All inputs are generated into HBM before the timed region.
 value = traversed edges of all programs over the timed batches / wall time
 (GTEPS); a traversed edge is a live out-edge of a vertex active in an
 iteration, counted by the engine.
+
+After the stream (outside the timed region) the query workload runs once as
+batches and once as a plain loop of single-program runs on the same GPU:
+256 BFS + 256 SSSP sources and 64 personalized-PageRank teleport sets on the
+final graph, and 128 small graph tenants (scale U{10..20}, edge factor
+U{8..32}, 8 insert batches, CC after each); detail.batched_queries reports
+queries/s of both and the speedup.
 """
 import argparse
 import concurrent.futures
@@ -93,6 +100,12 @@ def parse_args(argv):
     ap.add_argument("--resort-pct", type=int, default=0,
                     help="sort-compaction threshold in percent (-1 disables)")
     ap.add_argument("--programs", default="bfs,cc,sssp,pagerank-delta")
+    ap.add_argument("--queries", choices=["on", "off"], default="on",
+                    help="after the stream: batched vs plain query workload")
+    ap.add_argument("--q-bfs", type=int, default=256)
+    ap.add_argument("--q-sssp", type=int, default=256)
+    ap.add_argument("--q-ppr", type=int, default=64)
+    ap.add_argument("--q-graphs", type=int, default=128)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--cpu-batches", type=int, default=2,
@@ -241,8 +254,154 @@ def gpu_run(args):
         "roofline": roofline,
         "detail": detail,
     }
+    if args.queries == "on":
+        res["detail"]["batched_queries"] = queries_run(args, g, V, source)
     g.close()
     return res
+
+
+def _timed(fn):
+    import cytom_amd as C
+
+    C.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    C.synchronize()
+    return time.perf_counter() - t0, out
+
+
+def queries_run(args, g, V, source):
+    """Query workload on the graph left by the stream: BFS and SSSP from
+    sources uniform among vertices with in+out degree > 0 and personalized
+    PageRank from teleport sets of U{1..1024} such vertices, each run once as
+    one batch (cytom_batched_execute) and once as a plain per-query loop of
+    single-program runs on the same GPU; plus the small-graph tenants."""
+    import cytom_amd as C
+
+    rng = np.random.default_rng(args.seed + 100)
+    deg_o, deg_i = g.degrees()
+    live = np.nonzero(deg_o.astype(np.int64) + deg_i)[0].astype(np.uint32)
+    n_sssp = args.q_sssp if g.weighted else 0
+    sets = {
+        "bfs": [("bfs", int(s)) for s in rng.choice(live, args.q_bfs)],
+        "sssp": [("sssp", int(s)) for s in rng.choice(live, n_sssp)],
+        "ppr": [("ppr", rng.choice(live, int(rng.integers(1, 1025))))
+                for _ in range(args.q_ppr)],
+    }
+    out = {}
+    pids = {}
+    for kind, qs in sets.items():
+        if not qs:
+            continue
+        tb, st = _timed(lambda: g.batched_execute(qs))
+        sample = [0, len(qs) // 2, len(qs) - 1]
+        got = {i: g.read_query(i) for i in sample}
+        if kind not in pids:
+            pids[kind] = g.register(kind, int(live[0]))
+        pid = pids[kind]
+
+        def plain():
+            te = 0
+            for _, src in qs:
+                g.set_sources(pid, src)
+                te += g.execute(pid).traversed_edges
+            return te
+
+        tp, te_plain = _timed(plain)
+        same = True
+        for i in sample:
+            g.set_sources(pid, qs[i][1])
+            g.execute(pid)
+            same = same and bool(np.array_equal(got[i], g.read(pid)))
+        out[kind] = {"queries": len(qs), "batched_ms": round(tb * 1e3, 2),
+                     "plain_ms": round(tp * 1e3, 2), "speedup": round(tp / tb, 2),
+                     "batched_qps": round(len(qs) / tb, 1), "plain_qps": round(len(qs) / tp, 1),
+                     "batched_traversed_edges": st.traversed_edges,
+                     "plain_traversed_edges": te_plain, "sample_equal": same}
+        log("bench: %d %s queries: batched %.1f ms, plain loop %.1f ms" % (len(qs), kind,
+                                                                          tb * 1e3, tp * 1e3))
+    if args.q_graphs:
+        out["graphs"] = tenants_run(args)
+    tq = sum(v["queries"] for v in out.values())
+    tb = sum(v["batched_ms"] for v in out.values())
+    tp = sum(v["plain_ms"] for v in out.values())
+    out["total"] = {"queries": tq, "batched_ms": round(tb, 2), "plain_ms": round(tp, 2),
+                    "batched_qps": round(tq / tb * 1e3, 1) if tb else 0.0,
+                    "plain_qps": round(tq / tp * 1e3, 1) if tp else 0.0,
+                    "speedup": round(tp / tb, 2) if tb else 0.0}
+    return out
+
+
+def tenants_run(args):
+    """--q-graphs small graphs (scale U{10..20}, edge factor U{8..32}, own
+    RMAT seeds), each streamed in 8 equal insert batches with CC after every
+    batch: a plain loop of one context per graph against one context holding
+    their disjoint union (vertex ranges side by side), which runs every
+    tenant's batch b as one update and one CC run."""
+    import cytom_amd as C
+
+    rng = np.random.default_rng(args.seed + 200)
+    specs = [(int(rng.integers(10, 21)), int(rng.integers(8, 33))) for _ in range(args.q_graphs)]
+    nb = 8
+    dev = []  # per graph, per batch: (src, dst) device arrays
+    host = []  # per batch: union (src, dst) host arrays
+    base = 0
+    bases = []
+    for gi, (scale, ef) in enumerate(specs):
+        e_b = (ef << scale) // nb
+        seed = args.seed + 1000 + gi
+        bb = []
+        for b in range(nb):
+            s, d, _ = C.rmat_generate(scale, seed, b * e_b, e_b)
+            bb.append((s, d, e_b))
+        dev.append(bb)
+        bases.append(base)
+        base += 1 << scale
+    V_union = base
+    for b in range(nb):
+        us = np.concatenate([dev[gi][b][0].to_host() + np.uint32(bases[gi])
+                             for gi in range(len(specs))])
+        ud = np.concatenate([dev[gi][b][1].to_host() + np.uint32(bases[gi])
+                             for gi in range(len(specs))])
+        host.append((C.DeviceArray.from_host(us), C.DeviceArray.from_host(ud), len(us)))
+    e_total = sum(h[2] for h in host)
+
+    def plain():
+        labels = []
+        for gi, (scale, ef) in enumerate(specs):
+            G = C.Graph(1 << scale, edge_capacity=int((ef << scale) * 1.3) + 1024)
+            p = G.register("cc")
+            for b in range(nb):
+                s, d, n = dev[gi][b]
+                G.update_device(s, d, None, n)
+                G.execute(p)
+            labels.append(G.read(p) if gi < 4 else None)
+            G.close()
+        return labels
+
+    def union():
+        G = C.Graph(V_union, edge_capacity=int(e_total * 1.3) + 1024)
+        p = G.register("cc")
+        for b in range(nb):
+            s, d, n = host[b]
+            G.update_device(s, d, None, n)
+            G.execute(p)
+        lab = G.read(p)
+        G.close()
+        return lab
+
+    tp, lp = _timed(plain)
+    tb, lu = _timed(union)
+    same = True
+    for gi in range(min(4, len(specs))):
+        part = lu[bases[gi]:bases[gi] + (1 << specs[gi][0])].astype(np.int64) - bases[gi]
+        same = same and bool(np.array_equal(part, lp[gi].astype(np.int64)))
+    log("bench: %d graph tenants (%d edges): union %.1f ms, plain loop %.1f ms"
+        % (len(specs), e_total, tb * 1e3, tp * 1e3))
+    return {"queries": len(specs), "edges": e_total, "batched_ms": round(tb * 1e3, 2),
+            "plain_ms": round(tp * 1e3, 2), "speedup": round(tp / tb, 2),
+            "batched_qps": round(len(specs) / tb, 2), "plain_qps": round(len(specs) / tp, 2),
+            "sample_equal": same}
 
 
 def cpu_baseline(args, source):

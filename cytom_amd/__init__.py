@@ -416,6 +416,9 @@ class Graph:
                                                ctypes.byref(b)), "kernel_profile")
         return ms.value, n.value, b.value
 
+    def live_edge_count(self):
+        return self._check(lib().cytom_export_edges(self._h, None, None, None, 0), "export")
+
     def export_edges(self):
         n = self._check(lib().cytom_export_edges(self._h, None, None, None, 0), "export")
         s = np.empty(n, np.uint32)

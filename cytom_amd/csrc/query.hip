@@ -17,6 +17,11 @@
 //     personalized program with the same FP32 per-vertex arithmetic and
 //     exact fixed-point sums as the single-query program, so each query's
 //     ranks equal its single-query run.
+// Sparse frontiers push along out-edges (CSR); once a frontier's out-edges
+// exceed E / PULL_DIV the targets pull along in-edges (CSC) instead, which
+// needs no atomics for BFS and SSSP and keeps PPR sums in registers.
+// SSSP and PPR groups on graphs above VECTOR_MAX_EDGES run the tile-engine
+// program once per query instead (see run_seq_group).
 // The frontier's out-edges are split evenly over the wavefronts (a lane
 // finds its first edge by binary search over the frontier's degree prefix
 // sum), so hub vertices do not serialise the expansion.
@@ -35,9 +40,19 @@ struct QueryEngine {
   uint64_t* row_off = nullptr;  // V + 1
   uint32_t* col = nullptr;      // E
   uint32_t* wt = nullptr;       // E (weighted graphs)
+  uint64_t* in_off = nullptr;   // V + 1: in-edges by target (CSC)
+  uint32_t* in_src = nullptr;   // E
+  uint32_t* in_w = nullptr;     // E (weighted graphs)
+  uint32_t* heavy = nullptr;    // targets with more than HUB in-edges
+  uint32_t n_heavy = 0;
+  // in-edge segments of the heavy targets (at most SEG edges each)
+  uint32_t* seg_v = nullptr;
+  uint64_t* seg_b = nullptr;  // [2 * n_seg]: begin, end
+  uint32_t n_seg = 0;
   // frontier work space
   uint32_t* flist = nullptr;  // V
   uint32_t* nlist = nullptr;  // V
+  uint32_t* glist = nullptr;  // V
   uint64_t* fdeg = nullptr;   // V + 1
   uint64_t* fpre = nullptr;   // V + 1
   uint32_t* mark = nullptr;   // V
@@ -58,8 +73,18 @@ namespace {
 constexpr int QT = 256;          // threads per block
 constexpr uint32_t EPL = 8;      // edges per lane (scalar) or per lane group (vector)
 constexpr uint32_t BFS_W = 64;   // queries per BFS group (bits of a u64)
-constexpr uint32_t VEC_W = 16;   // queries per SSSP / PPR group
+constexpr uint32_t VEC_W = 16;   // queries per SSSP / PPR group (a power of two <= 32)
+constexpr uint32_t VEC_MASK = VEC_W - 1;
+constexpr uint32_t VEC_SLOTS = 64 / VEC_W;  // edges per wavefront step in the vector kernels
+constexpr uint64_t VEC_BITS = (1ull << VEC_W) - 1;
 constexpr uint16_t LEVEL_NONE = 0xFFFF;
+// In-degree above which a whole block pulls a vertex.
+constexpr uint64_t HUB = 256;
+// Heavy targets' in-edges are pulled in segments of this many edges, one
+// block each, combined with atomics.
+constexpr uint64_t SEG = 2048;
+// A frontier pulls (dense) once its out-edges exceed E / PULL_DIV.
+constexpr uint64_t PULL_DIV = 16;
 
 inline uint32_t blocks_for(uint64_t n, uint32_t per_block, uint32_t cap = 65536) {
   uint64_t b = (n + per_block - 1) / per_block;
@@ -92,6 +117,32 @@ __global__ void __launch_bounds__(QT) k_csr_fill(StoreView S, unsigned long long
       if (wt) wt[p] = S.wpool[off + i];
     }
   }
+}
+
+// Live edges into their target's CSC row.
+__global__ void __launch_bounds__(QT) k_csc_fill(StoreView S, unsigned long long* __restrict__ cursor,
+                                                 uint32_t* __restrict__ in_src,
+                                                 uint32_t* __restrict__ in_w) {
+  for (uint64_t t = blockIdx.x; t < S.n_tiles; t += gridDim.x) {
+    const uint32_t cnt = S.tile_cnt[t];
+    if (cnt == 0) continue;
+    const uint64_t off = S.tile_off[t];
+    const uint32_t xb = (uint32_t)(t % S.T) << S.log2vpt, yb = (uint32_t)(t / S.T) << S.log2vpt;
+    for (uint32_t i = threadIdx.x; i < cnt; i += QT) {
+      const uint32_t e = S.pool[off + i];
+      if (e == TOMB) continue;
+      const unsigned long long p = atomicAdd(&cursor[yb + (e >> 16)], 1ull);
+      in_src[p] = xb + (e & 0xFFFFu);
+      if (in_w) in_w[p] = S.wpool[off + i];
+    }
+  }
+}
+
+__global__ void k_heavy_list(const uint32_t* __restrict__ indeg, uint64_t V,
+                             uint32_t* __restrict__ list, unsigned int* __restrict__ counter) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    if (indeg[v] > HUB) list[atomicAdd(counter, 1u)] = (uint32_t)v;
 }
 
 // ------------------------------------------------------------- frontier
@@ -188,15 +239,15 @@ __global__ void __launch_bounds__(QT) k_sssp_expand(
     const uint32_t* __restrict__ wt, uint32_t* __restrict__ dist, uint32_t* __restrict__ mark,
     uint32_t* __restrict__ nlist, unsigned int* __restrict__ counter) {
   const uint64_t TE = fpre[n];
-  const uint32_t lane = threadIdx.x & 63, q = lane & 15, slot = lane >> 4;
+  const uint32_t lane = threadIdx.x & 63, q = lane & VEC_MASK, slot = lane / VEC_W;
   const uint64_t wave = ((uint64_t)blockIdx.x * QT + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * QT) >> 6;
-  for (uint64_t b = wave * 4 * EPL; b < TE; b += nwaves * 4 * EPL) {
+  for (uint64_t b = wave * VEC_SLOTS * EPL; b < TE; b += nwaves * VEC_SLOTS * EPL) {
     uint64_t e = b + slot;
-    const uint64_t end = min(b + 4 * EPL, TE);
+    const uint64_t end = min(b + VEC_SLOTS * EPL, TE);
     if (e >= end) continue;
     uint32_t i = find_entry(fpre, n, e);
-    for (; e < end; e += 4) {
+    for (; e < end; e += VEC_SLOTS) {
       while (fpre[i + 1] <= e) ++i;
       const uint32_t u = flist[i];
       const uint64_t p = row_off[u] + (e - fpre[i]);
@@ -210,7 +261,7 @@ __global__ void __launch_bounds__(QT) k_sssp_expand(
       }
       // One lane of the edge's 16 queues the target when any query improved.
       const uint64_t any = __ballot(improved);
-      if (q == 0 && ((any >> (slot * 16)) & 0xFFFFull)) {
+      if (q == 0 && ((any >> (slot * VEC_W)) & VEC_BITS)) {
         if (atomicCAS(&mark[v], 0u, 1u) == 0u) nlist[atomicAdd(counter, 1u)] = v;
       }
     }
@@ -235,102 +286,314 @@ __global__ void k_sssp_seed(const uint32_t* __restrict__ src, uint32_t nq,
 }
 
 // ------------------------------------------------------------------- PPR
-// pagerank-delta.cc:127-132 push form: every active (vertex, query) adds
-// round(delta / out_degree * 2^S) to each out-neighbour's fixed-point sum.
+// Frontier values of a group: contrib[v][q] = delta/out_degree while query
+// q is active at v, else 0 (the single-query program's contribution array),
+// as floats; each edge adds round(contrib * 2^S) to its target's exact
+// fixed-point sum (pagerank-delta.cc:127-132).
+__device__ inline long long ppr_fixed(float c, float qscale) {
+  return c != 0.f ? __float2ll_rn(c * qscale) : 0ll;
+}
+
 __global__ void __launch_bounds__(QT) k_ppr_expand(
     const uint32_t* __restrict__ flist, const uint64_t* __restrict__ fpre, uint32_t n,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ col,
-    const uint32_t* __restrict__ outdeg, const uint16_t* __restrict__ act,
-    const float* __restrict__ delta, long long* __restrict__ dsum, uint32_t* __restrict__ mark,
+    const float* __restrict__ contrib, long long* __restrict__ dsum, uint32_t* __restrict__ mark,
     uint32_t* __restrict__ nlist, unsigned int* __restrict__ counter, float qscale) {
   const uint64_t TE = fpre[n];
-  const uint32_t lane = threadIdx.x & 63, q = lane & 15, slot = lane >> 4;
+  const uint32_t lane = threadIdx.x & 63, q = lane & VEC_MASK, slot = lane / VEC_W;
   const uint64_t wave = ((uint64_t)blockIdx.x * QT + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * QT) >> 6;
-  for (uint64_t b = wave * 4 * EPL; b < TE; b += nwaves * 4 * EPL) {
+  for (uint64_t b = wave * VEC_SLOTS * EPL; b < TE; b += nwaves * VEC_SLOTS * EPL) {
     uint64_t e = b + slot;
-    const uint64_t end = min(b + 4 * EPL, TE);
+    const uint64_t end = min(b + VEC_SLOTS * EPL, TE);
     if (e >= end) continue;
     uint32_t i = find_entry(fpre, n, e);
-    for (; e < end; e += 4) {
+    for (; e < end; e += VEC_SLOTS) {
       while (fpre[i + 1] <= e) ++i;
       const uint32_t u = flist[i];
       const uint32_t v = col[row_off[u] + (e - fpre[i])];
-      bool pushed = false;
-      if ((act[u] >> q) & 1u) {
-        const long long c =
-            __float2ll_rn(delta[(uint64_t)u * VEC_W + q] / (float)outdeg[u] * qscale);
-        if (c != 0) {
-          atomicAdd(reinterpret_cast<unsigned long long*>(&dsum[(uint64_t)v * VEC_W + q]),
-                    (unsigned long long)c);
-          pushed = true;
-        }
-      }
-      const uint64_t any = __ballot(pushed);
-      if (q == 0 && ((any >> (slot * 16)) & 0xFFFFull)) {
+      const long long c = ppr_fixed(contrib[(uint64_t)u * VEC_W + q], qscale);
+      if (c != 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(&dsum[(uint64_t)v * VEC_W + q]),
+                  (unsigned long long)c);
+      const uint64_t any = __ballot(c != 0);
+      if (q == 0 && ((any >> (slot * VEC_W)) & VEC_BITS)) {
         if (atomicCAS(&mark[v], 0u, 1u) == 0u) nlist[atomicAdd(counter, 1u)] = v;
       }
     }
   }
 }
 
-// pagerank-delta.cc:142-174 (not the first-iteration rule): applied only
-// where a sum arrived, active iff |delta| > threshold * rank.
+// pagerank-delta.cc:142-174 (not the first-iteration rule) for one
+// (vertex, query) whose fixed-point sum is s_fx: applied only where a sum
+// arrived; writes the next contribution and returns whether the query stays
+// active at the vertex.
+__device__ inline bool ppr_apply_one(uint64_t k, long long s_fx, uint32_t od, double thr,
+                                     double qinv, float* rank, float* contrib_next) {
+  if (s_fx == 0) return false;
+  const float s = (float)((double)s_fx * qinv);
+  const float d = static_cast<float>(0.85 * (double)s);
+  const float r = rank[k] + d;
+  rank[k] = r;
+  const bool a = (double)fabsf(d) > thr * (double)r;
+  if (a) contrib_next[k] = od ? d / (float)od : 0.f;
+  return a;
+}
+
+// Apply over the vertices the push reached; the active ones form the next
+// frontier.  contrib_next was cleared for the whole group before.
 __global__ void k_ppr_apply(const uint32_t* __restrict__ nlist, uint32_t n, double thr, double qinv,
-                            float* __restrict__ rank, float* __restrict__ delta,
-                            long long* __restrict__ dsum, uint16_t* __restrict__ act,
-                            uint32_t* __restrict__ mark) {
-  const uint32_t q = threadIdx.x & 15;
+                            const uint32_t* __restrict__ outdeg, float* __restrict__ rank,
+                            float* __restrict__ contrib_next, long long* __restrict__ dsum,
+                            uint32_t* __restrict__ mark, uint32_t* __restrict__ flist,
+                            unsigned int* __restrict__ counter) {
+  const uint32_t q = threadIdx.x & VEC_MASK;
   const uint64_t nthr = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)n * VEC_W;
        t += nthr) {
     const uint32_t v = nlist[t / VEC_W];
     const uint64_t k = (uint64_t)v * VEC_W + q;
     const long long s_fx = dsum[k];
-    bool a = false;
-    if (s_fx != 0) {
-      const float s = (float)((double)s_fx * qinv);
-      const float d = static_cast<float>(0.85 * (double)s);
-      const float r = rank[k] + d;
-      rank[k] = r;
-      delta[k] = d;
-      dsum[k] = 0;
-      a = (double)fabsf(d) > thr * (double)r;
-    }
-    // 16 consecutive lanes hold one vertex: gather its active bits.
-    const uint64_t bits = __ballot(a);
+    dsum[k] = 0;
+    const bool a = ppr_apply_one(k, s_fx, outdeg[v], thr, qinv, rank, contrib_next);
+    // 16 consecutive lanes hold one vertex.
+    const uint64_t bits = (__ballot(a) >> (threadIdx.x & (63 & ~VEC_MASK))) & VEC_BITS;
     if (q == 0) {
-      const uint32_t sh = threadIdx.x & 48;
-      act[v] = (uint16_t)((bits >> sh) & 0xFFFFull);
       mark[v] = 0;
+      if (bits) flist[atomicAdd(counter, 1u)] = v;
     }
   }
 }
 
-__global__ void k_clear_act(const uint32_t* __restrict__ flist, uint32_t n, uint16_t* __restrict__ act) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    act[flist[i]] = 0;
-}
+// Pull form for dense frontiers: one 16-lane group per target sums its
+// in-edges' contributions in registers (exact, so equal to the push sums),
+// eight in-edges per step so their loads overlap, and applies; targets
+// above HUB in-edges are left to k_ppr_pull_heavy.
+constexpr int PULL_ILP = 8;
 
-// Keep the reached vertices that have an active query.
-__global__ void k_keep_active(const uint32_t* __restrict__ nlist, uint32_t n,
-                              const uint16_t* __restrict__ act, uint32_t* __restrict__ flist,
-                              unsigned int* __restrict__ counter) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t v = nlist[i];
-    if (act[v]) flist[atomicAdd(counter, 1u)] = v;
+__global__ void __launch_bounds__(QT) k_ppr_pull(
+    const uint64_t* __restrict__ in_off, const uint32_t* __restrict__ in_src, uint64_t V,
+    const uint32_t* __restrict__ outdeg, const float* __restrict__ contrib,
+    float* __restrict__ contrib_next, float* __restrict__ rank, double thr, double qinv,
+    float qscale, uint32_t* __restrict__ flist, unsigned int* __restrict__ counter) {
+  const uint32_t q = threadIdx.x & VEC_MASK;
+  const uint64_t ng = ((uint64_t)gridDim.x * blockDim.x) / VEC_W;
+  for (uint64_t v = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / VEC_W; v < V; v += ng) {
+    const uint64_t e0 = in_off[v], e1 = in_off[v + 1];
+    if (e1 - e0 > HUB) continue;  // uniform per group
+    long long acc = 0;
+    for (uint64_t e = e0; e < e1; e += PULL_ILP) {
+      uint32_t u[PULL_ILP];
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k) u[k] = in_src[min(e + k, e1 - 1)];
+      float c[PULL_ILP];
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k) c[k] = contrib[(uint64_t)u[k] * VEC_W + q];
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k)
+        if (e + k < e1) acc += ppr_fixed(c[k], qscale);
+    }
+    const bool a = ppr_apply_one(v * VEC_W + q, acc, outdeg[v], thr, qinv, rank, contrib_next);
+    const uint64_t bits = (__ballot(a) >> (threadIdx.x & (63 & ~VEC_MASK))) & VEC_BITS;
+    if (q == 0 && bits) flist[atomicAdd(counter, 1u)] = (uint32_t)v;
   }
 }
 
-// Teleport sets: rank = delta = (1-alpha)/|T| (|T| distinct), active.
+__global__ void k_clear_contrib(const uint32_t* __restrict__ list, uint32_t n,
+                                float* __restrict__ contrib) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)n * VEC_W;
+       t += (uint64_t)gridDim.x * blockDim.x)
+    contrib[(uint64_t)list[t / VEC_W] * VEC_W + (t & VEC_MASK)] = 0.f;
+}
+
+// Teleport sets: rank = delta = (1-alpha)/|T| (|T| distinct), active, so
+// contrib = delta / out_degree.
 __global__ void k_ppr_seed(const uint32_t* __restrict__ members, const uint32_t* __restrict__ qof,
-                           const float* __restrict__ r0, uint32_t n, float* __restrict__ rank,
-                           float* __restrict__ delta, uint16_t* __restrict__ act) {
+                           const float* __restrict__ r0, uint32_t n,
+                           const uint32_t* __restrict__ outdeg, float* __restrict__ rank,
+                           float* __restrict__ contrib) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t v = members[i], q = qof[i];
     rank[(uint64_t)v * VEC_W + q] = r0[q];
-    delta[(uint64_t)v * VEC_W + q] = r0[q];
-    atomicOr(reinterpret_cast<unsigned int*>(act + (v & ~1u)), (1u << q) << ((v & 1u) * 16));
+    const uint32_t od = outdeg[v];
+    contrib[(uint64_t)v * VEC_W + q] = od ? r0[q] / (float)od : 0.f;
+  }
+}
+
+// ------------------------------------------------------------ pull forms
+// BFS: one lane per target ORs its in-neighbours' frontier masks, eight at
+// a time, and stops once every query still missing the target reached it.
+__global__ void k_bfs_pull(const uint64_t* __restrict__ in_off, const uint32_t* __restrict__ in_src,
+                           uint64_t V, uint64_t qmask, const uint64_t* __restrict__ front,
+                           const uint64_t* __restrict__ vis, uint64_t* __restrict__ nxt,
+                           uint32_t* __restrict__ nlist, unsigned int* __restrict__ counter) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t e0 = in_off[v], e1 = in_off[v + 1];
+    if (e1 - e0 > HUB) continue;
+    const uint64_t want = qmask & ~vis[v];
+    if (!want) continue;
+    uint64_t m = 0;
+    for (uint64_t e = e0; e < e1 && (m & want) != want; e += PULL_ILP) {
+      uint32_t u[PULL_ILP];
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k) u[k] = in_src[min(e + k, e1 - 1)];
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k) m |= front[u[k]];  // clamped repeats add nothing
+    }
+    m &= want;
+    if (m) {
+      nxt[v] = m;
+      nlist[atomicAdd(counter, 1u)] = (uint32_t)v;
+    }
+  }
+}
+
+// SSSP: one 16-lane group per target takes the minimum over its in-edges
+// (sssp.cc:90-99), eight at a time, and lowers its distances in place.
+__global__ void __launch_bounds__(QT) k_sssp_pull(
+    const uint64_t* __restrict__ in_off, const uint32_t* __restrict__ in_src,
+    const uint32_t* __restrict__ in_w, uint64_t V, uint32_t* __restrict__ dist,
+    uint32_t* __restrict__ nlist, unsigned int* __restrict__ counter) {
+  const uint32_t q = threadIdx.x & VEC_MASK;
+  const uint64_t ng = ((uint64_t)gridDim.x * blockDim.x) / VEC_W;
+  for (uint64_t v = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / VEC_W; v < V; v += ng) {
+    const uint64_t e0 = in_off[v], e1 = in_off[v + 1];
+    if (e1 - e0 > HUB) continue;
+    uint32_t best = INF;
+    for (uint64_t e = e0; e < e1; e += PULL_ILP) {
+      uint32_t u[PULL_ILP], w[PULL_ILP];
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k) {
+        const uint64_t x = min(e + k, e1 - 1);
+        u[k] = in_src[x];
+        w[k] = in_w[x];
+      }
+      uint32_t du[PULL_ILP];
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k) du[k] = dist[(uint64_t)u[k] * VEC_W + q];
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k)
+        if (du[k] != INF) best = min(best, sat_add(du[k], w[k]));  // clamped repeats are harmless
+    }
+    uint32_t* dv = &dist[v * VEC_W + q];
+    const bool improved = best < *dv;
+    if (improved) *dv = best;
+    const uint64_t bits = (__ballot(improved) >> (threadIdx.x & (63 & ~VEC_MASK))) & VEC_BITS;
+    if (q == 0 && bits) nlist[atomicAdd(counter, 1u)] = (uint32_t)v;
+  }
+}
+
+// Heavy targets: one block per in-edge segment (16 edge lanes x 16 queries
+// for SSSP / PPR, 256 edge lanes for BFS), reduced in LDS, then combined
+// into the target with one atomic per query.
+__global__ void __launch_bounds__(QT) k_bfs_pull_seg(
+    const uint32_t* __restrict__ seg_v, const uint64_t* __restrict__ seg_b, uint32_t n_seg,
+    const uint32_t* __restrict__ in_src, uint64_t qmask, const uint64_t* __restrict__ front,
+    const uint64_t* __restrict__ vis, unsigned long long* __restrict__ nxt,
+    uint32_t* __restrict__ nlist, unsigned int* __restrict__ counter) {
+  __shared__ unsigned long long m_s;
+  for (uint32_t sg = blockIdx.x; sg < n_seg; sg += gridDim.x) {
+    const uint32_t v = seg_v[sg];
+    const uint64_t want = qmask & ~vis[v];
+    if (!want) continue;  // uniform
+    if (threadIdx.x == 0) m_s = 0;
+    __syncthreads();
+    const uint64_t e0 = seg_b[2 * sg], e1 = seg_b[2 * sg + 1];
+    uint64_t m = 0;
+    for (uint64_t e = e0 + threadIdx.x; e < e1; e += QT * 4) {
+      uint32_t u[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) u[k] = in_src[min(e + k * QT, e1 - 1)];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) m |= front[u[k]];
+    }
+    m &= want;
+    if (m) atomicOr(&m_s, (unsigned long long)m);
+    __syncthreads();
+    if (threadIdx.x == 0 && m_s) {
+      if (atomicOr(&nxt[v], m_s) == 0ull) nlist[atomicAdd(counter, 1u)] = v;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(QT) k_sssp_pull_seg(
+    const uint32_t* __restrict__ seg_v, const uint64_t* __restrict__ seg_b, uint32_t n_seg,
+    const uint32_t* __restrict__ in_src, const uint32_t* __restrict__ in_w,
+    uint32_t* __restrict__ dist, uint32_t* __restrict__ mark, uint32_t* __restrict__ nlist,
+    unsigned int* __restrict__ counter) {
+  __shared__ uint32_t best_s[VEC_W];
+  __shared__ uint32_t any_s;
+  const uint32_t q = threadIdx.x & VEC_MASK, el = threadIdx.x / VEC_W;
+  constexpr uint32_t LANES = QT / VEC_W;
+  for (uint32_t sg = blockIdx.x; sg < n_seg; sg += gridDim.x) {
+    const uint32_t v = seg_v[sg];
+    if (threadIdx.x < VEC_W) best_s[threadIdx.x] = INF;
+    if (threadIdx.x == 0) any_s = 0;
+    __syncthreads();
+    const uint64_t e0 = seg_b[2 * sg], e1 = seg_b[2 * sg + 1];
+    uint32_t best = INF;
+    for (uint64_t e = e0 + el; e < e1; e += LANES * 4) {
+      uint32_t u[4], w[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint64_t x = min(e + k * LANES, e1 - 1);
+        u[k] = in_src[x];
+        w[k] = in_w[x];
+      }
+      uint32_t du[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) du[k] = dist[(uint64_t)u[k] * VEC_W + q];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (du[k] != INF) best = min(best, sat_add(du[k], w[k]));
+    }
+    if (best != INF) atomicMin(&best_s[q], best);
+    __syncthreads();
+    if (threadIdx.x < VEC_W) {
+      const uint32_t b = best_s[threadIdx.x];
+      if (b != INF && b < atomicMin(&dist[(uint64_t)v * VEC_W + threadIdx.x], b)) any_s = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && any_s && atomicCAS(&mark[v], 0u, 1u) == 0u)
+      nlist[atomicAdd(counter, 1u)] = v;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(QT) k_ppr_pull_seg(
+    const uint32_t* __restrict__ seg_v, const uint64_t* __restrict__ seg_b, uint32_t n_seg,
+    const uint32_t* __restrict__ in_src, const float* __restrict__ contrib,
+    long long* __restrict__ dsum, float qscale) {
+  __shared__ long long part[QT];
+  const uint32_t q = threadIdx.x & VEC_MASK, el = threadIdx.x / VEC_W;
+  constexpr uint32_t LANES = QT / VEC_W;
+  for (uint32_t sg = blockIdx.x; sg < n_seg; sg += gridDim.x) {
+    const uint32_t v = seg_v[sg];
+    const uint64_t e0 = seg_b[2 * sg], e1 = seg_b[2 * sg + 1];
+    long long acc = 0;
+    for (uint64_t e = e0 + el; e < e1; e += LANES * 4) {
+      uint32_t u[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) u[k] = in_src[min(e + k * LANES, e1 - 1)];
+      float c[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) c[k] = contrib[(uint64_t)u[k] * VEC_W + q];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (e + k * LANES < e1) acc += ppr_fixed(c[k], qscale);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < VEC_W) {
+      long long s_fx = 0;
+      for (uint32_t k = 0; k < LANES; ++k) s_fx += part[k * VEC_W + q];
+      if (s_fx != 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(&dsum[(uint64_t)v * VEC_W + q]),
+                  (unsigned long long)s_fx);
+    }
+    __syncthreads();
   }
 }
 
@@ -374,12 +637,18 @@ void free_results(QueryEngine* qe) {
 }
 
 void free_snapshot(QueryEngine* qe) {
-  void* ps[] = {qe->row_off, qe->col, qe->wt, qe->flist, qe->nlist, qe->fdeg,
-                qe->fpre,    qe->mark, qe->counter, qe->scan_tmp};
+  void* ps[] = {qe->row_off, qe->col,     qe->wt,   qe->in_off,  qe->in_src,  qe->in_w,
+                qe->heavy,   qe->flist,   qe->nlist, qe->glist, qe->fdeg,   qe->fpre,    qe->mark,
+                qe->counter, qe->scan_tmp, qe->seg_v, qe->seg_b};
   for (void* p : ps)
     if (p) (void)hipFree(p);
-  qe->row_off = nullptr;
-  qe->col = qe->wt = qe->flist = qe->nlist = nullptr;
+  qe->row_off = qe->in_off = nullptr;
+  qe->col = qe->wt = qe->in_src = qe->in_w = qe->heavy = qe->flist = qe->nlist = nullptr;
+  qe->glist = nullptr;
+  qe->n_heavy = 0;
+  qe->seg_v = nullptr;
+  qe->seg_b = nullptr;
+  qe->n_seg = 0;
   qe->fdeg = qe->fpre = nullptr;
   qe->mark = nullptr;
   qe->counter = nullptr;
@@ -396,7 +665,11 @@ int snapshot(cytom_ctx* ctx, QueryEngine* qe) {
   free_snapshot(qe);
   if ((rc = dev_alloc(ctx, &qe->row_off, (V + 1) * 8)) || (rc = dev_alloc(ctx, &qe->col, E * 4)) ||
       (ctx->wpool && (rc = dev_alloc(ctx, &qe->wt, E * 4))) ||
+      (rc = dev_alloc(ctx, &qe->in_off, (V + 1) * 8)) || (rc = dev_alloc(ctx, &qe->in_src, E * 4)) ||
+      (ctx->wpool && (rc = dev_alloc(ctx, &qe->in_w, E * 4))) ||
+      (rc = dev_alloc(ctx, &qe->heavy, V * 4)) ||
       (rc = dev_alloc(ctx, &qe->flist, V * 4)) || (rc = dev_alloc(ctx, &qe->nlist, V * 4)) ||
+      (rc = dev_alloc(ctx, &qe->glist, V * 4)) ||
       (rc = dev_alloc(ctx, &qe->fdeg, (V + 1) * 8)) ||
       (rc = dev_alloc(ctx, &qe->fpre, (V + 1) * 8)) || (rc = dev_alloc(ctx, &qe->mark, V * 4)) ||
       (rc = dev_alloc(ctx, &qe->counter, 256)) ||
@@ -408,11 +681,48 @@ int snapshot(cytom_ctx* ctx, QueryEngine* qe) {
   CY_HIP(ctx, hipMemcpyAsync(qe->fdeg, qe->row_off, V * 8, hipMemcpyDeviceToDevice, st));
   k_csr_fill<<<blocks_for(ctx->n_tiles, 1, 16384), QT, 0, st>>>(
       ctx->view(), reinterpret_cast<unsigned long long*>(qe->fdeg), qe->col, qe->wt);
-  uint64_t total = 0;
+  // in_off from in-degrees the same way.
+  k_widen<<<blocks_for(V, 256), 256, 0, st>>>(ctx->indeg, qe->fdeg, V);
+  scan_exclusive_u64(qe->fdeg, qe->in_off, V, qe->scan_tmp, st);
+  CY_HIP(ctx, hipMemcpyAsync(qe->fdeg, qe->in_off, V * 8, hipMemcpyDeviceToDevice, st));
+  k_csc_fill<<<blocks_for(ctx->n_tiles, 1, 16384), QT, 0, st>>>(
+      ctx->view(), reinterpret_cast<unsigned long long*>(qe->fdeg), qe->in_src, qe->in_w);
+  CY_HIP(ctx, hipMemsetAsync(qe->counter, 0, 4, st));
+  k_heavy_list<<<blocks_for(V, 256, 16384), 256, 0, st>>>(ctx->indeg, V, qe->heavy, qe->counter);
+  uint64_t total = 0, total_in = 0;
   CY_HIP(ctx, hipMemcpyAsync(&total, qe->row_off + V, 8, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipMemcpyAsync(&total_in, qe->in_off + V, 8, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipMemcpyAsync(&qe->n_heavy, qe->counter, 4, hipMemcpyDeviceToHost, st));
   CY_HIP(ctx, hipMemsetAsync(qe->mark, 0, V * 4, st));
   CY_HIP(ctx, hipStreamSynchronize(st));
-  if (total != E) return set_err(ctx, CYTOM_EDEVICE, "degree sum differs from live edges");
+  if (total != E || total_in != E)
+    return set_err(ctx, CYTOM_EDEVICE, "degree sums differ from live edges");
+  // Segments of the heavy targets' in-edge ranges.
+  std::vector<uint32_t> hv(qe->n_heavy), sv;
+  std::vector<uint64_t> sb;
+  if (qe->n_heavy) {
+    CY_HIP(ctx, hipMemcpy(hv.data(), qe->heavy, hv.size() * 4, hipMemcpyDeviceToHost));
+    std::sort(hv.begin(), hv.end());
+    std::vector<uint64_t> off(V + 1);
+    CY_HIP(ctx, hipMemcpy(off.data(), qe->in_off, (V + 1) * 8, hipMemcpyDeviceToHost));
+    for (uint32_t v : hv) {
+      const uint64_t r[2] = {off[v], off[v + 1]};
+      for (uint64_t b = r[0]; b < r[1]; b += SEG) {
+        sv.push_back(v);
+        sb.push_back(b);
+        sb.push_back(std::min(b + SEG, r[1]));
+      }
+    }
+    CY_HIP(ctx, hipMemcpy(qe->heavy, hv.data(), hv.size() * 4, hipMemcpyHostToDevice));
+  }
+  qe->n_seg = (uint32_t)sv.size();
+  if ((rc = dev_alloc(ctx, &qe->seg_v, sv.size() * 4)) ||
+      (rc = dev_alloc(ctx, &qe->seg_b, sb.size() * 8)))
+    return rc;
+  if (!sv.empty()) {
+    CY_HIP(ctx, hipMemcpy(qe->seg_v, sv.data(), sv.size() * 4, hipMemcpyHostToDevice));
+    CY_HIP(ctx, hipMemcpy(qe->seg_b, sb.data(), sb.size() * 8, hipMemcpyHostToDevice));
+  }
   qe->E = E;
   qe->snap_batch = ctx->batches;
   return 0;
@@ -472,6 +782,7 @@ int run_bfs_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<uint32_t>& 
       hipMemcpyAsync(dsrc, src.data(), src.size() * 4, hipMemcpyHostToDevice, st))
     return done(set_err(ctx, CYTOM_EDEVICE, "BFS group set-up failed"));
   k_bfs_seed<<<1, 64, 0, st>>>(dsrc, (uint32_t)src.size(), vis, front, lev);
+  const uint64_t qmask = src.size() >= 64 ? ~0ull : ((1ull << src.size()) - 1);
   uint32_t n = 0;
   if ((rc = seed_frontier(ctx, qe, src, &n))) return done(rc);
   uint32_t level = 0;
@@ -483,10 +794,19 @@ int run_bfs_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<uint32_t>& 
     acc->traversed_edges += te;
     acc->active_vertices += n;
     CY_HIP(ctx, hipMemsetAsync(qe->counter, 0, 4, st));
-    if (te)
+    if (te * PULL_DIV > qe->E) {
+      k_bfs_pull<<<blocks_for(V, 256, 16384), 256, 0, st>>>(qe->in_off, qe->in_src, V, qmask,
+                                                             front, vis, nxt, qe->nlist,
+                                                             qe->counter);
+      if (qe->n_seg)
+        k_bfs_pull_seg<<<std::min<uint32_t>(qe->n_seg, 16384), QT, 0, st>>>(
+            qe->seg_v, qe->seg_b, qe->n_seg, qe->in_src, qmask, front, vis,
+            reinterpret_cast<unsigned long long*>(nxt), qe->nlist, qe->counter);
+    } else if (te) {
       k_bfs_expand<<<expand_blocks(ctx, te, 64 * EPL), QT, 0, st>>>(
           qe->flist, qe->fpre, n, qe->row_off, qe->col, front, vis,
           reinterpret_cast<unsigned long long*>(nxt), qe->nlist, qe->counter);
+    }
     k_clear_front<<<blocks_for(n, 256), 256, 0, st>>>(qe->flist, n, front);
     uint32_t m = 0;
     if ((rc = read_counter(ctx, qe, &m))) return done(rc);
@@ -524,10 +844,19 @@ int run_sssp_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<uint32_t>&
     acc->traversed_edges += te;
     acc->active_vertices += n;
     CY_HIP(ctx, hipMemsetAsync(qe->counter, 0, 4, st));
-    if (te)
-      k_sssp_expand<<<expand_blocks(ctx, te, 4 * EPL), QT, 0, st>>>(
+    const bool pull = te * PULL_DIV > qe->E;
+    if (pull) {
+      k_sssp_pull<<<blocks_for(V * VEC_W, 256, 16384), 256, 0, st>>>(
+          qe->in_off, qe->in_src, qe->in_w, V, dist, qe->nlist, qe->counter);
+      if (qe->n_seg)
+        k_sssp_pull_seg<<<std::min<uint32_t>(qe->n_seg, 16384), QT, 0, st>>>(
+            qe->seg_v, qe->seg_b, qe->n_seg, qe->in_src, qe->in_w, dist, qe->mark, qe->nlist,
+            qe->counter);
+    } else if (te) {
+      k_sssp_expand<<<expand_blocks(ctx, te, VEC_SLOTS * EPL), QT, 0, st>>>(
           qe->flist, qe->fpre, n, qe->row_off, qe->col, qe->wt, dist, qe->mark, qe->nlist,
           qe->counter);
+    }
     uint32_t m = 0;
     if ((rc = read_counter(ctx, qe, &m))) return done(rc);
     if (m) k_unmark<<<blocks_for(m, 256), 256, 0, st>>>(qe->nlist, m, qe->mark);
@@ -543,9 +872,8 @@ int run_ppr_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<std::vector
   hipStream_t st = ctx->stream;
   const uint64_t V = ctx->V;
   int rc;
-  float* delta = nullptr;
+  float* contrib[2] = {nullptr, nullptr};
   long long* dsum = nullptr;
-  uint16_t* act = nullptr;
   uint32_t *members = nullptr, *qof = nullptr;
   float* r0 = nullptr;
   std::vector<uint32_t> hm, hq, all;
@@ -561,59 +889,115 @@ int run_ppr_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<std::vector
       all.push_back(v);
     }
   }
-  if ((rc = dev_alloc(ctx, &delta, V * VEC_W * 4)) || (rc = dev_alloc(ctx, &dsum, V * VEC_W * 8)) ||
-      (rc = dev_alloc(ctx, &act, V * 2 + 4)) || (rc = dev_alloc(ctx, &members, hm.size() * 4)) ||
-      (rc = dev_alloc(ctx, &qof, hq.size() * 4)) || (rc = dev_alloc(ctx, &r0, VEC_W * 4))) {
-    (void)hipFree(delta), (void)hipFree(dsum), (void)hipFree(act);
+  auto release = [&]() {
+    (void)hipFree(contrib[0]), (void)hipFree(contrib[1]), (void)hipFree(dsum);
     (void)hipFree(members), (void)hipFree(qof), (void)hipFree(r0);
+  };
+  const size_t vec_bytes = V * VEC_W * 4;
+  if ((rc = dev_alloc(ctx, &contrib[0], vec_bytes)) || (rc = dev_alloc(ctx, &contrib[1], vec_bytes)) ||
+      (rc = dev_alloc(ctx, &dsum, V * VEC_W * 8)) ||
+      (rc = dev_alloc(ctx, &members, hm.size() * 4)) || (rc = dev_alloc(ctx, &qof, hq.size() * 4)) ||
+      (rc = dev_alloc(ctx, &r0, VEC_W * 4))) {
+    release();
     return rc;
   }
   auto done = [&](int r) {
     (void)hipStreamSynchronize(st);
-    (void)hipFree(delta), (void)hipFree(dsum), (void)hipFree(act);
-    (void)hipFree(members), (void)hipFree(qof), (void)hipFree(r0);
+    release();
     return r;
   };
-  if (hipMemsetAsync(rank, 0, V * VEC_W * 4, st) || hipMemsetAsync(delta, 0, V * VEC_W * 4, st) ||
-      hipMemsetAsync(dsum, 0, V * VEC_W * 8, st) || hipMemsetAsync(act, 0, V * 2 + 4, st) ||
+  if (hipMemsetAsync(rank, 0, vec_bytes, st) || hipMemsetAsync(contrib[0], 0, vec_bytes, st) ||
+      hipMemsetAsync(contrib[1], 0, vec_bytes, st) || hipMemsetAsync(dsum, 0, V * VEC_W * 8, st) ||
       hipMemcpyAsync(members, hm.data(), hm.size() * 4, hipMemcpyHostToDevice, st) ||
       hipMemcpyAsync(qof, hq.data(), hq.size() * 4, hipMemcpyHostToDevice, st) ||
       hipMemcpyAsync(r0, hr0.data(), VEC_W * 4, hipMemcpyHostToDevice, st))
     return done(set_err(ctx, CYTOM_EDEVICE, "PPR group set-up failed"));
   k_ppr_seed<<<blocks_for(hm.size(), 256), 256, 0, st>>>(members, qof, r0, (uint32_t)hm.size(),
-                                                         rank, delta, act);
+                                                         ctx->outdeg, rank, contrib[0]);
   uint32_t n = 0;
   if ((rc = seed_frontier(ctx, qe, all, &n))) return done(rc);
-  uint32_t* mark = qe->mark;
   const float qscale = ldexpf(1.f, ctx->prd_shift);
   const double qinv = ldexp(1.0, -ctx->prd_shift);
   uint32_t it = 0;
+  int c = 0;  // contrib[c]: the frontier's contributions; contrib[c ^ 1] is all zero
   while (n > 0 && it < ctx->max_iter) {
     uint64_t te = 0;
     if ((rc = frontier_prefix(ctx, qe, n, &te))) return done(rc);
     acc->traversed_edges += te;
     acc->active_vertices += n;
     CY_HIP(ctx, hipMemsetAsync(qe->counter, 0, 4, st));
-    if (te)
-      k_ppr_expand<<<expand_blocks(ctx, te, 4 * EPL), QT, 0, st>>>(
-          qe->flist, qe->fpre, n, qe->row_off, qe->col, ctx->outdeg, act, delta, dsum, mark,
-          qe->nlist, qe->counter, qscale);
-    k_clear_act<<<blocks_for(n, 256), 256, 0, st>>>(qe->flist, n, act);
-    uint32_t m = 0;
-    if ((rc = read_counter(ctx, qe, &m))) return done(rc);
-    CY_HIP(ctx, hipMemsetAsync(qe->counter, 0, 4, st));
-    if (m) {
-      k_ppr_apply<<<blocks_for((uint64_t)m * VEC_W, 256), 256, 0, st>>>(
-          qe->nlist, m, ctx->delta, qinv, rank, delta, dsum, act, mark);
-      k_keep_active<<<blocks_for(m, 256), 256, 0, st>>>(qe->nlist, m, act, qe->flist,
-                                                        qe->counter);
+    uint32_t next = 0;
+    if (te * PULL_DIV > qe->E) {
+      k_ppr_pull<<<blocks_for(V * VEC_W, 256, 16384), 256, 0, st>>>(
+          qe->in_off, qe->in_src, V, ctx->outdeg, contrib[c], contrib[c ^ 1], rank, ctx->delta,
+          qinv, qscale, qe->glist, qe->counter);
+      if (qe->n_seg) {
+        k_ppr_pull_seg<<<std::min<uint32_t>(qe->n_seg, 16384), QT, 0, st>>>(
+            qe->seg_v, qe->seg_b, qe->n_seg, qe->in_src, contrib[c], dsum, qscale);
+        k_ppr_apply<<<blocks_for((uint64_t)qe->n_heavy * VEC_W, 256), 256, 0, st>>>(
+            qe->heavy, qe->n_heavy, ctx->delta, qinv, ctx->outdeg, rank, contrib[c ^ 1], dsum,
+            qe->mark, qe->glist, qe->counter);
+      }
+      if ((rc = read_counter(ctx, qe, &next))) return done(rc);
+    } else {
+      if (te)
+        k_ppr_expand<<<expand_blocks(ctx, te, VEC_SLOTS * EPL), QT, 0, st>>>(
+            qe->flist, qe->fpre, n, qe->row_off, qe->col, contrib[c], dsum, qe->mark, qe->nlist,
+            qe->counter, qscale);
+      uint32_t m = 0;
+      if ((rc = read_counter(ctx, qe, &m))) return done(rc);
+      CY_HIP(ctx, hipMemsetAsync(qe->counter, 0, 4, st));
+      if (m)
+        k_ppr_apply<<<blocks_for((uint64_t)m * VEC_W, 256), 256, 0, st>>>(
+            qe->nlist, m, ctx->delta, qinv, ctx->outdeg, rank, contrib[c ^ 1], dsum, qe->mark,
+            qe->glist, qe->counter);
+      if ((rc = read_counter(ctx, qe, &next))) return done(rc);
     }
-    if ((rc = read_counter(ctx, qe, &n))) return done(rc);
+    // The frontier's contributions are spent: zero them for reuse.
+    k_clear_contrib<<<blocks_for((uint64_t)n * VEC_W, 256), 256, 0, st>>>(qe->flist, n, contrib[c]);
+    std::swap(qe->flist, qe->glist);
+    n = next;
+    c ^= 1;
     ++it;
     acc->iterations += 1;
   }
   if (n > 0) acc->converged = 0;  // stopped at the iteration cap
   return done(0);
+}
+
+// SSSP and PPR on large graphs: the vector groups trade their shared edge
+// reads for 16x wider random vertex-state reads, which loses to the tile
+// engine's LDS-windowed gather once the graph outgrows the caches.  Such
+// groups run the tile-engine program once per query instead (same values by
+// construction), each result copied into the group's buffer.
+constexpr uint64_t VECTOR_MAX_EDGES = 1ull << 24;
+constexpr int SEQ_ROUTE = 8;  // group_kind flag: one V-sized result per query
+
+int run_seq_group(cytom_ctx* ctx, int kind, const cytom_query* q, const uint32_t* idx, uint32_t n,
+                  uint32_t* buf, cytom_run_stats* acc) {
+  Program P;
+  P.kind = kind;
+  P.source = q[idx[0]].sources[0];
+  int rc = program_alloc(ctx, P);
+  if (rc) return rc;
+  for (uint32_t k = 0; k < n && !rc; ++k) {
+    const cytom_query& x = q[idx[k]];
+    cytom_run_stats rs{};
+    if ((rc = program_set_sources(ctx, P, x.sources, x.n_sources))) break;
+    if ((rc = run_program(ctx, P, &rs))) break;
+    acc->iterations += rs.iterations;
+    acc->traversed_edges += rs.traversed_edges;
+    acc->active_vertices += rs.active_vertices;
+    acc->slots_scanned += rs.slots_scanned;
+    if (!rs.converged) acc->converged = 0;
+    const void* src = prd_like(kind) ? (const void*)P.rank : (const void*)P.cur;
+    if (hipMemcpyAsync(buf + (uint64_t)k * ctx->V, src, ctx->V * 4, hipMemcpyDeviceToDevice,
+                       ctx->stream) != hipSuccess)
+      rc = set_err(ctx, CYTOM_EDEVICE, "query result copy failed");
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  program_free(P);
+  return rc;
 }
 
 }  // namespace
@@ -646,7 +1030,7 @@ int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run
   int rc;
   if ((rc = snapshot(ctx, qe))) return rc;
   hipStream_t st = ctx->stream;
-  CY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+  CY_HIP(ctx, hipEventRecord(ctx->ev[0], st));
   cytom_run_stats acc{};
   acc.converged = 1;
   qe->queries.assign(nq, QueryEngine::Q{0, 0, 0});
@@ -657,6 +1041,7 @@ int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run
     std::vector<uint32_t> idx;
     for (uint32_t i = 0; i < nq; ++i)
       if ((int)q[i].kind == kind) idx.push_back(i);
+    const bool seq = kind != CYTOM_BFS && qe->E > VECTOR_MAX_EDGES;
     for (size_t g0 = 0; g0 < idx.size(); g0 += width) {
       const size_t g1 = std::min(idx.size(), g0 + width);
       const uint32_t gid = (uint32_t)qe->group_buf.size();
@@ -665,9 +1050,12 @@ int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run
       if (hipMalloc(&buf, bytes) != hipSuccess)
         return set_err(ctx, CYTOM_ENOMEM, "query result allocation failed");
       qe->group_buf.push_back(buf);
-      qe->group_kind.push_back(kind);
+      qe->group_kind.push_back(seq ? kind | SEQ_ROUTE : kind);
       for (size_t k = g0; k < g1; ++k) qe->queries[idx[k]] = QueryEngine::Q{kind, gid, (uint32_t)(k - g0)};
-      if (kind == CYTOM_BFS) {
+      if (seq) {
+        rc = run_seq_group(ctx, kind, q, idx.data() + g0, (uint32_t)(g1 - g0),
+                           static_cast<uint32_t*>(buf), &acc);
+      } else if (kind == CYTOM_BFS) {
         std::vector<uint32_t> src;
         for (size_t k = g0; k < g1; ++k) src.push_back(q[idx[k]].sources[0]);
         rc = run_bfs_group(ctx, qe, src, static_cast<uint16_t*>(buf), &acc);
@@ -684,10 +1072,10 @@ int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run
       if (rc) return rc;
     }
   }
-  CY_HIP(ctx, hipEventRecord(ctx->ev[3], st));
-  CY_HIP(ctx, hipEventSynchronize(ctx->ev[3]));
+  CY_HIP(ctx, hipEventRecord(ctx->ev[1], st));
+  CY_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
   float ms = 0.f;
-  CY_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+  CY_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
   acc.t_compute_ms = ms;
   acc.reexecuted = 0;
   if (stats) *stats = acc;
@@ -702,9 +1090,15 @@ int read_query(cytom_ctx* ctx, uint32_t query, void* out, size_t bytes) {
   if (!out || bytes < V * 4) return set_err(ctx, CYTOM_EINVAL, "output buffer too small");
   const QueryEngine::Q& x = qe->queries[query];
   hipStream_t st = ctx->stream;
+  const void* buf = qe->group_buf[x.group];
+  if (qe->group_kind[x.group] & SEQ_ROUTE) {
+    const hipError_t e =
+        hipMemcpy(out, static_cast<const uint32_t*>(buf) + (uint64_t)x.lane * V, V * 4,
+                  hipMemcpyDeviceToHost);
+    return e == hipSuccess ? 0 : hip_fail(ctx, e, "read_query");
+  }
   uint32_t* tmp = nullptr;
   CY_HIP(ctx, hipMalloc(&tmp, V * 4));
-  const void* buf = qe->group_buf[x.group];
   if (x.kind == CYTOM_BFS)
     k_extract_lev<<<blocks_for(V, 256, 16384), 256, 0, st>>>(static_cast<const uint16_t*>(buf),
                                                              x.lane, V, tmp);

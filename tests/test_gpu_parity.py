@@ -359,3 +359,50 @@ def test_batched_ppr_iteration_cap_and_errors():
         gu.batched_execute([("sssp", [0])])
     r = gu.batched_execute([("bfs", [0]), ("ppr", [0])])
     assert list(gu.read_query(0)[:4]) == [0, 1, 2, 0xFFFFFFFF]
+
+
+def test_batched_queries_with_hub_vertices():
+    # In-degree above the hub threshold (2048) takes the block-per-vertex
+    # pull path; a dense frontier switches from push to pull.
+    scale, V = 12, 1 << 12
+    rng = np.random.default_rng(8)
+    s, d = O.rmat_edges(scale, 8, 0, 30000)
+    hub_in = rng.integers(0, V, 6000).astype(np.uint32)
+    hub_out = rng.integers(0, V, 6000).astype(np.uint32)
+    s = np.concatenate([s, hub_in, np.full(6000, 7, np.uint32)])
+    d = np.concatenate([d, np.full(6000, 7, np.uint32), hub_out])
+    w = (1 + rng.integers(0, 255, len(s))).astype(np.uint32)
+    g = C.Graph(V, vertices_per_tile=1024, weighted=True)
+    g.update(s, d, w)
+    srcs = rng.integers(0, V, 40)
+    queries = ([("bfs", x) for x in srcs] + [("sssp", x) for x in srcs[:20]] +
+               [("ppr", rng.integers(0, V, int(k))) for k in (1, 3, 40, 300)])
+    g.batched_execute(queries)
+    pid = {k: g.register(k, 0) for k in ("bfs", "sssp", "ppr")}
+    for i, (kind, src) in enumerate(queries):
+        g.set_sources(pid[kind], src)
+        g.execute(pid[kind])
+        assert np.array_equal(g.read_query(i), g.read(pid[kind])), (i, kind)
+
+
+def test_batched_queries_large_graph_route():
+    # Above 2^24 live edges SSSP and PPR groups run the tile-engine program
+    # per query; BFS keeps the bit-parallel groups.  Values still equal the
+    # single-query runs.
+    scale, V = 20, 1 << 20
+    g = C.Graph(V, weighted=True)
+    n = 1 << 22
+    for b in range(5):
+        s, d, w = C.rmat_generate(scale, 12, b * n, n, weights=True)
+        g.update_device(s, d, w, n)
+    assert g.live_edge_count() > (1 << 24)
+    rng = np.random.default_rng(2)
+    src = rng.integers(0, 64, 6)  # low ids: high degree under RMAT
+    queries = ([("sssp", x) for x in src[:3]] + [("bfs", x) for x in src[3:]] +
+               [("ppr", src), ("ppr", [int(src[0])])])
+    g.batched_execute(queries)
+    pid = {k: g.register(k, 0) for k in ("bfs", "sssp", "ppr")}
+    for i, (kind, sv) in enumerate(queries):
+        g.set_sources(pid[kind], sv)
+        g.execute(pid[kind])
+        assert np.array_equal(g.read_query(i), g.read(pid[kind])), (i, kind)
