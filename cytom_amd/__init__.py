@@ -104,6 +104,11 @@ SIGNATURES = {
     "cytom_program_init": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cytom_program_set_sources": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _U32]),
     "cytom_batched_execute": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
+    "cytom_ppr_weighted": (ctypes.c_int, [_U64, _U64, _VP, _VP, _VP, _VP, ctypes.c_float,
+                                          ctypes.c_float, _U32, _VP, _VP]),
+    "cytom_ppr_adjoint": (ctypes.c_int, [_U64, _U64, _VP, _VP, _VP, _VP, _VP, ctypes.c_float,
+                                         ctypes.c_float, _U32, _VP, _VP, _VP]),
+    "cytom_ppr_last_error": (ctypes.c_char_p, []),
     "cytom_read_query": (ctypes.c_int, [_VP, _U32, _VP, ctypes.c_size_t]),
     "cytom_update_batch": (ctypes.c_int, [_VP, _VP, _VP, _VP, _U64, _VP, _VP, _VP, _U64,
                                           ctypes.POINTER(BatchStats)]),

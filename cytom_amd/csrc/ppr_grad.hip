@@ -1,0 +1,427 @@
+// Differentiable weighted personalized PageRank (no reference counterpart).
+//
+// For an explicit edge list with positive FP32 weights w_e and a teleport
+// vector s, with out-row-normalised transition P_e = w_e / W_u (W_u = sum of
+// the weights leaving u; rows without out-edges drop their mass):
+//   forward   r = (1 - a) s + a P^T r
+//   adjoint   lambda = g + a P lambda            (g = dL/dr)
+//   dL/ds_v = (1 - a) lambda_v
+//   dL/dw_e = a r_u (lambda_v - mu_u) / W_u     for e = (u -> v),
+//             mu_u = sum_{e' leaving u} P_e' lambda_dst(e')
+// Both linear systems are solved by Jacobi sweeps in FP32 with a fixed
+// summation order (CSC rows for the forward, CSR rows for the adjoint), so
+// results are deterministic.  The operator owns its CSR/CSC copies of the
+// edge list; the tile store is not involved.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "devutil.h"
+#include "internal.h"
+
+namespace cytom {
+namespace {
+
+constexpr int GT = 256;
+constexpr uint64_t ROW_LIGHT = 64;  // rows above this many entries get a wavefront each
+
+inline uint32_t gblocks(uint64_t n, uint32_t per, uint32_t cap = 65536) {
+  const uint64_t b = (n + per - 1) / per;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, cap));
+}
+
+__global__ void k_count(const uint32_t* __restrict__ key, uint64_t E, uint32_t* __restrict__ cnt) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[key[e]], 1u);
+}
+
+__global__ void k_widen32(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
+__global__ void k_iota(uint32_t* __restrict__ x, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    x[i] = (uint32_t)i;
+}
+
+__global__ void k_gather_u32(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ in,
+                             uint64_t E, uint32_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = in[perm[i]];
+}
+
+__global__ void k_long_rows(const uint64_t* __restrict__ off, uint64_t R, uint64_t minlen,
+                            uint32_t* __restrict__ list, unsigned int* __restrict__ n) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * blockDim.x)
+    if (off[r + 1] - off[r] > minlen) list[atomicAdd(n, 1u)] = (uint32_t)r;
+}
+
+// W_u: sum of the weights leaving u, in CSR order (one thread per row; long
+// rows by a wavefront each).
+__global__ void k_row_sum(const uint64_t* __restrict__ off, uint64_t R,
+                          const uint32_t* __restrict__ perm, const float* __restrict__ w,
+                          float* __restrict__ W) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t b = off[r], e = off[r + 1];
+    if (e - b > ROW_LIGHT) continue;
+    float s = 0.f;
+    for (uint64_t i = b; i < e; ++i) s += w[perm[i]];
+    W[r] = s;
+  }
+}
+
+__device__ inline float wave_sum_f32(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void k_row_sum_long(const uint64_t* __restrict__ off, const uint32_t* __restrict__ rows,
+                               uint32_t nrows, const uint32_t* __restrict__ perm,
+                               const float* __restrict__ w, float* __restrict__ W) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t k = wave; k < nrows; k += nw) {
+    const uint32_t r = rows[k];
+    float s = 0.f;
+    for (uint64_t i = off[r] + lane; i < off[r + 1]; i += 64) s += w[perm[i]];
+    s = wave_sum_f32(s);
+    if (lane == 0) W[r] = s;
+  }
+}
+
+// Transition probability of every edge in the given row order:
+// p[i] = w[perm[i]] / W[src[perm[i]]].
+__global__ void k_prob(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ src,
+                       const float* __restrict__ w, const float* __restrict__ W, uint64_t E,
+                       float* __restrict__ p) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t e = perm[i];
+    p[i] = w[e] / W[src[e]];
+  }
+}
+
+__device__ inline void block_max_to(float d, unsigned int* out) {
+  __shared__ float wmax[GT / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) d = fmaxf(d, __shfl_xor(d, o, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = 0.f;
+    for (int i = 0; i < GT / 64; ++i) m = fmaxf(m, wmax[i]);
+    if (m > 0.f) atomicMax(out, __float_as_uint(m));
+  }
+}
+
+// One Jacobi sweep y_r = base_r + a * sum_{i in row r} p_i x_{nb_i}; the
+// largest |y_r - x_r| goes to *diff (as float bits, non-negative).
+__global__ void k_sweep(const uint64_t* __restrict__ off, uint64_t R, const uint32_t* __restrict__ nb,
+                        const float* __restrict__ p, const float* __restrict__ x,
+                        const float* __restrict__ base, float base_scale, float a,
+                        float* __restrict__ y, unsigned int* __restrict__ diff) {
+  float dmax = 0.f;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t b = off[r], e = off[r + 1];
+    if (e - b > ROW_LIGHT) continue;
+    float acc = 0.f;
+    for (uint64_t i = b; i < e; ++i) acc += p[i] * x[nb[i]];
+    const float v = base_scale * base[r] + a * acc;
+    y[r] = v;
+    dmax = fmaxf(dmax, fabsf(v - x[r]));
+  }
+  block_max_to(dmax, diff);
+}
+
+__global__ void k_sweep_long(const uint64_t* __restrict__ off, const uint32_t* __restrict__ rows,
+                             uint32_t nrows, const uint32_t* __restrict__ nb,
+                             const float* __restrict__ p, const float* __restrict__ x,
+                             const float* __restrict__ base, float base_scale, float a,
+                             float* __restrict__ y, unsigned int* __restrict__ diff) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t k = wave; k < nrows; k += nw) {
+    const uint32_t r = rows[k];
+    float acc = 0.f;
+    for (uint64_t i = off[r] + lane; i < off[r + 1]; i += 64) acc += p[i] * x[nb[i]];
+    acc = wave_sum_f32(acc);
+    if (lane == 0) {
+      const float v = base_scale * base[r] + a * acc;
+      y[r] = v;
+      const float d = fabsf(v - x[r]);
+      if (d > 0.f) atomicMax(diff, __float_as_uint(d));
+    }
+  }
+}
+
+// mu_u = sum over u's out-edges of p * lambda_dst (CSR order).
+__global__ void k_mu(const uint64_t* __restrict__ off, uint64_t R, const uint32_t* __restrict__ nb,
+                     const float* __restrict__ p, const float* __restrict__ lam,
+                     float* __restrict__ mu) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t b = off[r], e = off[r + 1];
+    if (e - b > ROW_LIGHT) continue;
+    float acc = 0.f;
+    for (uint64_t i = b; i < e; ++i) acc += p[i] * lam[nb[i]];
+    mu[r] = acc;
+  }
+}
+
+__global__ void k_mu_long(const uint64_t* __restrict__ off, const uint32_t* __restrict__ rows,
+                          uint32_t nrows, const uint32_t* __restrict__ nb,
+                          const float* __restrict__ p, const float* __restrict__ lam,
+                          float* __restrict__ mu) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t k = wave; k < nrows; k += nw) {
+    const uint32_t r = rows[k];
+    float acc = 0.f;
+    for (uint64_t i = off[r] + lane; i < off[r + 1]; i += 64) acc += p[i] * lam[nb[i]];
+    acc = wave_sum_f32(acc);
+    if (lane == 0) mu[r] = acc;
+  }
+}
+
+// dL/dw_e = a r_u (lambda_v - mu_u) / W_u, in the caller's edge order.
+__global__ void k_dw(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint64_t E,
+                     const float* __restrict__ r, const float* __restrict__ lam,
+                     const float* __restrict__ mu, const float* __restrict__ W, float a,
+                     float* __restrict__ dw) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = src[e], v = dst[e];
+    dw[e] = a * r[u] * (lam[v] - mu[u]) / W[u];
+  }
+}
+
+__global__ void k_scale(const float* __restrict__ x, uint64_t n, float c, float* __restrict__ y) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    y[i] = c * x[i];
+}
+
+// ----------------------------------------------------------------- host
+struct Rows {
+  uint64_t* off = nullptr;   // R + 1
+  uint32_t* perm = nullptr;  // edge ids by row
+  uint32_t* nb = nullptr;    // neighbour (the other endpoint) per entry
+  float* p = nullptr;        // transition probability per entry
+  uint32_t* longr = nullptr;
+  uint32_t nlong = 0;
+};
+
+struct Op {
+  uint64_t V = 0, E = 0;
+  Rows out, in;  // CSR by source, CSC by target
+  float* W = nullptr;
+  unsigned int* scratch = nullptr;  // [0] diff, [1] count
+  void* tmp = nullptr;
+  std::vector<void*> owned;
+  int err = 0;
+  std::string msg;
+  ~Op() {
+    for (void* q : owned)
+      if (q) (void)hipFree(q);
+  }
+  template <typename T>
+  T* alloc(size_t bytes) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(bytes, 4)) != hipSuccess) {
+      err = CYTOM_ENOMEM;
+      msg = "device allocation failed";
+      return nullptr;
+    }
+    owned.push_back(q);
+    return static_cast<T*>(q);
+  }
+};
+
+int build_rows(Op& op, Rows& R, const uint32_t* key, const uint32_t* other, hipStream_t st) {
+  const uint64_t V = op.V, E = op.E;
+  uint32_t* cnt = op.alloc<uint32_t>(V * 4);
+  uint64_t* wide = op.alloc<uint64_t>((V + 1) * 8);
+  R.off = op.alloc<uint64_t>((V + 1) * 8);
+  R.perm = op.alloc<uint32_t>(E * 4);
+  R.nb = op.alloc<uint32_t>(E * 4);
+  R.p = op.alloc<float>(E * 4);
+  R.longr = op.alloc<uint32_t>(V * 4);
+  // stable radix sort of (key, edge id): ids increase inside every row, so
+  // every row sums in a fixed order
+  uint32_t* ids = op.alloc<uint32_t>(E * 4);
+  uint32_t* k_out = op.alloc<uint32_t>(E * 4);
+  uint32_t* k_tmp = op.alloc<uint32_t>(E * 4);
+  uint32_t* v_tmp = op.alloc<uint32_t>(E * 4);
+  void* hist = op.alloc<char>(radix_hist_bytes(E));
+  void* stmp = op.alloc<char>(scan_tmp_bytes(std::max<uint64_t>(E, 256 * ((E + 4095) / 4096) + 1)) +
+                              256);
+  if (op.err) return op.err;
+  if (hipMemsetAsync(cnt, 0, V * 4, st) != hipSuccess) return CYTOM_EDEVICE;
+  k_count<<<gblocks(E, GT, 16384), GT, 0, st>>>(key, E, cnt);
+  k_widen32<<<gblocks(V, GT, 16384), GT, 0, st>>>(cnt, wide, V);
+  scan_exclusive_u64(wide, R.off, V, op.tmp, st);
+  k_iota<<<gblocks(E, GT, 16384), GT, 0, st>>>(ids, E);
+  uint32_t bits = 1;
+  while ((1ull << bits) < V) ++bits;
+  radix_sort_pairs(key, ids, k_out, R.perm, k_tmp, v_tmp, E, bits, hist, stmp, st);
+  k_gather_u32<<<gblocks(E, GT, 16384), GT, 0, st>>>(R.perm, other, E, R.nb);
+  // rows handled by a wavefront each in the sweeps
+  if (hipMemsetAsync(op.scratch + 1, 0, 4, st) != hipSuccess) return CYTOM_EDEVICE;
+  k_long_rows<<<gblocks(V, GT, 16384), GT, 0, st>>>(R.off, V, ROW_LIGHT, R.longr, op.scratch + 1);
+  if (hipMemcpyAsync(&R.nlong, op.scratch + 1, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return CYTOM_EDEVICE;
+  return 0;
+}
+
+int build_op(Op& op, uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+             const float* w, hipStream_t st) {
+  op.V = V;
+  op.E = E;
+  op.scratch = op.alloc<unsigned int>(256);
+  op.tmp = op.alloc<char>(scan_tmp_bytes(V + 1) + 256);
+  op.W = op.alloc<float>(V * 4);
+  if (op.err) return op.err;
+  int rc;
+  if ((rc = build_rows(op, op.out, src, dst, st)) || (rc = build_rows(op, op.in, dst, src, st)))
+    return rc;
+  if (hipMemsetAsync(op.W, 0, V * 4, st) != hipSuccess) return CYTOM_EDEVICE;
+  k_row_sum<<<gblocks(V, GT, 16384), GT, 0, st>>>(op.out.off, V, op.out.perm, w, op.W);
+  if (op.out.nlong)
+    k_row_sum_long<<<gblocks((uint64_t)op.out.nlong * 64, GT, 16384), GT, 0, st>>>(
+        op.out.off, op.out.longr, op.out.nlong, op.out.perm, w, op.W);
+  k_prob<<<gblocks(E, GT, 16384), GT, 0, st>>>(op.out.perm, src, w, op.W, E, op.out.p);
+  k_prob<<<gblocks(E, GT, 16384), GT, 0, st>>>(op.in.perm, src, w, op.W, E, op.in.p);
+  return hipStreamSynchronize(st) == hipSuccess ? 0 : CYTOM_EDEVICE;
+}
+
+// Jacobi solve of x = base_scale * base + a * M x over the rows of R; x is
+// the initial guess on entry and the solution on exit.
+int solve(Op& op, const Rows& R, const float* base, float base_scale, float a, float tol,
+          uint32_t max_iter, float* x, float* y, uint32_t* iters, hipStream_t st) {
+  const uint64_t V = op.V;
+  uint32_t it = 0;
+  for (; it < max_iter; ++it) {
+    if (hipMemsetAsync(op.scratch, 0, 4, st) != hipSuccess) return CYTOM_EDEVICE;
+    k_sweep<<<gblocks(V, GT, 16384), GT, 0, st>>>(R.off, V, R.nb, R.p, x, base, base_scale, a, y,
+                                                  op.scratch);
+    if (R.nlong)
+      k_sweep_long<<<gblocks((uint64_t)R.nlong * 64, GT, 16384), GT, 0, st>>>(
+          R.off, R.longr, R.nlong, R.nb, R.p, x, base, base_scale, a, y, op.scratch);
+    std::swap(x, y);
+    // Check every 8 sweeps (and at the cap) to keep the host out of the loop.
+    if ((it & 7) == 7 || it + 1 == max_iter) {
+      unsigned int bits = 0;
+      if (hipMemcpyAsync(&bits, op.scratch, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return CYTOM_EDEVICE;
+      float d;
+      std::memcpy(&d, &bits, 4);
+      if (d <= tol) {
+        ++it;
+        break;
+      }
+    }
+  }
+  // An odd number of swaps leaves the solution in the caller's y buffer.
+  if (it & 1) {
+    if (hipMemcpyAsync(y, x, V * 4, hipMemcpyDeviceToDevice, st) != hipSuccess) return CYTOM_EDEVICE;
+  }
+  *iters = it;
+  return hipStreamSynchronize(st) == hipSuccess ? 0 : CYTOM_EDEVICE;
+}
+
+int check_inputs(uint64_t V, uint64_t E, const void* src, const void* dst, const void* w) {
+  if (V == 0 || V > (1ull << 31) || E > 0xFFFFFFFFull) return CYTOM_EINVAL;
+  if (E && (!src || !dst || !w)) return CYTOM_EINVAL;
+  return 0;
+}
+
+thread_local std::string g_ppr_err;
+
+}  // namespace
+}  // namespace cytom
+
+using namespace cytom;
+
+extern "C" const char* cytom_ppr_last_error(void) { return g_ppr_err.c_str(); }
+
+extern "C" int cytom_ppr_weighted(uint64_t V, uint64_t E, const uint32_t* d_src,
+                                  const uint32_t* d_dst, const float* d_w, const float* d_s,
+                                  float alpha, float tol, uint32_t max_iter, float* d_r,
+                                  uint32_t* iterations) {
+  if (check_inputs(V, E, d_src, d_dst, d_w) || !d_s || !d_r || !(alpha >= 0.f && alpha < 1.f))
+    return CYTOM_EINVAL;
+  hipStream_t st = nullptr;
+  if (hipStreamCreate(&st) != hipSuccess) return CYTOM_EDEVICE;
+  int rc;
+  {
+    Op op;
+    float* y = op.alloc<float>(V * 4);
+    if (op.err) {
+      rc = op.err;
+    } else if ((rc = build_op(op, V, E, d_src, d_dst, d_w, st)) == 0) {
+      // x0 = (1 - a) s
+      k_scale<<<gblocks(V, GT, 16384), GT, 0, st>>>(d_s, V, 1.f - alpha, d_r);
+      uint32_t it = 0;
+      rc = solve(op, op.in, d_s, 1.f - alpha, alpha, tol, max_iter, d_r, y, &it, st);
+      if (iterations) *iterations = it;
+    }
+    if (rc) g_ppr_err = op.msg.empty() ? "weighted PPR forward failed" : op.msg;
+  }
+  (void)hipStreamDestroy(st);
+  return rc;
+}
+
+extern "C" int cytom_ppr_adjoint(uint64_t V, uint64_t E, const uint32_t* d_src,
+                                 const uint32_t* d_dst, const float* d_w, const float* d_r,
+                                 const float* d_dL_dr, float alpha, float tol, uint32_t max_iter,
+                                 float* d_dL_dw, float* d_dL_ds, uint32_t* iterations) {
+  if (check_inputs(V, E, d_src, d_dst, d_w) || !d_r || !d_dL_dr || !(alpha >= 0.f && alpha < 1.f))
+    return CYTOM_EINVAL;
+  hipStream_t st = nullptr;
+  if (hipStreamCreate(&st) != hipSuccess) return CYTOM_EDEVICE;
+  int rc;
+  {
+    Op op;
+    float* lam = op.alloc<float>(V * 4);
+    float* y = op.alloc<float>(V * 4);
+    float* mu = op.alloc<float>(V * 4);
+    if (op.err) {
+      rc = op.err;
+    } else if ((rc = build_op(op, V, E, d_src, d_dst, d_w, st)) == 0) {
+      // lambda = g + a P lambda, from lambda0 = g
+      (void)hipMemcpyAsync(lam, d_dL_dr, V * 4, hipMemcpyDeviceToDevice, st);
+      uint32_t it = 0;
+      rc = solve(op, op.out, d_dL_dr, 1.f, alpha, tol, max_iter, lam, y, &it, st);
+      if (iterations) *iterations = it;
+      if (!rc) {
+        const float* L = lam;  // solve() leaves the solution in its x buffer
+        k_mu<<<gblocks(V, GT, 16384), GT, 0, st>>>(op.out.off, V, op.out.nb, op.out.p, L, mu);
+        if (op.out.nlong)
+          k_mu_long<<<gblocks((uint64_t)op.out.nlong * 64, GT, 16384), GT, 0, st>>>(
+              op.out.off, op.out.longr, op.out.nlong, op.out.nb, op.out.p, L, mu);
+        if (d_dL_dw && E)
+          k_dw<<<gblocks(E, GT, 16384), GT, 0, st>>>(d_src, d_dst, E, d_r, L, mu, op.W, alpha,
+                                                     d_dL_dw);
+        if (d_dL_ds) k_scale<<<gblocks(V, GT, 16384), GT, 0, st>>>(L, V, 1.f - alpha, d_dL_ds);
+        rc = hipStreamSynchronize(st) == hipSuccess ? 0 : CYTOM_EDEVICE;
+      }
+    }
+    if (rc) g_ppr_err = op.msg.empty() ? "weighted PPR adjoint failed" : op.msg;
+  }
+  (void)hipStreamDestroy(st);
+  return rc;
+}

@@ -165,6 +165,29 @@ int cytom_batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq,
  * float[V] (ResultsWriter layout, results-writer.h:54-67). */
 int cytom_read_query(cytom_ctx* ctx, uint32_t query, void* out, size_t bytes);
 
+/* ---- differentiable personalized PageRank ------------------------------- */
+
+/* Weighted personalized PageRank on an explicit edge list (device arrays;
+ * no reference counterpart).  P_e = w_e / W_u with W_u the sum of the
+ * positive weights leaving u (rows without out-edges drop their mass);
+ * r = (1-alpha) s + alpha P^T r, solved by FP32 Jacobi sweeps from
+ * (1-alpha) s until the largest change is <= tol or max_iterations sweeps.
+ * Deterministic (fixed summation order).  d_r receives float[V]. */
+int cytom_ppr_weighted(uint64_t n_vertices, uint64_t n_edges, const uint32_t* d_src,
+                       const uint32_t* d_dst, const float* d_w, const float* d_s, float alpha,
+                       float tol, uint32_t max_iterations, float* d_r, uint32_t* iterations);
+/* Reverse mode of cytom_ppr_weighted for a loss gradient dL/dr: solves
+ * lambda = dL/dr + alpha P lambda the same way and writes
+ * dL/dw_e = alpha r_u (lambda_v - sum_e' P_e' lambda_dst(e')) / W_u (d_dL_dw,
+ * float[E], may be NULL) and dL/ds = (1-alpha) lambda (d_dL_ds, float[V],
+ * may be NULL).  d_r is the forward result. */
+int cytom_ppr_adjoint(uint64_t n_vertices, uint64_t n_edges, const uint32_t* d_src,
+                      const uint32_t* d_dst, const float* d_w, const float* d_r,
+                      const float* d_dL_dr, float alpha, float tol, uint32_t max_iterations,
+                      float* d_dL_dw, float* d_dL_ds, uint32_t* iterations);
+/* Message of the last failed cytom_ppr_* call on this thread. */
+const char* cytom_ppr_last_error(void);
+
 /* ---- inspection --------------------------------------------------------- */
 
 /* VertexStore degrees (vertex-store.h:64-70). */

@@ -312,3 +312,65 @@ def test_ppr_delta_program_tiny_and_power_iteration():
         ref, _ = O.ppr_f64(V, s, d, tel)
         err = np.abs(r - ref).sum() / ref.sum()
         assert err < 2e-5, (tel[:3], err)
+
+
+# ------------------------------------------------- differentiable PPR (FP64)
+def _weighted_graph(V, E, seed):
+    rng = np.random.default_rng(seed)
+    s = rng.integers(0, V, E).astype(np.uint32)
+    d = rng.integers(0, V, E).astype(np.uint32)
+    w = rng.uniform(1e-3, 1.0, E)
+    t = np.zeros(V)
+    t[rng.choice(V, 16, replace=False)] = 1.0 / 16
+    return s, d, w, t
+
+
+def test_ppr_grad_oracle_dense_equals_iterative():
+    from oracle import ppr_grad as G
+    V = 256
+    s, d, w, t = _weighted_graph(V, 2000, 1)
+    r_it = G.forward(V, s, d, w, t)
+    r_de = G.forward(V, s, d, w, t, dense=True)
+    assert np.allclose(r_it, r_de, rtol=1e-11, atol=1e-14)
+    _, g = G.loss_sq(r_de, np.arange(0, V, 3))
+    dw_it, ds_it, _ = G.adjoint(V, s, d, w, r_de, g)
+    dw_de, ds_de, _ = G.adjoint(V, s, d, w, r_de, g, dense=True)
+    assert np.allclose(dw_it, dw_de, rtol=1e-9, atol=1e-14)
+    assert np.allclose(ds_it, ds_de, rtol=1e-9, atol=1e-14)
+
+
+def test_ppr_grad_oracle_matches_finite_differences():
+    # Central differences, h = 1e-3, on sampled edges and teleport entries.
+    from oracle import ppr_grad as G
+    V = 512
+    s, d, w, t = _weighted_graph(V, 4000, 2)
+    targets = np.random.default_rng(3).integers(0, V, 64)
+
+    def L(wv, tv):
+        return G.loss_sq(G.forward(V, s, d, wv, tv, dense=True), targets)[0]
+
+    r = G.forward(V, s, d, w, t, dense=True)
+    _, g = G.loss_sq(r, targets)
+    dw, ds, _ = G.adjoint(V, s, d, w, r, g, dense=True)
+    h = 1e-3
+    for e in np.random.default_rng(4).choice(len(w), 24, replace=False):
+        wp, wm = w.copy(), w.copy()
+        wp[e] += h
+        wm[e] -= h
+        fd = (L(wp, t) - L(wm, t)) / (2 * h)
+        assert abs(fd - dw[e]) <= 1e-6 * max(1.0, abs(dw[e])) + 1e-5 * abs(dw[e]), (e, fd, dw[e])
+    for v in (0, 5, 77):
+        tp, tm = t.copy(), t.copy()
+        tp[v] += h
+        tm[v] -= h
+        fd = (L(w, tp) - L(w, tm)) / (2 * h)
+        assert abs(fd - ds[v]) <= 1e-9 + 1e-6 * abs(ds[v])
+
+
+def test_ppr_grad_oracle_uniform_teleport_is_pagerank():
+    from oracle import ppr_grad as G
+    V = 1 << 9
+    s, d = O.rmat_edges(9, 3, 0, 3000)
+    r = G.forward(V, s, d, np.ones(len(s)), np.full(V, 1.0 / V))
+    pr, _ = O.pagerank_f64(V, s, d)
+    assert np.allclose(r * V, pr, rtol=1e-9)
