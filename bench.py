@@ -80,7 +80,21 @@ def measured_traffic(args, kernel):
 
 
 def parse(argv=None):
-    return parse_args(sys.argv[1:] if argv is None else argv)
+    args = parse_args(sys.argv[1:] if argv is None else argv)
+    if args.config == "rmat27":
+        # Preset: every knob the 2^31-update stream needs, unless given.
+        given = set(a.split("=")[0] for a in (sys.argv[1:] if argv is None else argv))
+        preset = {"--scale": ("scale", 27), "--batches": ("batches", 128),
+                  "--programs": ("programs", "cc,pagerank-delta"),
+                  "--warmup": ("warmup", 120), "--steps": ("steps", 8),
+                  "--warmup-programs": ("warmup_programs", "cc"),
+                  "--cpu-baseline": ("cpu_baseline", "off"), "--queries": ("queries", "off"),
+                  "--ppr-grad": ("ppr_grad", "off"),
+                  "--delete-fraction": ("delete_fraction", 838860 / 16777216)}
+        for flag, (attr, val) in preset.items():
+            if flag not in given:
+                setattr(args, attr, val)
+    return args
 
 
 def parse_args(argv):
@@ -100,12 +114,20 @@ def parse_args(argv):
     ap.add_argument("--resort-pct", type=int, default=0,
                     help="sort-compaction threshold in percent (-1 disables)")
     ap.add_argument("--programs", default="bfs,cc,sssp,pagerank-delta")
+    ap.add_argument("--warmup-programs", default="all",
+                    help="programs run after each warmup batch (default: all); programs "
+                         "re-initialised every batch lose nothing when skipped")
+    ap.add_argument("--config", choices=["rmat22", "rmat27"], default="rmat22",
+                    help="rmat27: the RMAT-27 stream (2^31 updates in 128 batches of 2^24, "
+                         "CC + PageRank-delta), last 8 batches timed")
     ap.add_argument("--queries", choices=["on", "off"], default="on",
                     help="after the stream: batched vs plain query workload")
     ap.add_argument("--q-bfs", type=int, default=256)
     ap.add_argument("--q-sssp", type=int, default=256)
     ap.add_argument("--q-ppr", type=int, default=64)
     ap.add_argument("--q-graphs", type=int, default=128)
+    ap.add_argument("--ppr-grad", choices=["on", "off"], default="on",
+                    help="time the differentiable PPR forward/adjoint on RMAT-20")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--cpu-batches", type=int, default=2,
@@ -150,19 +172,22 @@ def gpu_run(args):
     # the live set on the host and materialised on the device.
     stream = C.Stream(args.seed)
     batches = []
-    src_all = []
+    need_source = any(n in ("bfs", "sssp") for n in names)
+    outdeg = np.zeros(V, np.int64) if need_source else None
     for b in range(args.batches):
         first, didx = stream.next(n_ins, n_del)
         s, d, w = C.rmat_generate(args.scale, args.seed, first, n_ins, weights=weighted)
         di = C.DeviceArray.from_host(didx)
         ds, dd, dw = C.rmat_gather(args.scale, args.seed, di, len(didx), weights=weighted)
         batches.append((s, d, w, ds, dd, dw, len(didx)))
-        src_all.append(s.to_host())
+        if need_source:
+            outdeg += np.bincount(s.to_host(), minlength=V)
+        if (b + 1) % 32 == 0:
+            log("bench: generated %d/%d batches" % (b + 1, args.batches))
     # BFS/SSSP source: highest out-degree vertex of the insert stream, ties to
     # the lowest id.
-    outdeg = np.bincount(np.concatenate(src_all), minlength=V)
-    source = int(np.argmax(outdeg))
-    del src_all
+    source = int(np.argmax(outdeg)) if need_source else 0
+    del outdeg
     log("bench: generated %d batches (%d inserts + %d deletions each) in %.1fs; source=%d"
         % (args.batches, n_ins, n_del, time.perf_counter() - t_setup, source))
     g = C.Graph(V, vertices_per_tile=args.vertices_per_tile, weighted=weighted,
@@ -175,12 +200,17 @@ def gpu_run(args):
     warm = min(args.warmup, args.batches)
     steps = min(args.steps, args.batches - warm)
 
-    def step(b, acc):
+    warm_names = names if args.warmup_programs == "all" else \
+        [p.strip() for p in args.warmup_programs.split(",") if p.strip()]
+
+    def step(b, acc, run=names):
         s, d, w, ds, dd, dw, nd = batches[b]
         bs = g.update_device(s, d, w, n_ins, ds, dd, dw, nd)
         acc["ingest_ms"] += bs.t_ingest_ms
         acc["deleted"] += bs.deleted
         for n, p in pids.items():
+            if n not in run:
+                continue
             st = g.execute(p)
             acc["te"][n] += st.traversed_edges
             acc["ms"][n] += st.t_compute_ms
@@ -195,7 +225,9 @@ def gpu_run(args):
 
     wacc = new_acc()
     for b in range(warm):
-        step(b, wacc)
+        step(b, wacc, warm_names)
+        if (b + 1) % 16 == 0:
+            log("bench: warmup batch %d/%d" % (b + 1, warm))
     log("bench: warmup %d batches done" % warm)
     g.set_profiling(True)
     acc = new_acc()
@@ -322,6 +354,7 @@ def queries_run(args, g, V, source):
                                                                           tb * 1e3, tp * 1e3))
     if args.q_graphs:
         out["graphs"] = tenants_run(args)
+    grad = ppr_grad_run(args) if args.ppr_grad == "on" else None
     tq = sum(v["queries"] for v in out.values())
     tb = sum(v["batched_ms"] for v in out.values())
     tp = sum(v["plain_ms"] for v in out.values())
@@ -329,7 +362,40 @@ def queries_run(args, g, V, source):
                     "batched_qps": round(tq / tb * 1e3, 1) if tb else 0.0,
                     "plain_qps": round(tq / tp * 1e3, 1) if tp else 0.0,
                     "speedup": round(tp / tb, 2) if tb else 0.0}
+    if grad:
+        out["ppr_grad"] = grad
     return out
+
+
+def ppr_grad_run(args):
+    """Differentiable weighted PPR on RMAT-20 (2^24 edges, U(0,1] weights,
+    256 teleport vertices, loss = sum of r_i^2 over 1,024 targets): device
+    time of one forward and one adjoint (each builds its CSR/CSC rows)."""
+    import cytom_amd as C
+    from cytom_amd import autograd as A
+
+    scale, V, E = 20, 1 << 20, 1 << 24
+    s, d, _ = C.rmat_generate(scale, args.seed + 300, 0, E)
+    rng = np.random.default_rng(args.seed + 300)
+    w = (1.0 - rng.random(E)).astype(np.float32)
+    t = np.zeros(V, np.float32)
+    t[rng.choice(V, 256, replace=False)] = 1.0 / 256
+    targets = rng.integers(0, V, 1024)
+    G = A.PPRGraph(V, s.to_host(), d.to_host(), w)
+    t_dev = C.DeviceArray.from_host(t)
+    r_dev = C.DeviceArray(V, np.float32)
+    tf, sweeps_f = _timed(lambda: G.forward_device(t_dev, r_dev, tol=1e-10))
+    r = r_dev.to_host()
+    g = np.zeros(V, np.float32)
+    np.add.at(g, targets, 2.0 * r[targets])
+    g_dev = C.DeviceArray.from_host(g)
+    dw = C.DeviceArray(E, np.float32)
+    ds = C.DeviceArray(V, np.float32)
+    ta, sweeps_a = _timed(lambda: G.adjoint_device(r_dev, g_dev, dw, ds, tol=1e-10))
+    log("bench: PPR gradient on RMAT-20: forward %.1f ms (%d sweeps), adjoint %.1f ms (%d sweeps)"
+        % (tf * 1e3, sweeps_f, ta * 1e3, sweeps_a))
+    return {"edges": E, "forward_ms": round(tf * 1e3, 2), "forward_sweeps": sweeps_f,
+            "adjoint_ms": round(ta * 1e3, 2), "adjoint_sweeps": sweeps_a}
 
 
 def tenants_run(args):
