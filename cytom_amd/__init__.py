@@ -11,6 +11,8 @@ import os
 
 import numpy as np
 
+from .formats import PRD_VERTEX as PRD_RECORD  # results-file vertex of PRD/PPR
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcytom_hip.so")
 
@@ -114,6 +116,9 @@ SIGNATURES = {
                                           ctypes.POINTER(BatchStats)]),
     "cytom_update_batch_device": (ctypes.c_int, [_VP, _VP, _VP, _VP, _U64, _VP, _VP, _VP, _U64,
                                                  ctypes.POINTER(BatchStats)]),
+    "cytom_update_batch_records": (ctypes.c_int, [_VP, _VP, _U32, _U64, _U32, _VP, _U64,
+                                                  ctypes.POINTER(BatchStats)]),
+    "cytom_read_results": (ctypes.c_int64, [_VP, ctypes.c_int, _VP, ctypes.c_size_t]),
     "cytom_execute": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(RunStats)]),
     "cytom_read_values": (ctypes.c_int, [_VP, ctypes.c_int, _VP, ctypes.c_size_t]),
     "cytom_read_degrees": (ctypes.c_int, [_VP, _VP, _VP]),
@@ -167,6 +172,20 @@ def _p(a):
 
 def _u32(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.uint32)
+
+
+
+
+def _records(a):
+    """Host edge records as a C-contiguous array of 8- or 12-byte items."""
+    a = np.asarray(a)
+    if a.dtype.names is None:
+        if a.ndim != 2 or a.shape[1] not in (2, 3) or a.dtype.itemsize != 4:
+            raise ValueError("records must be (n, 2|3) 4-byte arrays or a record dtype")
+        a = np.ascontiguousarray(a).view(np.dtype((np.void, 4 * a.shape[1]))).reshape(-1)
+    elif a.dtype.itemsize not in (8, 12):
+        raise ValueError("record size must be 8 or 12 bytes")
+    return np.ascontiguousarray(a)
 
 
 def device_count():
@@ -353,6 +372,39 @@ class Graph:
             self._h, _p(src), _p(dst), _p(w), n_ins, _p(dsrc), _p(ddst), _p(dw), n_del,
             ctypes.byref(st)), "update_device")
         return st
+
+    def update_records(self, records, del_records=None, default_weight=1):
+        """One batch from host records in the reference's binary edge layouts:
+        8-byte {src, tgt} (file_edge_t / edge_compact_t) or 12-byte
+        {src, tgt, float weight} (edge_compact_weighted_t), e.g. slices of an
+        np.memmap of the file.  Deletions use the same record size."""
+        rec = _records(records)
+        drec = _records(del_records) if del_records is not None else rec[:0]
+        if len(rec) and len(drec) and drec.dtype.itemsize != rec.dtype.itemsize:
+            raise ValueError("insert and delete records differ in size")
+        size = rec.dtype.itemsize if len(rec) else drec.dtype.itemsize
+        st = BatchStats()
+        self._check(lib().cytom_update_batch_records(
+            self._h, rec.ctypes.data if len(rec) else None, size, len(rec), default_weight,
+            drec.ctypes.data if len(drec) else None, len(drec), ctypes.byref(st)),
+            "update_records")
+        return st
+
+    def read_results(self, pid):
+        """Vertex values in the results_<n> layout (see cytom_read_results):
+        uint32 (BFS/CC), float32 (SSSP) or a (delta_sum, rank, delta) float32
+        record array (PRD/PPR), V entries."""
+        need = self._check(lib().cytom_read_results(self._h, pid, None, 0), "read_results")
+        if self.kinds[pid] in (PRD, PPR):
+            out = np.empty(self.V, PRD_RECORD)
+        elif self.kinds[pid] == SSSP:
+            out = np.empty(self.V, np.float32)
+        else:
+            out = np.empty(self.V, np.uint32)
+        assert out.nbytes == need
+        self._check(lib().cytom_read_results(self._h, pid, out.ctypes.data, out.nbytes),
+                    "read_results")
+        return out
 
     def execute(self, pid):
         st = RunStats()

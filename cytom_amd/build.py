@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libcytom_hip.so")
-SOURCES = ["prims.hip", "rmat.hip", "store.hip", "exec.hip", "query.hip", "ppr_grad.hip",
+SOURCES = ["prims.hip", "rmat.hip", "store.hip", "exec.hip", "query.hip", "ppr_grad.hip", "io.hip",
            "engine.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

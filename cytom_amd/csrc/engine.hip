@@ -61,6 +61,7 @@ void free_ctx(cytom_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& P : c->progs) program_free(P);
   query_free(c);
+  io_free(c);
   void* ps[] = {c->tile_off, c->tile_cap, c->tile_cnt, c->tile_live, c->tile_sorted,
                 c->pool,     c->wpool,    c->outdeg,   c->indeg,    c->d_misc,
                 c->d_iter};
@@ -76,7 +77,8 @@ void free_ctx(cytom_ctx* c) {
                      &c->g_tab_head, &c->g_node_next, &c->g_claimed,   &c->g_tile_flag,
                      &c->g_del_tiles, &c->g_found,    &c->g_tile_scan, &c->g_nch,
                      &c->g_cbase,    &c->g_chunks,    &c->g_colj,      &c->g_jbase,
-                     &c->g_jobs,     &c->g_rs_e,      &c->g_rs_w,      &c->g_rs_off};
+                     &c->g_jobs,     &c->g_rs_e,      &c->g_rs_w,      &c->g_rs_off,
+                     &c->g_rec};
   for (GrowBuf* b : bufs) free_buf(*b);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -292,6 +294,15 @@ int cytom_update_batch(cytom_ctx* ctx, const uint32_t* src, const uint32_t* dst,
                       n_del, stats);
 }
 
+int cytom_update_batch_records(cytom_ctx* ctx, const void* records, uint32_t record_bytes,
+                               uint64_t n_ins, uint32_t default_weight,
+                               const void* del_records, uint64_t n_del,
+                               cytom_batch_stats* stats) {
+  CY_CHECK_CTX(ctx);
+  return ingest_records(ctx, records, record_bytes, n_ins, default_weight, del_records, n_del,
+                        stats);
+}
+
 int cytom_execute(cytom_ctx* ctx, int prog_id, cytom_run_stats* stats) {
   CY_CHECK_CTX(ctx);
   if (prog_id < 0 || prog_id >= (int)ctx->progs.size())
@@ -321,6 +332,13 @@ int cytom_read_values(cytom_ctx* ctx, int prog_id, void* out, size_t bytes) {
   CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   CY_HIP(ctx, hipMemcpy(out, src, ctx->V * 4, hipMemcpyDeviceToHost));
   return 0;
+}
+
+int64_t cytom_read_results(cytom_ctx* ctx, int prog_id, void* out, size_t bytes) {
+  CY_CHECK_CTX(ctx);
+  if (prog_id < 0 || prog_id >= (int)ctx->progs.size())
+    return set_err(ctx, CYTOM_EINVAL, "bad program id");
+  return read_results(ctx, ctx->progs[prog_id], out, bytes);
 }
 
 int cytom_read_degrees(cytom_ctx* ctx, uint32_t* out_degree, uint32_t* in_degree) {

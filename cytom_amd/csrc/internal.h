@@ -141,7 +141,12 @@ struct cytom_ctx {
   cytom::GrowBuf g_keys, g_vals, g_keys2, g_vals2, g_flags, g_scan, g_seg_start, g_seg_tile,
       g_reloc_cap, g_reloc_list, g_ins_slot, g_scan_tmp, g_hist, g_in_src, g_in_dst, g_in_w,
       g_del_src, g_del_dst, g_del_w, g_tab_key, g_tab_head, g_node_next, g_claimed,
-      g_tile_flag, g_del_tiles, g_found, g_tile_scan, g_rs_e, g_rs_w, g_rs_off;
+      g_tile_flag, g_del_tiles, g_found, g_tile_scan, g_rs_e, g_rs_w, g_rs_off, g_rec;
+  // pinned staging of host records (two buffers used in turn)
+  void* h_stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  bool stage_busy[2] = {false, false};
+  int stage_next = 0;
   // execution work lists
   cytom::GrowBuf g_nch, g_cbase, g_chunks, g_colj, g_jbase, g_jobs;
   bool jobs_dirty = true;
@@ -249,6 +254,17 @@ int configure_gather(cytom_ctx* ctx);
 int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run_stats* stats);
 int read_query(cytom_ctx* ctx, uint32_t query, void* out, size_t bytes);
 void query_free(cytom_ctx* ctx);
+
+// ------------------------------------------------------------ host records, results
+// Inserts/deletes records {u32 src, u32 tgt[, f32 weight]} (rec_bytes 8 or 12)
+// from host memory through the pinned staging buffers.
+int ingest_records(cytom_ctx* ctx, const void* rec, uint32_t rec_bytes, uint64_t n_ins,
+                   uint32_t default_w, const void* del_rec, uint64_t n_del,
+                   cytom_batch_stats* stats);
+// Writes the program's vertex values in the reference's results layout; with
+// out == nullptr returns the byte count only.
+int64_t read_results(cytom_ctx* ctx, const Program& P, void* out, size_t bytes);
+void io_free(cytom_ctx* ctx);
 // Re-execution tests of the found deletions against the program's values;
 // sets *flag (device) to 1 when a re-initialisation is needed.
 void launch_reexec_check(cytom_ctx* ctx, Program& P, const uint32_t* found, uint64_t n_found,

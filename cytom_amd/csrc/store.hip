@@ -382,13 +382,26 @@ __global__ void k_del_finish(const uint32_t* __restrict__ src, const uint32_t* _
     const unsigned long long c = claimed[i];
     if (c == EMPTY64) continue;
     const uint32_t s = src[i], d = dst[i];
-    atomicSub(&S.outdeg[s], 1u);
-    atomicSub(&S.indeg[d], 1u);
     atomicSub(&S.tile_live[tile_of(s, d, S.log2vpt, S.T)], 1u);
     const unsigned long long k = atomicAdd(&misc[M_NFOUND], 1ull);
     found[3 * k + 0] = s;
     found[3 * k + 1] = d;
     found[3 * k + 2] = S.wpool ? S.wpool[c] : 0u;
+  }
+}
+
+// Degrees of the found deletions (MetaTileManager::deleteEdge,
+// meta-tile-manager.h:127-140).  Runs after the re-execution checks and hooks:
+// the reference evaluates edgeChanged(DELETE) while the deleted edge still
+// counts in its source's out-degree (its random deletions,
+// meta-tile-manager.h:200-228, leave the degrees alone, tile-manager.h:175-189),
+// so rank / out_degree never divides by zero there.
+__global__ void k_del_degrees(const uint32_t* __restrict__ found, uint64_t n,
+                              uint32_t* __restrict__ outdeg, uint32_t* __restrict__ indeg) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    atomicSub(&outdeg[found[3 * k + 0]], 1u);
+    atomicSub(&indeg[found[3 * k + 1]], 1u);
   }
 }
 
@@ -814,6 +827,9 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
       launch_hooks(ctx, P, d_src, d_dst, d_w, ins_slot, n_ins, found, n_found);
     }
   }
+  if (n_found)
+    k_del_degrees<<<grid_for(n_found, 256), 256, 0, st>>>(found, n_found, ctx->outdeg,
+                                                           ctx->indeg);
   // 5. compaction of tiles with many tombstones, then sort-compaction of
   // tiles with long unsorted tails
   if (n_del) {

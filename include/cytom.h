@@ -131,6 +131,20 @@ int cytom_update_batch_device(cytom_ctx* ctx, const uint32_t* d_src, const uint3
                               const uint32_t* d_w, uint64_t n_ins, const uint32_t* d_del_src,
                               const uint32_t* d_del_dst, const uint32_t* d_del_w,
                               uint64_t n_del, cytom_batch_stats* stats);
+/* The same batch from host records in the reference's binary edge layouts:
+ * record_bytes 8 is file_edge_t {int32 src, int32 tgt} / edge_compact_t
+ * {u32 src, u32 tgt} (util/datatypes.h:41-44, 55-58; the binary generator's
+ * input, jobs.h:52-75), 12 is edge_compact_weighted_t {u32 src, u32 tgt,
+ * float weight} (util/datatypes.h:60-64; the weighted edge logs,
+ * edges-writer.h:150-160).  Deletions use the same layout.  Records are
+ * copied through pinned staging buffers owned by the context (the caller's
+ * pages may be an mmap of the file).  8-byte records insert default_weight
+ * into a weighted graph; 12-byte weights must be integral in [1, 2^24].
+ * Deletions without weights match any weight. */
+int cytom_update_batch_records(cytom_ctx* ctx, const void* records, uint32_t record_bytes,
+                               uint64_t n_ins, uint32_t default_weight,
+                               const void* del_records, uint64_t n_del,
+                               cytom_batch_stats* stats);
 
 /* ---- execution ---------------------------------------------------------- */
 
@@ -141,6 +155,13 @@ int cytom_execute(cytom_ctx* ctx, int prog_id, cytom_run_stats* stats);
 /* Copies the program's values (the `next` array written by ResultsWriter,
  * results-writer.h:54-67): BFS/CC/SSSP uint32[V], PRD/PPR float rank[V]. */
 int cytom_read_values(cytom_ctx* ctx, int prog_id, void* out, size_t bytes);
+/* The program's vertex values in the reference's results_<n> file layout
+ * (ResultsWriter::run, results-writer.h:46-81, raw VertexType[V]): BFS/CC
+ * uint32, SSSP float (FLT_MAX unreached, examples/sssp.cc:21), PRD/PPR
+ * {float delta_sum, rank, delta} (examples/pagerank-delta.cc:30-33).  V is
+ * the context's vertex count (cytom_geometry).  Returns the bytes written,
+ * or with out == NULL the bytes needed; negative on error. */
+int64_t cytom_read_results(cytom_ctx* ctx, int prog_id, void* out, size_t bytes);
 
 /* ---- batched queries ---------------------------------------------------- */
 

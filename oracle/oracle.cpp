@@ -753,8 +753,6 @@ int64_t orc_update(orc_graph* g, const uint32_t* src, const uint32_t* dst, const
     });
     for (uint64_t i = 0; i < n_del; ++i) {
       if (hit_slot[i] < 0) continue;
-      g->outdeg[dsrc[i]] -= 1;
-      g->indeg[ddst[i]] -= 1;
       found.push_back(DelRec{dsrc[i], ddst[i], hit_w[i]});
     }
   }
@@ -842,6 +840,15 @@ int64_t orc_update(orc_graph* g, const uint32_t* src, const uint32_t* dst, const
         P.fixpoint = false;
       }
     }
+  }
+  // 4. Degrees drop after the hooks: the reference evaluates edgeChanged(DELETE)
+  // while the deleted edge still counts in its source's out-degree (its random
+  // deletions, meta-tile-manager.h:200-228, leave the degrees unchanged,
+  // tile-manager.h:175-189), so rank / out_degree never divides by zero there.
+  // The degrees then follow MetaTileManager::deleteEdge (meta-tile-manager.h:127-140).
+  for (const DelRec& e : found) {
+    g->outdeg[e.src] -= 1;
+    g->indeg[e.dst] -= 1;
   }
   return (int64_t)found.size();
 }

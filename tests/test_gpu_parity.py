@@ -56,7 +56,7 @@ def test_rmat_gather_by_index_matches_oracle():
 
 # ------------------------------------------------------------------ streams
 def run_stream_parity(scale, vpt, batches, n_ins, n_del, seed=2, edge_capacity=0,
-                      max_iterations=100, tuning=None):
+                      max_iterations=100, tuning=None, delete_first=False):
     V = 1 << scale
     st = O.Stream(seed)
     og = O.Graph(V, vertices_per_tile=min(vpt, 65536), weighted=True,
@@ -70,7 +70,7 @@ def run_stream_parity(scale, vpt, batches, n_ins, n_del, seed=2, edge_capacity=0
     oid = {n: og.register(k, s) for n, k, _, s in progs}
     gid = {n: gg.register(k, s) for n, _, k, s in progs}
     for b in range(batches):
-        first, didx = st.next(n_ins, n_del if b else 0)
+        first, didx = st.next(n_ins, n_del if (b or delete_first) else 0)
         s, d, w, ds, dd, dw = O.batch_edges(scale, seed, first, n_ins, didx, True)
         nf = og.update(s, d, w, ds, dd, dw)
         bs = gg.update(s, d, w, ds, dd, dw)
@@ -103,6 +103,13 @@ def test_stream_parity_small_tiles():
 
 def test_stream_parity_wide_tiles():
     run_stream_parity(scale=13, vpt=4096, batches=6, n_ins=8000, n_del=800)
+
+
+def test_stream_parity_deletions_in_first_batch():
+    # Deletions before the first run, at a rate that takes the last out-edge
+    # of many vertices: the PageRank-delta DELETE hook and re-execution test
+    # see the out-degree with the deleted edge still counted.
+    run_stream_parity(scale=11, vpt=256, batches=4, n_ins=3000, n_del=1500, delete_first=True)
 
 
 def test_stream_parity_with_pool_growth():

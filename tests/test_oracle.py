@@ -236,6 +236,34 @@ def test_prd_closed_form_tiny():
     assert abs(r2[10] - 0.15) < 1e-7
 
 
+def test_prd_deleting_a_last_out_edge():
+    # The DELETE hook divides by the source's out-degree with the deleted edge
+    # still counted (the reference's random deletions leave the degrees
+    # alone until after edgeChanged); afterwards the degrees drop.  Batch 1
+    # inserts and deletes 3 -> 1, the only out-edge of 3, before any run
+    # (ranks still 0): the result is the closed form of 0 -> 1, 0 -> 2.
+    g = O.Graph(64, vertices_per_tile=64)
+    p = g.register(O.PRD)
+    u32 = lambda *x: np.array(x, np.uint32)  # noqa: E731
+    assert g.update(u32(0, 0, 3), u32(1, 2, 1), dsrc=u32(3), ddst=u32(1)) == 1
+    outd, ind = g.degrees()
+    assert outd[3] == 0 and outd[0] == 2 and ind[1] == 1
+    g.execute(p)
+    r = g.read(p)
+    assert np.all(np.isfinite(r))
+    assert abs(r[0] - 0.15) < 1e-7 and abs(r[3] - 0.15) < 1e-7
+    assert abs(r[1] - 0.21375) < 1e-6 and abs(r[2] - 0.21375) < 1e-6
+    # after a run: deleting 4 -> 0, the only out-edge of 4, re-executes
+    # (0.15 / 1 exceeds delta * rank[0]) and lands on the closed form again
+    g.update(u32(4), u32(0))
+    g.execute(p)
+    g.update(u32(), u32(), dsrc=u32(4), ddst=u32(0))
+    assert g.last_reexec(p) == 1
+    g.execute(p)
+    r = g.read(p)
+    assert abs(r[1] - 0.21375) < 1e-6 and abs(r[4] - 0.15) < 1e-7
+
+
 def test_prd_converges_to_power_iteration():
     scale = 10
     V = 1 << scale
