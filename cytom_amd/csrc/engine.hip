@@ -33,8 +33,9 @@ int ensure(cytom_ctx* ctx, GrowBuf& b, size_t bytes) {
     b.p = nullptr;
     b.bytes = 0;
   }
-  size_t want = std::max<size_t>(bytes, 256);
-  want = want + want / 4;  // amortise growth
+  // Doubling keeps re-allocations (a device-wide synchronisation each) to a
+  // few per buffer over a stream whose graph keeps growing.
+  const size_t want = std::max<size_t>(bytes, 65536) * 2;
   if (hipMalloc(&b.p, want) != hipSuccess) {
     b.p = nullptr;
     return set_err(ctx, CYTOM_ENOMEM, "scratch allocation failed");
@@ -68,6 +69,7 @@ void free_ctx(cytom_ctx* c) {
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->h_misc) (void)hipHostFree(c->h_misc);
+  if (c->h_scalar) (void)hipHostFree(c->h_scalar);
   if (c->h_iter) (void)hipHostFree(c->h_iter);
   GrowBuf* bufs[] = {&c->g_keys,     &c->g_vals,      &c->g_keys2,     &c->g_vals2,
                      &c->g_flags,    &c->g_scan,      &c->g_seg_start, &c->g_seg_tile,
@@ -159,6 +161,7 @@ int cytom_create(const cytom_config* cfg, cytom_ctx** out) {
             hipMalloc(&c->indeg, V * 4) == hipSuccess &&
             hipMalloc(&c->d_misc, 64 * 8) == hipSuccess &&
             hipHostMalloc(&c->h_misc, 64 * 8, hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc(&c->h_scalar, 8 * 8, hipHostMallocDefault) == hipSuccess &&
             hipMalloc(&c->d_iter, sizeof(IterStats) * (c->max_iter + 2)) == hipSuccess &&
             hipHostMalloc(&c->h_iter, sizeof(IterStats) * (c->max_iter + 2),
                           hipHostMallocDefault) == hipSuccess;

@@ -392,6 +392,18 @@ __device__ inline void apply_tail(unsigned long long te, unsigned long long cnt,
   }
 }
 
+// An iteration whose frontier (nit - 1) is empty gathered nothing and set no
+// changed flag (hooks set a changed flag only together with an active bit),
+// so its apply only has to clear the next active bitmap.  Uniform per grid.
+__device__ inline bool empty_frontier(const IterStats* nit, uint64_t* __restrict__ act_next,
+                                      uint64_t nwords) {
+  if ((nit - 1)->count != 0) return false;
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+       w += (uint64_t)gridDim.x * blockDim.x)
+    act_next[w] = 0;
+  return true;
+}
+
 // Apply kernels: each wavefront takes APPLY_WORDS bitmap words (64
 // consecutive vertices each, one per lane) per step and issues all of their
 // loads before using any, so a wave keeps several memory round trips in
@@ -408,6 +420,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_min(
     uint8_t* __restrict__ rowact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
+  if (empty_frontier(nit, act_next, nwords)) return;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t stride = (((uint64_t)gridDim.x * blockDim.x) >> 6) * APPLY_WORDS;
   unsigned long long te = 0, cnt = 0, upd = 0;
@@ -471,6 +484,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
     const uint32_t* __restrict__ outdeg, IterStats* nit) {
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
+  if (!first && empty_frontier(nit, act_next, nwords)) return;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t stride = (((uint64_t)gridDim.x * blockDim.x) >> 6) * APPLY_WORDS;
   unsigned long long te = 0, cnt = 0, upd = 0;
@@ -822,10 +836,13 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
                                                      P.act[P.cur_act], ctx->outdeg,
                                                      P.rowact[P.cur_act], P.delta, P.contrib,
                                                      P.cur, P.fval, ctx->d_iter);
-  // Iterations are launched in growing groups with one host check per
-  // group; once the frontier is empty the remaining launches of a group are
-  // no-ops (the gather returns at once, apply finds nothing changed).
-  uint32_t it = 0, launched = 0, group = 2;
+  // Iterations are launched in groups with one host check per group.  The
+  // first group is as long as the previous run of the program (PageRank-delta
+  // runs take a similar number of iterations batch after batch), later ones
+  // grow 4, 8, 16.  Launches past convergence are no-ops: the gather returns
+  // at once and the apply only clears the next bitmap.
+  uint32_t it = 0, launched = 0;
+  uint32_t group = std::min<uint32_t>(std::max<uint32_t>(P.last_iters, 2), 64), next = 4;
   bool converged = false;
   const bool prof = ctx->profile && ctx->kev.size() >= 2 * (size_t)maxit;
   while (!converged && launched < maxit) {
@@ -875,8 +892,10 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
         break;
       }
     }
-    group = std::min<uint32_t>(group * 2, 16);
+    group = next;
+    next = std::min<uint32_t>(next * 2, 16);
   }
+  P.last_iters = it;
   CY_HIP(ctx, hipEventRecord(ctx->ev[3], st));
   CY_HIP(ctx, hipMemcpyAsync(ctx->h_iter, ctx->d_iter, sizeof(IterStats) * (it + 1),
                              hipMemcpyDeviceToHost, st));

@@ -91,6 +91,7 @@ struct Program {
   bool fixpoint = false;
   bool frontier_stale = true;
   int last_reexec = 0;
+  uint32_t last_iters = 0;  // iterations of the previous run (first launch group)
 };
 
 // Per-iteration device counters.
@@ -137,6 +138,7 @@ struct cytom_ctx {
   // device scalars: [0] pool_top, [1..] scratch counters
   unsigned long long* d_misc = nullptr;
   unsigned long long* h_misc = nullptr;  // pinned mirror
+  unsigned long long* h_scalar = nullptr;  // pinned landing slots of small reads (8)
   // ingest scratch
   cytom::GrowBuf g_keys, g_vals, g_keys2, g_vals2, g_flags, g_scan, g_seg_start, g_seg_tile,
       g_reloc_cap, g_reloc_list, g_ins_slot, g_scan_tmp, g_hist, g_in_src, g_in_dst, g_in_w,

@@ -582,11 +582,12 @@ int resort_tiles(cytom_ctx* ctx, uint32_t* n_sorted) {
   scan_exclusive_u32(flag, rank, nt, ctx->g_scan_tmp.p, st);
   scan_exclusive_u64(size, off, nt, ctx->g_scan_tmp.p, st);
   k_flag_emit<<<grid_for(nt, 256), 256, 0, st>>>(flag, rank, nt, list);
-  uint32_t n_sel = 0;
-  uint64_t total = 0;
-  CY_HIP(ctx, hipMemcpyAsync(&n_sel, rank + nt, 4, hipMemcpyDeviceToHost, st));
-  CY_HIP(ctx, hipMemcpyAsync(&total, off + nt, 8, hipMemcpyDeviceToHost, st));
+  ctx->h_scalar[0] = 0;
+  CY_HIP(ctx, hipMemcpyAsync(ctx->h_scalar, rank + nt, 4, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipMemcpyAsync(ctx->h_scalar + 1, off + nt, 8, hipMemcpyDeviceToHost, st));
   CY_HIP(ctx, hipStreamSynchronize(st));
+  const uint32_t n_sel = (uint32_t)ctx->h_scalar[0];
+  const uint64_t total = ctx->h_scalar[1];
   if (n_sel == 0) return 0;
   // Keys are (rank, tombstone bit, lv source bits) in 32 bits.
   const uint32_t rmax = 1u << (31 - lv);
