@@ -31,7 +31,7 @@ namespace cytom {
 namespace {
 
 constexpr int GATHER_THREADS = 1024;
-constexpr int APPLY_THREADS = 256;
+constexpr int APPLY_THREADS = 1024;
 constexpr int GATHER_ROWS = 8;  // 64-slot rows per wavefront step
 
 inline uint32_t grid_for(uint64_t n, uint32_t threads, uint32_t cap = 65536) {
@@ -830,8 +830,10 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
   CY_HIP(ctx, hipMemsetAsync(P.rowact[P.cur_act], 0, ctx->T, st));
   CY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
   const uint64_t words = ctx->V >> 6;
+  // One 1024-thread block per CU walks the bitmap words: measured faster than
+  // more, smaller blocks (fewer block tails and end-of-block reductions).
   const uint32_t apply_blocks = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>((uint64_t)ctx->num_cus * 4, (words + 3) / 4));
+      1, std::min<uint64_t>((uint64_t)ctx->num_cus, (words + 3) / 4));
   k_frontier<<<apply_blocks, APPLY_THREADS, 0, st>>>(P.kind, ctx->V, ctx->log2vpt,
                                                      P.act[P.cur_act], ctx->outdeg,
                                                      P.rowact[P.cur_act], P.delta, P.contrib,
