@@ -271,8 +271,11 @@ __device__ inline void gather_rows(const GatherArgs<KIND>& A, uint32_t* win,
 #pragma unroll
       for (int u = 0; u < GATHER_ROWS; ++u) val[u] = A.fval[s[u]];
     }
+    // Issued unconditionally (past the end the clamped descriptors point at
+    // the last row, a valid address): a conditional block here makes the
+    // compiler wait for these loads before the atomics below.
     uint32_t en[GATHER_ROWS], wtn[GATHER_ROWS], xbn[GATHER_ROWS], vmn = 0;
-    if (more) load_rows<KIND>(A, rows, R, rn, en, wtn, xbn, vmn);
+    load_rows<KIND>(A, rows, R, rn, en, wtn, xbn, vmn);
     if (KIND == CYTOM_PRD) {
 #pragma unroll
       for (int u = 0; u < GATHER_ROWS; ++u)
