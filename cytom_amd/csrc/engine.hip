@@ -80,7 +80,7 @@ void free_ctx(cytom_ctx* c) {
                      &c->g_del_tiles, &c->g_found,    &c->g_tile_scan, &c->g_nch,
                      &c->g_cbase,    &c->g_chunks,    &c->g_colj,      &c->g_jbase,
                      &c->g_jobs,     &c->g_rs_e,      &c->g_rs_w,      &c->g_rs_off,
-                     &c->g_rec};
+                     &c->g_rec,      &c->g_del_sig};
   for (GrowBuf* b : bufs) free_buf(*b);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);

@@ -143,7 +143,7 @@ struct cytom_ctx {
   cytom::GrowBuf g_keys, g_vals, g_keys2, g_vals2, g_flags, g_scan, g_seg_start, g_seg_tile,
       g_reloc_cap, g_reloc_list, g_ins_slot, g_scan_tmp, g_hist, g_in_src, g_in_dst, g_in_w,
       g_del_src, g_del_dst, g_del_w, g_tab_key, g_tab_head, g_node_next, g_claimed,
-      g_tile_flag, g_del_tiles, g_found, g_tile_scan, g_rs_e, g_rs_w, g_rs_off, g_rec;
+      g_tile_flag, g_del_tiles, g_found, g_tile_scan, g_rs_e, g_rs_w, g_rs_off, g_rec, g_del_sig;
   // pinned staging of host records (two buffers used in turn)
   void* h_stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};

@@ -264,11 +264,18 @@ __device__ inline uint64_t tab_hash(uint64_t key, uint32_t log_h) {
   return (key * 0x9E3779B97F4A7C15ull) >> (64 - log_h);
 }
 
+// Per-tile 256-bit signature of the batch's deletion keys: a slot whose
+// signature bit is clear cannot match, and skips the table probe (tiles
+// hold one or two requests on average, so almost every slot does).
+__device__ inline uint32_t sig_bit(uint64_t key) {
+  return (uint32_t)((key * 0xC2B2AE3D27D4EB4Full) >> 56);
+}
+
 __global__ void k_del_build(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                             uint64_t m, unsigned long long* __restrict__ tab_key,
                             uint32_t* __restrict__ tab_head, uint32_t log_h,
                             uint32_t* __restrict__ node_next, uint32_t* __restrict__ tile_flag,
-                            uint32_t lv, uint32_t T) {
+                            unsigned long long* __restrict__ tile_sig, uint32_t lv, uint32_t T) {
   const uint64_t hmask = (1ull << log_h) - 1;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * blockDim.x) {
@@ -281,7 +288,9 @@ __global__ void k_del_build(const uint32_t* __restrict__ src, const uint32_t* __
       h = (h + 1) & hmask;
     }
     node_next[i] = atomicExch(&tab_head[h], (uint32_t)i);
-    tile_flag[tile_of(s, d, lv, T)] = 1u;
+    const uint32_t t = tile_of(s, d, lv, T), b = sig_bit(key);
+    tile_flag[t] = 1u;
+    atomicOr(&tile_sig[4 * (uint64_t)t + (b >> 6)], 1ull << (b & 63u));
   }
 }
 
@@ -324,6 +333,7 @@ __global__ void __launch_bounds__(256) k_del_match(const uint64_t* __restrict__ 
                                                    uint32_t log_h,
                                                    const uint32_t* __restrict__ node_next,
                                                    const uint32_t* __restrict__ dw,
+                                                   const unsigned long long* __restrict__ tile_sig,
                                                    unsigned long long* __restrict__ claimed) {
   const uint64_t hmask = (1ull << log_h) - 1;
   const uint32_t nchunks = *nchunks_p;
@@ -336,6 +346,9 @@ __global__ void __launch_bounds__(256) k_del_match(const uint64_t* __restrict__ 
     const uint32_t xb = (t % S.T) * S.vpt, yb = (t / S.T) * S.vpt;
     const uint64_t off = S.tile_off[t] + s0;
     const uint32_t n = min(DEL_CH, S.tile_cnt[t] - s0);
+    const unsigned long long sg[4] = {tile_sig[4 * (uint64_t)t], tile_sig[4 * (uint64_t)t + 1],
+                                      tile_sig[4 * (uint64_t)t + 2],
+                                      tile_sig[4 * (uint64_t)t + 3]};
     for (uint32_t base = 0; base < n; base += 64 * DM_UNROLL) {
       uint32_t le[DM_UNROLL];
 #pragma unroll
@@ -346,7 +359,8 @@ __global__ void __launch_bounds__(256) k_del_match(const uint64_t* __restrict__ 
       for (int u = 0; u < DM_UNROLL; ++u) {
         e[u] = pack_edge(xb + (le[u] & 0xFFFFu), yb + (le[u] >> 16));
         h[u] = tab_hash(e[u], log_h);
-        k[u] = tab_key[h[u]];
+        const uint32_t b = sig_bit(e[u]);
+        k[u] = (sg[b >> 6] >> (b & 63u)) & 1ull ? tab_key[h[u]] : EMPTY64;
       }
 #pragma unroll
       for (int u = 0; u < DM_UNROLL; ++u) {
@@ -458,28 +472,78 @@ __global__ void k_resort_cut(const uint32_t* __restrict__ flag, const uint32_t* 
       atomicMin(&misc[M_RS_CUT], (unsigned long long)rank[t]);
 }
 
-// One block per selected tile: keys (rank, tombstone bit, source) and the
-// slot's position in the staging copy.
+// The staging copy of the selected tiles is cut into RS_BLOCK-slot pieces,
+// one per block of k_resort_gather; bfirst[b] = the selected tile holding
+// piece b's first slot (tiles lie back to back in list order).  Hot tiles
+// span many pieces, small ones share a piece.
+constexpr uint32_t RS_BLOCK = 1024;
+constexpr uint32_t RS_TILES = RS_BLOCK / RESORT_MIN_TAIL + 2;  // tiles one piece can touch
+
+__global__ void k_resort_pieces(const uint32_t* __restrict__ list, uint32_t nproc,
+                                const uint64_t* __restrict__ off,
+                                const uint32_t* __restrict__ tile_cnt,
+                                uint32_t* __restrict__ bfirst) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nproc;
+       r += gridDim.x * blockDim.x) {
+    const uint32_t t = list[r];
+    const uint64_t b0 = off[t], b1 = b0 + tile_cnt[t];
+    for (uint64_t b = (b0 + RS_BLOCK - 1) / RS_BLOCK; b * RS_BLOCK < b1; ++b) bfirst[b] = r;
+  }
+}
+
+// Keys (rank, tombstone bit, source) and staging copies of one piece of
+// selected slots per block.
 __global__ void __launch_bounds__(256) k_resort_gather(StoreView S, const uint32_t* __restrict__ list,
                                                        uint32_t nproc, const uint64_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ bfirst,
+                                                       uint32_t nslots,
                                                        uint32_t* __restrict__ keys,
                                                        uint32_t* __restrict__ vals,
                                                        uint32_t* __restrict__ e_st,
                                                        uint32_t* __restrict__ w_st) {
+  __shared__ uint32_t s_rank[RS_TILES], s_base[RS_TILES];
+  __shared__ uint64_t s_src[RS_TILES];
+  __shared__ uint32_t s_n;
   const uint32_t lv = S.log2vpt;
-  for (uint32_t r = blockIdx.x; r < nproc; r += gridDim.x) {
-    const uint32_t t = list[r];
-    const uint32_t cnt = S.tile_cnt[t];
-    const uint64_t src = S.tile_off[t];
-    const uint32_t base = (uint32_t)off[t];
-    for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
-      const uint32_t e = S.pool[src + i];
-      const uint32_t low = e == TOMB ? (1u << lv) : (e & 0xFFFFu);
-      keys[base + i] = (r << (lv + 1)) | low;
-      vals[base + i] = base + i;
-      e_st[base + i] = e;
-      if (S.wpool) w_st[base + i] = S.wpool[src + i];
+  for (uint32_t b = blockIdx.x; (uint64_t)b * RS_BLOCK < nslots; b += gridDim.x) {
+    const uint32_t k0 = b * RS_BLOCK, k1 = min(k0 + RS_BLOCK, nslots);
+    if (threadIdx.x < RS_TILES) {
+      const uint32_t r = bfirst[b] + threadIdx.x;
+      const bool in = r < nproc && (threadIdx.x == 0 || off[list[r]] < k1);
+      if (in) {
+        const uint32_t t = list[r];
+        s_rank[threadIdx.x] = r;
+        s_base[threadIdx.x] = (uint32_t)off[t];
+        s_src[threadIdx.x] = S.tile_off[t];
+      }
+      // the tiles of a piece are consecutive in the list: count them
+      const unsigned long long m = __ballot(in);
+      if (threadIdx.x == 0) s_n = (uint32_t)__popcll(m);
     }
+    __syncthreads();
+    const uint32_t n = s_n;
+    uint32_t e[RS_BLOCK / 256], wv[RS_BLOCK / 256], j[RS_BLOCK / 256];
+#pragma unroll
+    for (int u = 0; u < (int)(RS_BLOCK / 256); ++u) {
+      const uint32_t k = min(k0 + u * 256 + threadIdx.x, k1 - 1);
+      uint32_t jj = 0;
+      for (uint32_t q = 1; q < n; ++q) jj = s_base[q] <= k ? q : jj;
+      j[u] = jj;
+      const uint64_t slot = s_src[jj] + (k - s_base[jj]);
+      e[u] = S.pool[slot];
+      wv[u] = S.wpool ? S.wpool[slot] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < (int)(RS_BLOCK / 256); ++u) {
+      const uint32_t k = k0 + u * 256 + threadIdx.x;
+      if (k >= k1) continue;
+      const uint32_t low = e[u] == TOMB ? (1u << lv) : (e[u] & 0xFFFFu);
+      keys[k] = (s_rank[j[u]] << (lv + 1)) | low;
+      vals[k] = k;
+      e_st[k] = e[u];
+      if (S.wpool) w_st[k] = wv[u];
+    }
+    __syncthreads();
   }
 }
 
@@ -626,8 +690,11 @@ int resort_tiles(cytom_ctx* ctx, uint32_t* n_sorted) {
   auto* svals = static_cast<uint32_t*>(ctx->g_vals2.p);
   auto* e_st = static_cast<uint32_t*>(ctx->g_rs_e.p);
   auto* w_st = static_cast<uint32_t*>(ctx->g_rs_w.p);
-  k_resort_gather<<<std::min<uint32_t>(nproc, 16384), 256, 0, st>>>(ctx->view(), list, nproc, off,
-                                                                    keys, vals, e_st, w_st);
+  const uint32_t pieces = (uint32_t)((nslots + RS_BLOCK - 1) / RS_BLOCK);
+  auto* bfirst = static_cast<uint32_t*>(ctx->g_flags.p);  // free until the sort below
+  k_resort_pieces<<<grid_for(nproc, 256), 256, 0, st>>>(list, nproc, off, ctx->tile_cnt, bfirst);
+  k_resort_gather<<<std::min<uint32_t>(pieces, 16384), 256, 0, st>>>(
+      ctx->view(), list, nproc, off, bfirst, (uint32_t)nslots, keys, vals, e_st, w_st);
   uint32_t rank_bits = 0;
   while ((1ull << rank_bits) < nproc) ++rank_bits;
   radix_sort_pairs(keys, vals, skeys, svals, static_cast<uint32_t*>(ctx->g_flags.p),
@@ -761,6 +828,7 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
         (rc = ensure(ctx, ctx->g_node_next, n_del * 4)) ||
         (rc = ensure(ctx, ctx->g_claimed, n_del * 8)) ||
         (rc = ensure(ctx, ctx->g_tile_flag, ctx->n_tiles * 4)) ||
+        (rc = ensure(ctx, ctx->g_del_sig, ctx->n_tiles * 32)) ||
         (rc = ensure(ctx, ctx->g_tile_scan, (ctx->n_tiles + 1) * 16 + 256)) ||
         (rc = ensure(ctx, ctx->g_del_tiles, ctx->n_tiles * 4)) ||
         (rc = ensure(ctx, ctx->g_found, n_del * 12)) ||
@@ -785,8 +853,11 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
     CY_HIP(ctx, hipMemsetAsync(tab_head, 0xFF, H * 4, st));
     CY_HIP(ctx, hipMemsetAsync(claimed, 0xFF, n_del * 8, st));
     CY_HIP(ctx, hipMemsetAsync(tile_flag, 0, ctx->n_tiles * 4, st));
+    auto* tile_sig = static_cast<unsigned long long*>(ctx->g_del_sig.p);
+    CY_HIP(ctx, hipMemsetAsync(tile_sig, 0, ctx->n_tiles * 32, st));
     k_del_build<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, tab_key, tab_head,
-                                                      log_h, node_next, tile_flag, lv, T);
+                                                      log_h, node_next, tile_flag, tile_sig,
+                                                      lv, T);
     scan_exclusive_u32(tile_flag, tscan, ctx->n_tiles, ctx->g_scan_tmp.p, st);
     k_flag_emit<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(tile_flag, tscan, ctx->n_tiles,
                                                             del_tiles);
@@ -797,7 +868,7 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
     k_del_chunks<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(dnch, dcbase, ctx->n_tiles, dchunks);
     k_del_match<<<(uint32_t)ctx->num_cus * 4, 256, 0, st>>>(
         dchunks, dcbase + ctx->n_tiles, ctx->view(), tab_key, tab_head, log_h, node_next,
-        ctx->weighted ? d_dw : nullptr, claimed);
+        ctx->weighted ? d_dw : nullptr, tile_sig, claimed);
     k_del_finish<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, claimed,
                                                        ctx->view(), found, misc);
     CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st));
