@@ -866,7 +866,7 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
                                                            dnch);
     scan_exclusive_u32(dnch, dcbase, ctx->n_tiles, ctx->g_scan_tmp.p, st);
     k_del_chunks<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(dnch, dcbase, ctx->n_tiles, dchunks);
-    k_del_match<<<(uint32_t)ctx->num_cus * 4, 256, 0, st>>>(
+    k_del_match<<<(uint32_t)ctx->num_cus * 8, 256, 0, st>>>(
         dchunks, dcbase + ctx->n_tiles, ctx->view(), tab_key, tab_head, log_h, node_next,
         ctx->weighted ? d_dw : nullptr, tile_sig, claimed);
     k_del_finish<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, claimed,
