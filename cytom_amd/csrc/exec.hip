@@ -672,6 +672,8 @@ void launch_gather(cytom_ctx* ctx, Program& P, int it) {
   k_gather<KIND><<<gather_grid(ctx, lds), GATHER_THREADS, lds, ctx->stream>>>(A);
 }
 
+}  // namespace
+
 int build_jobs(cytom_ctx* ctx) {
   hipStream_t st = ctx->stream;
   int rc;
@@ -710,8 +712,6 @@ int build_jobs(cytom_ctx* ctx) {
   ctx->jobs_dirty = false;
   return 0;
 }
-
-}  // namespace
 
 int program_alloc(cytom_ctx* ctx, Program& P) {
   const uint64_t V = ctx->V;

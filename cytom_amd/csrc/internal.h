@@ -251,6 +251,9 @@ int program_set_sources(cytom_ctx* ctx, Program& P, const uint32_t* sources, uin
 int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* st);
 // Sets the gather kernels' LDS limit and persistent grid for the context.
 int configure_gather(cytom_ctx* ctx);
+// Rebuilds the work lists of the current store (chunks of chunk_slots slots
+// per tile in g_chunks, n_chunks of them, and the column jobs).
+int build_jobs(cytom_ctx* ctx);
 
 // ------------------------------------------------------------ batched queries
 int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run_stats* stats);

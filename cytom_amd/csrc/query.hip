@@ -98,43 +98,67 @@ __global__ void k_widen(const uint32_t* __restrict__ in, uint64_t* __restrict__ 
     out[i] = in[i];
 }
 
-// Live edges of every tile into their source's CSR row (the order inside a
-// row is immaterial: every kernel below reduces exactly).
-__global__ void __launch_bounds__(QT) k_csr_fill(StoreView S, unsigned long long* __restrict__ cursor,
-                                                 uint32_t* __restrict__ col,
-                                                 uint32_t* __restrict__ wt) {
-  for (uint64_t t = blockIdx.x; t < S.n_tiles; t += gridDim.x) {
-    const uint32_t cnt = S.tile_cnt[t];
-    if (cnt == 0) continue;
-    const uint64_t off = S.tile_off[t];
-    const uint32_t xb = (uint32_t)(t % S.T) << S.log2vpt, yb = (uint32_t)(t / S.T) << S.log2vpt;
-    for (uint32_t i = threadIdx.x; i < cnt; i += QT) {
-      const uint32_t e = S.pool[off + i];
-      if (e == TOMB) continue;
-      const uint32_t s = xb + (e & 0xFFFFu);
-      const unsigned long long p = atomicAdd(&cursor[s], 1ull);
-      col[p] = yb + (e >> 16);
-      if (wt) wt[p] = S.wpool[off + i];
+// Snapshot build: the live slots of every work-list chunk are compacted into
+// (src, dst[, w]) arrays (one wavefront per chunk, ballot ranks), then a
+// stable radix sort of (source, export index) gives the CSR order and one of
+// (target, export index) the CSC order.  No per-vertex cursors: hub vertices
+// cost no atomic contention.
+__global__ void k_chunk_live(const uint64_t* __restrict__ chunks, uint64_t n_chunks, StoreView S,
+                             uint32_t CH, uint64_t* __restrict__ live) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < n_chunks;
+       c += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
+    const uint64_t ch = chunks[c];
+    const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
+    const uint32_t n = min(CH, S.tile_cnt[t] - s0);
+    const uint64_t off = S.tile_off[t] + s0;
+    uint32_t k = 0;
+    for (uint32_t i = lane; i < n; i += 64) k += S.pool[off + i] != TOMB ? 1u : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o, 64);
+    if (lane == 0) live[c] = k;
+  }
+}
+
+__global__ void k_chunk_export(const uint64_t* __restrict__ chunks, uint64_t n_chunks,
+                               StoreView S, uint32_t CH, const uint64_t* __restrict__ base,
+                               uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                               uint32_t* __restrict__ w, uint32_t* __restrict__ idx) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < n_chunks;
+       c += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
+    const uint64_t ch = chunks[c];
+    const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
+    const uint32_t n = min(CH, S.tile_cnt[t] - s0);
+    const uint64_t off = S.tile_off[t] + s0;
+    const uint32_t xb = (t % S.T) << S.log2vpt, yb = (t / S.T) << S.log2vpt;
+    uint64_t p = base[c];
+    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+      const uint32_t i = i0 + lane;
+      const uint32_t e = i < n ? S.pool[off + i] : TOMB;
+      const bool live = e != TOMB;
+      const uint64_t m = __ballot(live);
+      if (live) {
+        const uint64_t q = p + __popcll(m & ((1ull << lane) - 1ull));
+        src[q] = xb + (e & 0xFFFFu);
+        dst[q] = yb + (e >> 16);
+        if (w) w[q] = S.wpool[off + i];
+        idx[q] = (uint32_t)q;
+      }
+      p += __popcll(m);
     }
   }
 }
 
-// Live edges into their target's CSC row.
-__global__ void __launch_bounds__(QT) k_csc_fill(StoreView S, unsigned long long* __restrict__ cursor,
-                                                 uint32_t* __restrict__ in_src,
-                                                 uint32_t* __restrict__ in_w) {
-  for (uint64_t t = blockIdx.x; t < S.n_tiles; t += gridDim.x) {
-    const uint32_t cnt = S.tile_cnt[t];
-    if (cnt == 0) continue;
-    const uint64_t off = S.tile_off[t];
-    const uint32_t xb = (uint32_t)(t % S.T) << S.log2vpt, yb = (uint32_t)(t / S.T) << S.log2vpt;
-    for (uint32_t i = threadIdx.x; i < cnt; i += QT) {
-      const uint32_t e = S.pool[off + i];
-      if (e == TOMB) continue;
-      const unsigned long long p = atomicAdd(&cursor[yb + (e >> 16)], 1ull);
-      in_src[p] = xb + (e & 0xFFFFu);
-      if (in_w) in_w[p] = S.wpool[off + i];
-    }
+// out[i] = in[perm[i]] (and the weights alongside).
+__global__ void k_permute(const uint32_t* __restrict__ perm, uint64_t n,
+                          const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                          const uint32_t* __restrict__ win, uint32_t* __restrict__ wout) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t j = perm[i];
+    out[i] = in[j];
+    if (wout) wout[i] = win[j];
   }
 }
 
@@ -656,6 +680,56 @@ void free_snapshot(QueryEngine* qe) {
   qe->snap_batch = ~0ull;
 }
 
+// CSR (col, wt) and CSC (in_src, in_w) rows of the live edges by export and
+// two stable radix sorts; temporaries are freed before returning.
+int fill_rows(cytom_ctx* ctx, QueryEngine* qe, uint64_t E) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  if (ctx->jobs_dirty && (rc = build_jobs(ctx))) return rc;
+  if (E > 0xFFFFFFFFull) return set_err(ctx, CYTOM_ERANGE, "snapshot above 2^32 edges");
+  const uint64_t nc = ctx->n_chunks;
+  uint32_t bits = 0;
+  while ((1ull << bits) < ctx->V) ++bits;
+  uint64_t *live = nullptr, *base = nullptr;
+  uint32_t *src = nullptr, *dst = nullptr, *w = nullptr, *idx = nullptr, *ko = nullptr,
+           *vo = nullptr, *kt = nullptr, *vt = nullptr;
+  void *hist = nullptr, *tmp = nullptr;
+  auto done = [&](int r) {
+    (void)hipStreamSynchronize(st);
+    void* ps[] = {live, base, src, dst, w, idx, ko, vo, kt, vt, hist, tmp};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+    return r;
+  };
+  const uint64_t n = std::max<uint64_t>(E, 1);
+  if ((rc = dev_alloc(ctx, &live, (nc + 1) * 8)) || (rc = dev_alloc(ctx, &base, (nc + 1) * 8)) ||
+      (rc = dev_alloc(ctx, &src, n * 4)) || (rc = dev_alloc(ctx, &dst, n * 4)) ||
+      (ctx->wpool && (rc = dev_alloc(ctx, &w, n * 4))) || (rc = dev_alloc(ctx, &idx, n * 4)) ||
+      (rc = dev_alloc(ctx, &ko, n * 4)) || (rc = dev_alloc(ctx, &vo, n * 4)) ||
+      (rc = dev_alloc(ctx, &kt, n * 4)) || (rc = dev_alloc(ctx, &vt, n * 4)) ||
+      (rc = dev_alloc(ctx, (char**)&hist, radix_hist_bytes(n))) ||
+      (rc = dev_alloc(ctx, (char**)&tmp,
+                      scan_tmp_bytes(std::max<uint64_t>({n, nc + 1, 256 * ((n + 4095) / 4096) + 1})) +
+                          256)))
+    return done(rc);
+  const auto* chunks = static_cast<const uint64_t*>(ctx->g_chunks.p);
+  if (nc) {
+    k_chunk_live<<<blocks_for(nc, 4, 16384), 256, 0, st>>>(chunks, nc, ctx->view(),
+                                                          ctx->chunk_slots, live);
+    scan_exclusive_u64(live, base, nc, tmp, st);
+    k_chunk_export<<<blocks_for(nc, 4, 16384), 256, 0, st>>>(chunks, nc, ctx->view(),
+                                                            ctx->chunk_slots, base, src, dst, w,
+                                                            idx);
+  }
+  if (E) {
+    radix_sort_pairs(src, idx, ko, vo, kt, vt, E, bits, hist, tmp, st);
+    k_permute<<<blocks_for(E, 256, 65536), 256, 0, st>>>(vo, E, dst, qe->col, w, qe->wt);
+    radix_sort_pairs(dst, idx, ko, vo, kt, vt, E, bits, hist, tmp, st);
+    k_permute<<<blocks_for(E, 256, 65536), 256, 0, st>>>(vo, E, src, qe->in_src, w, qe->in_w);
+  }
+  return done(0);
+}
+
 // CSR snapshot of the live graph (rebuilt after every update batch).
 int snapshot(cytom_ctx* ctx, QueryEngine* qe) {
   if (qe->snap_batch == ctx->batches && qe->row_off) return 0;
@@ -675,18 +749,12 @@ int snapshot(cytom_ctx* ctx, QueryEngine* qe) {
       (rc = dev_alloc(ctx, &qe->counter, 256)) ||
       (rc = dev_alloc(ctx, (char**)&qe->scan_tmp, scan_tmp_bytes(V + 1) + 256)))
     return rc;
-  // row_off = exclusive scan of out-degrees; fdeg doubles as the cursor.
+  // row_off / in_off = exclusive scans of the out- / in-degrees.
   k_widen<<<blocks_for(V, 256), 256, 0, st>>>(ctx->outdeg, qe->fdeg, V);
   scan_exclusive_u64(qe->fdeg, qe->row_off, V, qe->scan_tmp, st);
-  CY_HIP(ctx, hipMemcpyAsync(qe->fdeg, qe->row_off, V * 8, hipMemcpyDeviceToDevice, st));
-  k_csr_fill<<<blocks_for(ctx->n_tiles, 1, 16384), QT, 0, st>>>(
-      ctx->view(), reinterpret_cast<unsigned long long*>(qe->fdeg), qe->col, qe->wt);
-  // in_off from in-degrees the same way.
   k_widen<<<blocks_for(V, 256), 256, 0, st>>>(ctx->indeg, qe->fdeg, V);
   scan_exclusive_u64(qe->fdeg, qe->in_off, V, qe->scan_tmp, st);
-  CY_HIP(ctx, hipMemcpyAsync(qe->fdeg, qe->in_off, V * 8, hipMemcpyDeviceToDevice, st));
-  k_csc_fill<<<blocks_for(ctx->n_tiles, 1, 16384), QT, 0, st>>>(
-      ctx->view(), reinterpret_cast<unsigned long long*>(qe->fdeg), qe->in_src, qe->in_w);
+  if ((rc = fill_rows(ctx, qe, E))) return rc;
   CY_HIP(ctx, hipMemsetAsync(qe->counter, 0, 4, st));
   k_heavy_list<<<blocks_for(V, 256, 16384), 256, 0, st>>>(ctx->indeg, V, qe->heavy, qe->counter);
   uint64_t total = 0, total_in = 0;
