@@ -58,6 +58,9 @@ struct QueryEngine {
   uint32_t* mark = nullptr;   // V
   unsigned int* counter = nullptr;  // [0] next-list length
   void* scan_tmp = nullptr;
+  // BFS group masks (allocated by the first BFS group)
+  uint64_t *bvis = nullptr, *bfront = nullptr, *bnxt = nullptr;
+  uint32_t* bsrc = nullptr;
   // results: per query (kind, group, lane)
   struct Q {
     int kind;
@@ -663,7 +666,8 @@ void free_results(QueryEngine* qe) {
 void free_snapshot(QueryEngine* qe) {
   void* ps[] = {qe->row_off, qe->col,     qe->wt,   qe->in_off,  qe->in_src,  qe->in_w,
                 qe->heavy,   qe->flist,   qe->nlist, qe->glist, qe->fdeg,   qe->fpre,    qe->mark,
-                qe->counter, qe->scan_tmp, qe->seg_v, qe->seg_b};
+                qe->counter, qe->scan_tmp, qe->seg_v, qe->seg_b, qe->bvis, qe->bfront,
+                qe->bnxt,    qe->bsrc};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   qe->row_off = qe->in_off = nullptr;
@@ -677,6 +681,8 @@ void free_snapshot(QueryEngine* qe) {
   qe->mark = nullptr;
   qe->counter = nullptr;
   qe->scan_tmp = nullptr;
+  qe->bvis = qe->bfront = qe->bnxt = nullptr;
+  qe->bsrc = nullptr;
   qe->snap_batch = ~0ull;
 }
 
@@ -801,8 +807,9 @@ int frontier_prefix(cytom_ctx* ctx, QueryEngine* qe, uint32_t n, uint64_t* te) {
   hipStream_t st = ctx->stream;
   k_frontier_deg<<<blocks_for(n, 256), 256, 0, st>>>(qe->flist, n, ctx->outdeg, qe->fdeg);
   scan_exclusive_u64(qe->fdeg, qe->fpre, n, qe->scan_tmp, st);
-  CY_HIP(ctx, hipMemcpyAsync(te, qe->fpre + n, 8, hipMemcpyDeviceToHost, st));
+  CY_HIP(ctx, hipMemcpyAsync(ctx->h_scalar + 2, qe->fpre + n, 8, hipMemcpyDeviceToHost, st));
   CY_HIP(ctx, hipStreamSynchronize(st));
+  *te = ctx->h_scalar[2];
   return 0;
 }
 
@@ -813,8 +820,11 @@ uint32_t expand_blocks(cytom_ctx* ctx, uint64_t te, uint32_t edges_per_wave) {
 }
 
 int read_counter(cytom_ctx* ctx, QueryEngine* qe, uint32_t* n) {
-  CY_HIP(ctx, hipMemcpyAsync(n, qe->counter, 4, hipMemcpyDeviceToHost, ctx->stream));
+  ctx->h_scalar[3] = 0;
+  CY_HIP(ctx, hipMemcpyAsync(ctx->h_scalar + 3, qe->counter, 4, hipMemcpyDeviceToHost,
+                             ctx->stream));
   CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *n = (uint32_t)ctx->h_scalar[3];
   return 0;
 }
 
@@ -833,16 +843,16 @@ int run_bfs_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<uint32_t>& 
   hipStream_t st = ctx->stream;
   const uint64_t V = ctx->V;
   int rc;
-  uint64_t *vis = nullptr, *front = nullptr, *nxt = nullptr;
-  uint32_t* dsrc = nullptr;
-  if ((rc = dev_alloc(ctx, &vis, V * 8)) || (rc = dev_alloc(ctx, &front, V * 8)) ||
-      (rc = dev_alloc(ctx, &nxt, V * 8)) || (rc = dev_alloc(ctx, &dsrc, src.size() * 4))) {
-    (void)hipFree(vis), (void)hipFree(front), (void)hipFree(nxt), (void)hipFree(dsrc);
+  // visited / frontier / next masks live with the snapshot (one group at a time)
+  if (!qe->bvis && ((rc = dev_alloc(ctx, &qe->bvis, V * 8)) ||
+                    (rc = dev_alloc(ctx, &qe->bfront, V * 8)) ||
+                    (rc = dev_alloc(ctx, &qe->bnxt, V * 8)) ||
+                    (rc = dev_alloc(ctx, &qe->bsrc, BFS_W * 4))))
     return rc;
-  }
+  uint64_t *vis = qe->bvis, *front = qe->bfront, *nxt = qe->bnxt;
+  uint32_t* dsrc = qe->bsrc;
   auto done = [&](int r) {
     (void)hipStreamSynchronize(st);
-    (void)hipFree(vis), (void)hipFree(front), (void)hipFree(nxt), (void)hipFree(dsrc);
     return r;
   };
   if (hipMemsetAsync(vis, 0, V * 8, st) || hipMemsetAsync(front, 0, V * 8, st) ||
