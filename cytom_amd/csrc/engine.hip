@@ -364,16 +364,12 @@ int cytom_set_tuning(cytom_ctx* ctx, const cytom_tuning* t) {
   if (!t) return set_err(ctx, CYTOM_EINVAL, "null tuning");
   if (t->chunk_slots && (t->chunk_slots % 64 || t->chunk_slots > 1024))
     return set_err(ctx, CYTOM_EINVAL, "chunk_slots must be a multiple of 64 up to 1024");
-  if (t->job_chunks > 512) return set_err(ctx, CYTOM_EINVAL, "job_chunks above 512");
+  if (t->job_chunks > (1u << 20)) return set_err(ctx, CYTOM_EINVAL, "job_chunks above 2^20");
   if (t->blocks_per_cu > 16) return set_err(ctx, CYTOM_EINVAL, "blocks_per_cu above 16");
   if (t->source_bands && (t->source_bands & (t->source_bands - 1)))
     return set_err(ctx, CYTOM_EINVAL, "source_bands must be a power of two");
-  const uint32_t cs = t->chunk_slots ? t->chunk_slots : ctx->chunk_slots;
-  const uint32_t jc = t->job_chunks ? t->job_chunks : ctx->job_chunks;
-  if ((uint64_t)cs * jc > (1u << 17))
-    return set_err(ctx, CYTOM_EINVAL, "job_chunks * chunk_slots above 131072 slots per job");
-  ctx->chunk_slots = cs;
-  ctx->job_chunks = jc;
+  if (t->chunk_slots) ctx->chunk_slots = t->chunk_slots;
+  if (t->job_chunks) ctx->job_chunks = t->job_chunks;
   if (t->blocks_per_cu) ctx->gather_blocks = (uint32_t)ctx->num_cus * t->blocks_per_cu;
   if (t->source_bands) ctx->source_bands = t->source_bands;
   if (t->resort_pct) ctx->resort_pct = t->resort_pct;

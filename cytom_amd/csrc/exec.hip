@@ -201,8 +201,7 @@ struct GatherArgs {
   float qscale;  // 2^prd_shift
 };
 
-constexpr uint32_t MAX_JOB_CHUNKS = GATHER_THREADS;  // one metadata pass per job
-// Row descriptors of one job held in LDS (job_chunks * chunk_slots / 64).
+// Row descriptors of one pass of a job held in LDS.
 constexpr uint32_t MAX_JOB_ROWS = 2048;
 
 // The active chunks of a job form one list of 64-slot rows: a chunk of n
@@ -319,6 +318,10 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
     A.rowact_clear[i] = 0;
   if (A.it->count == 0) return;  // empty frontier (uniform)
   const uint32_t njobs = *A.njobs;
+  // A job's chunks are taken in passes of at most PASS chunks (one thread
+  // each, at most MAX_JOB_ROWS rows); the window lives across the passes and
+  // is flushed once per job.
+  const uint32_t PASS = min((uint32_t)GATHER_THREADS, (MAX_JOB_ROWS * 64u) / A.CH);
   unsigned long long scanned = 0;
   for (;;) {
     if (threadIdx.x == 0) s_job = (uint32_t)atomicAdd(&A.it->job_cursor, 1ull);
@@ -326,37 +329,45 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
     const uint32_t j = s_job;
     if (j >= njobs) break;  // uniform: every thread read s_job after the barrier
     const uint4 job = A.jobs[j];
-    const uint32_t c0 = job.x, nc = job.y - job.x, y = job.z;
-    // Row table of the job's chunks whose source row has an active vertex
-    // (TileManager::isActive, tile-manager.h:403-455).  One scan packs the
-    // rows (bits 24+) and slots (bits 0-23) before each chunk.
-    uint32_t flag = 0, n = 0, x = 0;
-    uint64_t off = 0;
-    if (threadIdx.x < nc) {
-      const uint64_t ch = A.chunks[c0 + threadIdx.x];
-      const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
-      x = t - y * T;
-      flag = A.rowact[x];
-      off = A.S.tile_off[t] + s0;
-      n = min(A.CH, A.S.tile_cnt[t] - s0);
+    const uint32_t y = job.z;
+    bool cleared = false;  // uniform
+    for (uint32_t cs = job.x; cs < job.y; cs += PASS) {
+      const uint32_t nc = min(PASS, job.y - cs);
+      // Row table of the pass's chunks whose source row has an active vertex
+      // (TileManager::isActive, tile-manager.h:403-455).  One scan packs the
+      // rows (bits 24+) and slots (bits 0-23) before each chunk.
+      uint32_t flag = 0, n = 0, x = 0;
+      uint64_t off = 0;
+      if (threadIdx.x < nc) {
+        const uint64_t ch = A.chunks[cs + threadIdx.x];
+        const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
+        x = t - y * T;
+        flag = A.rowact[x];
+        off = A.S.tile_off[t] + s0;
+        n = min(A.CH, A.S.tile_cnt[t] - s0);
+      }
+      const uint32_t nrows = flag ? (n + 63) >> 6 : 0u;
+      const uint64_t packed = ((uint64_t)nrows << 24) | (uint64_t)(flag ? n : 0u);
+      uint64_t tot;
+      const uint64_t ex = block_scan_excl<GATHER_THREADS, uint64_t>(packed, &tot);
+      const uint32_t R = (uint32_t)(tot >> 24);
+      if (R == 0) continue;  // uniform; the scan's barriers ordered s_rows / s_job reuse
+      for (uint32_t k = 0, r = (uint32_t)(ex >> 24); k < nrows; ++k, ++r)
+        s_rows[r] = row_desc(off + 64 * k, x, min(64u, n - 64 * k));
+      scanned += tot & 0xFFFFFFull;
+      if (!cleared) {
+        if (KIND == CYTOM_PRD) {
+          for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) winq[i] = 0ull;
+        } else {
+          for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) win[i] = INF;
+        }
+        cleared = true;
+      }
+      __syncthreads();
+      gather_rows<KIND>(A, win, winq, s_rows, R);
+      __syncthreads();
     }
-    const uint32_t nrows = flag ? (n + 63) >> 6 : 0u;
-    const uint64_t packed = ((uint64_t)nrows << 24) | (uint64_t)(flag ? n : 0u);
-    uint64_t tot;
-    const uint64_t ex = block_scan_excl<GATHER_THREADS, uint64_t>(packed, &tot);
-    const uint32_t R = (uint32_t)(tot >> 24);
-    if (R == 0) continue;  // uniform; the scan's barriers ordered s_job reuse
-    for (uint32_t j = 0, r = (uint32_t)(ex >> 24); j < nrows; ++j, ++r)
-      s_rows[r] = row_desc(off + 64 * j, x, min(64u, n - 64 * j));
-    scanned += tot & 0xFFFFFFull;
-    if (KIND == CYTOM_PRD) {
-      for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) winq[i] = 0ull;
-    } else {
-      for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) win[i] = INF;
-    }
-    __syncthreads();
-    gather_rows<KIND>(A, win, winq, s_rows, R);
-    __syncthreads();
+    if (!cleared) continue;  // no active row in the job
     const uint64_t base = (uint64_t)y * vpt;
     for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) {
       if (KIND == CYTOM_PRD) {
@@ -697,14 +708,19 @@ int build_jobs(cytom_ctx* ctx) {
   if ((rc = ensure(ctx, ctx->g_chunks, std::max<uint64_t>(n_chunks, 1) * 8))) return rc;
   k_emit_chunks<<<grid_for(nt, 256), 256, 0, st>>>(nch, cbase, nt, ctx->chunk_slots,
                                                    static_cast<uint64_t*>(ctx->g_chunks.p));
-  k_col_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, ctx->T, NB, ctx->job_chunks, colj);
+  // Jobs of job_chunks chunks, or sized from the store: at least 2^17 slots,
+  // and about eight jobs per CU when the store is large (fewer column flushes).
+  const uint32_t JC =
+      ctx->job_chunks ? ctx->job_chunks
+                      : (uint32_t)std::max<uint64_t>((1u << 17) / ctx->chunk_slots,
+                                                     n_chunks / ((uint64_t)ctx->num_cus * 8));
+  k_col_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, ctx->T, NB, JC, colj);
   scan_exclusive_u32(colj, jbase, cells, ctx->g_scan_tmp.p, st);
   CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc + 33, jbase + cells, 4, hipMemcpyDeviceToHost, st));
   CY_HIP(ctx, hipStreamSynchronize(st));
   const uint64_t n_jobs = (uint32_t)ctx->h_misc[33];
   if ((rc = ensure(ctx, ctx->g_jobs, std::max<uint64_t>(n_jobs, 1) * 16))) return rc;
-  k_emit_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, colj, jbase, ctx->T, NB,
-                                                    ctx->job_chunks,
+  k_emit_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, colj, jbase, ctx->T, NB, JC,
                                                     static_cast<uint4*>(ctx->g_jobs.p));
   ctx->n_chunks = n_chunks;
   ctx->n_cells = cells;

@@ -154,7 +154,7 @@ struct cytom_ctx {
   bool jobs_dirty = true;
   uint64_t n_chunks = 0, n_cells = 0, n_jobs = 0;
   uint32_t chunk_slots = 512;   // slots per chunk
-  uint32_t job_chunks = 256;    // chunks per column job (job_chunks * chunk_slots <= 2^17)
+  uint32_t job_chunks = 0;      // chunks per column job (0: sized from the store)
   uint32_t source_bands = 1;    // source-row bands swept one after the other
   // A tile is sort-compacted once its unsorted tail reaches 256 slots and
   // resort_pct percent of its sorted prefix (CYTOM_RESORT_OFF = never).

@@ -226,7 +226,8 @@ int64_t cytom_export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_
  * run.py:115-132).  Zero fields keep the current value. */
 typedef struct {
   uint32_t chunk_slots;   /* edge slots per work chunk (multiple of 64, <= 1024) */
-  uint32_t job_chunks;    /* chunks per column job (<= 512; job_chunks * chunk_slots <= 131072) */
+  uint32_t job_chunks;    /* chunks per column job (<= 2^20; reported 0 = sized from the
+                             store: at least 131072 slots, ~8 jobs per CU for large stores) */
   uint32_t blocks_per_cu; /* persistent gather blocks per compute unit */
   uint32_t source_bands;  /* source-row bands swept in turn (power of two) */
   uint32_t resort_pct;    /* sort-compact a tile once its unsorted tail reaches this

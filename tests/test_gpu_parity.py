@@ -140,8 +140,11 @@ def test_launch_shapes_agree_at_rmat20():
     # hooks relax concurrently, so only their fixpoint values are compared.
     scale, V = 20, 1 << 20
     st = O.Stream(2)
+    # 64-slot chunks in jobs of 8192 chunks: heavy columns take several
+    # 1024-chunk passes per job.
     shapes = [dict(source_bands=1), dict(source_bands=8, resort_pct=C.RESORT_OFF),
-              dict(source_bands=32, chunk_slots=256, job_chunks=16, resort_pct=5)]
+              dict(source_bands=32, chunk_slots=256, job_chunks=16, resort_pct=5),
+              dict(chunk_slots=64, job_chunks=8192)]
     graphs = []
     for t in shapes:
         g = C.Graph(V, weighted=True)
