@@ -255,8 +255,9 @@ def run_jobs(jobs, store=None, out=print, seed=RMAT_SEED):
         kinds.append(k)
     source = _source(jobs[0])
     summaries = [None] * len(jobs)
-    for idxs in group_jobs(jobs):
+    for store, idxs in enumerate(group_jobs(jobs)):
         for s in run_group(jobs, idxs, kinds, source, out, seed):
+            s["store"] = store  # jobs with the same index shared one tile store
             summaries[s["job"]] = s
     return summaries
 

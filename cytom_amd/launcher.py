@@ -10,10 +10,12 @@ ConfigLists), turns each into the 40-key job of run.py:107-148
 ``python -m cytom_amd.execute``:
 
 * ``--sequential`` / ``--profile``: one process per configuration, one after
-  the other (run.py:905-914 profile_all);
-* ``--concurrent``: the configurations of one graph run as the jobs of one
-  process, sharing its edge stream (run.py:496-596 groups them per dataset);
-  graphs follow one another on the one device;
+  the other, each recording its device usage as that algorithm's profile on
+  that graph (run.py:905-914 profile_all; cytom_amd/scheduler.py);
+* ``--concurrent``: the device scheduler of cytom_amd/scheduler.py
+  (run.py:415-681): per graph, windows of configurations run as the jobs of
+  one process, sharing its tile store, and several such processes share the
+  GPU while their predicted device usage and memory fit;
 * ``--print-only``: prints the job command lines instead of running them;
 * ``--algorithm-store`` with ``--add/--delete/--clear/--list``: the registry
   of cytom_amd/algorithm_store.py (run.py:683-759).
@@ -27,13 +29,13 @@ import argparse
 import math
 import os
 import shlex
-import subprocess
 import sys
 import time
 from collections import OrderedDict
 
 from .algorithm_store import AlgorithmStore
 from .jobs import job_argv
+from .scheduler import Algo, Scheduler, task_command
 
 # ---------------------------------------------------------------- scripts/config.py
 GRAPH_SETTINGS_DELIM = {
@@ -265,23 +267,7 @@ def job_values(args, c):
 
 def job_command(job_lists):
     """Command line of one process running the given jobs."""
-    cmd = [sys.executable, "-m", "cytom_amd.execute"]
-    for j in job_lists:
-        cmd += j
-    return cmd
-
-
-def _run_task(task_id, cmd, algos, graph, out):
-    env = dict(os.environ)
-    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    env["PYTHONPATH"] = root + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
-    out("-------Now launch task %d : %s on %s" % (task_id, algos, graph))
-    t0 = time.time()
-    rc = subprocess.call(cmd, env=env)
-    out("------------------------------------------")
-    out("Task %d has finished:  %s on %s (%.1f s, exit %d)"
-        % (task_id, algos, graph, time.time() - t0, rc))
-    return rc
+    return task_command(job_lists)
 
 
 def tasks(args, store):
@@ -340,13 +326,21 @@ def main(argv=None, out=print):
         for _, _, jl in todo:
             out(" ".join(shlex.quote(x) for x in job_command(jl)))
         return 0
+    algos = []
+    for c in configurations(args, store):
+        job = job_values(args, c)
+        algos.append(Algo(c["algorithm"], c["graph"], job_argv(job), job))
+    base = os.path.join(out_root(), "logs", "%s-%d" % (time.strftime("%Y%m%d-%H%M%S"),
+                                                       os.getpid()))
+    run_logs, k = base, 1
+    while os.path.exists(run_logs):  # one directory per launcher run
+        run_logs, k = "%s-%d" % (base, k), k + 1
+    sched = Scheduler(out=out, log_root=run_logs)
     t0 = time.time()
-    failed = 0
-    for task_id, (algos, graph, jl) in enumerate(todo):
-        if _run_task(task_id, job_command(jl), algos, graph, out):
-            failed += 1
+    failed = sched.run_all(algos) if args.concurrent else sched.profile_all(algos)
     out("----------Overall status--------------")
-    out("Tasks: %d, failed: %d, running time (s): %.1f" % (len(todo), failed, time.time() - t0))
+    out("Tasks: %d, failed: %d, running time (s): %.1f"
+        % (len(sched.finished), failed, time.time() - t0))
     return 1 if failed else 0
 
 

@@ -12,6 +12,7 @@ checked against the oracle through the C-ABI.
   graph.
 """
 import collections
+import json
 import os
 
 import numpy as np
@@ -258,3 +259,36 @@ def test_launcher_rmat_jobs_in_process(tmp_path, monkeypatch, capsys):
     stats = [l.split() for l in out.splitlines() if len(l.split()) == 2 and
              all(x.isdigit() for x in l.split())]
     assert sum(int(c) for _, c in stats) == 65536
+
+
+def test_launcher_profile_then_concurrent_tasks(tmp_path, monkeypatch):
+    # --profile runs each configuration alone and records its device usage;
+    # --concurrent then schedules windows of them as child processes that
+    # share the GPU, and learns from their measured usage.
+    monkeypatch.setenv("CYTOM_ROOT", str(tmp_path))
+    monkeypatch.setenv("CYTOM_ALGORITHM_STORE", str(tmp_path / "store.json"))
+    state = tmp_path / "scheduler-state.json"
+    monkeypatch.setenv("CYTOM_SCHEDULER_STATE", str(state))
+    flags = ["--algorithm", "bfs,cc", "--graph", "rmat-16,rmat-20", "--max-edges", "65536",
+             "--insertion-batch-size", "16384"]
+    for mode, n_tasks in (("--profile", 4), ("--concurrent", None)):
+        print("launcher %s: rmat-16 and rmat-20, 65536 edges, bfs + cc" % mode, flush=True)
+        out = []
+        assert launcher.main(flags + [mode], out=out.append) == 0, "\n".join(out)
+        done = [o for o in out if o.startswith("Task ") and "has finished" in o]
+        assert all(o.endswith("exit 0)") for o in done)
+        if n_tasks:
+            assert len(done) == n_tasks
+        summaries = [json.loads(o) for o in out if o.startswith("{")]
+        assert len(summaries) == 4 and all(s["executions"] == 4 for s in summaries)
+        assert {s["algorithm"] for s in summaries} == {"bfs", "cc"}
+    st = json.loads(state.read_text())
+    prof = st["profile_data"]
+    assert sorted(prof) == ["rmat-16", "rmat-20"]
+    for g in prof.values():
+        assert sorted(g) == ["bfs", "cc"]
+        assert all(0.0 < v["gpu_usage"] <= 100.0 and v["running_time"] > 0 for v in g.values())
+    assert len(st["model_X"]) >= 1 and len(st["model_X"]) == len(st["model_y"])
+    # one log per task, in a directory per launcher run
+    runs = sorted(os.listdir(tmp_path / "logs"))
+    assert len(runs) == 2 and len(os.listdir(tmp_path / "logs" / runs[0])) == 4

@@ -25,6 +25,7 @@ def roots(tmp_path, monkeypatch):
     monkeypatch.setenv("CYTOM_ROOT", str(tmp_path / "root"))
     monkeypatch.setenv("CYTOM_DATA_ROOT", str(tmp_path / "data"))
     monkeypatch.setenv("CYTOM_ALGORITHM_STORE", str(tmp_path / "store.json"))
+    monkeypatch.setenv("CYTOM_SCHEDULER_STATE", str(tmp_path / "scheduler-state.json"))
     return tmp_path
 
 
