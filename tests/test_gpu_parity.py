@@ -213,6 +213,57 @@ def test_medium_stream_against_checkers():
         assert l1n(gg.read(ids[3]), og.read(op)) <= PRD_L1_TOL
 
 
+def test_rmat22_stream_full_size():
+    # The measured workload at its full size: RMAT-22, 64 batches of 943,719
+    # inserts + 104,857 deletions drawn from the live edges, BFS/SSSP from
+    # vertex 0, CC and PageRank-delta after every batch (53.7 M live edges at
+    # the end).  Size-independent checks: after batches 32 and 64 the
+    # BFS/CC/SSSP values equal the textbook checkers on the exported live
+    # graph (bit-exact); the final PageRank-delta run from init equals the
+    # oracle's run from init on the same graph within the L1 bound, and when
+    # the stream's last run re-initialised it, the stream's ranks equal that
+    # run bit for bit (fixed-point sums make it deterministic).
+    import os
+    scale, V, n_ins, n_del = 22, 1 << 22, 943719, 104857
+    st = C.Stream(2)
+    g = C.Graph(V, weighted=True, max_iterations=100, delta=1e-7,
+                edge_capacity=3 * (1 << 26))
+    ids = {n: g.register(n, 0) for n in ("bfs", "cc", "sssp", "pagerank-delta")}
+    last = {}
+    for b in range(64):
+        first, didx = st.next(n_ins, n_del)
+        s, d, w = C.rmat_generate(scale, 2, first, n_ins, weights=True)
+        di = C.DeviceArray.from_host(didx)
+        ds, dd, dw = C.rmat_gather(scale, 2, di, len(didx), weights=True)
+        bs = g.update_device(s, d, w, n_ins, ds, dd, dw, len(didx))
+        assert bs.deleted == len(didx) and bs.missing == 0, b  # drawn from live edges
+        for n, pid in ids.items():
+            last[n] = g.execute(pid)
+            assert last[n].converged == 1 or n == "pagerank-delta", (b, n)
+        if b in (31, 63):
+            print("rmat22 stream: checkers after batch %d" % (b + 1), flush=True)
+            ls, ld, lw = g.export_edges()
+            assert len(ls) == bs.live_edges
+            assert np.array_equal(g.read(ids["bfs"]), O.check_bfs(V, ls, ld, 0)), b
+            assert np.array_equal(g.read(ids["cc"]), O.check_cc(V, ls, ld)), b
+            assert np.array_equal(g.read(ids["sssp"]), O.check_sssp(V, ls, ld, lw, 0)), b
+    assert bs.live_edges == 64 * (n_ins - n_del)
+    prd = ids["pagerank-delta"]
+    stream_ranks = g.read(prd)
+    g.init(prd)
+    g.execute(prd)
+    ranks = g.read(prd)
+    if last["pagerank-delta"].reexecuted:
+        assert np.array_equal(stream_ranks, ranks)
+    print("rmat22 stream: oracle PageRank-delta on the final graph", flush=True)
+    og = O.Graph(V, vertices_per_tile=16384, max_iterations=100, delta=1e-7,
+                 threads=min(16, os.cpu_count() or 1))
+    og.update(ls, ld)
+    op = og.register(O.PRD)
+    og.execute(op)
+    assert l1n(ranks, og.read(op)) <= PRD_L1_TOL
+
+
 # ------------------------------------------------------------------ edge cases
 def test_empty_and_missing_and_invalid():
     g = C.Graph(256, vertices_per_tile=64, weighted=True)
