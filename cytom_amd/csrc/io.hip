@@ -68,13 +68,26 @@ __global__ void k_results_prd(const long long* __restrict__ dsum, const float* _
   }
 }
 
+// Both pinned buffers and both events, or none of them (a partial failure is
+// released so the next call tries again).
 int ensure_stage(cytom_ctx* ctx) {
-  if (ctx->h_stage[0]) return 0;
+  if (ctx->h_stage[0] && ctx->h_stage[1] && ctx->stage_ev[0] && ctx->stage_ev[1]) return 0;
+  io_free(ctx);
   for (int b = 0; b < 2; ++b) {
-    if (hipHostMalloc(&ctx->h_stage[b], STAGE_BYTES, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc(&ctx->h_stage[b], STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+      ctx->h_stage[b] = nullptr;
+      io_free(ctx);
       return set_err(ctx, CYTOM_ENOMEM, "pinned staging allocation failed");
-    CY_HIP(ctx, hipEventCreateWithFlags(&ctx->stage_ev[b], hipEventDisableTiming));
+    }
+    const hipError_t e = hipEventCreateWithFlags(&ctx->stage_ev[b], hipEventDisableTiming);
+    if (e != hipSuccess) {
+      ctx->stage_ev[b] = nullptr;
+      io_free(ctx);
+      return hip_fail(ctx, e, "staging event");
+    }
   }
+  ctx->stage_busy[0] = ctx->stage_busy[1] = false;
+  ctx->stage_next = 0;
   return 0;
 }
 
