@@ -223,6 +223,10 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp
                         hipStream_t st);
 void scan_exclusive_u64(const uint64_t* in, uint64_t* out, uint64_t n, void* tmp,
                         hipStream_t st);
+// The same with the length read from device memory (*n_dev <= n_max) when
+// the scan runs; the launch and tmp are sized for n_max.
+void scan_exclusive_u64_devn(const uint64_t* in, uint64_t* out, const uint32_t* n_dev,
+                             uint64_t n_max, void* tmp, hipStream_t st);
 // Stable LSD radix sort of (key, value) pairs on the low key_bits bits.
 // Result lands in (k_out, v_out); (k_tmp, v_tmp) are scratch of n entries;
 // hist must hold radix_hist_bytes(n) bytes.

@@ -1,6 +1,8 @@
 // Device primitives: exclusive scan and stable LSD radix sort (gfx950,
 // 64-lane wavefronts).  Used by ingestion (sort of updates by tile key,
 // segment heads) and by the executor work-list builder.
+#include <algorithm>
+
 #include "internal.h"
 #include "devutil.h"
 
@@ -13,9 +15,13 @@ constexpr uint64_t SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 2048
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+// n_dev, when given, caps the length at run time (the launch is sized for n):
+// elements from *n_dev on count as zero and are not written.
 template <typename T>
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const T* __restrict__ in,
-                                                              uint64_t n, T* __restrict__ bsum) {
+                                                              uint64_t n, T* __restrict__ bsum,
+                                                              const uint32_t* __restrict__ n_dev) {
+  if (n_dev) n = std::min<uint64_t>(n, *n_dev);
   const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
   T s = 0;
 #pragma unroll
@@ -30,7 +36,9 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const T* __restric
 template <typename T>
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(const T* __restrict__ in,
                                                             T* __restrict__ out, uint64_t n,
-                                                            const T* __restrict__ boff) {
+                                                            const T* __restrict__ boff,
+                                                            const uint32_t* __restrict__ n_dev) {
+  if (n_dev) n = std::min<uint64_t>(n, *n_dev);
   const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
   T v[SCAN_ITEMS];
   T s = 0;
@@ -50,17 +58,21 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(const T* __restrict_
     if (i < n) out[i] = p;
     p += v[k];
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = off + tot;
+  // The block holding position n (or the last block) writes the total: the
+  // elements from n on are zero, so its block total is the prefix at n.
+  if (blockIdx.x == std::min<uint64_t>(n / SCAN_TILE, gridDim.x - 1) && threadIdx.x == 0)
+    out[n] = off + tot;
 }
 
 template <typename T>
-void scan_impl(const T* in, T* out, uint64_t n, void* tmp, hipStream_t st) {
+void scan_impl(const T* in, T* out, uint64_t n, void* tmp, hipStream_t st,
+               const uint32_t* n_dev = nullptr) {
   if (n == 0) {
     hipMemsetAsync(out, 0, sizeof(T), st);
     return;
   }
   if (n <= SCAN_TILE) {
-    k_scan_down<T><<<1, SCAN_THREADS, 0, st>>>(in, out, n, nullptr);
+    k_scan_down<T><<<1, SCAN_THREADS, 0, st>>>(in, out, n, nullptr, n_dev);
     return;
   }
   uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -68,9 +80,9 @@ void scan_impl(const T* in, T* out, uint64_t n, void* tmp, hipStream_t st) {
   T* boff = reinterpret_cast<T*>(static_cast<char*>(tmp) + align256(nb * sizeof(T)));
   void* rest =
       static_cast<char*>(tmp) + align256(nb * sizeof(T)) + align256((nb + 1) * sizeof(T));
-  k_scan_reduce<T><<<(uint32_t)nb, SCAN_THREADS, 0, st>>>(in, n, bsum);
-  scan_impl<T>(bsum, boff, nb, rest, st);
-  k_scan_down<T><<<(uint32_t)nb, SCAN_THREADS, 0, st>>>(in, out, n, boff);
+  k_scan_reduce<T><<<(uint32_t)nb, SCAN_THREADS, 0, st>>>(in, n, bsum, n_dev);
+  scan_impl<T>(bsum, boff, nb, rest, st);  // block sums past *n_dev are zero
+  k_scan_down<T><<<(uint32_t)nb, SCAN_THREADS, 0, st>>>(in, out, n, boff, n_dev);
 }
 
 // ------------------------------------------------------------ radix sort
@@ -157,6 +169,11 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp
 void scan_exclusive_u64(const uint64_t* in, uint64_t* out, uint64_t n, void* tmp,
                         hipStream_t st) {
   scan_impl<uint64_t>(in, out, n, tmp, st);
+}
+
+void scan_exclusive_u64_devn(const uint64_t* in, uint64_t* out, const uint32_t* n_dev,
+                             uint64_t n_max, void* tmp, hipStream_t st) {
+  scan_impl<uint64_t>(in, out, std::max<uint64_t>(n_max, 1), tmp, st, n_dev);
 }
 
 size_t radix_hist_bytes(uint64_t n) {

@@ -26,12 +26,25 @@
 // finds its first edge by binary search over the frontier's degree prefix
 // sum), so hub vertices do not serialise the expansion.
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "devutil.h"
 #include "internal.h"
 
 namespace cytom {
+
+// Device-side control of a BFS group's levels: every kernel of a level reads
+// the frontier size and the push/pull choice here, so the host enqueues
+// several levels between synchronisations instead of reading counts back
+// after each one.
+struct BfsCtl {
+  uint32_t n;      // frontier size of the next level (0: the group is done)
+  uint32_t next;   // next-list length (the expansion's append counter)
+  uint32_t mode;   // this level: 0 nothing, 1 push along out-edges, 2 pull along in-edges
+  uint32_t level;  // levels run so far
+  unsigned long long te, active, iters;  // run statistics
+};
 
 struct QueryEngine {
   // CSR snapshot
@@ -61,6 +74,7 @@ struct QueryEngine {
   // BFS group masks (allocated by the first BFS group)
   uint64_t *bvis = nullptr, *bfront = nullptr, *bnxt = nullptr;
   uint32_t* bsrc = nullptr;
+  BfsCtl* bctl = nullptr;
   // results: per query (kind, group, lane)
   struct Q {
     int kind;
@@ -172,6 +186,31 @@ __global__ void k_heavy_list(const uint32_t* __restrict__ indeg, uint64_t V,
     if (indeg[v] > HUB) list[atomicAdd(counter, 1u)] = (uint32_t)v;
 }
 
+// In-edge segments of the heavy targets: counts, then (after a scan) the
+// segments [b, min(b + SEG, end)) of each target's in-edge range.
+__global__ void k_seg_count(const uint32_t* __restrict__ heavy, uint32_t n,
+                            const uint64_t* __restrict__ in_off, uint64_t* __restrict__ nseg) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t v = heavy[i];
+    nseg[i] = (in_off[v + 1] - in_off[v] + SEG - 1) / SEG;
+  }
+}
+
+__global__ void k_seg_emit(const uint32_t* __restrict__ heavy, uint32_t n,
+                           const uint64_t* __restrict__ in_off, const uint64_t* __restrict__ sbase,
+                           uint32_t* __restrict__ seg_v, uint64_t* __restrict__ seg_b) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t v = heavy[i];
+    const uint64_t e1 = in_off[v + 1];
+    uint64_t k = sbase[i];
+    for (uint64_t e = in_off[v]; e < e1; e += SEG, ++k) {
+      seg_v[k] = v;
+      seg_b[2 * k] = e;
+      seg_b[2 * k + 1] = min(e + SEG, e1);
+    }
+  }
+}
+
 // ------------------------------------------------------------- frontier
 __global__ void k_frontier_deg(const uint32_t* __restrict__ flist, uint32_t n,
                                const uint32_t* __restrict__ outdeg, uint64_t* __restrict__ fdeg) {
@@ -190,15 +229,46 @@ __device__ inline uint32_t find_entry(const uint64_t* fpre, uint32_t n, uint64_t
   return lo;
 }
 
+// Frontier out-degrees of a BFS level (the frontier size from the control
+// block).
+__global__ void k_bfs_deg(const uint32_t* __restrict__ flist, const BfsCtl* __restrict__ c,
+                          const uint32_t* __restrict__ outdeg, uint64_t* __restrict__ fdeg) {
+  const uint32_t n = c->n;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    fdeg[i] = outdeg[flist[i]];
+}
+
+// Level start (one thread): a frontier whose out-edges (fpre[n]) exceed
+// E / PULL_DIV pulls along in-edges, a smaller one pushes; statistics; the
+// next-list counter restarts.
+__global__ void k_bfs_level(BfsCtl* c, const uint64_t* __restrict__ fpre, uint64_t E) {
+  const uint32_t n = c->n;
+  c->next = 0;
+  if (n == 0) {
+    c->mode = 0;
+    return;
+  }
+  const uint64_t te = fpre[n];
+  c->mode = te * PULL_DIV > E ? 2u : (te ? 1u : 0u);
+  c->level += 1;
+  c->te += te;
+  c->active += n;
+  c->iters += 1;
+}
+
+// Level end (one thread): the next list becomes the frontier.
+__global__ void k_bfs_level_end(BfsCtl* c) { c->n = c->next; }
+
 // Edge layout of the expansion: wavefront w owns edges
 // [w*64*EPL, (w+1)*64*EPL) of the frontier's concatenated out-edges, lane l
 // takes edges l, l+64, ... of that range (coalesced column reads).
 __global__ void __launch_bounds__(QT) k_bfs_expand(
-    const uint32_t* __restrict__ flist, const uint64_t* __restrict__ fpre, uint32_t n,
+    const uint32_t* __restrict__ flist, const uint64_t* __restrict__ fpre, BfsCtl* c,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ col,
     const uint64_t* __restrict__ front, const uint64_t* __restrict__ vis,
-    unsigned long long* __restrict__ nxt, uint32_t* __restrict__ nlist,
-    unsigned int* __restrict__ counter) {
+    unsigned long long* __restrict__ nxt, uint32_t* __restrict__ nlist) {
+  if (c->mode != 1u) return;  // uniform
+  const uint32_t n = c->n;
   const uint64_t TE = fpre[n];
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * QT + threadIdx.x) >> 6;
@@ -215,7 +285,7 @@ __global__ void __launch_bounds__(QT) k_bfs_expand(
       const uint64_t m = front[u] & ~vis[v];
       if (m) {
         const unsigned long long old = atomicOr(&nxt[v], (unsigned long long)m);
-        if (old == 0ull) nlist[atomicAdd(counter, 1u)] = v;
+        if (old == 0ull) nlist[atomicAdd(&c->next, 1u)] = v;
       }
     }
   }
@@ -223,9 +293,11 @@ __global__ void __launch_bounds__(QT) k_bfs_expand(
 
 // New levels of the vertices reached this round; the reached set becomes
 // the next frontier.  front[] of the old frontier was cleared before.
-__global__ void k_bfs_apply(const uint32_t* __restrict__ nlist, uint32_t n, uint16_t level,
+__global__ void k_bfs_apply(const uint32_t* __restrict__ nlist, const BfsCtl* __restrict__ c,
                             uint64_t* __restrict__ nxt, uint64_t* __restrict__ vis,
                             uint64_t* __restrict__ front, uint16_t* __restrict__ lev) {
+  const uint32_t n = c->next;
+  const uint16_t level = (uint16_t)c->level;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t v = nlist[i];
     const uint64_t nb = nxt[v] & ~vis[v];
@@ -241,8 +313,9 @@ __global__ void k_bfs_apply(const uint32_t* __restrict__ nlist, uint32_t n, uint
   }
 }
 
-__global__ void k_clear_front(const uint32_t* __restrict__ flist, uint32_t n,
+__global__ void k_clear_front(const uint32_t* __restrict__ flist, const BfsCtl* __restrict__ c,
                               uint64_t* __restrict__ front) {
+  const uint32_t n = c->n;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     front[flist[i]] = 0;
 }
@@ -453,7 +526,8 @@ __global__ void k_ppr_seed(const uint32_t* __restrict__ members, const uint32_t*
 __global__ void k_bfs_pull(const uint64_t* __restrict__ in_off, const uint32_t* __restrict__ in_src,
                            uint64_t V, uint64_t qmask, const uint64_t* __restrict__ front,
                            const uint64_t* __restrict__ vis, uint64_t* __restrict__ nxt,
-                           uint32_t* __restrict__ nlist, unsigned int* __restrict__ counter) {
+                           uint32_t* __restrict__ nlist, BfsCtl* c) {
+  if (c->mode != 2u) return;  // uniform
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t e0 = in_off[v], e1 = in_off[v + 1];
@@ -471,7 +545,7 @@ __global__ void k_bfs_pull(const uint64_t* __restrict__ in_off, const uint32_t* 
     m &= want;
     if (m) {
       nxt[v] = m;
-      nlist[atomicAdd(counter, 1u)] = (uint32_t)v;
+      nlist[atomicAdd(&c->next, 1u)] = (uint32_t)v;
     }
   }
 }
@@ -518,8 +592,9 @@ __global__ void __launch_bounds__(QT) k_bfs_pull_seg(
     const uint32_t* __restrict__ seg_v, const uint64_t* __restrict__ seg_b, uint32_t n_seg,
     const uint32_t* __restrict__ in_src, uint64_t qmask, const uint64_t* __restrict__ front,
     const uint64_t* __restrict__ vis, unsigned long long* __restrict__ nxt,
-    uint32_t* __restrict__ nlist, unsigned int* __restrict__ counter) {
+    uint32_t* __restrict__ nlist, BfsCtl* c) {
   __shared__ unsigned long long m_s;
+  if (c->mode != 2u) return;  // uniform
   for (uint32_t sg = blockIdx.x; sg < n_seg; sg += gridDim.x) {
     const uint32_t v = seg_v[sg];
     const uint64_t want = qmask & ~vis[v];
@@ -539,7 +614,7 @@ __global__ void __launch_bounds__(QT) k_bfs_pull_seg(
     if (m) atomicOr(&m_s, (unsigned long long)m);
     __syncthreads();
     if (threadIdx.x == 0 && m_s) {
-      if (atomicOr(&nxt[v], m_s) == 0ull) nlist[atomicAdd(counter, 1u)] = v;
+      if (atomicOr(&nxt[v], m_s) == 0ull) nlist[atomicAdd(&c->next, 1u)] = v;
     }
     __syncthreads();
   }
@@ -667,7 +742,7 @@ void free_snapshot(QueryEngine* qe) {
   void* ps[] = {qe->row_off, qe->col,     qe->wt,   qe->in_off,  qe->in_src,  qe->in_w,
                 qe->heavy,   qe->flist,   qe->nlist, qe->glist, qe->fdeg,   qe->fpre,    qe->mark,
                 qe->counter, qe->scan_tmp, qe->seg_v, qe->seg_b, qe->bvis, qe->bfront,
-                qe->bnxt,    qe->bsrc};
+                qe->bnxt,    qe->bsrc,    qe->bctl};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   qe->row_off = qe->in_off = nullptr;
@@ -683,6 +758,7 @@ void free_snapshot(QueryEngine* qe) {
   qe->scan_tmp = nullptr;
   qe->bvis = qe->bfront = qe->bnxt = nullptr;
   qe->bsrc = nullptr;
+  qe->bctl = nullptr;
   qe->snap_batch = ~0ull;
 }
 
@@ -771,32 +847,25 @@ int snapshot(cytom_ctx* ctx, QueryEngine* qe) {
   CY_HIP(ctx, hipStreamSynchronize(st));
   if (total != E || total_in != E)
     return set_err(ctx, CYTOM_EDEVICE, "degree sums differ from live edges");
-  // Segments of the heavy targets' in-edge ranges.
-  std::vector<uint32_t> hv(qe->n_heavy), sv;
-  std::vector<uint64_t> sb;
-  if (qe->n_heavy) {
-    CY_HIP(ctx, hipMemcpy(hv.data(), qe->heavy, hv.size() * 4, hipMemcpyDeviceToHost));
-    std::sort(hv.begin(), hv.end());
-    std::vector<uint64_t> off(V + 1);
-    CY_HIP(ctx, hipMemcpy(off.data(), qe->in_off, (V + 1) * 8, hipMemcpyDeviceToHost));
-    for (uint32_t v : hv) {
-      const uint64_t r[2] = {off[v], off[v + 1]};
-      for (uint64_t b = r[0]; b < r[1]; b += SEG) {
-        sv.push_back(v);
-        sb.push_back(b);
-        sb.push_back(std::min(b + SEG, r[1]));
-      }
-    }
-    CY_HIP(ctx, hipMemcpy(qe->heavy, hv.data(), hv.size() * 4, hipMemcpyHostToDevice));
+  // Segments of the heavy targets' in-edge ranges, built on the device (the
+  // frontier scratch arrays fdeg / fpre are free here).  Their order does not
+  // matter: the pull kernels combine segments with OR, min or exact sums.
+  const uint32_t nh = qe->n_heavy;
+  uint64_t n_seg = 0;
+  if (nh) {
+    k_seg_count<<<blocks_for(nh, 256), 256, 0, st>>>(qe->heavy, nh, qe->in_off, qe->fdeg);
+    scan_exclusive_u64(qe->fdeg, qe->fpre, nh, qe->scan_tmp, st);
+    CY_HIP(ctx, hipMemcpyAsync(ctx->h_scalar + 4, qe->fpre + nh, 8, hipMemcpyDeviceToHost, st));
+    CY_HIP(ctx, hipStreamSynchronize(st));
+    n_seg = ctx->h_scalar[4];
   }
-  qe->n_seg = (uint32_t)sv.size();
-  if ((rc = dev_alloc(ctx, &qe->seg_v, sv.size() * 4)) ||
-      (rc = dev_alloc(ctx, &qe->seg_b, sb.size() * 8)))
+  if (n_seg > 0xFFFFFFFFull) return set_err(ctx, CYTOM_ERANGE, "too many in-edge segments");
+  qe->n_seg = (uint32_t)n_seg;
+  if ((rc = dev_alloc(ctx, &qe->seg_v, n_seg * 4)) || (rc = dev_alloc(ctx, &qe->seg_b, n_seg * 16)))
     return rc;
-  if (!sv.empty()) {
-    CY_HIP(ctx, hipMemcpy(qe->seg_v, sv.data(), sv.size() * 4, hipMemcpyHostToDevice));
-    CY_HIP(ctx, hipMemcpy(qe->seg_b, sb.data(), sb.size() * 8, hipMemcpyHostToDevice));
-  }
+  if (n_seg)
+    k_seg_emit<<<blocks_for(nh, 256), 256, 0, st>>>(qe->heavy, nh, qe->in_off, qe->fpre, qe->seg_v,
+                                                    qe->seg_b);
   qe->E = E;
   qe->snap_batch = ctx->batches;
   return 0;
@@ -838,6 +907,10 @@ int seed_frontier(cytom_ctx* ctx, QueryEngine* qe, std::vector<uint32_t> v, uint
   return 0;
 }
 
+// Levels are enqueued in rounds without host round trips: each kernel reads
+// the frontier size and the push/pull choice from the control block, and a
+// level whose frontier is empty does nothing.  The host reads the control
+// block after each round (8 levels, then 4) to see whether the group is done.
 int run_bfs_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<uint32_t>& src,
                   uint16_t* lev, cytom_run_stats* acc) {
   hipStream_t st = ctx->stream;
@@ -847,54 +920,62 @@ int run_bfs_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<uint32_t>& 
   if (!qe->bvis && ((rc = dev_alloc(ctx, &qe->bvis, V * 8)) ||
                     (rc = dev_alloc(ctx, &qe->bfront, V * 8)) ||
                     (rc = dev_alloc(ctx, &qe->bnxt, V * 8)) ||
-                    (rc = dev_alloc(ctx, &qe->bsrc, BFS_W * 4))))
+                    (rc = dev_alloc(ctx, &qe->bsrc, BFS_W * 4)) ||
+                    (rc = dev_alloc(ctx, &qe->bctl, sizeof(BfsCtl)))))
     return rc;
   uint64_t *vis = qe->bvis, *front = qe->bfront, *nxt = qe->bnxt;
   uint32_t* dsrc = qe->bsrc;
+  BfsCtl* c = qe->bctl;
   auto done = [&](int r) {
     (void)hipStreamSynchronize(st);
     return r;
   };
+  std::vector<uint32_t> seeds(src);
+  std::sort(seeds.begin(), seeds.end());
+  seeds.erase(std::unique(seeds.begin(), seeds.end()), seeds.end());
+  const BfsCtl init{(uint32_t)seeds.size(), 0, 0, 0, 0, 0, 0};
   if (hipMemsetAsync(vis, 0, V * 8, st) || hipMemsetAsync(front, 0, V * 8, st) ||
       hipMemsetAsync(nxt, 0, V * 8, st) || hipMemsetAsync(lev, 0xFF, V * BFS_W * 2, st) ||
-      hipMemcpyAsync(dsrc, src.data(), src.size() * 4, hipMemcpyHostToDevice, st))
+      hipMemcpyAsync(dsrc, src.data(), src.size() * 4, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(qe->flist, seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(c, &init, sizeof(BfsCtl), hipMemcpyHostToDevice, st))
     return done(set_err(ctx, CYTOM_EDEVICE, "BFS group set-up failed"));
   k_bfs_seed<<<1, 64, 0, st>>>(dsrc, (uint32_t)src.size(), vis, front, lev);
   const uint64_t qmask = src.size() >= 64 ? ~0ull : ((1ull << src.size()) - 1);
-  uint32_t n = 0;
-  if ((rc = seed_frontier(ctx, qe, src, &n))) return done(rc);
-  uint32_t level = 0;
-  while (n > 0) {
-    if (level + 1 >= LEVEL_NONE) return done(set_err(ctx, CYTOM_ERANGE, "BFS depth above 65534"));
-    ++level;
-    uint64_t te = 0;
-    if ((rc = frontier_prefix(ctx, qe, n, &te))) return done(rc);
-    acc->traversed_edges += te;
-    acc->active_vertices += n;
-    CY_HIP(ctx, hipMemsetAsync(qe->counter, 0, 4, st));
-    if (te * PULL_DIV > qe->E) {
+  const uint32_t fixed = (uint32_t)ctx->num_cus * 4;  // grid of the per-frontier kernels
+  uint32_t* lists[2] = {qe->flist, qe->nlist};
+  uint32_t enqueued = 0;
+  BfsCtl h{};
+  for (uint32_t round = 8;; round = 4) {
+    if (enqueued + round + 1 >= LEVEL_NONE)
+      return done(set_err(ctx, CYTOM_ERANGE, "BFS depth above 65534"));
+    for (uint32_t k = 0; k < round; ++k, ++enqueued) {
+      uint32_t* fl = lists[enqueued & 1];
+      uint32_t* nl = lists[(enqueued + 1) & 1];
+      k_bfs_deg<<<fixed, 256, 0, st>>>(fl, c, ctx->outdeg, qe->fdeg);
+      scan_exclusive_u64_devn(qe->fdeg, qe->fpre, &c->n, V, qe->scan_tmp, st);
+      k_bfs_level<<<1, 1, 0, st>>>(c, qe->fpre, qe->E);
       k_bfs_pull<<<blocks_for(V, 256, 16384), 256, 0, st>>>(qe->in_off, qe->in_src, V, qmask,
-                                                             front, vis, nxt, qe->nlist,
-                                                             qe->counter);
+                                                             front, vis, nxt, nl, c);
       if (qe->n_seg)
         k_bfs_pull_seg<<<std::min<uint32_t>(qe->n_seg, 16384), QT, 0, st>>>(
             qe->seg_v, qe->seg_b, qe->n_seg, qe->in_src, qmask, front, vis,
-            reinterpret_cast<unsigned long long*>(nxt), qe->nlist, qe->counter);
-    } else if (te) {
-      k_bfs_expand<<<expand_blocks(ctx, te, 64 * EPL), QT, 0, st>>>(
-          qe->flist, qe->fpre, n, qe->row_off, qe->col, front, vis,
-          reinterpret_cast<unsigned long long*>(nxt), qe->nlist, qe->counter);
+            reinterpret_cast<unsigned long long*>(nxt), nl, c);
+      k_bfs_expand<<<(uint32_t)ctx->num_cus * 16, QT, 0, st>>>(
+          fl, qe->fpre, c, qe->row_off, qe->col, front, vis,
+          reinterpret_cast<unsigned long long*>(nxt), nl);
+      k_clear_front<<<fixed, 256, 0, st>>>(fl, c, front);
+      k_bfs_apply<<<fixed, 256, 0, st>>>(nl, c, nxt, vis, front, lev);
+      k_bfs_level_end<<<1, 1, 0, st>>>(c);
     }
-    k_clear_front<<<blocks_for(n, 256), 256, 0, st>>>(qe->flist, n, front);
-    uint32_t m = 0;
-    if ((rc = read_counter(ctx, qe, &m))) return done(rc);
-    if (m)
-      k_bfs_apply<<<blocks_for(m, 256), 256, 0, st>>>(qe->nlist, m, (uint16_t)level, nxt, vis,
-                                                      front, lev);
-    std::swap(qe->flist, qe->nlist);
-    n = m;
-    acc->iterations += 1;
+    CY_HIP(ctx, hipMemcpyAsync(ctx->h_scalar, c, sizeof(BfsCtl), hipMemcpyDeviceToHost, st));
+    CY_HIP(ctx, hipStreamSynchronize(st));
+    std::memcpy(&h, ctx->h_scalar, sizeof(BfsCtl));
+    if (h.n == 0) break;
   }
+  acc->traversed_edges += h.te;
+  acc->active_vertices += h.active;
+  acc->iterations += h.iters;
   return done(0);
 }
 
