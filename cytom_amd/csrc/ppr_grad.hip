@@ -23,23 +23,25 @@ namespace cytom {
 namespace {
 
 constexpr int GT = 256;
-constexpr uint64_t ROW_LIGHT = 64;  // rows above this many entries get a wavefront each
+constexpr uint64_t ROW_LIGHT = 64;  // rows above this many entries are summed by segments
+constexpr uint64_t SEGL = 1024;     // entries per segment of a long row (one wavefront each)
 
 inline uint32_t gblocks(uint64_t n, uint32_t per, uint32_t cap = 65536) {
   const uint64_t b = (n + per - 1) / per;
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, cap));
 }
 
-__global__ void k_count(const uint32_t* __restrict__ key, uint64_t E, uint32_t* __restrict__ cnt) {
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
-       e += (uint64_t)gridDim.x * blockDim.x)
-    atomicAdd(&cnt[key[e]], 1u);
-}
-
-__global__ void k_widen32(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    out[i] = in[i];
+// Row offsets from the sorted keys: position i writes off[r] = i for the rows
+// r in (skey[i-1], skey[i]] (from 0 at i = 0, up to V at i = E), so every
+// entry of off[0..V] is written once, with no atomics.
+__global__ void k_row_bounds(const uint32_t* __restrict__ skey, uint64_t E, uint64_t V,
+                             uint64_t* __restrict__ off) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= E;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t lo = i == 0 ? 0 : (uint64_t)skey[i - 1] + 1;
+    const uint64_t hi = i == E ? V : (uint64_t)skey[i];
+    for (uint64_t r = lo; r <= hi; ++r) off[r] = i;
+  }
 }
 
 __global__ void k_iota(uint32_t* __restrict__ x, uint64_t n) {
@@ -62,8 +64,8 @@ __global__ void k_long_rows(const uint64_t* __restrict__ off, uint64_t R, uint64
     if (off[r + 1] - off[r] > minlen) list[atomicAdd(n, 1u)] = (uint32_t)r;
 }
 
-// W_u: sum of the weights leaving u, in CSR order (one thread per row; long
-// rows by a wavefront each).
+// W_u: sum of the weights leaving u, in CSR order (one thread per row; the
+// long rows by segments, k_seg_wsum then k_rows_from_parts).
 __global__ void k_row_sum(const uint64_t* __restrict__ off, uint64_t R,
                           const uint32_t* __restrict__ perm, const float* __restrict__ w,
                           float* __restrict__ W) {
@@ -81,21 +83,6 @@ __device__ inline float wave_sum_f32(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
-}
-
-__global__ void k_row_sum_long(const uint64_t* __restrict__ off, const uint32_t* __restrict__ rows,
-                               uint32_t nrows, const uint32_t* __restrict__ perm,
-                               const float* __restrict__ w, float* __restrict__ W) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t k = wave; k < nrows; k += nw) {
-    const uint32_t r = rows[k];
-    float s = 0.f;
-    for (uint64_t i = off[r] + lane; i < off[r + 1]; i += 64) s += w[perm[i]];
-    s = wave_sum_f32(s);
-    if (lane == 0) W[r] = s;
-  }
 }
 
 // Transition probability of every edge in the given row order:
@@ -123,8 +110,9 @@ __device__ inline void block_max_to(float d, unsigned int* out) {
   }
 }
 
-// One Jacobi sweep y_r = base_r + a * sum_{i in row r} p_i x_{nb_i}; the
-// largest |y_r - x_r| goes to *diff (as float bits, non-negative).
+// One Jacobi sweep y_r = base_r + a * sum_{i in row r} p_i x_{nb_i} over the
+// rows of at most ROW_LIGHT entries (one thread each); the largest
+// |y_r - x_r| goes to *diff (as float bits, non-negative).
 __global__ void k_sweep(const uint64_t* __restrict__ off, uint64_t R, const uint32_t* __restrict__ nb,
                         const float* __restrict__ p, const float* __restrict__ x,
                         const float* __restrict__ base, float base_scale, float a,
@@ -143,26 +131,63 @@ __global__ void k_sweep(const uint64_t* __restrict__ off, uint64_t R, const uint
   block_max_to(dmax, diff);
 }
 
-__global__ void k_sweep_long(const uint64_t* __restrict__ off, const uint32_t* __restrict__ rows,
-                             uint32_t nrows, const uint32_t* __restrict__ nb,
-                             const float* __restrict__ p, const float* __restrict__ x,
-                             const float* __restrict__ base, float base_scale, float a,
-                             float* __restrict__ y, unsigned int* __restrict__ diff) {
+// Long rows are cut into segments of SEGL entries.  Counts per long row, then
+// (after a scan into sbase) the segment bounds.
+__global__ void k_lrow_nseg(const uint32_t* __restrict__ rows, uint32_t nrows,
+                            const uint64_t* __restrict__ off, uint64_t* __restrict__ cnt) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nrows; k += gridDim.x * blockDim.x) {
+    const uint32_t r = rows[k];
+    cnt[k] = (off[r + 1] - off[r] + SEGL - 1) / SEGL;
+  }
+}
+
+__global__ void k_lrow_emit(const uint32_t* __restrict__ rows, uint32_t nrows,
+                            const uint64_t* __restrict__ off, const uint64_t* __restrict__ sbase,
+                            uint64_t* __restrict__ seg_b) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nrows; k += gridDim.x * blockDim.x) {
+    const uint32_t r = rows[k];
+    const uint64_t e1 = off[r + 1];
+    uint64_t j = sbase[k];
+    for (uint64_t b = off[r]; b < e1; b += SEGL, ++j) {
+      seg_b[2 * j] = b;
+      seg_b[2 * j + 1] = min(b + SEGL, e1);
+    }
+  }
+}
+
+// part_j = sum over segment j of p_i x_{nb_i}: one wavefront per segment,
+// lanes strided over it, then a fixed-order wavefront reduction.
+__global__ void k_seg_partial(const uint64_t* __restrict__ seg_b, uint32_t nseg,
+                              const uint32_t* __restrict__ nb, const float* __restrict__ p,
+                              const float* __restrict__ x, float* __restrict__ part) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t k = wave; k < nrows; k += nw) {
+  for (uint64_t k = wave; k < nseg; k += nw) {
+    float acc = 0.f;
+    for (uint64_t i = seg_b[2 * k] + lane; i < seg_b[2 * k + 1]; i += 64) acc += p[i] * x[nb[i]];
+    acc = wave_sum_f32(acc);
+    if (lane == 0) part[k] = acc;
+  }
+}
+
+// Long rows of a sweep: y_r = base_r + a * (their segment partials in
+// segment order), the largest change to *diff.
+__global__ void k_sweep_rows(const uint32_t* __restrict__ rows, uint32_t nrows,
+                             const uint64_t* __restrict__ sbase, const float* __restrict__ part,
+                             const float* __restrict__ x, const float* __restrict__ base,
+                             float base_scale, float a, float* __restrict__ y,
+                             unsigned int* __restrict__ diff) {
+  float dmax = 0.f;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nrows; k += gridDim.x * blockDim.x) {
     const uint32_t r = rows[k];
     float acc = 0.f;
-    for (uint64_t i = off[r] + lane; i < off[r + 1]; i += 64) acc += p[i] * x[nb[i]];
-    acc = wave_sum_f32(acc);
-    if (lane == 0) {
-      const float v = base_scale * base[r] + a * acc;
-      y[r] = v;
-      const float d = fabsf(v - x[r]);
-      if (d > 0.f) atomicMax(diff, __float_as_uint(d));
-    }
+    for (uint64_t j = sbase[k]; j < sbase[k + 1]; ++j) acc += part[j];
+    const float v = base_scale * base[r] + a * acc;
+    y[r] = v;
+    dmax = fmaxf(dmax, fabsf(v - x[r]));
   }
+  block_max_to(dmax, diff);
 }
 
 // mu_u = sum over u's out-edges of p * lambda_dst (CSR order).
@@ -179,19 +204,30 @@ __global__ void k_mu(const uint64_t* __restrict__ off, uint64_t R, const uint32_
   }
 }
 
-__global__ void k_mu_long(const uint64_t* __restrict__ off, const uint32_t* __restrict__ rows,
-                          uint32_t nrows, const uint32_t* __restrict__ nb,
-                          const float* __restrict__ p, const float* __restrict__ lam,
-                          float* __restrict__ mu) {
+// Weight sums of the long rows' segments (w in CSR order through perm).
+__global__ void k_seg_wsum(const uint64_t* __restrict__ seg_b, uint32_t nseg,
+                           const uint32_t* __restrict__ perm, const float* __restrict__ w,
+                           float* __restrict__ part) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t k = wave; k < nrows; k += nw) {
-    const uint32_t r = rows[k];
+  for (uint64_t k = wave; k < nseg; k += nw) {
     float acc = 0.f;
-    for (uint64_t i = off[r] + lane; i < off[r + 1]; i += 64) acc += p[i] * lam[nb[i]];
+    for (uint64_t i = seg_b[2 * k] + lane; i < seg_b[2 * k + 1]; i += 64) acc += w[perm[i]];
     acc = wave_sum_f32(acc);
-    if (lane == 0) mu[r] = acc;
+    if (lane == 0) part[k] = acc;
+  }
+}
+
+// out[r] for the long rows: their segment partials added in segment order
+// (mu from the partials of p * lambda, W from the weight partials).
+__global__ void k_rows_from_parts(const uint32_t* __restrict__ rows, uint32_t nrows,
+                                  const uint64_t* __restrict__ sbase,
+                                  const float* __restrict__ part, float* __restrict__ out) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nrows; k += gridDim.x * blockDim.x) {
+    float acc = 0.f;
+    for (uint64_t j = sbase[k]; j < sbase[k + 1]; ++j) acc += part[j];
+    out[rows[k]] = acc;
   }
 }
 
@@ -221,6 +257,10 @@ struct Rows {
   float* p = nullptr;        // transition probability per entry
   uint32_t* longr = nullptr;
   uint32_t nlong = 0;
+  uint64_t* sbase = nullptr;  // nlong + 1: first segment of each long row
+  uint64_t* seg_b = nullptr;  // [2 * nseg]: segment begin, end
+  float* part = nullptr;      // nseg partial sums
+  uint32_t nseg = 0;
 };
 
 struct Op {
@@ -251,7 +291,6 @@ struct Op {
 
 int build_rows(Op& op, Rows& R, const uint32_t* key, const uint32_t* other, hipStream_t st) {
   const uint64_t V = op.V, E = op.E;
-  uint32_t* cnt = op.alloc<uint32_t>(V * 4);
   uint64_t* wide = op.alloc<uint64_t>((V + 1) * 8);
   R.off = op.alloc<uint64_t>((V + 1) * 8);
   R.perm = op.alloc<uint32_t>(E * 4);
@@ -268,21 +307,35 @@ int build_rows(Op& op, Rows& R, const uint32_t* key, const uint32_t* other, hipS
   void* stmp = op.alloc<char>(scan_tmp_bytes(std::max<uint64_t>(E, 256 * ((E + 4095) / 4096) + 1)) +
                               256);
   if (op.err) return op.err;
-  if (hipMemsetAsync(cnt, 0, V * 4, st) != hipSuccess) return CYTOM_EDEVICE;
-  k_count<<<gblocks(E, GT, 16384), GT, 0, st>>>(key, E, cnt);
-  k_widen32<<<gblocks(V, GT, 16384), GT, 0, st>>>(cnt, wide, V);
-  scan_exclusive_u64(wide, R.off, V, op.tmp, st);
   k_iota<<<gblocks(E, GT, 16384), GT, 0, st>>>(ids, E);
   uint32_t bits = 1;
   while ((1ull << bits) < V) ++bits;
   radix_sort_pairs(key, ids, k_out, R.perm, k_tmp, v_tmp, E, bits, hist, stmp, st);
+  k_row_bounds<<<gblocks(E + 1, GT, 16384), GT, 0, st>>>(k_out, E, V, R.off);
   k_gather_u32<<<gblocks(E, GT, 16384), GT, 0, st>>>(R.perm, other, E, R.nb);
-  // rows handled by a wavefront each in the sweeps
+  // long rows, summed by segments of SEGL entries in the sweeps
   if (hipMemsetAsync(op.scratch + 1, 0, 4, st) != hipSuccess) return CYTOM_EDEVICE;
   k_long_rows<<<gblocks(V, GT, 16384), GT, 0, st>>>(R.off, V, ROW_LIGHT, R.longr, op.scratch + 1);
   if (hipMemcpyAsync(&R.nlong, op.scratch + 1, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     return CYTOM_EDEVICE;
+  if (R.nlong) {
+    R.sbase = op.alloc<uint64_t>((R.nlong + 1) * 8);
+    if (op.err) return op.err;
+    k_lrow_nseg<<<gblocks(R.nlong, GT, 16384), GT, 0, st>>>(R.longr, R.nlong, R.off, wide);
+    scan_exclusive_u64(wide, R.sbase, R.nlong, op.tmp, st);
+    uint64_t nseg = 0;
+    if (hipMemcpyAsync(&nseg, R.sbase + R.nlong, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return CYTOM_EDEVICE;
+    if (nseg > 0xFFFFFFFFull) return CYTOM_ERANGE;
+    R.nseg = (uint32_t)nseg;
+    R.seg_b = op.alloc<uint64_t>(nseg * 16);
+    R.part = op.alloc<float>(nseg * 4);
+    if (op.err) return op.err;
+    k_lrow_emit<<<gblocks(R.nlong, GT, 16384), GT, 0, st>>>(R.longr, R.nlong, R.off, R.sbase,
+                                                           R.seg_b);
+  }
   return 0;
 }
 
@@ -299,9 +352,12 @@ int build_op(Op& op, uint64_t V, uint64_t E, const uint32_t* src, const uint32_t
     return rc;
   if (hipMemsetAsync(op.W, 0, V * 4, st) != hipSuccess) return CYTOM_EDEVICE;
   k_row_sum<<<gblocks(V, GT, 16384), GT, 0, st>>>(op.out.off, V, op.out.perm, w, op.W);
-  if (op.out.nlong)
-    k_row_sum_long<<<gblocks((uint64_t)op.out.nlong * 64, GT, 16384), GT, 0, st>>>(
-        op.out.off, op.out.longr, op.out.nlong, op.out.perm, w, op.W);
+  if (op.out.nlong) {
+    k_seg_wsum<<<gblocks((uint64_t)op.out.nseg * 64, GT, 16384), GT, 0, st>>>(
+        op.out.seg_b, op.out.nseg, op.out.perm, w, op.out.part);
+    k_rows_from_parts<<<gblocks(op.out.nlong, GT, 16384), GT, 0, st>>>(op.out.longr, op.out.nlong,
+                                                              op.out.sbase, op.out.part, op.W);
+  }
   k_prob<<<gblocks(E, GT, 16384), GT, 0, st>>>(op.out.perm, src, w, op.W, E, op.out.p);
   k_prob<<<gblocks(E, GT, 16384), GT, 0, st>>>(op.in.perm, src, w, op.W, E, op.in.p);
   return hipStreamSynchronize(st) == hipSuccess ? 0 : CYTOM_EDEVICE;
@@ -317,9 +373,12 @@ int solve(Op& op, const Rows& R, const float* base, float base_scale, float a, f
     if (hipMemsetAsync(op.scratch, 0, 4, st) != hipSuccess) return CYTOM_EDEVICE;
     k_sweep<<<gblocks(V, GT, 16384), GT, 0, st>>>(R.off, V, R.nb, R.p, x, base, base_scale, a, y,
                                                   op.scratch);
-    if (R.nlong)
-      k_sweep_long<<<gblocks((uint64_t)R.nlong * 64, GT, 16384), GT, 0, st>>>(
-          R.off, R.longr, R.nlong, R.nb, R.p, x, base, base_scale, a, y, op.scratch);
+    if (R.nlong) {
+      k_seg_partial<<<gblocks((uint64_t)R.nseg * 64, GT, 16384), GT, 0, st>>>(
+          R.seg_b, R.nseg, R.nb, R.p, x, R.part);
+      k_sweep_rows<<<gblocks(R.nlong, GT, 16384), GT, 0, st>>>(
+          R.longr, R.nlong, R.sbase, R.part, x, base, base_scale, a, y, op.scratch);
+    }
     std::swap(x, y);
     // Check every 8 sweeps (and at the cap) to keep the host out of the loop.
     if ((it & 7) == 7 || it + 1 == max_iter) {
@@ -410,9 +469,12 @@ extern "C" int cytom_ppr_adjoint(uint64_t V, uint64_t E, const uint32_t* d_src,
       if (!rc) {
         const float* L = lam;  // solve() leaves the solution in its x buffer
         k_mu<<<gblocks(V, GT, 16384), GT, 0, st>>>(op.out.off, V, op.out.nb, op.out.p, L, mu);
-        if (op.out.nlong)
-          k_mu_long<<<gblocks((uint64_t)op.out.nlong * 64, GT, 16384), GT, 0, st>>>(
-              op.out.off, op.out.longr, op.out.nlong, op.out.nb, op.out.p, L, mu);
+        if (op.out.nlong) {
+          k_seg_partial<<<gblocks((uint64_t)op.out.nseg * 64, GT, 16384), GT, 0, st>>>(
+              op.out.seg_b, op.out.nseg, op.out.nb, op.out.p, L, op.out.part);
+          k_rows_from_parts<<<gblocks(op.out.nlong, GT, 16384), GT, 0, st>>>(
+              op.out.longr, op.out.nlong, op.out.sbase, op.out.part, mu);
+        }
         if (d_dL_dw && E)
           k_dw<<<gblocks(E, GT, 16384), GT, 0, st>>>(d_src, d_dst, E, d_r, L, mu, op.W, alpha,
                                                      d_dL_dw);
