@@ -110,6 +110,14 @@ SIGNATURES = {
                                           ctypes.c_float, _U32, _VP, _VP]),
     "cytom_ppr_adjoint": (ctypes.c_int, [_U64, _U64, _VP, _VP, _VP, _VP, _VP, ctypes.c_float,
                                          ctypes.c_float, _U32, _VP, _VP, _VP]),
+    "cytom_ppr_graph_create": (ctypes.c_int, [_U64, _U64, _VP, _VP, _VP,
+                                              ctypes.POINTER(ctypes.c_void_p)]),
+    "cytom_ppr_graph_set_weights": (ctypes.c_int, [_VP, _VP]),
+    "cytom_ppr_graph_forward": (ctypes.c_int, [_VP, _VP, ctypes.c_float, ctypes.c_float, _U32,
+                                               _VP, _VP]),
+    "cytom_ppr_graph_adjoint": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_float, ctypes.c_float,
+                                               _U32, _VP, _VP, _VP]),
+    "cytom_ppr_graph_destroy": (ctypes.c_int, [_VP]),
     "cytom_ppr_last_error": (ctypes.c_char_p, []),
     "cytom_read_query": (ctypes.c_int, [_VP, _U32, _VP, ctypes.c_size_t]),
     "cytom_update_batch": (ctypes.c_int, [_VP, _VP, _VP, _VP, _U64, _VP, _VP, _VP, _U64,

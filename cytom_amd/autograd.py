@@ -2,7 +2,8 @@
 
 ``ppr(src, dst, w, s, alpha)`` returns r with r = (1-alpha) s + alpha P^T r,
 P_e = w_e / (sum of the weights leaving src_e); its backward calls the
-library's adjoint kernel (cytom_ppr_adjoint) for dL/dw and dL/ds.
+library's adjoint (cytom_ppr_graph_adjoint) on the operator the forward built,
+for dL/dw and dL/ds.
 
 Both directions run in libcytom_hip.so on the GPU; torch only does the
 autograd bookkeeping on host tensors (the library owns its device buffers,
@@ -26,34 +27,47 @@ def _check(rc, what):
 
 
 class PPRGraph:
-    """Edge list and weights resident in HBM for repeated forward/adjoint calls."""
+    """The operator on one edge list (cytom_ppr_graph): the CSR/CSC rows are
+    built once, set_weights only recomputes the transition probabilities, and
+    forward / adjoint calls reuse both."""
 
     def __init__(self, n_vertices, src, dst, w):
         self.V = int(n_vertices)
         self.E = len(src)
-        self.src = DeviceArray.from_host(np.ascontiguousarray(src, np.uint32))
-        self.dst = DeviceArray.from_host(np.ascontiguousarray(dst, np.uint32))
-        self.w = DeviceArray.from_host(np.ascontiguousarray(w, np.float32))
+        src_d = DeviceArray.from_host(np.ascontiguousarray(src, np.uint32))
+        dst_d = DeviceArray.from_host(np.ascontiguousarray(dst, np.uint32))
+        w_d = DeviceArray.from_host(np.ascontiguousarray(w, np.float32))
+        h = ctypes.c_void_p()
+        _check(lib().cytom_ppr_graph_create(self.V, self.E, src_d.ptr, dst_d.ptr, w_d.ptr,
+                                            ctypes.byref(h)), "ppr_graph_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cytom_ppr_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
     def set_weights(self, w):
-        self.w = DeviceArray.from_host(np.ascontiguousarray(w, np.float32))
+        w_d = DeviceArray.from_host(np.ascontiguousarray(w, np.float32))
+        _check(lib().cytom_ppr_graph_set_weights(self._h, w_d.ptr), "ppr_graph_set_weights")
 
     def forward_device(self, s_dev, r_dev, alpha=0.85, tol=DEFAULT_TOL,
                        max_iter=DEFAULT_MAX_ITER):
         it = ctypes.c_uint32()
-        _check(lib().cytom_ppr_weighted(self.V, self.E, self.src.ptr, self.dst.ptr, self.w.ptr,
-                                        s_dev.ptr, alpha, tol, max_iter, r_dev.ptr,
-                                        ctypes.byref(it)), "ppr_weighted")
+        _check(lib().cytom_ppr_graph_forward(self._h, s_dev.ptr, alpha, tol, max_iter, r_dev.ptr,
+                                             ctypes.byref(it)), "ppr_graph_forward")
         return it.value
 
     def adjoint_device(self, r_dev, g_dev, dw_dev, ds_dev, alpha=0.85, tol=DEFAULT_TOL,
                        max_iter=DEFAULT_MAX_ITER):
         it = ctypes.c_uint32()
-        _check(lib().cytom_ppr_adjoint(self.V, self.E, self.src.ptr, self.dst.ptr, self.w.ptr,
-                                       r_dev.ptr, g_dev.ptr, alpha, tol, max_iter,
-                                       dw_dev.ptr if dw_dev else None,
-                                       ds_dev.ptr if ds_dev else None, ctypes.byref(it)),
-               "ppr_adjoint")
+        _check(lib().cytom_ppr_graph_adjoint(self._h, r_dev.ptr, g_dev.ptr, alpha, tol, max_iter,
+                                             dw_dev.ptr if dw_dev else None,
+                                             ds_dev.ptr if ds_dev else None, ctypes.byref(it)),
+               "ppr_graph_adjoint")
         return it.value
 
     def forward(self, s, alpha=0.85, tol=DEFAULT_TOL, max_iter=DEFAULT_MAX_ITER):

@@ -14,6 +14,7 @@
 // edge list; the tile store is not involved.
 #include <algorithm>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "devutil.h"
@@ -275,23 +276,32 @@ struct Op {
   ~Op() {
     for (void* q : owned)
       if (q) (void)hipFree(q);
+    for (void* q : temps)
+      if (q) (void)hipFree(q);
   }
+  std::vector<void*> temps;  // build-only buffers, freed by drop_temps
   template <typename T>
-  T* alloc(size_t bytes) {
+  T* alloc(size_t bytes, bool temp = false) {
     void* q = nullptr;
     if (hipMalloc(&q, std::max<size_t>(bytes, 4)) != hipSuccess) {
       err = CYTOM_ENOMEM;
       msg = "device allocation failed";
       return nullptr;
     }
-    owned.push_back(q);
+    (temp ? temps : owned).push_back(q);
     return static_cast<T*>(q);
+  }
+  void drop_temps(hipStream_t st) {
+    (void)hipStreamSynchronize(st);
+    for (void* q : temps)
+      if (q) (void)hipFree(q);
+    temps.clear();
   }
 };
 
 int build_rows(Op& op, Rows& R, const uint32_t* key, const uint32_t* other, hipStream_t st) {
   const uint64_t V = op.V, E = op.E;
-  uint64_t* wide = op.alloc<uint64_t>((V + 1) * 8);
+  uint64_t* wide = op.alloc<uint64_t>((V + 1) * 8, true);
   R.off = op.alloc<uint64_t>((V + 1) * 8);
   R.perm = op.alloc<uint32_t>(E * 4);
   R.nb = op.alloc<uint32_t>(E * 4);
@@ -299,13 +309,13 @@ int build_rows(Op& op, Rows& R, const uint32_t* key, const uint32_t* other, hipS
   R.longr = op.alloc<uint32_t>(V * 4);
   // stable radix sort of (key, edge id): ids increase inside every row, so
   // every row sums in a fixed order
-  uint32_t* ids = op.alloc<uint32_t>(E * 4);
-  uint32_t* k_out = op.alloc<uint32_t>(E * 4);
-  uint32_t* k_tmp = op.alloc<uint32_t>(E * 4);
-  uint32_t* v_tmp = op.alloc<uint32_t>(E * 4);
-  void* hist = op.alloc<char>(radix_hist_bytes(E));
-  void* stmp = op.alloc<char>(scan_tmp_bytes(std::max<uint64_t>(E, 256 * ((E + 4095) / 4096) + 1)) +
-                              256);
+  uint32_t* ids = op.alloc<uint32_t>(E * 4, true);
+  uint32_t* k_out = op.alloc<uint32_t>(E * 4, true);
+  uint32_t* k_tmp = op.alloc<uint32_t>(E * 4, true);
+  uint32_t* v_tmp = op.alloc<uint32_t>(E * 4, true);
+  void* hist = op.alloc<char>(radix_hist_bytes(E), true);
+  void* stmp = op.alloc<char>(
+      scan_tmp_bytes(std::max<uint64_t>(E, 256 * ((E + 4095) / 4096) + 1)) + 256, true);
   if (op.err) return op.err;
   k_iota<<<gblocks(E, GT, 16384), GT, 0, st>>>(ids, E);
   uint32_t bits = 1;
@@ -339,17 +349,48 @@ int build_rows(Op& op, Rows& R, const uint32_t* key, const uint32_t* other, hipS
   return 0;
 }
 
-int build_op(Op& op, uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
-             const float* w, hipStream_t st) {
+// Largest endpoint id of the edge list, for the range check.
+__global__ void k_max_id(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                         uint64_t E, unsigned int* __restrict__ out) {
+  uint32_t m = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    m = max(m, max(a[i], b[i]));
+  if (m) atomicMax(out, m);
+}
+
+// CSR and CSC rows of the edge list (src, dst are the operator's own copies).
+int build_structure(Op& op, uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                    hipStream_t st) {
   op.V = V;
   op.E = E;
   op.scratch = op.alloc<unsigned int>(256);
   op.tmp = op.alloc<char>(scan_tmp_bytes(V + 1) + 256);
   op.W = op.alloc<float>(V * 4);
   if (op.err) return op.err;
+  if (E) {
+    if (hipMemsetAsync(op.scratch + 2, 0, 4, st) != hipSuccess) return CYTOM_EDEVICE;
+    k_max_id<<<gblocks(E, GT, 4096), GT, 0, st>>>(src, dst, E, op.scratch + 2);
+    unsigned int mx = 0;
+    if (hipMemcpyAsync(&mx, op.scratch + 2, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return CYTOM_EDEVICE;
+    if (mx >= V) {
+      op.msg = "edge endpoint out of range";
+      return CYTOM_EINVAL;
+    }
+  }
   int rc;
   if ((rc = build_rows(op, op.out, src, dst, st)) || (rc = build_rows(op, op.in, dst, src, st)))
     return rc;
+  op.drop_temps(st);
+  return 0;
+}
+
+// W_u and the transition probabilities of both row orders for weights w
+// (in the caller's edge order).
+int set_weights(Op& op, const uint32_t* src, const float* w, hipStream_t st) {
+  const uint64_t V = op.V, E = op.E;
   if (hipMemsetAsync(op.W, 0, V * 4, st) != hipSuccess) return CYTOM_EDEVICE;
   k_row_sum<<<gblocks(V, GT, 16384), GT, 0, st>>>(op.out.off, V, op.out.perm, w, op.W);
   if (op.out.nlong) {
@@ -358,8 +399,10 @@ int build_op(Op& op, uint64_t V, uint64_t E, const uint32_t* src, const uint32_t
     k_rows_from_parts<<<gblocks(op.out.nlong, GT, 16384), GT, 0, st>>>(op.out.longr, op.out.nlong,
                                                               op.out.sbase, op.out.part, op.W);
   }
-  k_prob<<<gblocks(E, GT, 16384), GT, 0, st>>>(op.out.perm, src, w, op.W, E, op.out.p);
-  k_prob<<<gblocks(E, GT, 16384), GT, 0, st>>>(op.in.perm, src, w, op.W, E, op.in.p);
+  if (E) {
+    k_prob<<<gblocks(E, GT, 16384), GT, 0, st>>>(op.out.perm, src, w, op.W, E, op.out.p);
+    k_prob<<<gblocks(E, GT, 16384), GT, 0, st>>>(op.in.perm, src, w, op.W, E, op.in.p);
+  }
   return hipStreamSynchronize(st) == hipSuccess ? 0 : CYTOM_EDEVICE;
 }
 
@@ -410,37 +453,140 @@ int check_inputs(uint64_t V, uint64_t E, const void* src, const void* dst, const
 
 thread_local std::string g_ppr_err;
 
+int fail(int rc, const Op* op, const char* what) {
+  g_ppr_err = (op && !op->msg.empty()) ? op->msg : what;
+  return rc;
+}
+
 }  // namespace
 }  // namespace cytom
 
 using namespace cytom;
 
+// The operator on one edge list: its own copies of the endpoints, the CSR /
+// CSC rows, the current transition probabilities, the solver vectors and a
+// stream.
+struct cytom_ppr_graph {
+  Op op;
+  hipStream_t st = nullptr;
+  uint32_t* src = nullptr;
+  uint32_t* dst = nullptr;
+  float* y = nullptr;    // solver work vector
+  float* lam = nullptr;  // adjoint solution
+  float* mu = nullptr;   // adjoint row sums
+  ~cytom_ppr_graph() {
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
+  }
+};
+
 extern "C" const char* cytom_ppr_last_error(void) { return g_ppr_err.c_str(); }
 
+extern "C" int cytom_ppr_graph_create(uint64_t V, uint64_t E, const uint32_t* d_src,
+                                      const uint32_t* d_dst, const float* d_w,
+                                      cytom_ppr_graph** out) {
+  if (!out) return CYTOM_EINVAL;
+  *out = nullptr;
+  if (check_inputs(V, E, d_src, d_dst, d_w)) return fail(CYTOM_EINVAL, nullptr, "invalid edge list");
+  auto* g = new (std::nothrow) cytom_ppr_graph();
+  if (!g) return fail(CYTOM_ENOMEM, nullptr, "host allocation failed");
+  if (hipStreamCreate(&g->st) != hipSuccess) {
+    g->st = nullptr;
+    delete g;
+    return fail(CYTOM_EDEVICE, nullptr, "stream creation failed");
+  }
+  Op& op = g->op;
+  g->src = op.alloc<uint32_t>(E * 4);
+  g->dst = op.alloc<uint32_t>(E * 4);
+  g->y = op.alloc<float>(V * 4);
+  g->lam = op.alloc<float>(V * 4);
+  g->mu = op.alloc<float>(V * 4);
+  int rc = op.err;
+  if (!rc && E &&
+      (hipMemcpyAsync(g->src, d_src, E * 4, hipMemcpyDeviceToDevice, g->st) != hipSuccess ||
+       hipMemcpyAsync(g->dst, d_dst, E * 4, hipMemcpyDeviceToDevice, g->st) != hipSuccess))
+    rc = CYTOM_EDEVICE;
+  if (!rc) rc = build_structure(op, V, E, g->src, g->dst, g->st);
+  if (!rc) rc = set_weights(op, g->src, d_w, g->st);
+  if (rc) {
+    fail(rc, &op, "weighted PPR graph build failed");
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return 0;
+}
+
+extern "C" int cytom_ppr_graph_set_weights(cytom_ppr_graph* g, const float* d_w) {
+  if (!g || (g->op.E && !d_w)) return fail(CYTOM_EINVAL, nullptr, "invalid arguments");
+  const int rc = set_weights(g->op, g->src, d_w, g->st);
+  return rc ? fail(rc, &g->op, "weighted PPR weight update failed") : 0;
+}
+
+extern "C" int cytom_ppr_graph_forward(cytom_ppr_graph* g, const float* d_s, float alpha,
+                                       float tol, uint32_t max_iter, float* d_r,
+                                       uint32_t* iterations) {
+  if (!g || !d_s || !d_r || !(alpha >= 0.f && alpha < 1.f))
+    return fail(CYTOM_EINVAL, nullptr, "invalid arguments");
+  Op& op = g->op;
+  // x0 = (1 - a) s
+  k_scale<<<gblocks(op.V, GT, 16384), GT, 0, g->st>>>(d_s, op.V, 1.f - alpha, d_r);
+  uint32_t it = 0;
+  const int rc = solve(op, op.in, d_s, 1.f - alpha, alpha, tol, max_iter, d_r, g->y, &it, g->st);
+  if (iterations) *iterations = it;
+  return rc ? fail(rc, &op, "weighted PPR forward failed") : 0;
+}
+
+extern "C" int cytom_ppr_graph_adjoint(cytom_ppr_graph* g, const float* d_r,
+                                       const float* d_dL_dr, float alpha, float tol,
+                                       uint32_t max_iter, float* d_dL_dw, float* d_dL_ds,
+                                       uint32_t* iterations) {
+  if (!g || !d_r || !d_dL_dr || !(alpha >= 0.f && alpha < 1.f))
+    return fail(CYTOM_EINVAL, nullptr, "invalid arguments");
+  Op& op = g->op;
+  hipStream_t st = g->st;
+  // lambda = g + a P lambda, from lambda0 = g
+  if (hipMemcpyAsync(g->lam, d_dL_dr, op.V * 4, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return fail(CYTOM_EDEVICE, &op, "weighted PPR adjoint failed");
+  uint32_t it = 0;
+  int rc = solve(op, op.out, d_dL_dr, 1.f, alpha, tol, max_iter, g->lam, g->y, &it, st);
+  if (iterations) *iterations = it;
+  if (!rc) {
+    const float* L = g->lam;  // solve() leaves the solution in its x buffer
+    k_mu<<<gblocks(op.V, GT, 16384), GT, 0, st>>>(op.out.off, op.V, op.out.nb, op.out.p, L, g->mu);
+    if (op.out.nlong) {
+      k_seg_partial<<<gblocks((uint64_t)op.out.nseg * 64, GT, 16384), GT, 0, st>>>(
+          op.out.seg_b, op.out.nseg, op.out.nb, op.out.p, L, op.out.part);
+      k_rows_from_parts<<<gblocks(op.out.nlong, GT, 16384), GT, 0, st>>>(
+          op.out.longr, op.out.nlong, op.out.sbase, op.out.part, g->mu);
+    }
+    if (d_dL_dw && op.E)
+      k_dw<<<gblocks(op.E, GT, 16384), GT, 0, st>>>(g->src, g->dst, op.E, d_r, L, g->mu, op.W,
+                                                    alpha, d_dL_dw);
+    if (d_dL_ds) k_scale<<<gblocks(op.V, GT, 16384), GT, 0, st>>>(L, op.V, 1.f - alpha, d_dL_ds);
+    rc = hipStreamSynchronize(st) == hipSuccess ? 0 : CYTOM_EDEVICE;
+  }
+  return rc ? fail(rc, &op, "weighted PPR adjoint failed") : 0;
+}
+
+extern "C" int cytom_ppr_graph_destroy(cytom_ppr_graph* g) {
+  delete g;
+  return 0;
+}
+
+// One-shot forms: build the operator for this call only.
 extern "C" int cytom_ppr_weighted(uint64_t V, uint64_t E, const uint32_t* d_src,
                                   const uint32_t* d_dst, const float* d_w, const float* d_s,
                                   float alpha, float tol, uint32_t max_iter, float* d_r,
                                   uint32_t* iterations) {
   if (check_inputs(V, E, d_src, d_dst, d_w) || !d_s || !d_r || !(alpha >= 0.f && alpha < 1.f))
-    return CYTOM_EINVAL;
-  hipStream_t st = nullptr;
-  if (hipStreamCreate(&st) != hipSuccess) return CYTOM_EDEVICE;
-  int rc;
-  {
-    Op op;
-    float* y = op.alloc<float>(V * 4);
-    if (op.err) {
-      rc = op.err;
-    } else if ((rc = build_op(op, V, E, d_src, d_dst, d_w, st)) == 0) {
-      // x0 = (1 - a) s
-      k_scale<<<gblocks(V, GT, 16384), GT, 0, st>>>(d_s, V, 1.f - alpha, d_r);
-      uint32_t it = 0;
-      rc = solve(op, op.in, d_s, 1.f - alpha, alpha, tol, max_iter, d_r, y, &it, st);
-      if (iterations) *iterations = it;
-    }
-    if (rc) g_ppr_err = op.msg.empty() ? "weighted PPR forward failed" : op.msg;
-  }
-  (void)hipStreamDestroy(st);
+    return fail(CYTOM_EINVAL, nullptr, "invalid arguments");
+  cytom_ppr_graph* g = nullptr;
+  int rc = cytom_ppr_graph_create(V, E, d_src, d_dst, d_w, &g);
+  if (!rc) rc = cytom_ppr_graph_forward(g, d_s, alpha, tol, max_iter, d_r, iterations);
+  cytom_ppr_graph_destroy(g);
   return rc;
 }
 
@@ -449,41 +595,12 @@ extern "C" int cytom_ppr_adjoint(uint64_t V, uint64_t E, const uint32_t* d_src,
                                  const float* d_dL_dr, float alpha, float tol, uint32_t max_iter,
                                  float* d_dL_dw, float* d_dL_ds, uint32_t* iterations) {
   if (check_inputs(V, E, d_src, d_dst, d_w) || !d_r || !d_dL_dr || !(alpha >= 0.f && alpha < 1.f))
-    return CYTOM_EINVAL;
-  hipStream_t st = nullptr;
-  if (hipStreamCreate(&st) != hipSuccess) return CYTOM_EDEVICE;
-  int rc;
-  {
-    Op op;
-    float* lam = op.alloc<float>(V * 4);
-    float* y = op.alloc<float>(V * 4);
-    float* mu = op.alloc<float>(V * 4);
-    if (op.err) {
-      rc = op.err;
-    } else if ((rc = build_op(op, V, E, d_src, d_dst, d_w, st)) == 0) {
-      // lambda = g + a P lambda, from lambda0 = g
-      (void)hipMemcpyAsync(lam, d_dL_dr, V * 4, hipMemcpyDeviceToDevice, st);
-      uint32_t it = 0;
-      rc = solve(op, op.out, d_dL_dr, 1.f, alpha, tol, max_iter, lam, y, &it, st);
-      if (iterations) *iterations = it;
-      if (!rc) {
-        const float* L = lam;  // solve() leaves the solution in its x buffer
-        k_mu<<<gblocks(V, GT, 16384), GT, 0, st>>>(op.out.off, V, op.out.nb, op.out.p, L, mu);
-        if (op.out.nlong) {
-          k_seg_partial<<<gblocks((uint64_t)op.out.nseg * 64, GT, 16384), GT, 0, st>>>(
-              op.out.seg_b, op.out.nseg, op.out.nb, op.out.p, L, op.out.part);
-          k_rows_from_parts<<<gblocks(op.out.nlong, GT, 16384), GT, 0, st>>>(
-              op.out.longr, op.out.nlong, op.out.sbase, op.out.part, mu);
-        }
-        if (d_dL_dw && E)
-          k_dw<<<gblocks(E, GT, 16384), GT, 0, st>>>(d_src, d_dst, E, d_r, L, mu, op.W, alpha,
-                                                     d_dL_dw);
-        if (d_dL_ds) k_scale<<<gblocks(V, GT, 16384), GT, 0, st>>>(L, V, 1.f - alpha, d_dL_ds);
-        rc = hipStreamSynchronize(st) == hipSuccess ? 0 : CYTOM_EDEVICE;
-      }
-    }
-    if (rc) g_ppr_err = op.msg.empty() ? "weighted PPR adjoint failed" : op.msg;
-  }
-  (void)hipStreamDestroy(st);
+    return fail(CYTOM_EINVAL, nullptr, "invalid arguments");
+  cytom_ppr_graph* g = nullptr;
+  int rc = cytom_ppr_graph_create(V, E, d_src, d_dst, d_w, &g);
+  if (!rc)
+    rc = cytom_ppr_graph_adjoint(g, d_r, d_dL_dr, alpha, tol, max_iter, d_dL_dw, d_dL_ds,
+                                 iterations);
+  cytom_ppr_graph_destroy(g);
   return rc;
 }

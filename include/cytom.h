@@ -206,6 +206,25 @@ int cytom_ppr_adjoint(uint64_t n_vertices, uint64_t n_edges, const uint32_t* d_s
                       const uint32_t* d_dst, const float* d_w, const float* d_r,
                       const float* d_dL_dr, float alpha, float tol, uint32_t max_iterations,
                       float* d_dL_dw, float* d_dL_ds, uint32_t* iterations);
+/* The same operator on a fixed edge list, for repeated calls (no reference
+ * counterpart; for training loops that keep a graph and change weights or
+ * teleport vectors).  cytom_ppr_graph_create copies the endpoints, checks
+ * them against n_vertices and builds the CSR / CSC rows once;
+ * cytom_ppr_graph_set_weights recomputes only the transition probabilities
+ * (float[E] in the edge order given at creation); forward and adjoint then
+ * behave as cytom_ppr_weighted / cytom_ppr_adjoint without rebuilding the
+ * rows.  Calls on one graph are serialised on its own stream; device arrays
+ * passed in must be ready when the call is made. */
+typedef struct cytom_ppr_graph cytom_ppr_graph;
+int cytom_ppr_graph_create(uint64_t n_vertices, uint64_t n_edges, const uint32_t* d_src,
+                           const uint32_t* d_dst, const float* d_w, cytom_ppr_graph** out);
+int cytom_ppr_graph_set_weights(cytom_ppr_graph* g, const float* d_w);
+int cytom_ppr_graph_forward(cytom_ppr_graph* g, const float* d_s, float alpha, float tol,
+                            uint32_t max_iterations, float* d_r, uint32_t* iterations);
+int cytom_ppr_graph_adjoint(cytom_ppr_graph* g, const float* d_r, const float* d_dL_dr,
+                            float alpha, float tol, uint32_t max_iterations, float* d_dL_dw,
+                            float* d_dL_ds, uint32_t* iterations);
+int cytom_ppr_graph_destroy(cytom_ppr_graph* g);
 /* Message of the last failed cytom_ppr_* call on this thread. */
 const char* cytom_ppr_last_error(void);
 

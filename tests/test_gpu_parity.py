@@ -536,6 +536,46 @@ def test_ppr_grad_finite_differences_gpu():
     assert _rel(grad[picks], fd) < 2e-2
 
 
+def test_ppr_graph_handle_matches_one_shot_calls():
+    # The graph handle builds its rows once; forward and adjoint through it
+    # must equal the one-shot entry points bit for bit (same rows, same
+    # summation order), and set_weights must equal a fresh build with those
+    # weights.  An endpoint >= V is rejected before any kernel indexes by it.
+    import ctypes
+    from cytom_amd import autograd as A
+    V, E = 1 << 12, 40000
+    s, d, w, t = _weighted(V, E, 6, n_tel=32)
+    rng = np.random.default_rng(7)
+    g = np.zeros(V, np.float32)
+    g[rng.integers(0, V, 256)] = 1.0
+    src, dst = C.DeviceArray.from_host(s), C.DeviceArray.from_host(d)
+    wd, td, gd = (C.DeviceArray.from_host(x) for x in (w, t, g))
+    r1, r2 = C.DeviceArray(V, np.float32), C.DeviceArray(V, np.float32)
+    it = ctypes.c_uint32()
+    lib = C.lib()
+    assert lib.cytom_ppr_weighted(V, E, src.ptr, dst.ptr, wd.ptr, td.ptr, 0.85, 0.0, 200, r1.ptr,
+                                  ctypes.byref(it)) == 0
+    G = A.PPRGraph(V, s, d, w)
+    G.forward_device(td, r2, tol=0.0, max_iter=200)
+    assert np.array_equal(r1.to_host(), r2.to_host())
+    dw1, ds1 = C.DeviceArray(E, np.float32), C.DeviceArray(V, np.float32)
+    dw2, ds2 = C.DeviceArray(E, np.float32), C.DeviceArray(V, np.float32)
+    assert lib.cytom_ppr_adjoint(V, E, src.ptr, dst.ptr, wd.ptr, r1.ptr, gd.ptr, 0.85, 0.0, 200,
+                                 dw1.ptr, ds1.ptr, ctypes.byref(it)) == 0
+    G.adjoint_device(r2, gd, dw2, ds2, tol=0.0, max_iter=200)
+    assert np.array_equal(dw1.to_host(), dw2.to_host())
+    assert np.array_equal(ds1.to_host(), ds2.to_host())
+    w2 = (w * rng.uniform(0.5, 2.0, E)).astype(np.float32)
+    G.set_weights(w2)
+    assert np.array_equal(G.forward(t, tol=0.0, max_iter=200),
+                          A.PPRGraph(V, s, d, w2).forward(t, tol=0.0, max_iter=200))
+    G.close()
+    bad = d.copy()
+    bad[123] = V
+    with pytest.raises(C.CytomError):
+        A.PPRGraph(V, s, bad, w)
+
+
 def test_ppr_grad_rmat20_against_fp64_adjoint():
     # RMAT-20 (2^24 edges), U(0,1] weights, 256 teleport vertices, loss over
     # 1,024 targets: the GPU adjoint against the FP64 iterative adjoint.
