@@ -216,6 +216,17 @@ class ProcessHandle:
         self.log = log
         self.path = path
 
+    def terminate(self, grace=5.0):
+        """Stops the child (SIGTERM, then SIGKILL after `grace` seconds)."""
+        if self.popen.poll() is None:
+            self.popen.terminate()
+            try:
+                self.popen.wait(timeout=grace)
+            except subprocess.TimeoutExpired:
+                self.popen.kill()
+                self.popen.wait()
+        self.poll()
+
     def poll(self):
         rc = self.popen.poll()
         if rc is not None and not self.log.closed:
@@ -446,17 +457,28 @@ class Scheduler:
                 return
             self.sleep(self.poll_interval)
 
+    def stop_all(self):
+        """Terminates the running tasks (used when the scheduler itself is
+        interrupted, so no child keeps the GPU)."""
+        for t in list(self.alive.values()):
+            if hasattr(t.handle, "terminate"):
+                t.handle.terminate()
+        self.alive.clear()
+
     def run_all(self, algos):
         """Schedules every configuration, waits for all tasks and saves the
         profiles and the model's data (sched_func and exit, run.py:387-412,
         631-670).  Returns the number of failed tasks."""
         self.add_algos(algos)
-        while self.n_pending:
-            self.schedule()
-            if self.n_pending:
+        try:
+            while self.n_pending:
+                self.schedule()
+                if self.n_pending:
+                    self.wait_any()
+            while self.alive:
                 self.wait_any()
-        while self.alive:
-            self.wait_any()
+        finally:
+            self.stop_all()
         self.check_stats()
         self.state.save()
         return self.failed
@@ -464,12 +486,15 @@ class Scheduler:
     def profile_all(self, algos):
         """One task per configuration, one after the other, each recording its
         single-run profile (CytomManager.profile_all, run.py:905-916)."""
-        for a in algos:
-            usage = self.profile_usage([a.algorithm], a.graph)
-            self.launch([a], a.graph, usage[0], usage[0], estimate_device_bytes([a.job]),
-                        single=True)
-            while self.alive:
-                self.wait_any()
+        try:
+            for a in algos:
+                usage = self.profile_usage([a.algorithm], a.graph)
+                self.launch([a], a.graph, usage[0], usage[0], estimate_device_bytes([a.job]),
+                            single=True)
+                while self.alive:
+                    self.wait_any()
+        finally:
+            self.stop_all()
         self.check_stats()
         self.state.save()
         return self.failed

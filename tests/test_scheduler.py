@@ -271,3 +271,30 @@ def test_state_round_trip(tmp_path):
     assert (st2.profile_data, st2.model_X, st2.model_y) == (st.profile_data, st.model_X,
                                                             st.model_y)
     assert not os.path.exists(p + ".tmp")
+
+
+def test_interrupt_stops_running_tasks(tmp_path):
+    # An exception inside the scheduling loop (say KeyboardInterrupt while
+    # waiting) terminates every running child before it propagates.
+    clock = Clock()
+
+    class Handle(FakeHandle):
+        stopped = []
+
+        def terminate(self):
+            Handle.stopped.append(self.task.names)
+
+    class Spawn(FakeSpawn):
+        def __call__(self, task):
+            self.launched.append(task.names)
+            return Handle(self.clock, self.duration, 0.4, task)
+
+    def boom(dt):
+        raise KeyboardInterrupt
+
+    sp = Spawn(clock)
+    sch, sp, _ = _sched(tmp_path, {**_prof("g1", bfs=40), **_prof("g2", bfs=40)}, spawn=sp)
+    sch.sleep = boom
+    with pytest.raises(KeyboardInterrupt):
+        sch.run_all(_algos("g1", ["bfs"]) + _algos("g2", ["bfs"]))
+    assert sorted(map(tuple, Handle.stopped)) == [("bfs",), ("bfs",)] and not sch.alive
