@@ -61,6 +61,27 @@ __global__ void k_init_min(int kind, uint64_t V, uint32_t source, uint32_t* __re
   }
 }
 
+// Two SSSP queries (KIND_SSSP_PAIR): distance pairs, each source at 0 in its
+// own component, both sources active.
+__global__ void k_init_pair(uint64_t V, uint32_t s0, uint32_t s1, uint2* __restrict__ cur,
+                            uint2* __restrict__ nxt, uint64_t* __restrict__ act,
+                            uint8_t* __restrict__ changed) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint2 val = make_uint2(v == s0 ? 0u : INF, v == s1 ? 0u : INF);
+    cur[v] = val;
+    nxt[v] = val;
+    changed[v] = 0;
+    if ((v & 63) == 0) {
+      const uint64_t w = v >> 6;
+      uint64_t word = 0;
+      if ((s0 >> 6) == w) word |= 1ull << (s0 & 63);
+      if ((s1 >> 6) == w) word |= 1ull << (s1 & 63);
+      act[w] = word;
+    }
+  }
+}
+
 // pagerank-delta.cc:246-270: rank 0, delta 1, all active.
 // pagerank-delta.cc:246-270 (rank 0, delta 1, all active); for PPR every
 // vertex starts at rank = delta = 0, inactive, before k_init_ppr_set.
@@ -114,8 +135,14 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_frontier(
       ++cnt;
     }
     // Frontier values: what the gather reads per edge instead of the bitmap.
-    if (prd_like(kind)) contrib[v] = (a && od) ? delta[v] / (float)od : 0.f;
-    else fval[v] = a ? cur[v] : INF;
+    if (prd_like(kind)) {
+      contrib[v] = (a && od) ? delta[v] / (float)od : 0.f;
+    } else if (kind == KIND_SSSP_PAIR) {
+      reinterpret_cast<uint2*>(fval)[v] =
+          a ? reinterpret_cast<const uint2*>(cur)[v] : make_uint2(INF, INF);
+    } else {
+      fval[v] = a ? cur[v] : INF;
+    }
     if (lane == 0 && word) rowact[v >> log2vpt] = 1;
   }
   te = block_sum<APPLY_THREADS, unsigned long long>(te);
@@ -232,7 +259,7 @@ __device__ inline void load_rows(const GatherArgs<KIND>& A, const uint64_t* rows
     const bool v = r0 + u < R && lane <= (uint32_t)(d[u] & 63u);
     const uint64_t idx = (d[u] >> 22) + (v ? lane : 0u);
     e[u] = A.S.pool[idx];
-    if (KIND == CYTOM_SSSP) wt[u] = A.S.wpool[idx];
+    if (KIND == CYTOM_SSSP || KIND == KIND_SSSP_PAIR) wt[u] = A.S.wpool[idx];
     xb[u] = ((uint32_t)(d[u] >> 6) & 0xFFFFu) << A.S.log2vpt;
     vmask |= (v ? 1u : 0u) << u;
   }
@@ -263,9 +290,13 @@ __device__ inline void gather_rows(const GatherArgs<KIND>& A, uint32_t* win,
     }
     float c[GATHER_ROWS];
     uint32_t val[GATHER_ROWS];
+    uint2 val2[GATHER_ROWS];
     if (KIND == CYTOM_PRD) {
 #pragma unroll
       for (int u = 0; u < GATHER_ROWS; ++u) c[u] = A.contrib[s[u]];
+    } else if (KIND == KIND_SSSP_PAIR) {
+#pragma unroll
+      for (int u = 0; u < GATHER_ROWS; ++u) val2[u] = reinterpret_cast<const uint2*>(A.fval)[s[u]];
     } else {
 #pragma unroll
       for (int u = 0; u < GATHER_ROWS; ++u) val[u] = A.fval[s[u]];
@@ -280,6 +311,13 @@ __device__ inline void gather_rows(const GatherArgs<KIND>& A, uint32_t* win,
       for (int u = 0; u < GATHER_ROWS; ++u)
         if (ok[u] && c[u] != 0.f)  // pagerank-delta.cc:127-132, as exact fixed point
           atomicAdd(&winq[e[u] >> 16], (unsigned long long)__float2ll_rn(c[u] * A.qscale));
+    } else if (KIND == KIND_SSSP_PAIR) {
+#pragma unroll
+      for (int u = 0; u < GATHER_ROWS; ++u) {  // sssp.cc:90-99 for each query of the pair
+        const uint32_t cx = sat_add(val2[u].x, wt[u]), cy = sat_add(val2[u].y, wt[u]);
+        if (ok[u] && cx != INF) atomicMin(&win[2 * (e[u] >> 16)], cx);
+        if (ok[u] && cy != INF) atomicMin(&win[2 * (e[u] >> 16) + 1], cy);
+      }
     } else {
 #pragma unroll
       for (int u = 0; u < GATHER_ROWS; ++u) {
@@ -358,6 +396,8 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
       if (!cleared) {
         if (KIND == CYTOM_PRD) {
           for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) winq[i] = 0ull;
+        } else if (KIND == KIND_SSSP_PAIR) {
+          for (uint32_t i = threadIdx.x; i < 2 * vpt; i += GATHER_THREADS) win[i] = INF;
         } else {
           for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) win[i] = INF;
         }
@@ -376,6 +416,15 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
           atomicAdd(reinterpret_cast<unsigned long long*>(&A.dsum[base + i]), q);
           A.changed[base + i] = 1;
         }
+      } else if (KIND == KIND_SSSP_PAIR) {
+        bool ch = false;
+#pragma unroll
+        for (uint32_t k = 0; k < 2; ++k) {
+          const uint32_t val = win[2 * i + k];
+          uint32_t* dst = &A.nxt[2 * (base + i) + k];
+          if (val != INF && val < *dst) ch |= val < atomicMin(dst, val);
+        }
+        if (ch) A.changed[base + i] = 1;
       } else {
         const uint32_t val = win[i];
         if (val != INF && val < A.nxt[base + i]) {
@@ -482,6 +531,45 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_min(
         act_next[w] = word;
         if (word) rowact_next[v >> log2vpt] = 1;
       }
+    }
+  }
+  apply_tail(te, cnt, upd, nit);
+}
+
+// Apply of a pair run: a vertex is active when either distance dropped; its
+// frontier pair carries the dropped components and INF for the others.
+__global__ void __launch_bounds__(APPLY_THREADS) k_apply_pair(
+    uint64_t V, uint32_t log2vpt, uint2* __restrict__ cur, const uint2* __restrict__ nxt,
+    uint2* __restrict__ fval, uint8_t* __restrict__ changed,
+    const uint64_t* __restrict__ act_cur, uint64_t* __restrict__ act_next,
+    uint8_t* __restrict__ rowact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nwords = V >> 6;
+  if (empty_frontier(nit, act_next, nwords)) return;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long te = 0, cnt = 0, upd = 0;
+  for (uint64_t w = wave; w < nwords; w += stride) {
+    const uint64_t v = (w << 6) | lane;
+    const bool was = (act_cur[w] >> lane) & 1ull;
+    bool a = false;
+    if (changed[v]) {
+      changed[v] = 0;
+      ++upd;
+      const uint2 n = nxt[v], c = cur[v];
+      if (n.x != c.x || n.y != c.y) {
+        cur[v] = n;
+        fval[v] = make_uint2(n.x != c.x ? n.x : INF, n.y != c.y ? n.y : INF);
+        a = true;
+        te += outdeg[v];
+        ++cnt;
+      }
+    }
+    if (was && !a) fval[v] = make_uint2(INF, INF);
+    const uint64_t word = __ballot(a);
+    if (lane == 0) {
+      act_next[w] = word;
+      if (word) rowact_next[v >> log2vpt] = 1;
     }
   }
   apply_tail(te, cnt, upd, nit);
@@ -679,7 +767,7 @@ void launch_gather(cytom_ctx* ctx, Program& P, int it) {
   A.changed = P.changed;
   A.it = ctx->d_iter + it;
   A.CH = ctx->chunk_slots;
-  const size_t lds = (size_t)ctx->vpt * (KIND == CYTOM_PRD ? 8 : 4);
+  const size_t lds = (size_t)ctx->vpt * (KIND == CYTOM_PRD || KIND == KIND_SSSP_PAIR ? 8 : 4);
   k_gather<KIND><<<gather_grid(ctx, lds), GATHER_THREADS, lds, ctx->stream>>>(A);
 }
 
@@ -737,8 +825,9 @@ int program_alloc(cytom_ctx* ctx, Program& P) {
     ok = ok && alloc((void**)&P.rank, V * 4) && alloc((void**)&P.delta, V * 4) &&
          alloc((void**)&P.dsum, V * 8) && alloc((void**)&P.contrib, V * 4);
   } else {
-    ok = ok && alloc((void**)&P.cur, V * 4) && alloc((void**)&P.nxt, V * 4) &&
-         alloc((void**)&P.fval, V * 4);
+    const size_t vb = P.kind == KIND_SSSP_PAIR ? 8 : 4;  // one u32, or a pair, per vertex
+    ok = ok && alloc((void**)&P.cur, V * vb) && alloc((void**)&P.nxt, V * vb) &&
+         alloc((void**)&P.fval, V * vb);
   }
   for (int i = 0; i < 2 && ok; ++i)
     ok = alloc((void**)&P.act[i], (V / 64) * 8) && alloc((void**)&P.rowact[i], ctx->T);
@@ -778,6 +867,10 @@ int program_init(cytom_ctx* ctx, Program& P) {
     if (ppr && P.n_teleport)
       k_init_ppr_set<<<grid_for(P.n_teleport, 256), 256, 0, st>>>(P.teleport, P.n_teleport,
                                                                   P.rank, P.delta, P.act[0]);
+  } else if (P.kind == KIND_SSSP_PAIR) {
+    k_init_pair<<<grid_for(ctx->V, 256), 256, 0, st>>>(
+        ctx->V, P.source, P.source2, reinterpret_cast<uint2*>(P.cur),
+        reinterpret_cast<uint2*>(P.nxt), P.act[0], P.changed);
   } else {
     k_init_min<<<grid_for(ctx->V, 256), 256, 0, st>>>(P.kind, ctx->V, P.source, P.cur, P.nxt,
                                                       P.act[0], P.changed);
@@ -865,7 +958,8 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
   uint32_t it = 0, launched = 0;
   uint32_t group = std::min<uint32_t>(std::max<uint32_t>(P.last_iters, 2), 64), next = 4;
   bool converged = false;
-  const bool prof = ctx->profile && ctx->kev.size() >= 2 * (size_t)maxit;
+  const bool prof = ctx->profile && ctx->kev.size() >= 2 * (size_t)maxit &&
+                    P.kind <= CYTOM_PPR;  // the per-kind profile slots
   while (!converged && launched < maxit) {
     const uint32_t n = std::min(group, maxit - launched);
     for (uint32_t k = 0; k < n; ++k, ++launched) {
@@ -875,6 +969,7 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
         case CYTOM_BFS: launch_gather<CYTOM_BFS>(ctx, P, i); break;
         case CYTOM_CC: launch_gather<CYTOM_CC>(ctx, P, i); break;
         case CYTOM_SSSP: launch_gather<CYTOM_SSSP>(ctx, P, i); break;
+        case KIND_SSSP_PAIR: launch_gather<KIND_SSSP_PAIR>(ctx, P, i); break;
         default: launch_gather<CYTOM_PRD>(ctx, P, i); break;
       }
       if (prof) CY_HIP(ctx, hipEventRecord(ctx->kev[2 * i + 1], st));
@@ -893,6 +988,11 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
         k_apply_min<CYTOM_CC><<<apply_blocks, APPLY_THREADS, 0, st>>>(
             ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
             P.rowact[nx], ctx->outdeg, ctx->d_iter + i + 1);
+      } else if (P.kind == KIND_SSSP_PAIR) {
+        k_apply_pair<<<apply_blocks, APPLY_THREADS, 0, st>>>(
+            ctx->V, ctx->log2vpt, reinterpret_cast<uint2*>(P.cur),
+            reinterpret_cast<const uint2*>(P.nxt), reinterpret_cast<uint2*>(P.fval), P.changed,
+            P.act[P.cur_act], P.act[nx], P.rowact[nx], ctx->outdeg, ctx->d_iter + i + 1);
       } else {
         k_apply_min<CYTOM_SSSP><<<apply_blocks, APPLY_THREADS, 0, st>>>(
             ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],

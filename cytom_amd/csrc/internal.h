@@ -70,10 +70,17 @@ struct StoreView {
 // PageRank-delta and its personalized form share the gather and apply.
 __host__ __device__ inline bool prd_like(int kind) { return kind == CYTOM_PRD || kind == CYTOM_PPR; }
 
+// Internal program kind of the batched queries: two SSSP queries in one run,
+// with a (u32, u32) pair of distances per vertex in cur / nxt / fval (each
+// array V pairs) and a pair window in the gather.  Never registered in a
+// context's program list, so the update hooks never see it.
+constexpr int KIND_SSSP_PAIR = 5;
+
 // Per-program device state.
 struct Program {
   int kind = 0;
   uint32_t source = 0;
+  uint32_t source2 = 0;          // second query's source (KIND_SSSP_PAIR)
   uint32_t* teleport = nullptr;  // PPR teleport set (device, distinct ids)
   uint32_t n_teleport = 0;
   uint32_t* cur = nullptr;  // BFS/CC/SSSP current values

@@ -1159,6 +1159,50 @@ int run_seq_group(cytom_ctx* ctx, int kind, const cytom_query* q, const uint32_t
   return rc;
 }
 
+// Distances of a pair run into two result rows.
+__global__ void k_split_pair(const uint2* __restrict__ cur, uint64_t V, uint32_t* __restrict__ a,
+                             uint32_t* __restrict__ b) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint2 x = cur[v];
+    a[v] = x.x;
+    b[v] = x.y;
+  }
+}
+
+// SSSP queries of the tile-engine route, two per run (KIND_SSSP_PAIR): one
+// pass over the slots and weights per iteration serves both, and each
+// query's distances are its own fixpoint, the same as a single run.  An odd
+// last query runs alone.
+int run_seq_pairs(cytom_ctx* ctx, const cytom_query* q, const uint32_t* idx, uint32_t n,
+                  uint32_t* buf, cytom_run_stats* acc) {
+  int rc = 0;
+  if (n >= 2) {
+    Program P;
+    P.kind = KIND_SSSP_PAIR;
+    if ((rc = program_alloc(ctx, P))) return rc;
+    for (uint32_t k = 0; k + 1 < n && !rc; k += 2) {
+      P.source = q[idx[k]].sources[0];
+      P.source2 = q[idx[k + 1]].sources[0];
+      cytom_run_stats rs{};
+      if ((rc = program_init(ctx, P)) || (rc = run_program(ctx, P, &rs))) break;
+      acc->iterations += rs.iterations;
+      acc->traversed_edges += rs.traversed_edges;
+      acc->active_vertices += rs.active_vertices;
+      acc->slots_scanned += rs.slots_scanned;
+      if (!rs.converged) acc->converged = 0;
+      k_split_pair<<<blocks_for(ctx->V, 256, 16384), 256, 0, ctx->stream>>>(
+          reinterpret_cast<const uint2*>(P.cur), ctx->V, buf + (uint64_t)k * ctx->V,
+          buf + (uint64_t)(k + 1) * ctx->V);
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    program_free(P);
+  }
+  if (!rc && (n & 1))
+    rc = run_seq_group(ctx, CYTOM_SSSP, q, idx + n - 1, 1, buf + (uint64_t)(n - 1) * ctx->V, acc);
+  return rc;
+}
+
 }  // namespace
 
 void query_free(cytom_ctx* ctx) {
@@ -1211,7 +1255,10 @@ int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run
       qe->group_buf.push_back(buf);
       qe->group_kind.push_back(seq ? kind | SEQ_ROUTE : kind);
       for (size_t k = g0; k < g1; ++k) qe->queries[idx[k]] = QueryEngine::Q{kind, gid, (uint32_t)(k - g0)};
-      if (seq) {
+      if (seq && kind == CYTOM_SSSP) {
+        rc = run_seq_pairs(ctx, q, idx.data() + g0, (uint32_t)(g1 - g0),
+                           static_cast<uint32_t*>(buf), &acc);
+      } else if (seq) {
         rc = run_seq_group(ctx, kind, q, idx.data() + g0, (uint32_t)(g1 - g0),
                            static_cast<uint32_t*>(buf), &acc);
       } else if (kind == CYTOM_BFS) {

@@ -459,7 +459,11 @@ def test_batched_queries_large_graph_route():
     assert g.live_edge_count() > (1 << 24)
     rng = np.random.default_rng(2)
     src = rng.integers(0, 64, 6)  # low ids: high degree under RMAT
-    queries = ([("sssp", x) for x in src[:3]] + [("bfs", x) for x in src[3:]] +
+    # SSSP runs two queries per tile-engine pass: five queries give two
+    # pairs (one repeating a source, one with a high-id vertex that reaches
+    # little) and a single run.
+    sssp = [int(x) for x in src[:3]] + [int(src[0]), V - 1]
+    queries = ([("sssp", x) for x in sssp] + [("bfs", x) for x in src[3:]] +
                [("ppr", src), ("ppr", [int(src[0])])])
     g.batched_execute(queries)
     pid = {k: g.register(k, 0) for k in ("bfs", "sssp", "ppr")}
