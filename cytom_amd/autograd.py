@@ -48,7 +48,10 @@ class PPRGraph:
             self._h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: the binding may already be gone
+            pass
 
     def set_weights(self, w):
         w_d = DeviceArray.from_host(np.ascontiguousarray(w, np.float32))
