@@ -302,9 +302,12 @@ class Stream:
         return first.value, idx[:m].copy()
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().cytom_stream_destroy(self._h)
-            self._h = None
+        try:
+            if getattr(self, "_h", None):
+                lib().cytom_stream_destroy(self._h)
+                self._h = None
+        except Exception:  # interpreter shutdown: the binding may already be gone
+            pass
 
 
 class Graph:
