@@ -446,6 +446,24 @@ def test_batched_queries_with_hub_vertices():
         assert np.array_equal(g.read_query(i), g.read(pid[kind])), (i, kind)
 
 
+def test_batched_bfs_deep_graph_rounds():
+    # A 700-vertex path plus sparse random shortcuts: BFS depths in the
+    # hundreds, so the batched BFS runs many rounds of device-driven levels
+    # (8, then 4 at a time, alternating frontier lists) and levels past the
+    # last one.  Levels equal the textbook BFS; 70 queries make two groups.
+    V = 1 << 12
+    rng = np.random.default_rng(12)
+    path = np.arange(700, dtype=np.uint32)
+    s = np.concatenate([path[:-1], rng.integers(0, V, 400).astype(np.uint32)])
+    d = np.concatenate([path[1:], rng.integers(700, V, 400).astype(np.uint32)])
+    g = C.Graph(V, vertices_per_tile=1024)
+    g.update(s, d)
+    srcs = np.concatenate([[0, 1, 350, 699], rng.integers(0, V, 66)]).astype(np.uint32)
+    g.batched_execute([("bfs", int(x)) for x in srcs])
+    for i, x in enumerate(srcs):
+        assert np.array_equal(g.read_query(i), O.check_bfs(V, s, d, int(x))), (i, int(x))
+
+
 def test_batched_queries_large_graph_route():
     # Above 2^24 live edges SSSP and PPR groups run the tile-engine program
     # per query; BFS keeps the bit-parallel groups.  Values still equal the
