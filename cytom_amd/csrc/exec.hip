@@ -1071,6 +1071,8 @@ int configure_gather(cytom_ctx* ctx) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, l4));
   CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_PRD>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, l8));
+  CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<KIND_SSSP_PAIR>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, l8));
   ctx->gather_blocks = 0;
   return 0;
 }
