@@ -538,7 +538,7 @@ def cpu_baseline(args, source):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the engine has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         return 2
     E, per_batch, n_ins, n_del = stream_plan(args)

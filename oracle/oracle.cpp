@@ -1,8 +1,8 @@
 // CPU oracle for the evolving-graph hot path -- TEST INFRASTRUCTURE ONLY.
 //
 // A plain C++ restatement of the reference's streaming ingest and
-// incremental executors under the semantics contract of this build (each
-// deviation from the literal reference code is a named switch below), and
+// incremental executors under a stated semantics contract (each deviation
+// from the literal reference code is a named switch below), and
 // textbook checkers.  It is the checker for the HIP library and the CPU
 // baseline timed beside it; it is never part of the product path.
 //

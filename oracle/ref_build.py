@@ -2,7 +2,7 @@
 
 The reference's hot-path headers cannot be compiled here: util.h pulls in
 schema_generated.h (flatbuffers) and tile-manager.h needs gtest, neither of
-which exists in this image.  The two self-contained pieces this build's
+which exists in this image.  The two self-contained pieces the update
 stream depends on are compiled instead, taken verbatim from the reference
 tree at build time:
 
