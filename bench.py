@@ -9,8 +9,10 @@ live edges).  After every batch the engine runs BFS and SSSP (u32 weights in
 PageRank-delta (damping 0.85, threshold 1e-7, 100-iteration cap).
 
 One step = one update batch: ingest + all four programs to convergence.
-The first --warmup batches are untimed; the next --steps batches are timed.
-All inputs are generated into HBM before the timed region.
+Every run streams all --batches batches; the last --steps batches are timed
+and the --warmup batches before them are run untimed (the batches before
+those are run untimed too, with --prefix-programs).  All inputs are generated
+into HBM before the timed region.
 
 value = traversed edges of all programs over the timed batches / wall time
 (GTEPS); a traversed edge is a live out-edge of a vertex active in an
@@ -61,19 +63,22 @@ def source_digest():
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gather_prd_rmat22_traffic.json")
 
 
+# The settings that define the streamed workload (the timed window does not:
+# every run streams all batches).
+WORKLOAD_KEYS = ("scale", "edge_factor", "batches", "delete_fraction", "programs",
+                 "vertices_per_tile", "chunk_slots", "job_chunks", "blocks_per_cu",
+                 "source_bands", "resort_pct", "seed")
+
+
 def measured_traffic(args, kernel):
     """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE passes over the default workload, or None
+    rocprofv3 FETCH_SIZE/WRITE_SIZE passes over the same workload, or None
     when the workload, the kernel or the HIP sources differ from the
     measured ones."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
     rec = json.load(open(TRAFFIC_FILE))
-    defaults = parse_args([])
-    same = all(getattr(args, k) == getattr(defaults, k)
-               for k in ("scale", "edge_factor", "batches", "programs", "steps", "warmup",
-                         "vertices_per_tile", "chunk_slots", "job_chunks", "blocks_per_cu",
-                         "source_bands", "resort_pct", "seed"))
+    same = rec.get("workload_key") == {k: getattr(args, k) for k in WORKLOAD_KEYS}
     if not same or rec.get("kernel") != kernel or rec.get("source_digest") != source_digest():
         return None
     return rec["hbm_bytes_per_launch"]
@@ -86,8 +91,8 @@ def parse(argv=None):
         given = set(a.split("=")[0] for a in (sys.argv[1:] if argv is None else argv))
         preset = {"--scale": ("scale", 27), "--batches": ("batches", 128),
                   "--programs": ("programs", "cc,pagerank-delta"),
-                  "--warmup": ("warmup", 120), "--steps": ("steps", 8),
-                  "--warmup-programs": ("warmup_programs", "cc"),
+                  "--warmup": ("warmup", 2), "--steps": ("steps", 8),
+                  "--prefix-programs": ("prefix_programs", "cc"),
                   "--cpu-baseline": ("cpu_baseline", "off"), "--queries": ("queries", "off"),
                   "--ppr-grad": ("ppr_grad", "off"),
                   # 67M tiles of ~29 edges: small chunks put more tiles in
@@ -117,9 +122,9 @@ def parse_args(argv):
     ap.add_argument("--resort-pct", type=int, default=0,
                     help="sort-compaction threshold in percent (-1 disables)")
     ap.add_argument("--programs", default="bfs,cc,sssp,pagerank-delta")
-    ap.add_argument("--warmup-programs", default="all",
-                    help="programs run after each warmup batch (default: all); programs "
-                         "re-initialised every batch lose nothing when skipped")
+    ap.add_argument("--prefix-programs", default="all",
+                    help="programs run after each batch before the warmup batches (default: "
+                         "all); programs re-initialised every batch lose nothing when skipped")
     ap.add_argument("--config", choices=["rmat22", "rmat27"], default="rmat22",
                     help="rmat27: the RMAT-27 stream (2^31 updates in 128 batches of 2^24, "
                          "CC + PageRank-delta), last 8 batches timed")
@@ -134,7 +139,8 @@ def parse_args(argv):
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--cpu-batches", type=int, default=2,
-                    help="timed batches of the CPU sample (the last ones of the stream)")
+                    help="timed batches of the CPU sample (the last ones of the stream, "
+                         "inside the GPU's timed window)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (the reference "
@@ -200,17 +206,18 @@ def gpu_run(args):
     pids = {}
     for n in names:
         pids[n] = g.register(n, source if n in ("bfs", "sssp") else 0)
-    warm = min(args.warmup, args.batches)
-    steps = min(args.steps, args.batches - warm)
-
-    warm_names = names if args.warmup_programs == "all" else \
-        [p.strip() for p in args.warmup_programs.split(",") if p.strip()]
+    steps = max(0, min(args.steps, args.batches))
+    warm = max(0, min(args.warmup, args.batches - steps))
+    prefix = args.batches - steps - warm
+    prefix_names = names if args.prefix_programs == "all" else \
+        [p.strip() for p in args.prefix_programs.split(",") if p.strip()]
 
     def step(b, acc, run=names):
         s, d, w, ds, dd, dw, nd = batches[b]
         bs = g.update_device(s, d, w, n_ins, ds, dd, dw, nd)
         acc["ingest_ms"] += bs.t_ingest_ms
         acc["deleted"] += bs.deleted
+        te = 0
         for n, p in pids.items():
             if n not in run:
                 continue
@@ -219,25 +226,33 @@ def gpu_run(args):
             acc["ms"][n] += st.t_compute_ms
             acc["iters"][n] += st.iterations
             acc["scanned"][n] += st.slots_scanned
+            te += st.traversed_edges
         acc["live"] = bs.live_edges
+        return te
 
     def new_acc():
         return {"ingest_ms": 0.0, "deleted": 0, "live": 0, "te": {n: 0 for n in names},
                 "ms": {n: 0.0 for n in names}, "iters": {n: 0 for n in names},
                 "scanned": {n: 0 for n in names}}
 
-    wacc = new_acc()
-    for b in range(warm):
-        step(b, wacc, warm_names)
+    pacc = new_acc()
+    for b in range(prefix):
+        step(b, pacc, prefix_names)
         if (b + 1) % 16 == 0:
-            log("bench: warmup batch %d/%d" % (b + 1, warm))
-    log("bench: warmup %d batches done" % warm)
+            log("bench: untimed batch %d/%d" % (b + 1, prefix))
+    wacc = new_acc()
+    for b in range(prefix, prefix + warm):
+        step(b, wacc)
+    log("bench: %d untimed + %d warmup batches done" % (prefix, warm))
     g.set_profiling(True)
     acc = new_acc()
+    per_batch_te, per_batch_s = [], []
     C.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
-        step(warm + k, acc)
+        tb0 = time.perf_counter()
+        per_batch_te.append(step(prefix + warm + k, acc))
+        per_batch_s.append(time.perf_counter() - tb0)  # execute() ends synchronised
         if (k + 1) % 8 == 0:
             log("bench: timed batch %d/%d" % (k + 1, steps))
     C.synchronize()
@@ -248,16 +263,26 @@ def gpu_run(args):
     dom = max(names, key=lambda n: prof[n][0])
     ms, launches, alg = prof[dom]
     achieved = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    cpu_n = max(0, min(args.cpu_batches, steps))
     detail = {
+        "batches_streamed": args.batches,
+        "timed_batches": [prefix + warm + 1, args.batches] if steps else [],
         "gteps_by_program": {n: (acc["te"][n] / (acc["ms"][n] * 1e-3) / 1e9
                                  if acc["ms"][n] else 0.0) for n in names},
         "traversed_edges_by_program": acc["te"],
         "iterations_by_program": acc["iters"],
         "slots_scanned_by_program": acc["scanned"],
+        "slots_per_traversed_edge": {n: round(acc["scanned"][n] / acc["te"][n], 3)
+                                     if acc["te"][n] else None for n in names},
         "compute_ms_by_program": {n: round(acc["ms"][n], 3) for n in names},
         "ingest_ms_total": round(acc["ingest_ms"], 3),
         "update_batches_per_s": steps / wall if wall > 0 else 0.0,
         "live_edges_final": acc["live"],
+        # The GPU rate over exactly the batches the CPU baseline times.
+        "gteps_last_batches": {
+            "batches": [args.batches - cpu_n + 1, args.batches] if cpu_n else [],
+            "value": (sum(per_batch_te[steps - cpu_n:]) / sum(per_batch_s[steps - cpu_n:]) / 1e9
+                      if cpu_n and sum(per_batch_s[steps - cpu_n:]) > 0 else None)},
         "gather_kernel_by_program": {
             n: {"ms": round(prof[n][0], 3), "launches": prof[n][1],
                 "alg_gbytes": round(prof[n][2] / 1e9, 3),
@@ -276,11 +301,15 @@ def gpu_run(args):
         "avg_launch_us": round(ms * 1e3 / launches, 2) if launches else None,
         "alg_bytes_per_launch": round(alg / launches) if launches else None,
     }
+    # Final values of every program, for the comparison against the CPU
+    # oracle on the same stream (done in the CPU leg).
+    final = {n: g.read(p) for n, p in pids.items()}
     res = {
         "value": te_total / wall / 1e9 if wall > 0 else 0.0,
         "ms_per_step": wall * 1e3 / steps if steps else 0.0,
         "steps": steps,
         "warmup": warm,
+        "prefix": prefix,
         "source": source,
         "n_ins": n_ins,
         "n_del": n_del,
@@ -288,6 +317,7 @@ def gpu_run(args):
         "tuning": g.get_tuning(),
         "roofline": roofline,
         "detail": detail,
+        "final": final,
     }
     if args.queries == "on":
         res["detail"]["batched_queries"] = queries_run(args, g, V, source)
@@ -397,12 +427,40 @@ def ppr_grad_run(args):
     dw = C.DeviceArray(E, np.float32)
     ds = C.DeviceArray(V, np.float32)
     ta, sweeps_a = _timed(lambda: G.adjoint_device(r_dev, g_dev, dw, ds, tol=1e-10))
+    # Central differences (h = 1e-3) of the loss on edges drawn with
+    # probability proportional to |dL/dw| (most edges' gradients are below the
+    # FP32 resolution of the loss), against the adjoint's gradient.
+    grad = dw.to_host()
+    n_fd = 64
+    pw = np.abs(grad).astype(np.float64)
+    picks = rng.choice(E, n_fd, replace=False, p=pw / pw.sum())
+
+    def loss(wv):
+        G.set_weights(wv)
+        G.forward_device(t_dev, r_dev, tol=1e-10)
+        rv = r_dev.to_host().astype(np.float64)
+        return float(np.sum(rv[targets] ** 2))
+
+    h = 1e-3
+    t_fd = time.perf_counter()
+    fd = []
+    for e in picks:
+        wp, wm = w.copy(), w.copy()
+        wp[e] += h
+        wm[e] -= h
+        fd.append((loss(wp) - loss(wm)) / (2 * h))
+    t_fd = time.perf_counter() - t_fd
+    fd = np.asarray(fd)
+    rel_fd = float(np.linalg.norm(grad[picks] - fd) / max(np.linalg.norm(fd), 1e-300))
     G.close()
     log("bench: PPR gradient on RMAT-20: build %.1f ms, forward %.1f ms (%d sweeps), adjoint %.1f ms "
-        "(%d sweeps)" % (tb * 1e3, tf * 1e3, sweeps_f, ta * 1e3, sweeps_a))
+        "(%d sweeps); central differences on %d edges: rel. error %.2e"
+        % (tb * 1e3, tf * 1e3, sweeps_f, ta * 1e3, sweeps_a, n_fd, rel_fd))
     return {"edges": E, "build_ms": round(tb * 1e3, 2), "forward_ms": round(tf * 1e3, 2),
             "forward_sweeps": sweeps_f, "adjoint_ms": round(ta * 1e3, 2),
-            "adjoint_sweeps": sweeps_a}
+            "adjoint_sweeps": sweeps_a,
+            "fd_check": {"edges": n_fd, "h": h, "rel_error_vs_central_differences": rel_fd,
+                         "seconds": round(t_fd, 2)}}
 
 
 def tenants_run(args):
@@ -439,32 +497,38 @@ def tenants_run(args):
         host.append((C.DeviceArray.from_host(us), C.DeviceArray.from_host(ud), len(us)))
     e_total = sum(h[2] for h in host)
 
+    # Contexts are created (and their CC registered) before either timed
+    # region, so both legs time only the update batches and CC runs.
+    t_ctx = time.perf_counter()
+    plain_graphs = []
+    for gi, (scale, ef) in enumerate(specs):
+        G = C.Graph(1 << scale, edge_capacity=int((ef << scale) * 1.3) + 1024)
+        plain_graphs.append((G, G.register("cc")))
+    G_union = C.Graph(V_union, edge_capacity=int(e_total * 1.3) + 1024)
+    p_union = G_union.register("cc")
+    C.synchronize()
+    t_ctx = time.perf_counter() - t_ctx
+
     def plain():
-        labels = []
-        for gi, (scale, ef) in enumerate(specs):
-            G = C.Graph(1 << scale, edge_capacity=int((ef << scale) * 1.3) + 1024)
-            p = G.register("cc")
+        for gi, (G, p) in enumerate(plain_graphs):
             for b in range(nb):
                 s, d, n = dev[gi][b]
                 G.update_device(s, d, None, n)
                 G.execute(p)
-            labels.append(G.read(p) if gi < 4 else None)
-            G.close()
-        return labels
+        return [G.read(p) for G, p in plain_graphs[:4]]
 
     def union():
-        G = C.Graph(V_union, edge_capacity=int(e_total * 1.3) + 1024)
-        p = G.register("cc")
         for b in range(nb):
             s, d, n = host[b]
-            G.update_device(s, d, None, n)
-            G.execute(p)
-        lab = G.read(p)
-        G.close()
-        return lab
+            G_union.update_device(s, d, None, n)
+            G_union.execute(p_union)
+        return G_union.read(p_union)
 
     tp, lp = _timed(plain)
     tb, lu = _timed(union)
+    for G, _ in plain_graphs:
+        G.close()
+    G_union.close()
     same = True
     for gi in range(min(4, len(specs))):
         part = lu[bases[gi]:bases[gi] + (1 << specs[gi][0])].astype(np.int64) - bases[gi]
@@ -473,20 +537,37 @@ def tenants_run(args):
         % (len(specs), e_total, tb * 1e3, tp * 1e3))
     return {"queries": len(specs), "edges": e_total, "batched_ms": round(tb * 1e3, 2),
             "plain_ms": round(tp * 1e3, 2), "speedup": round(tp / tb, 2),
+            "context_setup_ms_untimed": round(t_ctx * 1e3, 2),
             "batched_qps": round(len(specs) / tb, 2), "plain_qps": round(len(specs) / tp, 2),
             "sample_equal": same}
 
 
-def cpu_baseline(args, source):
+def cpu_model_name():
+    name = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return name
+
+
+def cpu_baseline(args, source, gpu_final):
     """The oracle (multi-threaded C++ restatement) on the host cores: the
-    last --cpu-batches batches of the same stream, ingest + all programs."""
+    last --cpu-batches batches of the same stream (inside the GPU's timed
+    window), ingest + all programs.  Afterwards, outside the CPU timing, the
+    GPU's final values are compared with the oracle's: BFS/CC/SSSP exactly,
+    PageRank-delta by normalised L1 against the oracle's FP32 run and against
+    its FP64 restatement on the same final graph."""
     import oracle as O
 
     E, per_batch, n_ins, n_del = stream_plan(args)
     V = 1 << args.scale
     names = [p.strip() for p in args.programs.split(",") if p.strip()]
     weighted = "sssp" in names
-    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    threads = args.cpu_threads or O.host_threads()
     t_setup = time.perf_counter()
     st = O.Stream(args.seed)
     plan = [st.next(n_ins, n_del) for _ in range(args.batches)]
@@ -497,7 +578,8 @@ def cpu_baseline(args, source):
 
     g = O.Graph(V, vertices_per_tile=65536, weighted=weighted, max_iterations=100, delta=1e-7,
                 threads=threads)
-    first_timed = max(0, args.batches - args.cpu_batches)
+    n_timed = max(1, min(args.cpu_batches, args.batches))
+    first_timed = args.batches - n_timed
     with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as ex:
         futs = [ex.submit(gen, b) for b in range(args.batches)]
         for b in range(first_timed):
@@ -514,25 +596,50 @@ def cpu_baseline(args, source):
         for n, p in pids.items():
             te += g.execute(p).traversed_edges
     wall = time.perf_counter() - t0
-    cpu_model = platform.processor() or "unknown"
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return {
+    nproc = os.cpu_count() or 1
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    out = {
         "value": te / wall / 1e9 if wall > 0 else 0.0,
         "unit": "GTEPS",
         "cores": threads,
         "kind": "port",
-        "sample": "last %d of %d batches of the same RMAT-%d stream (earlier batches ingested "
-                  "untimed, programs from init as after a re-execution), ingest + %s; "
-                  "%.1f s timed; %s" % (len(timed_inputs), args.batches, args.scale,
-                                        "/".join(names), wall, cpu_model),
+        "sample": "batches %d-%d of the %d-batch RMAT-%d stream (the last %d of the GPU's timed "
+                  "window; earlier batches ingested untimed, programs from init as after a "
+                  "re-execution), ingest + %s; %.1f s timed; %d threads (nproc %d, affinity %d, "
+                  "OMP_NUM_THREADS %s); %s"
+                  % (first_timed + 1, args.batches, args.batches, args.scale, n_timed,
+                     "/".join(names), wall, threads, nproc, affinity,
+                     os.environ.get("OMP_NUM_THREADS", "unset"), cpu_model_name()),
     }
+    # Parity of the GPU's final values against the oracle (not timed).
+    parity = {}
+    for n, p in pids.items():
+        if n not in gpu_final:
+            continue
+        ov, gv = g.read(p), gpu_final[n]
+        if n == "pagerank-delta":
+            parity["pagerank-delta_l1_vs_oracle_fp32"] = l1n(gv, ov)
+        else:
+            parity[n + "_equal"] = bool(np.array_equal(ov, gv))
+    if "pagerank-delta" in gpu_final:
+        t1 = time.perf_counter()
+        ls, ld, _ = g.live_edges()
+        r64, it64 = O.prd_f64(V, ls, ld, max_iterations=100, delta=1e-7, threads=threads)
+        parity["pagerank-delta_l1_vs_fp64"] = l1n(gpu_final["pagerank-delta"], r64)
+        parity["oracle_fp32_l1_vs_fp64"] = l1n(g.read(pids["pagerank-delta"]), r64)
+        parity["fp64_iterations"] = int(it64)
+        parity["pagerank-delta_fixed_point_residual"] = O.prd_residual(
+            V, ls, ld, gpu_final["pagerank-delta"], threads=threads)
+        log("bench: FP64 PageRank-delta on the final graph %.1fs" % (time.perf_counter() - t1))
+    out["final_parity"] = parity
+    return out
+
+
+def l1n(a, b):
+    """L1 distance of two rank vectors after L1 normalisation."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a / a.sum() - b / b.sum()).sum())
 
 
 def main():
@@ -547,7 +654,8 @@ def main():
     res = gpu_run(args)
     cpu = None
     if args.cpu_baseline == "auto":
-        cpu = cpu_baseline(args, res["source"])
+        cpu = cpu_baseline(args, res["source"], res["final"])
+        res["detail"]["final_parity"] = cpu.pop("final_parity")
     names = [p.strip() for p in args.programs.split(",") if p.strip()]
     line = {
         "metric": METRIC,
@@ -564,8 +672,9 @@ def main():
         "data": "synthetic (counter-based RMAT stream, seed %d)" % args.seed,
         "config": {
             "workload": "RMAT-%d stream, %d batches x %d updates (%d inserts + %d deletions); "
-                        "%s after every batch" % (args.scale, args.batches, per_batch, n_ins,
-                                                  n_del, ", ".join(names)),
+                        "%s after every batch; all %d batches streamed, batches %d-%d timed"
+                        % (args.scale, args.batches, per_batch, n_ins, n_del, ", ".join(names),
+                           args.batches, res["prefix"] + res["warmup"] + 1, args.batches),
             "scale": args.scale,
             "edge_factor": args.edge_factor,
             "vertices_per_tile": res["vpt"],

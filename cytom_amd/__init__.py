@@ -74,6 +74,7 @@ class Tuning(ctypes.Structure):
         ("source_bands", ctypes.c_uint32),
         ("resort_pct", ctypes.c_uint32),
         ("resort_max_slots", ctypes.c_uint32),
+        ("prd_frac_bits", ctypes.c_uint32),
     ]
 
 
@@ -180,6 +181,40 @@ def _p(a):
 
 def _u32(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _ids(a, name):
+    """Vertex ids as a contiguous 1-D uint32 array; rejects values that do not
+    fit in 32 bits (negative or above 2^32 - 1) instead of wrapping them."""
+    a = np.asarray(a)
+    if a.ndim != 1:
+        raise ValueError("%s must be one-dimensional" % name)
+    if a.size and a.dtype.kind in "iuf":
+        if a.dtype.kind == "f" and not np.array_equal(a, np.floor(a)):
+            raise ValueError("%s holds non-integral ids" % name)
+        if a.min() < 0 or a.max() > 0xFFFFFFFF:
+            raise ValueError("%s holds ids outside [0, 2^32)" % name)
+    elif a.size:
+        raise ValueError("%s must be numeric" % name)
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _weights(a, name):
+    """Edge weights as uint32; they must be integral and in [1, 2^32) (the
+    engine's u32 SSSP weights), so a float weight is never truncated."""
+    if a is None:
+        return None
+    a = _ids(a, name)
+    if a.size and a.min() == 0:
+        raise ValueError("%s holds zero weights (weights must be >= 1)" % name)
+    return a
+
+
+def _same_len(*named):
+    arrs = [(n, a) for n, a in named if a is not None]
+    if len(set(len(a) for _, a in arrs)) > 1:
+        raise ValueError("array lengths differ: " +
+                         ", ".join("%s=%d" % (n, len(a)) for n, a in arrs))
 
 
 
@@ -367,10 +402,16 @@ class Graph:
                     "set_sources")
 
     def update(self, src, dst, w=None, dsrc=None, ddst=None, dw=None):
-        src, dst, w = _u32(src), _u32(dst), _u32(w)
-        dsrc = _u32(dsrc if dsrc is not None else np.empty(0, np.uint32))
-        ddst = _u32(ddst if ddst is not None else np.empty(0, np.uint32))
-        dw = _u32(dw)
+        """One update batch from host arrays: inserts (src, dst[, w]) then
+        deletions (dsrc, ddst[, dw]).  Arrays of one side must have equal
+        lengths; weights must be positive integers below 2^32."""
+        src, dst = _ids(src, "src"), _ids(dst, "dst")
+        w = _weights(w, "w")
+        dsrc = _ids(dsrc if dsrc is not None else np.empty(0, np.uint32), "dsrc")
+        ddst = _ids(ddst if ddst is not None else np.empty(0, np.uint32), "ddst")
+        dw = _weights(dw, "dw")
+        _same_len(("src", src), ("dst", dst), ("w", w))
+        _same_len(("dsrc", dsrc), ("ddst", ddst), ("dw", dw))
         st = BatchStats()
         self._check(lib().cytom_update_batch(self._h, _p(src), _p(dst), _p(w), len(src), _p(dsrc),
                                              _p(ddst), _p(dw), len(dsrc), ctypes.byref(st)),
@@ -378,6 +419,12 @@ class Graph:
         return st
 
     def update_device(self, src, dst, w, n_ins, dsrc=None, ddst=None, dw=None, n_del=0):
+        """One update batch from device arrays (DeviceArray or raw pointers)
+        of at least n_ins / n_del elements."""
+        for name, a, n in (("src", src, n_ins), ("dst", dst, n_ins), ("w", w, n_ins),
+                           ("dsrc", dsrc, n_del), ("ddst", ddst, n_del), ("dw", dw, n_del)):
+            if isinstance(a, DeviceArray) and a.n < n:
+                raise ValueError("%s holds %d elements, %d needed" % (name, a.n, n))
         st = BatchStats()
         self._check(lib().cytom_update_batch_device(
             self._h, _p(src), _p(dst), _p(w), n_ins, _p(dsrc), _p(ddst), _p(dw), n_del,
@@ -459,10 +506,10 @@ class Graph:
         return o, i
 
     def set_tuning(self, chunk_slots=0, job_chunks=0, blocks_per_cu=0, source_bands=0,
-                   resort_pct=0, resort_max_slots=0):
+                   resort_pct=0, resort_max_slots=0, prd_frac_bits=0):
         """Zero keeps a field; resort_pct=RESORT_OFF disables sort-compaction."""
         t = Tuning(chunk_slots, job_chunks, blocks_per_cu, source_bands, resort_pct,
-                   resort_max_slots)
+                   resort_max_slots, prd_frac_bits)
         self._check(lib().cytom_set_tuning(self._h, ctypes.byref(t)), "tuning")
 
     def get_tuning(self):
@@ -470,7 +517,8 @@ class Graph:
         self._check(lib().cytom_get_tuning(self._h, ctypes.byref(t)), "tuning")
         return {"chunk_slots": t.chunk_slots, "job_chunks": t.job_chunks,
                 "blocks_per_cu": t.blocks_per_cu, "source_bands": t.source_bands,
-                "resort_pct": t.resort_pct, "resort_max_slots": t.resort_max_slots}
+                "resort_pct": t.resort_pct, "resort_max_slots": t.resort_max_slots,
+                "prd_frac_bits": t.prd_frac_bits}
 
     def set_profiling(self, enable=True):
         self._check(lib().cytom_set_profiling(self._h, 1 if enable else 0), "profiling")
@@ -487,11 +535,12 @@ class Graph:
     def live_edge_count(self):
         return self._check(lib().cytom_export_edges(self._h, None, None, None, 0), "export")
 
-    def export_edges(self):
+    def export_edges(self, weights=True):
+        """Live edges (src, dst, w); w is None with weights=False."""
         n = self._check(lib().cytom_export_edges(self._h, None, None, None, 0), "export")
         s = np.empty(n, np.uint32)
         d = np.empty(n, np.uint32)
-        w = np.empty(n, np.uint32)
+        w = np.empty(n, np.uint32) if weights else None
         m = self._check(lib().cytom_export_edges(self._h, _p(s), _p(d), _p(w), n), "export")
         assert m == n
         return s, d, w

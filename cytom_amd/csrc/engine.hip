@@ -89,6 +89,14 @@ void free_ctx(cytom_ctx* c) {
   delete c;
 }
 
+// Fraction bits of the PageRank-delta fixed-point sums that keep any
+// vertex's sum (at most 2V in magnitude) inside int64: 60 - ceil(log2 2V).
+int max_prd_shift(uint64_t V) {
+  int lg = 0;
+  while ((1ull << lg) < V) ++lg;
+  return 60 - (lg + 1);
+}
+
 #define CY_CHECK_CTX(ctx)                                   \
   do {                                                      \
     if (!(ctx)) return CYTOM_EINVAL;                        \
@@ -131,11 +139,7 @@ int cytom_create(const cytom_config* cfg, cytom_ctx** out) {
   c->log2vpt = 0;
   while ((1u << c->log2vpt) < vpt) ++c->log2vpt;
   c->n_tiles = (uint64_t)c->T * c->T;
-  {
-    int lg = 0;
-    while ((1ull << lg) < V) ++lg;
-    c->prd_shift = 60 - (lg + 1);
-  }
+  c->prd_shift = max_prd_shift(V);
   c->max_iter = cfg->max_iterations ? cfg->max_iterations : 100;
   c->delta = cfg->delta;
   c->weighted = cfg->weighted != 0;
@@ -368,6 +372,9 @@ int cytom_set_tuning(cytom_ctx* ctx, const cytom_tuning* t) {
   if (t->blocks_per_cu > 16) return set_err(ctx, CYTOM_EINVAL, "blocks_per_cu above 16");
   if (t->source_bands && (t->source_bands & (t->source_bands - 1)))
     return set_err(ctx, CYTOM_EINVAL, "source_bands must be a power of two");
+  if (t->prd_frac_bits && (t->prd_frac_bits < 16 || (int)t->prd_frac_bits > max_prd_shift(ctx->V)))
+    return set_err(ctx, CYTOM_EINVAL, "prd_frac_bits outside [16, 60 - ceil(log2 2V)]");
+  if (t->prd_frac_bits) ctx->prd_shift = (int)t->prd_frac_bits;
   if (t->chunk_slots) ctx->chunk_slots = t->chunk_slots;
   if (t->job_chunks) ctx->job_chunks = t->job_chunks;
   if (t->blocks_per_cu) ctx->gather_blocks = (uint32_t)ctx->num_cus * t->blocks_per_cu;
@@ -386,6 +393,7 @@ int cytom_get_tuning(const cytom_ctx* ctx, cytom_tuning* t) {
   t->source_bands = std::min(ctx->source_bands, ctx->T);
   t->resort_pct = ctx->resort_pct;
   t->resort_max_slots = ctx->resort_max_slots;
+  t->prd_frac_bits = (uint32_t)ctx->prd_shift;
   return 0;
 }
 

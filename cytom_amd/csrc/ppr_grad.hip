@@ -98,6 +98,26 @@ __global__ void k_prob(const uint32_t* __restrict__ perm, const uint32_t* __rest
   }
 }
 
+// |v - x| for the convergence test, with a NaN or infinite iterate mapped to
+// +inf (fmaxf would drop a NaN, and a diverged solve must not pass as a
+// converged one).
+__device__ inline float change_of(float v, float x) {
+  const float d = fabsf(v - x);
+  return d <= 3.402823466e38f ? d : __int_as_float(0x7f800000);
+}
+
+// Flags (*bad = 1) any weight that is not positive and finite.
+__global__ void k_check_weights(const float* __restrict__ w, uint64_t E,
+                                unsigned int* __restrict__ bad) {
+  bool b = false;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const float x = w[i];
+    b |= !(x > 0.f && x <= 3.402823466e38f);
+  }
+  if (b) *bad = 1u;
+}
+
 __device__ inline void block_max_to(float d, unsigned int* out) {
   __shared__ float wmax[GT / 64];
 #pragma unroll
@@ -127,7 +147,7 @@ __global__ void k_sweep(const uint64_t* __restrict__ off, uint64_t R, const uint
     for (uint64_t i = b; i < e; ++i) acc += p[i] * x[nb[i]];
     const float v = base_scale * base[r] + a * acc;
     y[r] = v;
-    dmax = fmaxf(dmax, fabsf(v - x[r]));
+    dmax = fmaxf(dmax, change_of(v, x[r]));
   }
   block_max_to(dmax, diff);
 }
@@ -186,7 +206,7 @@ __global__ void k_sweep_rows(const uint32_t* __restrict__ rows, uint32_t nrows,
     for (uint64_t j = sbase[k]; j < sbase[k + 1]; ++j) acc += part[j];
     const float v = base_scale * base[r] + a * acc;
     y[r] = v;
-    dmax = fmaxf(dmax, fabsf(v - x[r]));
+    dmax = fmaxf(dmax, change_of(v, x[r]));
   }
   block_max_to(dmax, diff);
 }
@@ -391,6 +411,19 @@ int build_structure(Op& op, uint64_t V, uint64_t E, const uint32_t* src, const u
 // (in the caller's edge order).
 int set_weights(Op& op, const uint32_t* src, const float* w, hipStream_t st) {
   const uint64_t V = op.V, E = op.E;
+  op.msg.clear();
+  if (E) {
+    unsigned int bad = 0;
+    if (hipMemsetAsync(op.scratch + 3, 0, 4, st) != hipSuccess) return CYTOM_EDEVICE;
+    k_check_weights<<<gblocks(E, GT, 4096), GT, 0, st>>>(w, E, op.scratch + 3);
+    if (hipMemcpyAsync(&bad, op.scratch + 3, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return CYTOM_EDEVICE;
+    if (bad) {
+      op.msg = "edge weights must be positive and finite";
+      return CYTOM_EINVAL;
+    }
+  }
   if (hipMemsetAsync(op.W, 0, V * 4, st) != hipSuccess) return CYTOM_EDEVICE;
   k_row_sum<<<gblocks(V, GT, 16384), GT, 0, st>>>(op.out.off, V, op.out.perm, w, op.W);
   if (op.out.nlong) {
@@ -411,6 +444,7 @@ int set_weights(Op& op, const uint32_t* src, const float* w, hipStream_t st) {
 int solve(Op& op, const Rows& R, const float* base, float base_scale, float a, float tol,
           uint32_t max_iter, float* x, float* y, uint32_t* iters, hipStream_t st) {
   const uint64_t V = op.V;
+  op.msg.clear();
   uint32_t it = 0;
   for (; it < max_iter; ++it) {
     if (hipMemsetAsync(op.scratch, 0, 4, st) != hipSuccess) return CYTOM_EDEVICE;
@@ -431,6 +465,10 @@ int solve(Op& op, const Rows& R, const float* base, float base_scale, float a, f
         return CYTOM_EDEVICE;
       float d;
       std::memcpy(&d, &bits, 4);
+      if (!(d <= 3.402823466e38f)) {
+        op.msg = "the iteration diverged (a non-finite iterate)";
+        return CYTOM_ERANGE;
+      }
       if (d <= tol) {
         ++it;
         break;

@@ -53,6 +53,12 @@ from .execute import GROUP_KEYS
 
 USAGE_LIMIT = 100.0   # run.py:577
 STOP_USAGE = 90.0     # run.py:505
+# Tasks alive at once, whatever the usage model predicts: every task is a
+# process with the GPU open, and a card serves only so many such processes.
+MAX_ALIVE = 8
+# Floor of a predicted usage (percent): a fitted line can go to zero or below
+# for small windows, which would admit any number of them.
+MIN_PREDICT = 1.0
 DEFAULT_USAGE = 50.0  # usage of an algorithm on a graph with no profile
 MEMORY_FRACTION = 0.9
 DEVICE_MEMORY_DEFAULT = 288 << 30  # MI355X HBM3E
@@ -277,8 +283,9 @@ class ProcessRunner:
 class Scheduler:
     def __init__(self, state=None, spawn=None, busy=device_busy_percent, memory_bytes=None,
                  clock=time.monotonic, sleep=time.sleep, out=print, poll_interval=0.5,
-                 log_root=None):
+                 log_root=None, max_alive=MAX_ALIVE):
         self.state = state if state is not None else SchedulerState()
+        self.max_alive = max(1, min(int(max_alive), 16))
         self.spawn = spawn or ProcessRunner(log_root or os.path.join(os.getcwd(), "logs"))
         self.busy = busy
         self.memory_limit = MEMORY_FRACTION * (memory_bytes if memory_bytes is not None
@@ -335,7 +342,8 @@ class Scheduler:
             self.fit_true_data(X, y)
 
     def predict(self, sum_profile):
-        return sum_profile if self.unfit else self.model.predict([[sum_profile]])[0]
+        p = sum_profile if self.unfit else self.model.predict([[sum_profile]])[0]
+        return max(MIN_PREDICT, float(p))
 
     # -- running tasks
     def add_algos(self, algos):
@@ -399,7 +407,8 @@ class Scheduler:
         device_bytes = estimate_device_bytes([a.job for a in algos])
         current = self.current_usage()
         avail = (current + predict <= USAGE_LIMIT
-                 and self.bytes_current + device_bytes <= self.memory_limit)
+                 and self.bytes_current + device_bytes <= self.memory_limit
+                 and len(self.alive) < self.max_alive)
         return avail, sum_profile, predict, device_bytes
 
     def schedule(self):

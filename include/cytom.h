@@ -190,9 +190,11 @@ int cytom_read_query(cytom_ctx* ctx, uint32_t query, void* out, size_t bytes);
 
 /* Weighted personalized PageRank on an explicit edge list (device arrays;
  * no reference counterpart).  P_e = w_e / W_u with W_u the sum of the
- * positive weights leaving u (rows without out-edges drop their mass);
+ * weights leaving u (rows without out-edges drop their mass); weights must
+ * be positive and finite (CYTOM_EINVAL otherwise).
  * r = (1-alpha) s + alpha P^T r, solved by FP32 Jacobi sweeps from
- * (1-alpha) s until the largest change is <= tol or max_iterations sweeps.
+ * (1-alpha) s until the largest change is <= tol or max_iterations sweeps;
+ * a non-finite iterate (from non-finite s) fails with CYTOM_ERANGE.
  * Deterministic (fixed summation order).  d_r receives float[V]. */
 int cytom_ppr_weighted(uint64_t n_vertices, uint64_t n_edges, const uint32_t* d_src,
                        const uint32_t* d_dst, const float* d_w, const float* d_s, float alpha,
@@ -252,6 +254,10 @@ typedef struct {
   uint32_t resort_pct;    /* sort-compact a tile once its unsorted tail reaches this
                              percentage of its sorted prefix; CYTOM_RESORT_OFF disables */
   uint32_t resort_max_slots; /* slots staged per sort-compaction pass (more tiles wait) */
+  uint32_t prd_frac_bits; /* fraction bits of the PageRank-delta fixed-point sums, in
+                             [16, 60 - ceil(log2 2V)] (the default is the maximum; lower
+                             values exist to measure the quantisation).  Set it before
+                             PageRank-delta programs are registered or re-initialise them. */
 } cytom_tuning;
 #define CYTOM_RESORT_OFF 0xFFFFFFFFu
 int cytom_set_tuning(cytom_ctx* ctx, const cytom_tuning* t);

@@ -96,6 +96,18 @@ def lib():
     L.orc_check_sssp.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _u32p, _u32p, _u32p,
                                  ctypes.c_uint32, _u32p]
     L.orc_check_cc.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _u32p, _u32p, _u32p]
+    L.orc_check_multi.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _u32p, _u32p,
+                                  ctypes.c_void_p, _u32p, ctypes.c_uint32, ctypes.c_int, _u32p]
+    L.orc_check_multi.restype = None
+    L.orc_checker_threads.argtypes = [ctypes.c_int]
+    L.orc_checker_threads.restype = None
+    L.orc_check_cc_mt.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _u32p, _u32p, ctypes.c_int,
+                                  _u32p]
+    L.orc_prd_f64_mt.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _u32p, _u32p,
+                                 ctypes.c_uint32, ctypes.c_double, ctypes.c_int, _f64p]
+    L.orc_prd_f64_mt.restype = ctypes.c_uint32
+    L.orc_prd_residual.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _u32p, _u32p, _f32p,
+                                   ctypes.c_double, ctypes.c_int, _f64p]
     L.orc_prd_f64.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _u32p, _u32p, ctypes.c_uint32,
                               ctypes.c_double, _f64p]
     L.orc_prd_f64.restype = ctypes.c_uint32
@@ -257,6 +269,11 @@ class Graph:
         return o, i
 
 
+def checker_threads(n):
+    """Threads the textbook checkers use to build their rows."""
+    lib().orc_checker_threads(int(n))
+
+
 def check_bfs(V, s, d, source):
     out = np.empty(V, np.uint32)
     lib().orc_check_bfs(V, len(s), _u32(s), _u32(d), source, out)
@@ -269,16 +286,49 @@ def check_sssp(V, s, d, w, source):
     return out
 
 
-def check_cc(V, s, d):
-    out = np.empty(V, np.uint32)
-    lib().orc_check_cc(V, len(s), _u32(s), _u32(d), out)
+def check_multi(V, s, d, sources, w=None, threads=1):
+    """Textbook BFS levels (w None) or Dijkstra distances from each source:
+    an (n, V) uint32 array."""
+    src = _u32(np.atleast_1d(sources))
+    out = np.empty((len(src), V), np.uint32)
+    w = _u32(w)
+    lib().orc_check_multi(V, len(s), _u32(s), _u32(d), _ptr(w), src, len(src), threads, out)
     return out
 
 
-def prd_f64(V, s, d, max_iterations=100, delta=1e-7):
+def check_cc(V, s, d, threads=1):
+    out = np.empty(V, np.uint32)
+    if threads > 1:
+        lib().orc_check_cc_mt(V, len(s), _u32(s), _u32(d), threads, out)
+    else:
+        lib().orc_check_cc(V, len(s), _u32(s), _u32(d), out)
+    return out
+
+
+def prd_f64(V, s, d, max_iterations=100, delta=1e-7, threads=1):
     out = np.empty(V, np.float64)
-    it = lib().orc_prd_f64(V, len(s), _u32(s), _u32(d), max_iterations, delta, out)
+    it = lib().orc_prd_f64_mt(V, len(s), _u32(s), _u32(d), max_iterations, delta, threads, out)
     return out, it
+
+
+def prd_residual(V, s, d, rank, alpha=0.85, threads=1):
+    """Normalised one-sweep residual of the PageRank fixed point over FP32
+    ranks: sum|r - (1-a) - a P^T r| / sum|r| (FP64)."""
+    out = np.zeros(2, np.float64)
+    r = np.ascontiguousarray(rank, dtype=np.float32)
+    lib().orc_prd_residual(V, len(s), _u32(s), _u32(d), r, alpha, threads, out)
+    return out[0] / out[1] if out[1] else 0.0
+
+
+def host_threads():
+    """Host threads a checker may use: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS when the environment sets it."""
+    import os
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
 
 
 def ppr_f64(V, s, d, teleport, alpha=0.85, tol=1e-13, max_iterations=10000):

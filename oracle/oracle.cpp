@@ -941,11 +941,68 @@ CSR build_csr(uint64_t V, uint64_t E, const uint32_t* a, const uint32_t* b, cons
   }
   return c;
 }
+
+// fn(i0, i1) on `threads` contiguous pieces of [0, n), one std::thread each.
+void par_ranges(int threads, uint64_t n, const std::function<void(uint64_t, uint64_t)>& fn) {
+  if (threads <= 1 || n < (1u << 16)) {
+    fn(0, n);
+    return;
+  }
+  std::vector<std::thread> ts;
+  for (int t = 0; t < threads; ++t) {
+    const uint64_t a = n * (uint64_t)t / (uint64_t)threads;
+    const uint64_t b = n * (uint64_t)(t + 1) / (uint64_t)threads;
+    ts.emplace_back([&fn, a, b] { fn(a, b); });
+  }
+  for (auto& t : ts) t.join();
+}
+
+// The same rows built by several threads: counts and cursors by relaxed
+// atomics, so the order inside a row depends on the schedule (the checkers
+// that use it do not depend on that order).  One thread = build_csr.
+CSR build_csr_mt(uint64_t V, uint64_t E, const uint32_t* a, const uint32_t* b,
+                 const uint32_t* w, int threads) {
+  if (threads <= 1) return build_csr(V, E, a, b, w);
+  CSR c;
+  std::vector<uint64_t> cur(V, 0);
+  par_ranges(threads, E, [&](uint64_t i0, uint64_t i1) {
+    for (uint64_t i = i0; i < i1; ++i) __atomic_fetch_add(&cur[a[i]], 1ull, __ATOMIC_RELAXED);
+  });
+  c.off.assign(V + 1, 0);
+  for (uint64_t v = 0; v < V; ++v) {
+    c.off[v + 1] = c.off[v] + cur[v];
+    cur[v] = c.off[v];
+  }
+  c.adj.resize(E);
+  if (w) c.wt.resize(E);
+  par_ranges(threads, E, [&](uint64_t i0, uint64_t i1) {
+    for (uint64_t i = i0; i < i1; ++i) {
+      const uint64_t p = __atomic_fetch_add(&cur[a[i]], 1ull, __ATOMIC_RELAXED);
+      c.adj[p] = b[i];
+      if (w) c.wt[p] = w[i];
+    }
+  });
+  return c;
+}
+
+std::vector<uint32_t> out_degrees_mt(uint64_t V, uint64_t E, const uint32_t* src, int threads) {
+  std::vector<uint32_t> deg(V, 0);
+  par_ranges(threads, E, [&](uint64_t i0, uint64_t i1) {
+    for (uint64_t i = i0; i < i1; ++i) __atomic_fetch_add(&deg[src[i]], 1u, __ATOMIC_RELAXED);
+  });
+  return deg;
+}
 }  // namespace
+
+namespace {
+int g_checker_threads = 1;  // threads that build the checkers' rows
+}
+
+void orc_checker_threads(int n) { g_checker_threads = n < 1 ? 1 : n; }
 
 void orc_check_bfs(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                    uint32_t source, uint32_t* out) {
-  CSR c = build_csr(V, E, src, dst, nullptr);
+  CSR c = build_csr_mt(V, E, src, dst, nullptr, g_checker_threads);
   std::fill(out, out + V, INF);
   std::vector<uint32_t> q;
   q.reserve(V);
@@ -965,7 +1022,7 @@ void orc_check_bfs(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* 
 
 void orc_check_sssp(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                     const uint32_t* w, uint32_t source, uint32_t* out) {
-  CSR c = build_csr(V, E, src, dst, w);
+  CSR c = build_csr_mt(V, E, src, dst, w, g_checker_threads);
   std::vector<uint64_t> dist(V, UINT64_MAX);
   using Item = std::pair<uint64_t, uint32_t>;
   std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
@@ -987,10 +1044,79 @@ void orc_check_sssp(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t*
   for (uint64_t v = 0; v < V; ++v) out[v] = dist[v] >= INF ? INF : (uint32_t)dist[v];
 }
 
+// Queue BFS / Dijkstra from several sources over one set of rows (built
+// once); the sources are split over `threads` threads, out is n x V.
+namespace {
+void bfs_rows(const CSR& c, uint64_t V, uint32_t source, uint32_t* out) {
+  std::fill(out, out + V, INF);
+  std::vector<uint32_t> q;
+  q.reserve(1024);
+  out[source] = 0;
+  q.push_back(source);
+  for (size_t h = 0; h < q.size(); ++h) {
+    const uint32_t u = q[h];
+    for (uint64_t p = c.off[u]; p < c.off[u + 1]; ++p) {
+      const uint32_t v = c.adj[p];
+      if (out[v] == INF) {
+        out[v] = out[u] + 1;
+        q.push_back(v);
+      }
+    }
+  }
+}
+
+void dijkstra_rows(const CSR& c, uint64_t V, uint32_t source, uint32_t* out) {
+  std::vector<uint64_t> dist(V, UINT64_MAX);
+  using Item = std::pair<uint64_t, uint32_t>;
+  std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
+  dist[source] = 0;
+  pq.push({0, source});
+  while (!pq.empty()) {
+    auto [d, u] = pq.top();
+    pq.pop();
+    if (d != dist[u]) continue;
+    for (uint64_t p = c.off[u]; p < c.off[u + 1]; ++p) {
+      const uint32_t v = c.adj[p];
+      const uint64_t nd = d + c.wt[p];
+      if (nd < dist[v]) {
+        dist[v] = nd;
+        pq.push({nd, v});
+      }
+    }
+  }
+  for (uint64_t v = 0; v < V; ++v) out[v] = dist[v] >= INF ? INF : (uint32_t)dist[v];
+}
+}  // namespace
+
+void orc_check_multi(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                     const uint32_t* w, const uint32_t* sources, uint32_t n, int threads,
+                     uint32_t* out) {
+  CSR c = build_csr_mt(V, E, src, dst, w, threads);
+  std::atomic<uint32_t> next{0};
+  const int nt = std::max(1, std::min<int>(threads, (int)n));
+  std::vector<std::thread> ts;
+  for (int t = 0; t < nt; ++t)
+    ts.emplace_back([&] {
+      for (uint32_t k = next++; k < n; k = next++) {
+        if (w) dijkstra_rows(c, V, sources[k], out + (uint64_t)k * V);
+        else bfs_rows(c, V, sources[k], out + (uint64_t)k * V);
+      }
+    });
+  for (auto& t : ts) t.join();
+}
+
 // label[v] = min({v} U {u : u reaches v}) over directed paths.
 void orc_check_cc(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                   uint32_t* out) {
-  CSR c = build_csr(V, E, src, dst, nullptr);
+  orc_check_cc_mt(V, E, src, dst, g_checker_threads, out);
+}
+
+// The same labels; the rows are built by `threads` threads, the search is
+// one depth-first sweep in increasing root order (each vertex is labelled by
+// the first, i.e. smallest, root that reaches it).
+void orc_check_cc_mt(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                     int threads, uint32_t* out) {
+  CSR c = build_csr_mt(V, E, src, dst, nullptr, threads);
   std::fill(out, out + V, INF);
   std::vector<uint32_t> stack;
   for (uint64_t u = 0; u < V; ++u) {
@@ -1013,50 +1139,97 @@ void orc_check_cc(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* d
 
 uint32_t orc_prd_f64(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                      uint32_t max_iterations, double delta, double* rank) {
-  CSR in = build_csr(V, E, dst, src, nullptr);  // in-edges, deterministic order
-  std::vector<uint32_t> outdeg(V, 0);
-  for (uint64_t i = 0; i < E; ++i) outdeg[src[i]]++;
+  return orc_prd_f64_mt(V, E, src, dst, max_iterations, delta, 1, rank);
+}
+
+// pagerank-delta.cc:127-174 in FP64 from init (rank 0, delta 1, all
+// active).  Each vertex's sum is taken over its in-edges in row order by
+// one thread, so the threads only split the vertex range.
+uint32_t orc_prd_f64_mt(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                        uint32_t max_iterations, double delta, int threads, double* rank) {
+  CSR in = build_csr_mt(V, E, dst, src, nullptr, threads);  // in-edges
+  std::vector<uint32_t> outdeg = out_degrees_mt(V, E, src, threads);
   std::vector<double> d(V, 1.0), s(V, 0.0);
   std::vector<uint8_t> act(V, 1), nact(V, 0);
   std::fill(rank, rank + V, 0.0);
   uint32_t it = 0;
   bool first = true;
   while (it < max_iterations) {
-    for (uint64_t v = 0; v < V; ++v) {
-      double acc = 0.0;
-      for (uint64_t p = in.off[v]; p < in.off[v + 1]; ++p) {
-        uint32_t u = in.adj[p];
-        if (act[u]) acc += d[u] / (double)outdeg[u];
-      }
-      s[v] = acc;
-    }
-    bool conv = true;
-    std::fill(nact.begin(), nact.end(), 0);
-    for (uint64_t v = 0; v < V; ++v) {
-      double dv;
-      if (first) {
-        dv = (1. - ALPHA) + ALPHA * s[v];
-        rank[v] += dv;
-        dv -= 1.0;
-      } else {
-        if (s[v] == 0.0) {
-          continue;
+    par_ranges(threads, V, [&](uint64_t v0, uint64_t v1) {
+      for (uint64_t v = v0; v < v1; ++v) {
+        double acc = 0.0;
+        for (uint64_t p = in.off[v]; p < in.off[v + 1]; ++p) {
+          uint32_t u = in.adj[p];
+          if (act[u]) acc += d[u] / (double)outdeg[u];
         }
-        dv = ALPHA * s[v];
-        rank[v] += dv;
+        s[v] = acc;
       }
-      d[v] = dv;
-      if (std::abs(dv) > delta * rank[v]) {
-        nact[v] = 1;
-        conv = false;
+    });
+    std::atomic<bool> conv{true};
+    par_ranges(threads, V, [&](uint64_t v0, uint64_t v1) {
+      bool c = true;
+      for (uint64_t v = v0; v < v1; ++v) {
+        nact[v] = 0;
+        double dv;
+        if (first) {
+          dv = (1. - ALPHA) + ALPHA * s[v];
+          rank[v] += dv;
+          dv -= 1.0;
+        } else {
+          if (s[v] == 0.0) continue;
+          dv = ALPHA * s[v];
+          rank[v] += dv;
+        }
+        d[v] = dv;
+        if (std::abs(dv) > delta * rank[v]) {
+          nact[v] = 1;
+          c = false;
+        }
       }
-    }
+      if (!c) conv = false;
+    });
     ++it;
     first = false;
     act.swap(nact);
     if (conv) break;
   }
   return it;
+}
+
+// One FP64 sweep of the PageRank fixed point over FP32 ranks:
+// out[0] = sum_v |r_v - (1-alpha) - alpha * sum_{u->v} r_u / outdeg(u)|,
+// out[1] = sum_v |r_v|.  A PageRank-delta run that pushes every delta it
+// computes ends on that fixed point (pagerank-delta.cc:142-174: rank_v =
+// (1-alpha) + alpha * sum_u (1 + pushed deltas of u) / outdeg(u)); deltas it
+// drops below the threshold leave a residual of at most alpha * delta *
+// sum_u r_u, plus the FP32 rounding of the ranks.
+void orc_prd_residual(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                      const float* rank, double alpha, int threads, double* out) {
+  CSR in = build_csr_mt(V, E, dst, src, nullptr, threads);
+  std::vector<uint32_t> outdeg = out_degrees_mt(V, E, src, threads);
+  const int nt = threads < 1 ? 1 : threads;
+  std::vector<double> part(2 * (size_t)nt, 0.0);
+  std::atomic<int> slot{0};
+  par_ranges(nt, V, [&](uint64_t v0, uint64_t v1) {
+    double res = 0.0, tot = 0.0;
+    for (uint64_t v = v0; v < v1; ++v) {
+      double acc = 0.0;
+      for (uint64_t p = in.off[v]; p < in.off[v + 1]; ++p) {
+        const uint32_t u = in.adj[p];
+        acc += (double)rank[u] / (double)outdeg[u];
+      }
+      res += std::abs((double)rank[v] - (1.0 - alpha) - alpha * acc);
+      tot += std::abs((double)rank[v]);
+    }
+    const int k = slot.fetch_add(1);
+    part[2 * k] = res;
+    part[2 * k + 1] = tot;
+  });
+  out[0] = out[1] = 0.0;
+  for (int k = 0; k < slot.load(); ++k) {
+    out[0] += part[2 * k];
+    out[1] += part[2 * k + 1];
+  }
 }
 
 uint32_t orc_ppr_f64(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,

@@ -107,16 +107,36 @@ uint64_t orc_live_edges(const orc_graph* g, uint32_t* src, uint32_t* dst, uint32
 void orc_degrees(const orc_graph* g, uint32_t* outdeg, uint32_t* indeg);
 
 /* ---- textbook checkers on an explicit edge list ------------------------ */
+/* Threads that build the checkers' adjacency rows (default 1); the searches
+ * themselves are sequential, so results do not depend on it. */
+void orc_checker_threads(int n_threads);
 void orc_check_bfs(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                    uint32_t source, uint32_t* out);
 void orc_check_sssp(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                     const uint32_t* w, uint32_t source, uint32_t* out);
 void orc_check_cc(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                   uint32_t* out);
+/* The same labels with the rows built by n_threads threads (billion-edge
+ * graphs); the depth-first sweep stays sequential. */
+void orc_check_cc_mt(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                     int n_threads, uint32_t* out);
+/* Queue BFS levels (w == NULL) or Dijkstra distances (u32 weights w) from
+ * each of n sources on one set of rows; out holds n x V values. */
+void orc_check_multi(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                     const uint32_t* w, const uint32_t* sources, uint32_t n, int n_threads,
+                     uint32_t* out);
 /* FP64 restatement of the PageRank-delta program from scratch (same cap and
  * activation threshold).  Returns the iteration count. */
 uint32_t orc_prd_f64(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
                      uint32_t max_iterations, double delta, double* rank);
+/* The same with the vertex range split over n_threads threads. */
+uint32_t orc_prd_f64_mt(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                        uint32_t max_iterations, double delta, int n_threads, double* rank);
+/* One FP64 sweep of the PageRank fixed point over FP32 ranks r:
+ * out[0] = sum_v |r_v - (1-alpha) - alpha sum_{u->v} r_u/outdeg(u)|,
+ * out[1] = sum_v |r_v|. */
+void orc_prd_residual(uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                      const float* rank, double alpha, int n_threads, double* out);
 /* FP64 power iteration of personalized PageRank:
  * r = (1-a) t + a * sum_{u->v} r_u / outdeg(u), t uniform over the distinct
  * teleport vertices; mass at vertices without out-edges is dropped. */

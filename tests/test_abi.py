@@ -44,7 +44,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(C.Config) == 8 + 4 + 4 + 8 + 8 + 4 + 4
     assert ctypes.sizeof(C.RunStats) == 4 * 4 + 8 * 3 + 8
     assert ctypes.sizeof(C.BatchStats) == 8 * 4 + 4 * 4 + 8
-    assert ctypes.sizeof(C.Tuning) == 4 * 6
+    assert ctypes.sizeof(C.Tuning) == 4 * 7
 
 
 def test_invalid_arguments_fail_without_a_device():
@@ -67,3 +67,29 @@ def test_stream_sampler_matches_oracle():
         fb, db = b.next(n_ins, n_del)
         assert fa == fb
         assert np.array_equal(da, db)
+
+
+def test_update_arguments_checked_before_the_library_is_called():
+    # A Graph shell with no context: every case below must be rejected by the
+    # binding itself (the library would read n*4 bytes from each array).
+    g = C.Graph.__new__(C.Graph)
+    g._h = None
+    u = lambda *v: np.array(v, np.uint32)
+    with pytest.raises(ValueError, match="lengths differ"):
+        g.update(u(1, 2, 3), u(4, 5))
+    with pytest.raises(ValueError, match="lengths differ"):
+        g.update(u(1, 2), u(4, 5), u(1))
+    with pytest.raises(ValueError, match="lengths differ"):
+        g.update(u(), u(), None, u(1, 2), u(3), None)
+    with pytest.raises(ValueError, match="lengths differ"):
+        g.update(u(), u(), None, u(1), u(3), u(1, 1))
+    with pytest.raises(ValueError, match="non-integral"):
+        g.update(u(1), u(2), np.array([1.5], np.float32))
+    with pytest.raises(ValueError, match="zero weights"):
+        g.update(u(1), u(2), np.array([0.0]))
+    with pytest.raises(ValueError, match="outside"):
+        g.update(np.array([-1]), u(2))
+    with pytest.raises(ValueError, match="outside"):
+        g.update(u(1), u(2), np.array([1 << 33]))
+    # integral floats are accepted as ids/weights
+    assert C._weights(np.array([3.0, 255.0]), "w").tolist() == [3, 255]

@@ -213,57 +213,6 @@ def test_medium_stream_against_checkers():
         assert l1n(gg.read(ids[3]), og.read(op)) <= PRD_L1_TOL
 
 
-def test_rmat22_stream_full_size():
-    # The measured workload at its full size: RMAT-22, 64 batches of 943,719
-    # inserts + 104,857 deletions drawn from the live edges, BFS/SSSP from
-    # vertex 0, CC and PageRank-delta after every batch (53.7 M live edges at
-    # the end).  Size-independent checks: after batches 32 and 64 the
-    # BFS/CC/SSSP values equal the textbook checkers on the exported live
-    # graph (bit-exact); the final PageRank-delta run from init equals the
-    # oracle's run from init on the same graph within the L1 bound, and when
-    # the stream's last run re-initialised it, the stream's ranks equal that
-    # run bit for bit (fixed-point sums make it deterministic).
-    import os
-    scale, V, n_ins, n_del = 22, 1 << 22, 943719, 104857
-    st = C.Stream(2)
-    g = C.Graph(V, weighted=True, max_iterations=100, delta=1e-7,
-                edge_capacity=3 * (1 << 26))
-    ids = {n: g.register(n, 0) for n in ("bfs", "cc", "sssp", "pagerank-delta")}
-    last = {}
-    for b in range(64):
-        first, didx = st.next(n_ins, n_del)
-        s, d, w = C.rmat_generate(scale, 2, first, n_ins, weights=True)
-        di = C.DeviceArray.from_host(didx)
-        ds, dd, dw = C.rmat_gather(scale, 2, di, len(didx), weights=True)
-        bs = g.update_device(s, d, w, n_ins, ds, dd, dw, len(didx))
-        assert bs.deleted == len(didx) and bs.missing == 0, b  # drawn from live edges
-        for n, pid in ids.items():
-            last[n] = g.execute(pid)
-            assert last[n].converged == 1 or n == "pagerank-delta", (b, n)
-        if b in (31, 63):
-            print("rmat22 stream: checkers after batch %d" % (b + 1), flush=True)
-            ls, ld, lw = g.export_edges()
-            assert len(ls) == bs.live_edges
-            assert np.array_equal(g.read(ids["bfs"]), O.check_bfs(V, ls, ld, 0)), b
-            assert np.array_equal(g.read(ids["cc"]), O.check_cc(V, ls, ld)), b
-            assert np.array_equal(g.read(ids["sssp"]), O.check_sssp(V, ls, ld, lw, 0)), b
-    assert bs.live_edges == 64 * (n_ins - n_del)
-    prd = ids["pagerank-delta"]
-    stream_ranks = g.read(prd)
-    g.init(prd)
-    g.execute(prd)
-    ranks = g.read(prd)
-    if last["pagerank-delta"].reexecuted:
-        assert np.array_equal(stream_ranks, ranks)
-    print("rmat22 stream: oracle PageRank-delta on the final graph", flush=True)
-    og = O.Graph(V, vertices_per_tile=16384, max_iterations=100, delta=1e-7,
-                 threads=min(16, os.cpu_count() or 1))
-    og.update(ls, ld)
-    op = og.register(O.PRD)
-    og.execute(op)
-    assert l1n(ranks, og.read(op)) <= PRD_L1_TOL
-
-
 # ------------------------------------------------------------------ edge cases
 def test_empty_and_missing_and_invalid():
     g = C.Graph(256, vertices_per_tile=64, weighted=True)
@@ -277,8 +226,14 @@ def test_empty_and_missing_and_invalid():
     with pytest.raises(C.CytomError) as ei:
         g.update(np.array([1], np.uint32), np.array([256], np.uint32), np.array([1], np.uint32))
     assert ei.value.code == C.EINVAL
-    with pytest.raises(C.CytomError):
+    # a zero weight: refused by the binding, and by the library itself
+    with pytest.raises(ValueError):
         g.update(np.array([1], np.uint32), np.array([2], np.uint32), np.array([0], np.uint32))
+    one, zero = C.DeviceArray.from_host(np.array([1], np.uint32)), C.DeviceArray.from_host(
+        np.array([0], np.uint32))
+    with pytest.raises(C.CytomError) as ei:
+        g.update_device(one, C.DeviceArray.from_host(np.array([2], np.uint32)), zero, 1)
+    assert ei.value.code == C.EINVAL
     assert g.export_edges()[0].size == 0
     # self loops, duplicates, and a deletion of an edge inserted in the same batch
     s = np.array([3, 3, 3, 5, 5, 7], np.uint32)
@@ -326,6 +281,45 @@ def test_prd_insert_only_hooks_path():
         og.execute(op)
         gg.execute(gp)
         assert l1n(gg.read(gp), og.read(op)) <= PRD_L1_TOL, b
+
+
+def test_prd_fixed_point_fraction_bits():
+    # The PageRank-delta neighbour sums are int64 fixed point with
+    # 60 - ceil(log2 2V) fraction bits: 43 at RMAT-16, 32 at RMAT-27.  Forcing
+    # 32 bits at RMAT-16 reproduces the RMAT-27 quantum (2^-32 per
+    # contribution) on a graph the FP64 restatement solves in seconds: the
+    # ranks must stay within the 1e-6 bound of the FP64 run and of the FP32
+    # oracle.  16 bits (quantum 1.5e-5) shows the check is sensitive to the
+    # quantisation: its error is far larger.
+    scale, V = 16, 1 << 16
+    st = O.Stream(21)
+    og = O.Graph(V, vertices_per_tile=16384)
+    graphs = {}
+    for bits in (0, 32, 16):
+        g = C.Graph(V)
+        if bits:
+            g.set_tuning(prd_frac_bits=bits)
+        assert g.get_tuning()["prd_frac_bits"] == (bits or 43)
+        graphs[bits] = (g, g.register(C.PRD))
+    op = og.register(O.PRD)
+    for b in range(3):
+        first, didx = st.next(300000, 30000 if b else 0)
+        s, d, _, ds, dd, _ = O.batch_edges(scale, 21, first, 300000, didx, False)
+        og.update(s, d, None, ds, dd, None)
+        for g, p in graphs.values():
+            g.update(s, d, None, ds, dd, None)
+            g.execute(p)
+        og.execute(op)
+    ls, ld, _ = og.live_edges()
+    r64, _ = O.prd_f64(V, ls, ld)
+    err = {bits: (l1n(g.read(p), r64), l1n(g.read(p), og.read(op)))
+           for bits, (g, p) in graphs.items()}
+    print("PageRank-delta L1 (vs FP64, vs FP32 oracle) by fraction bits:", err, flush=True)
+    for bits in (0, 32):
+        assert err[bits][0] <= PRD_L1_TOL and err[bits][1] <= PRD_L1_TOL, (bits, err[bits])
+    assert err[16][0] > 10 * max(err[32][0], 1e-9)
+    with pytest.raises(C.CytomError):
+        graphs[0][0].set_tuning(prd_frac_bits=44)  # above 60 - ceil(log2 2V) at V = 2^16
 
 
 # ------------------------------------------------------- personalized PageRank
@@ -530,6 +524,37 @@ def test_ppr_grad_forward_adjoint_and_autograd():
     assert torch.equal(r2, r.detach())
 
 
+def test_ppr_weights_must_be_positive_and_finite():
+    # A zero, negative, NaN or infinite weight is refused (P = w / W would be
+    # NaN or negative); a refused set_weights leaves the operator unchanged; a
+    # non-finite teleport vector makes the solve fail instead of reporting a
+    # converged NaN result.
+    from cytom_amd import autograd as A
+    V, E = 1 << 10, 5000
+    s, d, w, t = _weighted(V, E, 9)
+    for bad in (0.0, -1.0, np.nan, np.inf):
+        wb = w.copy()
+        wb[17] = bad
+        with pytest.raises(C.CytomError) as ei:
+            A.PPRGraph(V, s, d, wb)
+        assert ei.value.code == C.EINVAL, bad
+    G = A.PPRGraph(V, s, d, w)
+    r0 = G.forward(t)
+    wb = w.copy()
+    wb[3] = np.nan
+    with pytest.raises(C.CytomError) as ei:
+        G.set_weights(wb)
+    assert ei.value.code == C.EINVAL and "positive" in str(ei.value)
+    assert np.array_equal(G.forward(t), r0)
+    tb = t.copy()
+    tb[5] = np.nan
+    with pytest.raises(C.CytomError) as ei:
+        G.forward(tb)
+    assert ei.value.code == C.ERANGE
+    assert np.array_equal(G.forward(t), r0)
+    G.close()
+
+
 def test_ppr_grad_finite_differences_gpu():
     # Central differences of the FP32 forward (h = 1e-3) on sampled edges.
     from cytom_amd import autograd as A
@@ -596,28 +621,3 @@ def test_ppr_graph_handle_matches_one_shot_calls():
     bad[123] = V
     with pytest.raises(C.CytomError):
         A.PPRGraph(V, s, bad, w)
-
-
-def test_ppr_grad_rmat20_against_fp64_adjoint():
-    # RMAT-20 (2^24 edges), U(0,1] weights, 256 teleport vertices, loss over
-    # 1,024 targets: the GPU adjoint against the FP64 iterative adjoint.
-    from cytom_amd import autograd as A
-    from oracle import ppr_grad as G
-    scale, V = 20, 1 << 20
-    E = 1 << 24
-    s, d = O.rmat_edges(scale, 20, 0, E)
-    rng = np.random.default_rng(20)
-    w = (1.0 - rng.random(E)).astype(np.float32)
-    t = np.zeros(V, np.float32)
-    t[rng.choice(V, 256, replace=False)] = 1.0 / 256
-    targets = rng.integers(0, V, 1024)
-    Gd = A.PPRGraph(V, s, d, w)
-    r = Gd.forward(t)
-    g = np.zeros(V, np.float32)
-    np.add.at(g, targets, 2.0 * r[targets])
-    dw, _ = Gd.adjoint(r, g)
-    r_ref = G.forward(V, s, d, w.astype(np.float64), t.astype(np.float64), tol=1e-13)
-    _, g_ref = G.loss_sq(r_ref, targets)
-    dw_ref, _, _ = G.adjoint(V, s, d, w.astype(np.float64), r_ref, g_ref, tol=1e-13)
-    assert _rel(r, r_ref) < 1e-5
-    assert _rel(dw, dw_ref) < 1e-4

@@ -298,3 +298,27 @@ def test_interrupt_stops_running_tasks(tmp_path):
     with pytest.raises(KeyboardInterrupt):
         sch.run_all(_algos("g1", ["bfs"]) + _algos("g2", ["bfs"]))
     assert sorted(map(tuple, Handle.stopped)) == [("bfs",), ("bfs",)] and not sch.alive
+
+
+def test_alive_tasks_capped_when_the_model_predicts_nothing(tmp_path):
+    # A model fitted on windows that measured ~0 % predicts zero (or less)
+    # for every window, so usage never stops admission; the alive cap does.
+    # Forty single-algorithm datasets at 0 % profiled usage each.
+    profiles = {}
+    algos = []
+    for i in range(40):
+        g = "g%d" % i
+        profiles.update(_prof(g, bfs=0.0))
+        algos += _algos(g, ["bfs"])
+    sch, sp, _ = _sched(tmp_path, profiles, model=([[10.0], [20.0]], [-10.0, -20.0]))
+    assert sch.predict(0.0) == S.MIN_PREDICT  # floored, never <= 0
+    assert sch.run_all(algos) == 0
+    assert len(sp.launched) == 40
+    assert sp.max_alive == S.MAX_ALIVE <= 16
+
+
+def test_alive_cap_is_bounded_by_sixteen(tmp_path):
+    st = S.SchedulerState(str(tmp_path / "s.json"))
+    sch = S.Scheduler(state=st, spawn=FakeSpawn(Clock()), busy=lambda: 0.0, memory_bytes=1 << 40,
+                      max_alive=100)
+    assert sch.max_alive == 16
