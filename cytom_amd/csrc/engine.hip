@@ -462,7 +462,10 @@ int64_t cytom_stream_next(cytom_stream* s, uint64_t n_ins, uint64_t n_del, uint6
   if (!s || !ins_first || (n_del && !del_index)) return CYTOM_EINVAL;
   *ins_first = s->next_index;
   try {
-    s->live.reserve(s->live.size() + n_ins);
+    // Geometric growth: reserving the exact new size would copy the whole
+    // live list on every batch (16 GB per batch late in a 2^31-edge stream).
+    const size_t need = s->live.size() + n_ins;
+    if (need > s->live.capacity()) s->live.reserve(std::max(need, 2 * s->live.capacity()));
   } catch (...) {
     return CYTOM_ENOMEM;
   }

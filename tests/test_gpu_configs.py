@@ -133,11 +133,13 @@ def test_c2_rmat22_stream_checked_every_8_batches(say):
             e32, e64, eo = l1n(ranks, og.read(op)), l1n(ranks, r64), l1n(og.read(op), r64)
             say("rmat22 stream: PageRank-delta after batch %d: L1 vs FP32 oracle %.3g, vs FP64 "
                 "%.3g (oracle FP32 vs FP64 %.3g)" % (b + 1, e32, e64, eo))
-            assert e32 <= PRD_L1_TOL, (b, e32)
-            # FP32 arithmetic itself sits at some distance from FP64: the GPU
-            # may be no further from FP64 than the 1e-6 bound or twice the
-            # FP32 oracle's own distance, whichever is larger.
-            assert e64 <= max(PRD_L1_TOL, 2 * eo), (b, e64, eo)
+            # The GPU's neighbour sums are exact (int64 fixed point); the FP32
+            # oracle's float sums round in thread order, which at 40 M edges
+            # alone puts it ~1e-6 from FP64.  The bound: the GPU within 1e-6
+            # of the FP64 restatement, and within 1e-6 plus the oracle's own
+            # FP64 distance of the FP32 oracle (triangle inequality).
+            assert e64 <= PRD_L1_TOL, (b, e64)
+            assert e32 <= PRD_L1_TOL + eo, (b, e32, eo)
             del og
     assert bs.live_edges == 64 * (n_ins - n_del)
 
