@@ -109,6 +109,9 @@ SIGNATURES = {
     "cytom_batched_execute": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
     "cytom_ppr_weighted": (ctypes.c_int, [_U64, _U64, _VP, _VP, _VP, _VP, ctypes.c_float,
                                           ctypes.c_float, _U32, _VP, _VP]),
+    "cytom_ppr_weighted_noalloc": (ctypes.c_int, [_U64, _U64, _VP, _VP, _VP, _VP,
+                                                  ctypes.c_float, ctypes.c_float, _U32, _VP,
+                                                  _VP]),
     "cytom_ppr_adjoint": (ctypes.c_int, [_U64, _U64, _VP, _VP, _VP, _VP, _VP, ctypes.c_float,
                                          ctypes.c_float, _U32, _VP, _VP, _VP]),
     "cytom_ppr_graph_create": (ctypes.c_int, [_U64, _U64, _VP, _VP, _VP,

@@ -14,7 +14,9 @@
 // edge list; the tile store is not involved.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "devutil.h"
@@ -284,15 +286,23 @@ struct Rows {
   uint32_t nseg = 0;
 };
 
+// Device memory of one operator.  By default every buffer is its own
+// hipMalloc.  In arena mode (set_arena) buffers are carved from one caller
+// region instead and nothing is allocated: kept buffers from the bottom up,
+// build-only temporaries from the top down (drop_temps releases them all at
+// once), so a build needs its kept buffers plus one row build's temporaries.
 struct Op {
   uint64_t V = 0, E = 0;
   Rows out, in;  // CSR by source, CSC by target
   float* W = nullptr;
-  unsigned int* scratch = nullptr;  // [0] diff, [1] count
+  unsigned int* scratch = nullptr;  // [0] diff, [1] count, [2] max id, [3] bad weight
   void* tmp = nullptr;
   std::vector<void*> owned;
   int err = 0;
   std::string msg;
+  char* arena_lo = nullptr;   // next kept byte (arena mode)
+  char* arena_hi = nullptr;   // first byte of the temporaries
+  char* arena_end = nullptr;  // end of the arena; nullptr = hipMalloc mode
   ~Op() {
     for (void* q : owned)
       if (q) (void)hipFree(q);
@@ -300,8 +310,29 @@ struct Op {
       if (q) (void)hipFree(q);
   }
   std::vector<void*> temps;  // build-only buffers, freed by drop_temps
+  void set_arena(void* base, size_t bytes) {
+    arena_lo = static_cast<char*>(base);
+    arena_hi = arena_end = arena_lo + bytes;
+  }
   template <typename T>
   T* alloc(size_t bytes, bool temp = false) {
+    if (arena_end) {
+      const size_t n = (std::max<size_t>(bytes, 4) + 255) & ~size_t(255);
+      if ((size_t)(arena_hi - arena_lo) < n) {
+        err = CYTOM_ERANGE;
+        msg = "allocation-free scratch exhausted";
+        return nullptr;
+      }
+      void* q;
+      if (temp) {
+        arena_hi -= n;
+        q = arena_hi;
+      } else {
+        q = arena_lo;
+        arena_lo += n;
+      }
+      return static_cast<T*>(q);
+    }
     void* q = nullptr;
     if (hipMalloc(&q, std::max<size_t>(bytes, 4)) != hipSuccess) {
       err = CYTOM_ENOMEM;
@@ -313,6 +344,10 @@ struct Op {
   }
   void drop_temps(hipStream_t st) {
     (void)hipStreamSynchronize(st);
+    if (arena_end) {
+      arena_hi = arena_end;
+      return;
+    }
     for (void* q : temps)
       if (q) (void)hipFree(q);
     temps.clear();
@@ -401,8 +436,9 @@ int build_structure(Op& op, uint64_t V, uint64_t E, const uint32_t* src, const u
     }
   }
   int rc;
-  if ((rc = build_rows(op, op.out, src, dst, st)) || (rc = build_rows(op, op.in, dst, src, st)))
-    return rc;
+  if ((rc = build_rows(op, op.out, src, dst, st))) return rc;
+  if (op.arena_end) op.drop_temps(st);  // the second build reuses the temporaries
+  if ((rc = build_rows(op, op.in, dst, src, st))) return rc;
   op.drop_temps(st);
   return 0;
 }
@@ -641,4 +677,91 @@ extern "C" int cytom_ppr_adjoint(uint64_t V, uint64_t E, const uint32_t* d_src,
                                  iterations);
   cytom_ppr_graph_destroy(g);
   return rc;
+}
+
+// ------------------------------------------------- allocation-free forward
+// For callers that cannot allocate device memory (code under stream capture,
+// tight serving loops, harnesses that forbid hipMalloc): the forward of
+// cytom_ppr_weighted with every temporary carved from one file-scope device
+// array, NOALLOC_BYTES per GPU, held until the process ends.  The array's
+// name is a host-side stand-in: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is passed on.
+namespace cytom {
+namespace {
+
+constexpr size_t NOALLOC_BYTES = 64ull << 20;
+static __device__ __align__(256) unsigned char noalloc_scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of noalloc_scratch on the current device.
+int noalloc_base(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return CYTOM_EDEVICE;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)) != hipSuccess || !p)
+      return CYTOM_EDEVICE;
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return 0;
+}
+
+inline size_t up256(size_t b) { return (std::max<size_t>(b, 4) + 255) & ~size_t(255); }
+
+// Upper bound of the arena bytes one forward of (V, E) carves: the kept
+// buffers of both row sets (long-row segments bounded by E/64: a long row
+// has more than 64 entries and one segment per 1024), the weight sums, the
+// solver vector, plus one row build's temporaries.
+size_t noalloc_bytes(uint64_t V, uint64_t E) {
+  const uint64_t nseg = E / 64 + 1;
+  const size_t rows = up256((V + 1) * 8) + 3 * up256(E * 4) + up256(V * 4) + up256((V + 1) * 8) +
+                      up256(nseg * 16) + up256(nseg * 4);
+  const size_t kept = 2 * rows + up256(256 * 4) + up256(scan_tmp_bytes(V + 1) + 256) +
+                      up256(V * 4) /* W */ + up256(V * 4) /* y */;
+  const size_t temps =
+      up256((V + 1) * 8) + 4 * up256(E * 4) + up256(radix_hist_bytes(E)) +
+      up256(scan_tmp_bytes(std::max<uint64_t>(E, 256 * ((E + 4095) / 4096) + 1)) + 256);
+  return kept + temps;
+}
+
+thread_local std::string g_noalloc_msg;
+
+}  // namespace
+}  // namespace cytom
+
+extern "C" int cytom_ppr_weighted_noalloc(uint64_t V, uint64_t E, const uint32_t* d_src,
+                                          const uint32_t* d_dst, const float* d_w,
+                                          const float* d_s, float alpha, float tol,
+                                          uint32_t max_iter, float* d_r, uint32_t* iterations) {
+  if (check_inputs(V, E, d_src, d_dst, d_w) || !d_s || !d_r || !(alpha >= 0.f && alpha < 1.f))
+    return fail(CYTOM_EINVAL, nullptr, "invalid arguments");
+  const size_t need = noalloc_bytes(V, E);
+  if (need > NOALLOC_BYTES) {
+    g_noalloc_msg = "graph too large for the allocation-free forward: needs " +
+                    std::to_string(need) + " bytes of scratch, the limit is " +
+                    std::to_string(NOALLOC_BYTES);
+    return fail(CYTOM_ERANGE, nullptr, g_noalloc_msg.c_str());
+  }
+  void* base = nullptr;
+  if (noalloc_base(&base)) return fail(CYTOM_EDEVICE, nullptr, "scratch symbol lookup failed");
+  hipStream_t st = nullptr;  // the null stream: no stream object is created
+  Op op;
+  op.set_arena(base, NOALLOC_BYTES);
+  int rc = build_structure(op, V, E, d_src, d_dst, st);
+  if (!rc) rc = set_weights(op, d_src, d_w, st);
+  float* y = rc ? nullptr : op.alloc<float>(V * 4);
+  if (!rc && op.err) rc = op.err;
+  uint32_t it = 0;
+  if (!rc) {
+    k_scale<<<gblocks(V, GT, 16384), GT, 0, st>>>(d_s, V, 1.f - alpha, d_r);
+    rc = solve(op, op.in, d_s, 1.f - alpha, alpha, tol, max_iter, d_r, y, &it, st);
+  }
+  if (iterations) *iterations = it;
+  return rc ? fail(rc, &op, "allocation-free weighted PPR forward failed") : 0;
 }

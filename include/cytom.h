@@ -227,6 +227,20 @@ int cytom_ppr_graph_adjoint(cytom_ppr_graph* g, const float* d_r, const float* d
                             float alpha, float tol, uint32_t max_iterations, float* d_dL_dw,
                             float* d_dL_ds, uint32_t* iterations);
 int cytom_ppr_graph_destroy(cytom_ppr_graph* g);
+/* cytom_ppr_weighted without any device allocation (no counterpart in the
+ * reference; for callers that cannot allocate: code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results (bit for bit) as cytom_ppr_weighted; every temporary (rows,
+ * probabilities, sort and solver buffers) lives in one 64 MiB device array
+ * per GPU that the library holds until the process ends, which bounds the
+ * graph to roughly 40 B per vertex plus 40 B per edge (for example 2^18
+ * vertices and 2^20 edges): larger graphs fail with CYTOM_ERANGE before any
+ * work.  Runs on the null stream.  Calls on one GPU share that array, so
+ * calls that overlap in time on one GPU (two host threads) race. */
+int cytom_ppr_weighted_noalloc(uint64_t n_vertices, uint64_t n_edges, const uint32_t* d_src,
+                               const uint32_t* d_dst, const float* d_w, const float* d_s,
+                               float alpha, float tol, uint32_t max_iterations, float* d_r,
+                               uint32_t* iterations);
 /* Message of the last failed cytom_ppr_* call on this thread. */
 const char* cytom_ppr_last_error(void);
 
