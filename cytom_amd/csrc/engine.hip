@@ -79,7 +79,7 @@ void free_ctx(cytom_ctx* c) {
                      &c->g_tab_head, &c->g_node_next, &c->g_claimed,   &c->g_tile_flag,
                      &c->g_del_tiles, &c->g_found,    &c->g_tile_scan, &c->g_nch,
                      &c->g_cbase,    &c->g_chunks,    &c->g_colj,      &c->g_jbase,
-                     &c->g_jobs,     &c->g_rs_e,      &c->g_rs_w,      &c->g_rs_off,
+                     &c->g_jobs,     &c->g_bands,     &c->g_rs_e,      &c->g_rs_w,      &c->g_rs_off,
                      &c->g_rec,      &c->g_del_sig};
   for (GrowBuf* b : bufs) free_buf(*b);
   for (auto& e : c->ev)
@@ -372,6 +372,8 @@ int cytom_set_tuning(cytom_ctx* ctx, const cytom_tuning* t) {
   if (t->blocks_per_cu > 16) return set_err(ctx, CYTOM_EINVAL, "blocks_per_cu above 16");
   if (t->source_bands && (t->source_bands & (t->source_bands - 1)))
     return set_err(ctx, CYTOM_EINVAL, "source_bands must be a power of two");
+  if (t->source_bands > MAX_SOURCE_BANDS)
+    return set_err(ctx, CYTOM_EINVAL, "source_bands above 16");
   if (t->prd_frac_bits && (t->prd_frac_bits < 16 || (int)t->prd_frac_bits > max_prd_shift(ctx->V)))
     return set_err(ctx, CYTOM_EINVAL, "prd_frac_bits outside [16, 60 - ceil(log2 2V)]");
   if (t->prd_frac_bits) ctx->prd_shift = (int)t->prd_frac_bits;

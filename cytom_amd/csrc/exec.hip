@@ -171,35 +171,69 @@ __global__ void k_emit_chunks(const uint32_t* __restrict__ nch, const uint32_t* 
 }
 
 // Jobs are cut from cells: cell (b, y) holds the chunks of column y whose
-// source rows fall in band b (T/NB consecutive rows).  Cells are numbered
-// band-major, so the persistent grid sweeps the source bands one after the
-// other and every XCD's L2 holds only one band's source values at a time.
-__device__ inline void cell_range(const uint32_t* cbase, uint32_t T, uint32_t NB, uint32_t cell,
-                                  uint32_t* y, uint32_t* c_begin, uint32_t* c_end) {
-  const uint32_t rows = T / NB, b = cell / T;
+// source rows fall in band b (source rows [xb[b], xb[b+1])).  Cells are
+// numbered band-major, so the jobs of one band form one contiguous queue.
+// Band boundaries split the source rows into parts of about equal slot
+// counts; each band's queue is served first by the blocks whose number is
+// the band modulo the band count.  Workgroups are dispatched round-robin
+// over the XCDs, so with 8 bands each XCD works on one band: the source
+// values its jobs read (1/8 of them) stay in that XCD's L2.  A block whose
+// queue is empty takes jobs from the other bands.
+__device__ inline void cell_range(const uint32_t* cbase, const uint32_t* xb, uint32_t T,
+                                  uint32_t cell, uint32_t* y, uint32_t* c_begin,
+                                  uint32_t* c_end) {
+  const uint32_t b = cell / T;
   *y = cell % T;
-  const uint64_t t0 = (uint64_t)*y * T + (uint64_t)b * rows;
-  *c_begin = cbase[t0];
-  *c_end = cbase[t0 + rows];
+  const uint64_t t0 = (uint64_t)*y * T;
+  *c_begin = cbase[t0 + xb[b]];
+  *c_end = cbase[t0 + xb[b + 1]];
 }
 
-__global__ void k_col_jobs(const uint32_t* __restrict__ cbase, uint32_t T, uint32_t NB,
-                           uint32_t JC, uint32_t* __restrict__ colj) {
+// Slots per source row (summed over the columns).
+__global__ void k_row_slots(const uint32_t* __restrict__ cnt, uint32_t T,
+                            unsigned long long* __restrict__ rs) {
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < T; x += gridDim.x * blockDim.x) {
+    unsigned long long s = 0;
+    for (uint32_t y = 0; y < T; ++y) s += cnt[(uint64_t)y * T + x];
+    rs[x] = s;
+  }
+}
+
+// Band b starts at the first source row whose slot prefix reaches
+// b / NB of the total (bands may be empty).  One thread: T is at most 2^17.
+__global__ void k_band_bounds(const unsigned long long* __restrict__ rs, uint32_t T, uint32_t NB,
+                              uint32_t* __restrict__ xb) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  unsigned long long tot = 0;
+  for (uint32_t x = 0; x < T; ++x) tot += rs[x];
+  xb[0] = 0;
+  uint32_t b = 1;
+  unsigned long long pre = 0;
+  for (uint32_t x = 0; x < T && b < NB; ++x) {
+    while (b < NB && pre * NB >= tot * b) xb[b++] = x;
+    pre += rs[x];
+  }
+  while (b < NB) xb[b++] = T;
+  xb[NB] = T;
+}
+
+__global__ void k_col_jobs(const uint32_t* __restrict__ cbase, const uint32_t* __restrict__ xb,
+                           uint32_t T, uint32_t NB, uint32_t JC, uint32_t* __restrict__ colj) {
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < T * NB;
        c += gridDim.x * blockDim.x) {
     uint32_t y, c0, c1;
-    cell_range(cbase, T, NB, c, &y, &c0, &c1);
+    cell_range(cbase, xb, T, c, &y, &c0, &c1);
     colj[c] = (c1 - c0 + JC - 1) / JC;
   }
 }
 
-__global__ void k_emit_jobs(const uint32_t* __restrict__ cbase, const uint32_t* __restrict__ colj,
-                            const uint32_t* __restrict__ jbase, uint32_t T, uint32_t NB,
-                            uint32_t JC, uint4* __restrict__ jobs) {
+__global__ void k_emit_jobs(const uint32_t* __restrict__ cbase, const uint32_t* __restrict__ xb,
+                            const uint32_t* __restrict__ colj, const uint32_t* __restrict__ jbase,
+                            uint32_t T, uint32_t NB, uint32_t JC, uint4* __restrict__ jobs) {
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < T * NB;
        c += gridDim.x * blockDim.x) {
     uint32_t y, c_begin, c_end;
-    cell_range(cbase, T, NB, c, &y, &c_begin, &c_end);
+    cell_range(cbase, xb, T, c, &y, &c_begin, &c_end);
     const uint32_t nj = colj[c], b = jbase[c];
     for (uint32_t j = 0; j < nj; ++j) {
       const uint32_t c0 = c_begin + j * JC;
@@ -213,7 +247,8 @@ __global__ void k_emit_jobs(const uint32_t* __restrict__ cbase, const uint32_t* 
 template <int KIND>
 struct GatherArgs {
   const uint4* jobs;
-  const uint32_t* njobs;
+  const uint32_t* jbase;  // first job of every cell (cells band-major), cells + 1 entries
+  uint32_t nbands;
   const uint64_t* chunks;
   StoreView S;
   const uint8_t* rowact;
@@ -258,8 +293,11 @@ __device__ inline void load_rows(const GatherArgs<KIND>& A, const uint64_t* rows
   for (int u = 0; u < GATHER_ROWS; ++u) {
     const bool v = r0 + u < R && lane <= (uint32_t)(d[u] & 63u);
     const uint64_t idx = (d[u] >> 22) + (v ? lane : 0u);
-    e[u] = A.S.pool[idx];
-    if (KIND == CYTOM_SSSP || KIND == KIND_SSSP_PAIR) wt[u] = A.S.wpool[idx];
+    // Slots are read once per launch: non-temporal loads keep them from
+    // evicting the source values the L2 should hold.
+    e[u] = __builtin_nontemporal_load(&A.S.pool[idx]);
+    if (KIND == CYTOM_SSSP || KIND == KIND_SSSP_PAIR)
+      wt[u] = __builtin_nontemporal_load(&A.S.wpool[idx]);
     xb[u] = ((uint32_t)(d[u] >> 6) & 0xFFFFu) << A.S.log2vpt;
     vmask |= (v ? 1u : 0u) << u;
   }
@@ -355,17 +393,32 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x)
     A.rowact_clear[i] = 0;
   if (A.it->count == 0) return;  // empty frontier (uniform)
-  const uint32_t njobs = *A.njobs;
+  const uint32_t NB = A.nbands;
+  const uint32_t home = blockIdx.x % NB;
   // A job's chunks are taken in passes of at most PASS chunks (one thread
   // each, at most MAX_JOB_ROWS rows); the window lives across the passes and
   // is flushed once per job.
   const uint32_t PASS = min((uint32_t)GATHER_THREADS, (MAX_JOB_ROWS * 64u) / A.CH);
   unsigned long long scanned = 0;
   for (;;) {
-    if (threadIdx.x == 0) s_job = (uint32_t)atomicAdd(&A.it->job_cursor, 1ull);
+    if (threadIdx.x == 0) {
+      // own band's queue first, then the others in turn
+      uint32_t j = UINT32_MAX;
+      for (uint32_t k = 0; k < NB; ++k) {
+        const uint32_t b = (home + k) % NB;
+        const uint32_t lo = A.jbase[b * T], hi = A.jbase[(b + 1) * T];
+        if (lo >= hi) continue;
+        const uint32_t c = (uint32_t)atomicAdd(&A.it->band_cursor[b], 1ull);
+        if (lo + c < hi) {
+          j = lo + c;
+          break;
+        }
+      }
+      s_job = j;
+    }
     __syncthreads();
     const uint32_t j = s_job;
-    if (j >= njobs) break;  // uniform: every thread read s_job after the barrier
+    if (j == UINT32_MAX) break;  // uniform: every thread read s_job after the barrier
     const uint4 job = A.jobs[j];
     const uint32_t y = job.z;
     bool cleared = false;  // uniform
@@ -754,7 +807,8 @@ template <int KIND>
 void launch_gather(cytom_ctx* ctx, Program& P, int it) {
   GatherArgs<KIND> A;
   A.jobs = static_cast<const uint4*>(ctx->g_jobs.p);
-  A.njobs = static_cast<const uint32_t*>(ctx->g_jbase.p) + ctx->n_cells;  // total job count
+  A.jbase = static_cast<const uint32_t*>(ctx->g_jbase.p);
+  A.nbands = ctx->n_bands;
   A.chunks = static_cast<const uint64_t*>(ctx->g_chunks.p);
   A.S = ctx->view();
   A.rowact = P.rowact[P.cur_act];
@@ -777,7 +831,7 @@ int build_jobs(cytom_ctx* ctx) {
   hipStream_t st = ctx->stream;
   int rc;
   const uint64_t nt = ctx->n_tiles;
-  const uint32_t NB = std::min(ctx->source_bands, ctx->T);
+  const uint32_t NB = std::min(std::min(ctx->source_bands, ctx->T), MAX_SOURCE_BANDS);
   const uint64_t cells = (uint64_t)ctx->T * NB;
   if ((rc = ensure(ctx, ctx->g_nch, nt * 4)) || (rc = ensure(ctx, ctx->g_cbase, (nt + 1) * 4)) ||
       (rc = ensure(ctx, ctx->g_colj, cells * 4)) ||
@@ -788,6 +842,11 @@ int build_jobs(cytom_ctx* ctx) {
   auto* cbase = static_cast<uint32_t*>(ctx->g_cbase.p);
   auto* colj = static_cast<uint32_t*>(ctx->g_colj.p);
   auto* jbase = static_cast<uint32_t*>(ctx->g_jbase.p);
+  if ((rc = ensure(ctx, ctx->g_bands, 256 + (uint64_t)ctx->T * 8))) return rc;
+  auto* xb = static_cast<uint32_t*>(ctx->g_bands.p);
+  auto* rs = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->g_bands.p) + 256);
+  k_row_slots<<<grid_for(ctx->T, 256), 256, 0, st>>>(ctx->tile_cnt, ctx->T, rs);
+  k_band_bounds<<<1, 64, 0, st>>>(rs, ctx->T, NB, xb);
   k_nch<<<grid_for(nt, 256), 256, 0, st>>>(ctx->tile_cnt, nt, ctx->chunk_slots, nch);
   scan_exclusive_u32(nch, cbase, nt, ctx->g_scan_tmp.p, st);
   CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc + 32, cbase + nt, 4, hipMemcpyDeviceToHost, st));
@@ -802,16 +861,17 @@ int build_jobs(cytom_ctx* ctx) {
       ctx->job_chunks ? ctx->job_chunks
                       : (uint32_t)std::max<uint64_t>((1u << 17) / ctx->chunk_slots,
                                                      n_chunks / ((uint64_t)ctx->num_cus * 8));
-  k_col_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, ctx->T, NB, JC, colj);
+  k_col_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, xb, ctx->T, NB, JC, colj);
   scan_exclusive_u32(colj, jbase, cells, ctx->g_scan_tmp.p, st);
   CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc + 33, jbase + cells, 4, hipMemcpyDeviceToHost, st));
   CY_HIP(ctx, hipStreamSynchronize(st));
   const uint64_t n_jobs = (uint32_t)ctx->h_misc[33];
   if ((rc = ensure(ctx, ctx->g_jobs, std::max<uint64_t>(n_jobs, 1) * 16))) return rc;
-  k_emit_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, colj, jbase, ctx->T, NB, JC,
+  k_emit_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, xb, colj, jbase, ctx->T, NB, JC,
                                                     static_cast<uint4*>(ctx->g_jobs.p));
   ctx->n_chunks = n_chunks;
   ctx->n_cells = cells;
+  ctx->n_bands = NB;
   ctx->n_jobs = n_jobs;
   ctx->jobs_dirty = false;
   return 0;

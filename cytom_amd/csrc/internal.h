@@ -109,7 +109,9 @@ struct IterStats {
   unsigned long long updated;  // targets whose value the gather changed
   unsigned long long job_cursor;
   unsigned long long pad[3];
+  unsigned long long band_cursor[16];  // next job of each source band's queue
 };
+constexpr uint32_t MAX_SOURCE_BANDS = 16;
 
 struct GrowBuf {
   void* p = nullptr;
@@ -158,11 +160,13 @@ struct cytom_ctx {
   int stage_next = 0;
   // execution work lists
   cytom::GrowBuf g_nch, g_cbase, g_chunks, g_colj, g_jbase, g_jobs;
+  cytom::GrowBuf g_bands;  // source-band boundaries (bands + 1 source rows) and row slot sums
   bool jobs_dirty = true;
   uint64_t n_chunks = 0, n_cells = 0, n_jobs = 0;
+  uint32_t n_bands = 1;
   uint32_t chunk_slots = 512;   // slots per chunk
   uint32_t job_chunks = 0;      // chunks per column job (0: sized from the store)
-  uint32_t source_bands = 1;    // source-row bands swept one after the other
+  uint32_t source_bands = 1;    // source-row bands, one job queue each (see build_jobs)
   // A tile is sort-compacted once its unsorted tail reaches 256 slots and
   // resort_pct percent of its sorted prefix (CYTOM_RESORT_OFF = never).
   uint32_t resort_pct = 25;

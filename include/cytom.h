@@ -264,7 +264,8 @@ typedef struct {
   uint32_t job_chunks;    /* chunks per column job (<= 2^20; reported 0 = sized from the
                              store: at least 131072 slots, ~8 jobs per CU for large stores) */
   uint32_t blocks_per_cu; /* persistent gather blocks per compute unit */
-  uint32_t source_bands;  /* source-row bands swept in turn (power of two) */
+  uint32_t source_bands;  /* source-row bands of about equal slot counts, one job queue
+                             each, served first by every NB-th block (power of two <= 16) */
   uint32_t resort_pct;    /* sort-compact a tile once its unsorted tail reaches this
                              percentage of its sorted prefix; CYTOM_RESORT_OFF disables */
   uint32_t resort_max_slots; /* slots staged per sort-compaction pass (more tiles wait) */

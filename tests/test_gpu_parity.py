@@ -118,7 +118,7 @@ def test_stream_parity_with_pool_growth():
 
 
 @pytest.mark.parametrize("tuning", [
-    dict(source_bands=1), dict(source_bands=4), dict(source_bands=1024),
+    dict(source_bands=1), dict(source_bands=4), dict(source_bands=16),
     dict(chunk_slots=64, job_chunks=1, source_bands=2),
     dict(chunk_slots=1024, job_chunks=128, blocks_per_cu=1),
     dict(resort_pct=C.RESORT_OFF), dict(resort_pct=1), dict(resort_pct=1, resort_max_slots=700)])
@@ -143,7 +143,7 @@ def test_launch_shapes_agree_at_rmat20():
     # 64-slot chunks in jobs of 8192 chunks: heavy columns take several
     # 1024-chunk passes per job.
     shapes = [dict(source_bands=1), dict(source_bands=8, resort_pct=C.RESORT_OFF),
-              dict(source_bands=32, chunk_slots=256, job_chunks=16, resort_pct=5),
+              dict(source_bands=16, chunk_slots=256, job_chunks=16, resort_pct=5),
               dict(chunk_slots=64, job_chunks=8192)]
     graphs = []
     for t in shapes:
