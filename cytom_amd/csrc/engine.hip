@@ -79,7 +79,7 @@ void free_ctx(cytom_ctx* c) {
                      &c->g_tab_head, &c->g_node_next, &c->g_claimed,   &c->g_tile_flag,
                      &c->g_del_tiles, &c->g_found,    &c->g_tile_scan, &c->g_nch,
                      &c->g_cbase,    &c->g_chunks,    &c->g_colj,      &c->g_jbase,
-                     &c->g_jobs,     &c->g_bands,     &c->g_rs_e,      &c->g_rs_w,      &c->g_rs_off,
+                     &c->g_jobs,     &c->g_bands,     &c->g_grp,       &c->g_rs_e,      &c->g_rs_w,      &c->g_rs_off,
                      &c->g_rec,      &c->g_del_sig};
   for (GrowBuf* b : bufs) free_buf(*b);
   for (auto& e : c->ev)

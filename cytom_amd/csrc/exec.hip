@@ -40,6 +40,51 @@ inline uint32_t grid_for(uint64_t n, uint32_t threads, uint32_t cap = 65536) {
   return (uint32_t)std::min<uint64_t>(b, cap);
 }
 
+// ------------------------------------------------------ source groups
+// A source row (the vpt sources of one tile column x) is cut into 64 groups
+// of vpt / 64 consecutive sources.  Activity is kept as one byte per group
+// (gact[x * 64 + g] = 1 when a vertex of the group is active; plain stores,
+// so the apply needs no atomics), and every 64-slot row of the edge pool
+// carries the 64-bit mask of the groups its sources fall in (grp); the
+// gather reads a row only when the row's mask meets its source row's
+// active groups, so a sparse frontier skips most rows of an active source
+// row (TileManager::isActive, tile-manager.h:403-455, at group granularity).
+__device__ inline uint64_t word_groups(uint64_t word, uint64_t v0, uint32_t log2vpt) {
+  const uint32_t gshift = log2vpt - 6;  // log2 of the group width
+  const uint32_t g0 = (uint32_t)((v0 & ((1ull << log2vpt) - 1)) >> gshift);
+  if (gshift >= 6) return word ? 1ull << g0 : 0ull;  // the word lies in one group
+  const uint32_t gs = 1u << gshift;
+  const uint64_t gmask = (1ull << gs) - 1;
+  uint64_t m = 0;
+  for (uint32_t k = 0; k < 64 / gs; ++k)
+    if ((word >> (k * gs)) & gmask) m |= 1ull << (g0 + k);
+  return m;
+}
+
+// Marks the groups of one active-bitmap word (vertices v0 .. v0 + 63).
+__device__ inline void mark_groups(uint8_t* __restrict__ gact, uint64_t word, uint64_t v0,
+                                   uint32_t log2vpt) {
+  if (!word) return;
+  uint64_t m = word_groups(word, v0, log2vpt);
+  uint8_t* row = gact + ((v0 >> log2vpt) << 6);
+  while (m) {
+    row[__ffsll((unsigned long long)m) - 1] = 1;
+    m &= m - 1;
+  }
+}
+
+// The active groups of source row x as a 64-bit mask.  Each 8-byte word
+// holds 8 group bytes of value 0 or 1; multiplying by sum_j 2^(56-7j) moves
+// byte j's bit to bit 56+j (all other partial products land below bit 56 or
+// above bit 63, at distinct positions, so nothing carries into the top byte).
+__device__ inline uint64_t group_mask(const uint8_t* __restrict__ gact, uint32_t x) {
+  const uint64_t* p = reinterpret_cast<const uint64_t*>(gact + ((uint64_t)x << 6));
+  uint64_t m = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) m |= ((p[i] * 0x0102040810204080ull) >> 56) << (8 * i);
+  return m;
+}
+
 // ---------------------------------------------------------------- init
 // bfs.cc:170-197, cc.cc:171-189, sssp.cc:171-195 (+ source set to 0).
 __global__ void k_init_min(int kind, uint64_t V, uint32_t source, uint32_t* __restrict__ cur,
@@ -116,7 +161,7 @@ __global__ void k_init_ppr_set(const uint32_t* __restrict__ set, uint32_t n, flo
 // vertices.
 __global__ void __launch_bounds__(APPLY_THREADS) k_frontier(
     int kind, uint64_t V, uint32_t log2vpt, const uint64_t* __restrict__ act,
-    const uint32_t* __restrict__ outdeg, uint8_t* __restrict__ rowact,
+    const uint32_t* __restrict__ outdeg, uint8_t* __restrict__ gact,
     const float* __restrict__ delta, float* __restrict__ contrib,
     const uint32_t* __restrict__ cur, uint32_t* __restrict__ fval, IterStats* it) {
   const int lane = threadIdx.x & 63;
@@ -143,7 +188,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_frontier(
     } else {
       fval[v] = a ? cur[v] : INF;
     }
-    if (lane == 0 && word) rowact[v >> log2vpt] = 1;
+    if (lane == 0) mark_groups(gact, word, v, log2vpt);
   }
   te = block_sum<APPLY_THREADS, unsigned long long>(te);
   cnt = block_sum<APPLY_THREADS, unsigned long long>(cnt);
@@ -167,6 +212,34 @@ __global__ void k_emit_chunks(const uint32_t* __restrict__ nch, const uint32_t* 
        t += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t n = nch[t], b = cbase[t];
     for (uint32_t c = 0; c < n; ++c) chunks[b + c] = ((uint64_t)t << 32) | (uint64_t)(c * CH);
+  }
+}
+
+// Group mask of every 64-slot row of the store (tile blocks start at
+// multiples of 64 slots, so row r of a tile at slot offset o is grp[o/64 +
+// r]).  One wavefront per work chunk; tombstones and slots past the tile's
+// count add nothing.
+__global__ void k_row_groups(const uint64_t* __restrict__ chunks, uint64_t n_chunks,
+                             StoreView S, uint32_t CH, uint64_t* __restrict__ grp) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t gshift = S.log2vpt - 6;
+  for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < n_chunks;
+       c += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
+    const uint64_t ch = chunks[c];
+    const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
+    const uint32_t n = min(CH, S.tile_cnt[t] - s0);
+    const uint64_t off = S.tile_off[t] + s0;
+    for (uint32_t r = 0; r * 64 < n; ++r) {
+      const uint32_t i = r * 64 + lane;
+      uint64_t m = 0;
+      if (i < n) {
+        const uint32_t e = S.pool[off + i];
+        if (e != TOMB) m = 1ull << ((e & 0xFFFFu) >> gshift);
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) m |= __shfl_xor(m, o, 64);
+      if (lane == 0) grp[(off >> 6) + r] = m;
+    }
   }
 }
 
@@ -251,8 +324,9 @@ struct GatherArgs {
   uint32_t nbands;
   const uint64_t* chunks;
   StoreView S;
-  const uint8_t* rowact;
-  uint8_t* rowact_clear;
+  const uint8_t* gact;         // active groups, 64 bytes per source row
+  uint64_t* gact_clear;        // the next iteration's group bytes, cleared here (T * 8 words)
+  const uint64_t* grp;         // source groups of every 64-slot pool row
   const uint32_t* fval;
   uint32_t* nxt;
   const float* contrib;
@@ -390,8 +464,8 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
   __shared__ uint32_t s_job;
   const uint32_t vpt = A.S.vpt, T = A.S.T;
   // Clear next iteration's row activity (written by the apply kernel).
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x)
-    A.rowact_clear[i] = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 8 * T; i += gridDim.x * blockDim.x)
+    A.gact_clear[i] = 0;
   if (A.it->count == 0) return;  // empty frontier (uniform)
   const uint32_t NB = A.nbands;
   const uint32_t home = blockIdx.x % NB;
@@ -424,27 +498,39 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
     bool cleared = false;  // uniform
     for (uint32_t cs = job.x; cs < job.y; cs += PASS) {
       const uint32_t nc = min(PASS, job.y - cs);
-      // Row table of the pass's chunks whose source row has an active vertex
-      // (TileManager::isActive, tile-manager.h:403-455).  One scan packs the
-      // rows (bits 24+) and slots (bits 0-23) before each chunk.
-      uint32_t flag = 0, n = 0, x = 0;
+      // Row table of the pass's rows whose sources meet an active group of
+      // their source row.  One scan packs the rows (bits 24+) and slots
+      // (bits 0-23) before each chunk.
+      uint32_t n = 0, x = 0, keep = 0, nrows = 0, nslots = 0;
       uint64_t off = 0;
       if (threadIdx.x < nc) {
         const uint64_t ch = A.chunks[cs + threadIdx.x];
         const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
         x = t - y * T;
-        flag = A.rowact[x];
-        off = A.S.tile_off[t] + s0;
-        n = min(A.CH, A.S.tile_cnt[t] - s0);
+        const uint64_t m = group_mask(A.gact, x);
+        if (m) {
+          off = A.S.tile_off[t] + s0;
+          n = min(A.CH, A.S.tile_cnt[t] - s0);
+          const uint32_t nr = (n + 63) >> 6;
+          // PageRank-delta frontiers are dense in almost every iteration: its
+          // rows are taken whole once any group of their source row is active.
+          const bool whole = KIND == CYTOM_PRD || m == ~0ull;
+          for (uint32_t k = 0; k < nr; ++k) {
+            if (whole || (m & A.grp[(off >> 6) + k])) {
+              keep |= 1u << k;
+              ++nrows;
+              nslots += min(64u, n - 64 * k);
+            }
+          }
+        }
       }
-      const uint32_t nrows = flag ? (n + 63) >> 6 : 0u;
-      const uint64_t packed = ((uint64_t)nrows << 24) | (uint64_t)(flag ? n : 0u);
+      const uint64_t packed = ((uint64_t)nrows << 24) | (uint64_t)nslots;
       uint64_t tot;
       const uint64_t ex = block_scan_excl<GATHER_THREADS, uint64_t>(packed, &tot);
       const uint32_t R = (uint32_t)(tot >> 24);
       if (R == 0) continue;  // uniform; the scan's barriers ordered s_rows / s_job reuse
-      for (uint32_t k = 0, r = (uint32_t)(ex >> 24); k < nrows; ++k, ++r)
-        s_rows[r] = row_desc(off + 64 * k, x, min(64u, n - 64 * k));
+      for (uint32_t k = 0, r = (uint32_t)(ex >> 24); keep; ++k, keep >>= 1)
+        if (keep & 1u) s_rows[r++] = row_desc(off + 64 * k, x, min(64u, n - 64 * k));
       scanned += tot & 0xFFFFFFull;
       if (!cleared) {
         if (KIND == CYTOM_PRD) {
@@ -533,7 +619,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_min(
     uint64_t V, uint32_t log2vpt, uint32_t* __restrict__ cur, const uint32_t* __restrict__ nxt,
     uint32_t* __restrict__ fval, uint8_t* __restrict__ changed,
     const uint64_t* __restrict__ act_cur, uint64_t* __restrict__ act_next,
-    uint8_t* __restrict__ rowact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
+    uint8_t* __restrict__ gact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
   if (empty_frontier(nit, act_next, nwords)) return;
@@ -582,7 +668,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_min(
       const uint64_t word = __ballot(a);
       if (lane == 0) {
         act_next[w] = word;
-        if (word) rowact_next[v >> log2vpt] = 1;
+        mark_groups(gact_next, word, v, log2vpt);
       }
     }
   }
@@ -595,7 +681,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_pair(
     uint64_t V, uint32_t log2vpt, uint2* __restrict__ cur, const uint2* __restrict__ nxt,
     uint2* __restrict__ fval, uint8_t* __restrict__ changed,
     const uint64_t* __restrict__ act_cur, uint64_t* __restrict__ act_next,
-    uint8_t* __restrict__ rowact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
+    uint8_t* __restrict__ gact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
   if (empty_frontier(nit, act_next, nwords)) return;
@@ -622,7 +708,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_pair(
     const uint64_t word = __ballot(a);
     if (lane == 0) {
       act_next[w] = word;
-      if (word) rowact_next[v >> log2vpt] = 1;
+      mark_groups(gact_next, word, v, log2vpt);
     }
   }
   apply_tail(te, cnt, upd, nit);
@@ -635,7 +721,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
     uint64_t V, uint32_t log2vpt, int first, double thr, double qinv, float* __restrict__ rank,
     float* __restrict__ delta, long long* __restrict__ dsum, float* __restrict__ contrib,
     uint8_t* __restrict__ changed, const uint64_t* __restrict__ act_cur,
-    uint64_t* __restrict__ act_next, uint8_t* __restrict__ rowact_next,
+    uint64_t* __restrict__ act_next, uint8_t* __restrict__ gact_next,
     const uint32_t* __restrict__ outdeg, IterStats* nit) {
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
@@ -702,7 +788,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
       const uint64_t word = __ballot(a);
       if (lane == 0) {
         act_next[w] = word;
-        if (word) rowact_next[v >> log2vpt] = 1;
+        mark_groups(gact_next, word, v, log2vpt);
       }
     }
   }
@@ -811,8 +897,9 @@ void launch_gather(cytom_ctx* ctx, Program& P, int it) {
   A.nbands = ctx->n_bands;
   A.chunks = static_cast<const uint64_t*>(ctx->g_chunks.p);
   A.S = ctx->view();
-  A.rowact = P.rowact[P.cur_act];
-  A.rowact_clear = P.rowact[P.cur_act ^ 1];
+  A.gact = P.gact[P.cur_act];
+  A.gact_clear = reinterpret_cast<uint64_t*>(P.gact[P.cur_act ^ 1]);
+  A.grp = static_cast<const uint64_t*>(ctx->g_grp.p);
   A.fval = P.fval;
   A.nxt = P.nxt;
   A.contrib = P.contrib;
@@ -855,6 +942,13 @@ int build_jobs(cytom_ctx* ctx) {
   if ((rc = ensure(ctx, ctx->g_chunks, std::max<uint64_t>(n_chunks, 1) * 8))) return rc;
   k_emit_chunks<<<grid_for(nt, 256), 256, 0, st>>>(nch, cbase, nt, ctx->chunk_slots,
                                                    static_cast<uint64_t*>(ctx->g_chunks.p));
+  // Source groups of every 64-slot row, recomputed after each batch (the
+  // batch moved, added and tombstoned slots).
+  if ((rc = ensure(ctx, ctx->g_grp, (ctx->pool_size / 64 + 1) * 8))) return rc;
+  if (n_chunks)
+    k_row_groups<<<grid_for(n_chunks * 64, 256), 256, 0, st>>>(
+        static_cast<const uint64_t*>(ctx->g_chunks.p), n_chunks, ctx->view(), ctx->chunk_slots,
+        static_cast<uint64_t*>(ctx->g_grp.p));
   // Jobs of job_chunks chunks, or sized from the store: at least 2^17 slots,
   // and about eight jobs per CU when the store is large (fewer column flushes).
   const uint32_t JC =
@@ -890,7 +984,7 @@ int program_alloc(cytom_ctx* ctx, Program& P) {
          alloc((void**)&P.fval, V * vb);
   }
   for (int i = 0; i < 2 && ok; ++i)
-    ok = alloc((void**)&P.act[i], (V / 64) * 8) && alloc((void**)&P.rowact[i], ctx->T);
+    ok = alloc((void**)&P.act[i], (V / 64) * 8) && alloc((void**)&P.gact[i], (size_t)ctx->T * 64);
   ok = ok && alloc((void**)&P.changed, V);
   if (!ok) {
     program_free(P);
@@ -901,7 +995,7 @@ int program_alloc(cytom_ctx* ctx, Program& P) {
 
 void program_free(Program& P) {
   void* ps[] = {P.cur,     P.nxt,     P.fval,      P.rank,      P.delta,   P.dsum,
-                P.contrib, P.act[0],  P.act[1],    P.rowact[0], P.rowact[1], P.changed,
+                P.contrib, P.act[0],  P.act[1],    P.gact[0], P.gact[1], P.changed,
                 P.teleport};
   for (void* p : ps)
     if (p) (void)hipFree(p);
@@ -909,7 +1003,7 @@ void program_free(Program& P) {
   P.rank = P.delta = P.contrib = nullptr;
   P.dsum = nullptr;
   P.act[0] = P.act[1] = nullptr;
-  P.rowact[0] = P.rowact[1] = nullptr;
+  P.gact[0] = P.gact[1] = nullptr;
   P.changed = nullptr;
   P.teleport = nullptr;
   P.n_teleport = 0;
@@ -999,7 +1093,7 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
   if (ctx->jobs_dirty && (rc = build_jobs(ctx))) return rc;
   const uint32_t maxit = ctx->max_iter;
   CY_HIP(ctx, hipMemsetAsync(ctx->d_iter, 0, sizeof(IterStats) * (maxit + 2), st));
-  CY_HIP(ctx, hipMemsetAsync(P.rowact[P.cur_act], 0, ctx->T, st));
+  CY_HIP(ctx, hipMemsetAsync(P.gact[P.cur_act], 0, (size_t)ctx->T * 64, st));
   CY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
   const uint64_t words = ctx->V >> 6;
   // One 1024-thread block per CU walks the bitmap words: measured faster than
@@ -1008,7 +1102,7 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
       1, std::min<uint64_t>((uint64_t)ctx->num_cus, (words + 3) / 4));
   k_frontier<<<apply_blocks, APPLY_THREADS, 0, st>>>(P.kind, ctx->V, ctx->log2vpt,
                                                      P.act[P.cur_act], ctx->outdeg,
-                                                     P.rowact[P.cur_act], P.delta, P.contrib,
+                                                     P.gact[P.cur_act], P.delta, P.contrib,
                                                      P.cur, P.fval, ctx->d_iter);
   // Iterations are launched in groups with one host check per group.  The
   // first group is as long as the previous run of the program (PageRank-delta
@@ -1038,25 +1132,25 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
         k_apply_prd<<<apply_blocks, APPLY_THREADS, 0, st>>>(
             ctx->V, ctx->log2vpt, P.first_iter ? 1 : 0, ctx->delta, ldexp(1.0, -ctx->prd_shift),
             P.rank, P.delta, P.dsum,
-            P.contrib, P.changed, P.act[P.cur_act], P.act[nx], P.rowact[nx], ctx->outdeg,
+            P.contrib, P.changed, P.act[P.cur_act], P.act[nx], P.gact[nx], ctx->outdeg,
             ctx->d_iter + i + 1);
       } else if (P.kind == CYTOM_BFS) {
         k_apply_min<CYTOM_BFS><<<apply_blocks, APPLY_THREADS, 0, st>>>(
             ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
-            P.rowact[nx], ctx->outdeg, ctx->d_iter + i + 1);
+            P.gact[nx], ctx->outdeg, ctx->d_iter + i + 1);
       } else if (P.kind == CYTOM_CC) {
         k_apply_min<CYTOM_CC><<<apply_blocks, APPLY_THREADS, 0, st>>>(
             ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
-            P.rowact[nx], ctx->outdeg, ctx->d_iter + i + 1);
+            P.gact[nx], ctx->outdeg, ctx->d_iter + i + 1);
       } else if (P.kind == KIND_SSSP_PAIR) {
         k_apply_pair<<<apply_blocks, APPLY_THREADS, 0, st>>>(
             ctx->V, ctx->log2vpt, reinterpret_cast<uint2*>(P.cur),
             reinterpret_cast<const uint2*>(P.nxt), reinterpret_cast<uint2*>(P.fval), P.changed,
-            P.act[P.cur_act], P.act[nx], P.rowact[nx], ctx->outdeg, ctx->d_iter + i + 1);
+            P.act[P.cur_act], P.act[nx], P.gact[nx], ctx->outdeg, ctx->d_iter + i + 1);
       } else {
         k_apply_min<CYTOM_SSSP><<<apply_blocks, APPLY_THREADS, 0, st>>>(
             ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
-            P.rowact[nx], ctx->outdeg, ctx->d_iter + i + 1);
+            P.gact[nx], ctx->outdeg, ctx->d_iter + i + 1);
       }
       P.cur_act = nx;
       P.first_iter = false;

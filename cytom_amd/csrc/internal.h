@@ -91,7 +91,7 @@ struct Program {
   long long* dsum = nullptr;  // PRD next.delta_sum, fixed point scaled by 2^prd_shift
   float* contrib = nullptr; // PRD delta/out_degree while v is active, else 0
   uint64_t* act[2] = {nullptr, nullptr};  // active bitmaps (wave-sized words)
-  uint8_t* rowact[2] = {nullptr, nullptr};  // per source-row activity
+  uint8_t* gact[2] = {nullptr, nullptr};  // per source row: 64 group-activity bytes
   uint8_t* changed = nullptr;               // per-vertex changed flags
   int cur_act = 0;
   bool first_iter = true;
@@ -161,6 +161,7 @@ struct cytom_ctx {
   // execution work lists
   cytom::GrowBuf g_nch, g_cbase, g_chunks, g_colj, g_jbase, g_jobs;
   cytom::GrowBuf g_bands;  // source-band boundaries (bands + 1 source rows) and row slot sums
+  cytom::GrowBuf g_grp;    // source groups of every 64-slot pool row
   bool jobs_dirty = true;
   uint64_t n_chunks = 0, n_cells = 0, n_jobs = 0;
   uint32_t n_bands = 1;
