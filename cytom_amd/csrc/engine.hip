@@ -64,13 +64,12 @@ void free_ctx(cytom_ctx* c) {
   query_free(c);
   io_free(c);
   void* ps[] = {c->tile_off, c->tile_cap, c->tile_cnt, c->tile_live, c->tile_sorted,
-                c->pool,     c->wpool,    c->outdeg,   c->indeg,    c->d_misc,
-                c->d_iter};
+                c->pool,     c->wpool,    c->outdeg,   c->indeg,    c->d_misc};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->h_misc) (void)hipHostFree(c->h_misc);
   if (c->h_scalar) (void)hipHostFree(c->h_scalar);
-  if (c->h_iter) (void)hipHostFree(c->h_iter);
+  if (c->h_pflags) (void)hipHostFree(c->h_pflags);
   GrowBuf* bufs[] = {&c->g_keys,     &c->g_vals,      &c->g_keys2,     &c->g_vals2,
                      &c->g_flags,    &c->g_scan,      &c->g_seg_start, &c->g_seg_tile,
                      &c->g_reloc_cap, &c->g_reloc_list, &c->g_ins_slot, &c->g_scan_tmp,
@@ -80,11 +79,11 @@ void free_ctx(cytom_ctx* c) {
                      &c->g_del_tiles, &c->g_found,    &c->g_tile_scan, &c->g_nch,
                      &c->g_cbase,    &c->g_chunks,    &c->g_colj,      &c->g_jbase,
                      &c->g_jobs,     &c->g_bands,     &c->g_grp,       &c->g_rs_e,      &c->g_rs_w,      &c->g_rs_off,
+                     &c->g_pflags,
                      &c->g_rec,      &c->g_del_sig};
   for (GrowBuf* b : bufs) free_buf(*b);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
-  for (auto& e : c->kev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -164,11 +163,10 @@ int cytom_create(const cytom_config* cfg, cytom_ctx** out) {
             hipMalloc(&c->outdeg, V * 4) == hipSuccess &&
             hipMalloc(&c->indeg, V * 4) == hipSuccess &&
             hipMalloc(&c->d_misc, 64 * 8) == hipSuccess &&
-            hipHostMalloc(&c->h_misc, 64 * 8, hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc(&c->h_misc, 128 * 8, hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc(&c->h_pflags, 128 * 4, hipHostMallocDefault) == hipSuccess &&
             hipHostMalloc(&c->h_scalar, 8 * 8, hipHostMallocDefault) == hipSuccess &&
-            hipMalloc(&c->d_iter, sizeof(IterStats) * (c->max_iter + 2)) == hipSuccess &&
-            hipHostMalloc(&c->h_iter, sizeof(IterStats) * (c->max_iter + 2),
-                          hipHostMallocDefault) == hipSuccess;
+            true;
   if (!ok) return fail(CYTOM_ENOMEM);
   c->pool_size = cfg->edge_capacity ? cfg->edge_capacity : (1ull << 22);
   if (hipMalloc(&c->pool, c->pool_size * 4) != hipSuccess) return fail(CYTOM_ENOMEM);
@@ -401,15 +399,12 @@ int cytom_get_tuning(const cytom_ctx* ctx, cytom_tuning* t) {
 
 int cytom_set_profiling(cytom_ctx* ctx, int enable) {
   CY_CHECK_CTX(ctx);
-  if (enable && ctx->kev.size() < 2 * (size_t)ctx->max_iter) {
-    while (ctx->kev.size() < 2 * (size_t)ctx->max_iter) {
-      hipEvent_t e;
-      CY_HIP(ctx, hipEventCreate(&e));
-      ctx->kev.push_back(e);
-    }
-  }
   ctx->profile = enable != 0;
-  for (int k = 0; k < 4; ++k) {
+  for (auto& P : ctx->progs) {
+    int rc = program_events(ctx, P);
+    if (rc) return rc;
+  }
+  for (int k = 0; k < 5; ++k) {
     ctx->gather_ms[k] = 0;
     ctx->gather_launches[k] = 0;
     ctx->gather_alg_bytes[k] = 0;

@@ -87,9 +87,18 @@ __device__ inline uint64_t group_mask(const uint8_t* __restrict__ gact, uint32_t
 
 // ---------------------------------------------------------------- init
 // bfs.cc:170-197, cc.cc:171-189, sssp.cc:171-195 (+ source set to 0).
+// A gate (nullable) holds the batch's decision for a program, written on the
+// device by the update batch: 1 = re-initialise, 0 = run the insert hooks,
+// 2 = neither (the batch was refused).  Init kernels run when the gate is
+// null or 1, hook kernels when it is null or 0.
+__device__ inline bool gate_is(const unsigned long long* gate, unsigned long long v) {
+  return gate == nullptr || *gate == v;
+}
+
 __global__ void k_init_min(int kind, uint64_t V, uint32_t source, uint32_t* __restrict__ cur,
                            uint32_t* __restrict__ nxt, uint64_t* __restrict__ act,
-                           uint8_t* __restrict__ changed) {
+                           uint8_t* __restrict__ changed, const unsigned long long* gate) {
+  if (!gate_is(gate, 1)) return;
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t val = kind == CYTOM_CC ? (uint32_t)v : (v == source ? 0u : INF);
@@ -110,7 +119,8 @@ __global__ void k_init_min(int kind, uint64_t V, uint32_t source, uint32_t* __re
 // own component, both sources active.
 __global__ void k_init_pair(uint64_t V, uint32_t s0, uint32_t s1, uint2* __restrict__ cur,
                             uint2* __restrict__ nxt, uint64_t* __restrict__ act,
-                            uint8_t* __restrict__ changed) {
+                            uint8_t* __restrict__ changed, const unsigned long long* gate) {
+  if (!gate_is(gate, 1)) return;
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
     const uint2 val = make_uint2(v == s0 ? 0u : INF, v == s1 ? 0u : INF);
@@ -127,12 +137,15 @@ __global__ void k_init_pair(uint64_t V, uint32_t s0, uint32_t s1, uint2* __restr
   }
 }
 
-// pagerank-delta.cc:246-270: rank 0, delta 1, all active.
 // pagerank-delta.cc:246-270 (rank 0, delta 1, all active); for PPR every
 // vertex starts at rank = delta = 0, inactive, before k_init_ppr_set.
 __global__ void k_init_prd(uint64_t V, int personalized, float* __restrict__ rank,
                            float* __restrict__ delta, long long* __restrict__ dsum,
-                           uint64_t* __restrict__ act, uint8_t* __restrict__ changed) {
+                           uint64_t* __restrict__ act, uint8_t* __restrict__ changed,
+                           uint32_t* __restrict__ dctl, const unsigned long long* gate) {
+  if (!gate_is(gate, 1)) return;
+  // the first-iteration apply rule (pagerank-delta.cc:148-156) for the next run
+  if (blockIdx.x == 0 && threadIdx.x == 0) dctl[0] = personalized ? 0u : 1u;
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
     rank[v] = 0.f;
@@ -145,7 +158,9 @@ __global__ void k_init_prd(uint64_t V, int personalized, float* __restrict__ ran
 
 // PPR teleport set: rank = delta = (1-alpha)/|T|, active.
 __global__ void k_init_ppr_set(const uint32_t* __restrict__ set, uint32_t n, float* __restrict__ rank,
-                               float* __restrict__ delta, uint64_t* __restrict__ act) {
+                               float* __restrict__ delta, uint64_t* __restrict__ act,
+                               const unsigned long long* gate) {
+  if (!gate_is(gate, 1)) return;
   const float r0 = (float)((1. - 0.85) * (1.0 / (double)n));
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t v = set[i];
@@ -219,9 +234,11 @@ __global__ void k_emit_chunks(const uint32_t* __restrict__ nch, const uint32_t* 
 // multiples of 64 slots, so row r of a tile at slot offset o is grp[o/64 +
 // r]).  One wavefront per work chunk; tombstones and slots past the tile's
 // count add nothing.
-__global__ void k_row_groups(const uint64_t* __restrict__ chunks, uint64_t n_chunks,
-                             StoreView S, uint32_t CH, uint64_t* __restrict__ grp) {
+__global__ void k_row_groups(const uint64_t* __restrict__ chunks,
+                             const uint32_t* __restrict__ n_chunks_dev, StoreView S, uint32_t CH,
+                             uint64_t* __restrict__ grp) {
   const uint32_t lane = threadIdx.x & 63;
+  const uint64_t n_chunks = *n_chunks_dev;
   const uint32_t gshift = S.log2vpt - 6;
   for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < n_chunks;
        c += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
@@ -290,8 +307,17 @@ __global__ void k_band_bounds(const unsigned long long* __restrict__ rs, uint32_
   xb[NB] = T;
 }
 
+// Chunks per job: job_chunks when set, else max(2^17 slots, n_chunks / (8 CUs)).
+__global__ void k_job_length(const uint32_t* __restrict__ n_chunks, uint32_t job_chunks,
+                             uint32_t CH, uint32_t cus, uint32_t* __restrict__ jc) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  *jc = job_chunks ? job_chunks : max((1u << 17) / CH, *n_chunks / (cus * 8u));
+}
+
 __global__ void k_col_jobs(const uint32_t* __restrict__ cbase, const uint32_t* __restrict__ xb,
-                           uint32_t T, uint32_t NB, uint32_t JC, uint32_t* __restrict__ colj) {
+                           uint32_t T, uint32_t NB, const uint32_t* __restrict__ jc,
+                           uint32_t* __restrict__ colj) {
+  const uint32_t JC = *jc;
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < T * NB;
        c += gridDim.x * blockDim.x) {
     uint32_t y, c0, c1;
@@ -302,7 +328,9 @@ __global__ void k_col_jobs(const uint32_t* __restrict__ cbase, const uint32_t* _
 
 __global__ void k_emit_jobs(const uint32_t* __restrict__ cbase, const uint32_t* __restrict__ xb,
                             const uint32_t* __restrict__ colj, const uint32_t* __restrict__ jbase,
-                            uint32_t T, uint32_t NB, uint32_t JC, uint4* __restrict__ jobs) {
+                            uint32_t T, uint32_t NB, const uint32_t* __restrict__ jc,
+                            uint4* __restrict__ jobs) {
+  const uint32_t JC = *jc;
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < T * NB;
        c += gridDim.x * blockDim.x) {
     uint32_t y, c_begin, c_end;
@@ -718,13 +746,27 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_pair(
 // reference: ALPHA and the threshold are doubles).  Loads are unconditional:
 // most vertices carry a sum while PageRank-delta runs.
 __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
-    uint64_t V, uint32_t log2vpt, int first, double thr, double qinv, float* __restrict__ rank,
+    uint64_t V, uint32_t log2vpt, uint32_t* __restrict__ dctl, double thr, double qinv,
+    float* __restrict__ rank,
     float* __restrict__ delta, long long* __restrict__ dsum, float* __restrict__ contrib,
     uint8_t* __restrict__ changed, const uint64_t* __restrict__ act_cur,
     uint64_t* __restrict__ act_next, uint8_t* __restrict__ gact_next,
     const uint32_t* __restrict__ outdeg, IterStats* nit) {
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
+  // The first-iteration rule holds for this launch when init set dctl[0];
+  // every block reads it here and the last block to finish clears it.
+  const bool first = dctl[0] != 0;
+  __syncthreads();
+  if (first) {
+    if (threadIdx.x == 0) {
+      __threadfence();
+      if (atomicAdd(&dctl[1], 1u) == gridDim.x - 1) {
+        dctl[1] = 0;
+        dctl[0] = 0;
+      }
+    }
+  }
   if (!first && empty_frontier(nit, act_next, nwords)) return;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t stride = (((uint64_t)gridDim.x * blockDim.x) >> 6) * APPLY_WORDS;
@@ -797,7 +839,9 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
 
 // ------------------------------------------------------ deletion tests
 __global__ void k_reexec_min(int kind, const uint32_t* __restrict__ found, uint64_t n,
+                             const unsigned long long* __restrict__ n_dev,
                              const uint32_t* __restrict__ cur, unsigned long long* flag) {
+  n = min(n, (uint64_t)*n_dev);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t s = found[3 * i], d = found[3 * i + 1], w = found[3 * i + 2];
@@ -812,8 +856,10 @@ __global__ void k_reexec_min(int kind, const uint32_t* __restrict__ found, uint6
 
 // pagerank-delta.cc:190-206
 __global__ void k_reexec_prd(const uint32_t* __restrict__ found, uint64_t n,
+                             const unsigned long long* __restrict__ n_dev,
                              const float* __restrict__ rank, const uint32_t* __restrict__ outdeg,
                              double thr, unsigned long long* flag) {
+  n = min(n, (uint64_t)*n_dev);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t s = found[3 * i], d = found[3 * i + 1];
@@ -829,7 +875,8 @@ __global__ void k_hooks_min(int kind, const uint32_t* __restrict__ src,
                             const uint64_t* __restrict__ ins_slot, uint64_t n,
                             const uint32_t* __restrict__ pool, uint32_t* __restrict__ cur,
                             uint32_t* __restrict__ nxt, uint64_t* __restrict__ act,
-                            uint8_t* __restrict__ changed) {
+                            uint8_t* __restrict__ changed, const unsigned long long* gate) {
+  if (!gate_is(gate, 0)) return;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     if (pool[ins_slot[i]] == TOMB) continue;  // deleted within the batch
@@ -854,10 +901,13 @@ __global__ void k_hooks_min(int kind, const uint32_t* __restrict__ src,
 __global__ void k_hooks_prd(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                             const uint64_t* __restrict__ ins_slot, uint64_t n,
                             const uint32_t* __restrict__ found, uint64_t n_found,
+                            const unsigned long long* __restrict__ n_found_dev,
                             const uint32_t* __restrict__ pool, const float* __restrict__ rank,
                             const uint32_t* __restrict__ outdeg, long long* __restrict__ dsum,
                             uint64_t* __restrict__ act, uint8_t* __restrict__ changed, double thr,
-                            float qscale) {
+                            float qscale, const unsigned long long* gate) {
+  if (!gate_is(gate, 0)) return;
+  if (n_found_dev) n_found = min(n_found, (uint64_t)*n_found_dev);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n + n_found;
        i += (uint64_t)gridDim.x * blockDim.x) {
     if (i < n) {
@@ -906,7 +956,7 @@ void launch_gather(cytom_ctx* ctx, Program& P, int it) {
   A.dsum = P.dsum;
   A.qscale = ldexpf(1.f, ctx->prd_shift);
   A.changed = P.changed;
-  A.it = ctx->d_iter + it;
+  A.it = P.d_iter + it;
   A.CH = ctx->chunk_slots;
   const size_t lds = (size_t)ctx->vpt * (KIND == CYTOM_PRD || KIND == KIND_SSSP_PAIR ? 8 : 4);
   k_gather<KIND><<<gather_grid(ctx, lds), GATHER_THREADS, lds, ctx->stream>>>(A);
@@ -914,61 +964,81 @@ void launch_gather(cytom_ctx* ctx, Program& P, int it) {
 
 }  // namespace
 
-int build_jobs(cytom_ctx* ctx) {
-  hipStream_t st = ctx->stream;
+// Work lists of the current store, device-driven: every count stays on the
+// device (the kernels read it there), buffers are sized by upper bounds
+// (chunks <= pool_size / CH + n_tiles, jobs <= chunks / minimum job length +
+// cells), so the lists can be rebuilt without a host round trip.
+int prepare_jobs(cytom_ctx* ctx) {
   int rc;
   const uint64_t nt = ctx->n_tiles;
   const uint32_t NB = std::min(std::min(ctx->source_bands, ctx->T), MAX_SOURCE_BANDS);
   const uint64_t cells = (uint64_t)ctx->T * NB;
+  const uint64_t max_chunks = ctx->pool_size / ctx->chunk_slots + nt + 1;
+  const uint64_t jc_min = ctx->job_chunks ? ctx->job_chunks : (1u << 17) / ctx->chunk_slots;
+  const uint64_t max_jobs = max_chunks / jc_min + cells + 1;
   if ((rc = ensure(ctx, ctx->g_nch, nt * 4)) || (rc = ensure(ctx, ctx->g_cbase, (nt + 1) * 4)) ||
       (rc = ensure(ctx, ctx->g_colj, cells * 4)) ||
       (rc = ensure(ctx, ctx->g_jbase, (cells + 1) * 4)) ||
-      (rc = ensure(ctx, ctx->g_scan_tmp, scan_tmp_bytes(nt + 1) + 256)))
+      (rc = ensure(ctx, ctx->g_scan_tmp, scan_tmp_bytes(std::max<uint64_t>(nt, cells) + 1) + 256)) ||
+      (rc = ensure(ctx, ctx->g_bands, 256 + (uint64_t)ctx->T * 8)) ||
+      (rc = ensure(ctx, ctx->g_chunks, max_chunks * 8)) ||
+      (rc = ensure(ctx, ctx->g_grp, (ctx->pool_size / 64 + 1) * 8)) ||
+      (rc = ensure(ctx, ctx->g_jobs, max_jobs * 16)))
     return rc;
+  ctx->n_chunks = max_chunks;
+  ctx->n_cells = cells;
+  ctx->n_bands = NB;
+  ctx->n_jobs = max_jobs;
+  return 0;
+}
+
+int build_jobs(cytom_ctx* ctx) {
+  int rc;
+  if ((rc = prepare_jobs(ctx))) return rc;
+  enqueue_jobs(ctx);
+  CY_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+void enqueue_jobs(cytom_ctx* ctx) {
+  hipStream_t st = ctx->stream;
+  const uint64_t nt = ctx->n_tiles;
+  const uint32_t NB = ctx->n_bands;
+  const uint64_t cells = ctx->n_cells;
   auto* nch = static_cast<uint32_t*>(ctx->g_nch.p);
   auto* cbase = static_cast<uint32_t*>(ctx->g_cbase.p);
   auto* colj = static_cast<uint32_t*>(ctx->g_colj.p);
   auto* jbase = static_cast<uint32_t*>(ctx->g_jbase.p);
-  if ((rc = ensure(ctx, ctx->g_bands, 256 + (uint64_t)ctx->T * 8))) return rc;
   auto* xb = static_cast<uint32_t*>(ctx->g_bands.p);
+  uint32_t* jc = xb + 32;  // the job length chosen on the device
   auto* rs = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->g_bands.p) + 256);
+  auto* chunks = static_cast<uint64_t*>(ctx->g_chunks.p);
   k_row_slots<<<grid_for(ctx->T, 256), 256, 0, st>>>(ctx->tile_cnt, ctx->T, rs);
   k_band_bounds<<<1, 64, 0, st>>>(rs, ctx->T, NB, xb);
   k_nch<<<grid_for(nt, 256), 256, 0, st>>>(ctx->tile_cnt, nt, ctx->chunk_slots, nch);
   scan_exclusive_u32(nch, cbase, nt, ctx->g_scan_tmp.p, st);
-  CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc + 32, cbase + nt, 4, hipMemcpyDeviceToHost, st));
-  CY_HIP(ctx, hipStreamSynchronize(st));
-  const uint64_t n_chunks = (uint32_t)ctx->h_misc[32];
-  if ((rc = ensure(ctx, ctx->g_chunks, std::max<uint64_t>(n_chunks, 1) * 8))) return rc;
-  k_emit_chunks<<<grid_for(nt, 256), 256, 0, st>>>(nch, cbase, nt, ctx->chunk_slots,
-                                                   static_cast<uint64_t*>(ctx->g_chunks.p));
+  k_emit_chunks<<<grid_for(nt, 256), 256, 0, st>>>(nch, cbase, nt, ctx->chunk_slots, chunks);
   // Source groups of every 64-slot row, recomputed after each batch (the
   // batch moved, added and tombstoned slots).
-  if ((rc = ensure(ctx, ctx->g_grp, (ctx->pool_size / 64 + 1) * 8))) return rc;
-  if (n_chunks)
-    k_row_groups<<<grid_for(n_chunks * 64, 256), 256, 0, st>>>(
-        static_cast<const uint64_t*>(ctx->g_chunks.p), n_chunks, ctx->view(), ctx->chunk_slots,
-        static_cast<uint64_t*>(ctx->g_grp.p));
+  k_row_groups<<<grid_for(ctx->n_chunks * 64, 256), 256, 0, st>>>(
+      chunks, cbase + nt, ctx->view(), ctx->chunk_slots, static_cast<uint64_t*>(ctx->g_grp.p));
   // Jobs of job_chunks chunks, or sized from the store: at least 2^17 slots,
   // and about eight jobs per CU when the store is large (fewer column flushes).
-  const uint32_t JC =
-      ctx->job_chunks ? ctx->job_chunks
-                      : (uint32_t)std::max<uint64_t>((1u << 17) / ctx->chunk_slots,
-                                                     n_chunks / ((uint64_t)ctx->num_cus * 8));
-  k_col_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, xb, ctx->T, NB, JC, colj);
+  k_job_length<<<1, 64, 0, st>>>(cbase + nt, ctx->job_chunks, ctx->chunk_slots,
+                                 (uint32_t)ctx->num_cus, jc);
+  k_col_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, xb, ctx->T, NB, jc, colj);
   scan_exclusive_u32(colj, jbase, cells, ctx->g_scan_tmp.p, st);
-  CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc + 33, jbase + cells, 4, hipMemcpyDeviceToHost, st));
-  CY_HIP(ctx, hipStreamSynchronize(st));
-  const uint64_t n_jobs = (uint32_t)ctx->h_misc[33];
-  if ((rc = ensure(ctx, ctx->g_jobs, std::max<uint64_t>(n_jobs, 1) * 16))) return rc;
-  k_emit_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, xb, colj, jbase, ctx->T, NB, JC,
+  k_emit_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, xb, colj, jbase, ctx->T, NB, jc,
                                                     static_cast<uint4*>(ctx->g_jobs.p));
-  ctx->n_chunks = n_chunks;
-  ctx->n_cells = cells;
-  ctx->n_bands = NB;
-  ctx->n_jobs = n_jobs;
   ctx->jobs_dirty = false;
-  return 0;
+}
+
+int64_t jobs_chunk_count(cytom_ctx* ctx) {
+  uint32_t n = 0;
+  CY_HIP(ctx, hipMemcpyAsync(&n, static_cast<uint32_t*>(ctx->g_cbase.p) + ctx->n_tiles, 4,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return n;
 }
 
 int program_alloc(cytom_ctx* ctx, Program& P) {
@@ -986,6 +1056,11 @@ int program_alloc(cytom_ctx* ctx, Program& P) {
   for (int i = 0; i < 2 && ok; ++i)
     ok = alloc((void**)&P.act[i], (V / 64) * 8) && alloc((void**)&P.gact[i], (size_t)ctx->T * 64);
   ok = ok && alloc((void**)&P.changed, V);
+  const size_t nit = sizeof(IterStats) * (ctx->max_iter + 2);
+  ok = ok && alloc((void**)&P.d_iter, nit) && alloc((void**)&P.dctl, 64) &&
+       hipHostMalloc((void**)&P.h_iter, nit, hipHostMallocDefault) == hipSuccess;
+  for (int i = 0; i < 2 && ok; ++i) ok = hipEventCreate(&P.ev[i]) == hipSuccess;
+  ok = ok && hipMemsetAsync(P.dctl, 0, 64, ctx->stream) == hipSuccess;
   if (!ok) {
     program_free(P);
     return set_err(ctx, CYTOM_ENOMEM, "program state allocation failed");
@@ -999,6 +1074,17 @@ void program_free(Program& P) {
                 P.teleport};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+  if (P.d_iter) (void)hipFree(P.d_iter);
+  if (P.dctl) (void)hipFree(P.dctl);
+  if (P.h_iter) (void)hipHostFree(P.h_iter);
+  for (auto& e : P.ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : P.kev) (void)hipEventDestroy(e);
+  P.d_iter = nullptr;
+  P.dctl = nullptr;
+  P.h_iter = nullptr;
+  P.ev[0] = P.ev[1] = nullptr;
+  P.kev.clear();
   P.cur = P.nxt = P.fval = nullptr;
   P.rank = P.delta = P.contrib = nullptr;
   P.dsum = nullptr;
@@ -1009,25 +1095,26 @@ void program_free(Program& P) {
   P.n_teleport = 0;
 }
 
-int program_init(cytom_ctx* ctx, Program& P) {
+int program_init(cytom_ctx* ctx, Program& P, const unsigned long long* gate) {
   hipStream_t st = ctx->stream;
-  P.cur_act = 0;
-  P.first_iter = P.kind != CYTOM_PPR;  // the PRD first-iteration apply rule
-  P.fixpoint = false;
+  // Init writes the current buffers (act[cur_act]), so the host's buffer
+  // parity stays valid whether or not a gated init runs.
+  uint64_t* act = P.act[P.cur_act];
+  if (!gate) P.fixpoint = false;
   if (prd_like(P.kind)) {
     const int ppr = P.kind == CYTOM_PPR ? 1 : 0;
-    k_init_prd<<<grid_for(ctx->V, 256), 256, 0, st>>>(ctx->V, ppr, P.rank, P.delta, P.dsum,
-                                                      P.act[0], P.changed);
+    k_init_prd<<<grid_for(ctx->V, 256), 256, 0, st>>>(ctx->V, ppr, P.rank, P.delta, P.dsum, act,
+                                                      P.changed, P.dctl, gate);
     if (ppr && P.n_teleport)
       k_init_ppr_set<<<grid_for(P.n_teleport, 256), 256, 0, st>>>(P.teleport, P.n_teleport,
-                                                                  P.rank, P.delta, P.act[0]);
+                                                                  P.rank, P.delta, act, gate);
   } else if (P.kind == KIND_SSSP_PAIR) {
     k_init_pair<<<grid_for(ctx->V, 256), 256, 0, st>>>(
         ctx->V, P.source, P.source2, reinterpret_cast<uint2*>(P.cur),
-        reinterpret_cast<uint2*>(P.nxt), P.act[0], P.changed);
+        reinterpret_cast<uint2*>(P.nxt), act, P.changed, gate);
   } else {
     k_init_min<<<grid_for(ctx->V, 256), 256, 0, st>>>(P.kind, ctx->V, P.source, P.cur, P.nxt,
-                                                      P.act[0], P.changed);
+                                                      act, P.changed, gate);
   }
   CY_HIP(ctx, hipGetLastError());
   return 0;
@@ -1060,124 +1147,115 @@ int program_set_sources(cytom_ctx* ctx, Program& P, const uint32_t* sources, uin
   return program_init(ctx, P);
 }
 
-void launch_reexec_check(cytom_ctx* ctx, Program& P, const uint32_t* found, uint64_t n_found,
-                         unsigned long long* flag) {
+void launch_reexec_check(cytom_ctx* ctx, Program& P, const uint32_t* found, uint64_t n_max,
+                         const unsigned long long* n_dev, unsigned long long* flag) {
   if (prd_like(P.kind))
-    k_reexec_prd<<<grid_for(n_found, 256), 256, 0, ctx->stream>>>(found, n_found, P.rank,
-                                                                  ctx->outdeg, ctx->delta, flag);
+    k_reexec_prd<<<grid_for(n_max, 256), 256, 0, ctx->stream>>>(found, n_max, n_dev, P.rank,
+                                                                ctx->outdeg, ctx->delta, flag);
   else
-    k_reexec_min<<<grid_for(n_found, 256), 256, 0, ctx->stream>>>(P.kind, found, n_found, P.cur,
-                                                                  flag);
+    k_reexec_min<<<grid_for(n_max, 256), 256, 0, ctx->stream>>>(P.kind, found, n_max, n_dev,
+                                                                P.cur, flag);
 }
 
 void launch_hooks(cytom_ctx* ctx, Program& P, const uint32_t* d_src, const uint32_t* d_dst,
                   const uint32_t* d_w, const uint64_t* ins_slot, uint64_t n_ins,
-                  const uint32_t* found, uint64_t n_found) {
+                  const uint32_t* found, uint64_t n_found_max,
+                  const unsigned long long* n_found_dev, const unsigned long long* gate) {
   hipStream_t st = ctx->stream;
   if (prd_like(P.kind)) {
-    if (n_ins + n_found)
-      k_hooks_prd<<<grid_for(n_ins + n_found, 256), 256, 0, st>>>(
-          d_src, d_dst, ins_slot, n_ins, found, n_found, ctx->pool, P.rank, ctx->outdeg, P.dsum,
-          P.act[P.cur_act], P.changed, ctx->delta, ldexpf(1.f, ctx->prd_shift));
+    if (n_ins + n_found_max)
+      k_hooks_prd<<<grid_for(n_ins + n_found_max, 256), 256, 0, st>>>(
+          d_src, d_dst, ins_slot, n_ins, found, n_found_max, n_found_dev, ctx->pool, P.rank,
+          ctx->outdeg, P.dsum, P.act[P.cur_act], P.changed, ctx->delta,
+          ldexpf(1.f, ctx->prd_shift), gate);
   } else if (n_ins) {
     k_hooks_min<<<grid_for(n_ins, 256), 256, 0, st>>>(P.kind, d_src, d_dst, d_w, ins_slot, n_ins,
                                                       ctx->pool, P.cur, P.nxt, P.act[P.cur_act],
-                                                      P.changed);
+                                                      P.changed, gate);
   }
-  P.fixpoint = false;
 }
 
-int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
-  hipStream_t st = ctx->stream;
-  int rc;
-  if (ctx->jobs_dirty && (rc = build_jobs(ctx))) return rc;
-  const uint32_t maxit = ctx->max_iter;
-  CY_HIP(ctx, hipMemsetAsync(ctx->d_iter, 0, sizeof(IterStats) * (maxit + 2), st));
-  CY_HIP(ctx, hipMemsetAsync(P.gact[P.cur_act], 0, (size_t)ctx->T * 64, st));
-  CY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-  const uint64_t words = ctx->V >> 6;
+namespace {
+
+inline uint32_t apply_blocks_for(const cytom_ctx* ctx) {
   // One 1024-thread block per CU walks the bitmap words: measured faster than
   // more, smaller blocks (fewer block tails and end-of-block reductions).
-  const uint32_t apply_blocks = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>((uint64_t)ctx->num_cus, (words + 3) / 4));
-  k_frontier<<<apply_blocks, APPLY_THREADS, 0, st>>>(P.kind, ctx->V, ctx->log2vpt,
-                                                     P.act[P.cur_act], ctx->outdeg,
-                                                     P.gact[P.cur_act], P.delta, P.contrib,
-                                                     P.cur, P.fval, ctx->d_iter);
-  // Iterations are launched in groups with one host check per group.  The
-  // first group is as long as the previous run of the program (PageRank-delta
-  // runs take a similar number of iterations batch after batch), later ones
-  // grow 4, 8, 16.  Launches past convergence are no-ops: the gather returns
-  // at once and the apply only clears the next bitmap.
-  uint32_t it = 0, launched = 0;
-  uint32_t group = std::min<uint32_t>(std::max<uint32_t>(P.last_iters, 2), 64), next = 4;
-  bool converged = false;
-  const bool prof = ctx->profile && ctx->kev.size() >= 2 * (size_t)maxit &&
-                    P.kind <= CYTOM_PPR;  // the per-kind profile slots
-  while (!converged && launched < maxit) {
-    const uint32_t n = std::min(group, maxit - launched);
-    for (uint32_t k = 0; k < n; ++k, ++launched) {
-      const uint32_t i = launched;
-      if (prof) CY_HIP(ctx, hipEventRecord(ctx->kev[2 * i], st));
-      switch (P.kind) {
-        case CYTOM_BFS: launch_gather<CYTOM_BFS>(ctx, P, i); break;
-        case CYTOM_CC: launch_gather<CYTOM_CC>(ctx, P, i); break;
-        case CYTOM_SSSP: launch_gather<CYTOM_SSSP>(ctx, P, i); break;
-        case KIND_SSSP_PAIR: launch_gather<KIND_SSSP_PAIR>(ctx, P, i); break;
-        default: launch_gather<CYTOM_PRD>(ctx, P, i); break;
-      }
-      if (prof) CY_HIP(ctx, hipEventRecord(ctx->kev[2 * i + 1], st));
-      const int nx = P.cur_act ^ 1;
-      if (prd_like(P.kind)) {
-        k_apply_prd<<<apply_blocks, APPLY_THREADS, 0, st>>>(
-            ctx->V, ctx->log2vpt, P.first_iter ? 1 : 0, ctx->delta, ldexp(1.0, -ctx->prd_shift),
-            P.rank, P.delta, P.dsum,
-            P.contrib, P.changed, P.act[P.cur_act], P.act[nx], P.gact[nx], ctx->outdeg,
-            ctx->d_iter + i + 1);
-      } else if (P.kind == CYTOM_BFS) {
-        k_apply_min<CYTOM_BFS><<<apply_blocks, APPLY_THREADS, 0, st>>>(
-            ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
-            P.gact[nx], ctx->outdeg, ctx->d_iter + i + 1);
-      } else if (P.kind == CYTOM_CC) {
-        k_apply_min<CYTOM_CC><<<apply_blocks, APPLY_THREADS, 0, st>>>(
-            ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
-            P.gact[nx], ctx->outdeg, ctx->d_iter + i + 1);
-      } else if (P.kind == KIND_SSSP_PAIR) {
-        k_apply_pair<<<apply_blocks, APPLY_THREADS, 0, st>>>(
-            ctx->V, ctx->log2vpt, reinterpret_cast<uint2*>(P.cur),
-            reinterpret_cast<const uint2*>(P.nxt), reinterpret_cast<uint2*>(P.fval), P.changed,
-            P.act[P.cur_act], P.act[nx], P.gact[nx], ctx->outdeg, ctx->d_iter + i + 1);
-      } else {
-        k_apply_min<CYTOM_SSSP><<<apply_blocks, APPLY_THREADS, 0, st>>>(
-            ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
-            P.gact[nx], ctx->outdeg, ctx->d_iter + i + 1);
-      }
-      P.cur_act = nx;
-      P.first_iter = false;
+  const uint64_t words = ctx->V >> 6;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->num_cus,
+                                                             (words + 3) / 4));
+}
+
+}  // namespace
+
+void run_begin(cytom_ctx* ctx, Program& P) {
+  hipStream_t st = ctx->stream;
+  fill_u64(reinterpret_cast<uint64_t*>(P.d_iter),
+           sizeof(IterStats) / 8 * (uint64_t)(ctx->max_iter + 2), 0, st);
+  fill_u64(reinterpret_cast<uint64_t*>(P.gact[P.cur_act]), (uint64_t)ctx->T * 8, 0, st);
+  (void)hipEventRecord(P.ev[0], st);
+  k_frontier<<<apply_blocks_for(ctx), APPLY_THREADS, 0, st>>>(
+      P.kind, ctx->V, ctx->log2vpt, P.act[P.cur_act], ctx->outdeg, P.gact[P.cur_act], P.delta,
+      P.contrib, P.cur, P.fval, P.d_iter);
+}
+
+void run_iters(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t n) {
+  hipStream_t st = ctx->stream;
+  const uint32_t ab = apply_blocks_for(ctx);
+  const bool prof = ctx->profile && P.kev.size() >= 2 * (size_t)ctx->max_iter;
+  for (uint32_t i = lo; i < lo + n; ++i) {
+    if (prof) (void)hipEventRecord(P.kev[2 * i], st);
+    switch (P.kind) {
+      case CYTOM_BFS: launch_gather<CYTOM_BFS>(ctx, P, i); break;
+      case CYTOM_CC: launch_gather<CYTOM_CC>(ctx, P, i); break;
+      case CYTOM_SSSP: launch_gather<CYTOM_SSSP>(ctx, P, i); break;
+      case KIND_SSSP_PAIR: launch_gather<KIND_SSSP_PAIR>(ctx, P, i); break;
+      default: launch_gather<CYTOM_PRD>(ctx, P, i); break;
     }
-    const uint32_t lo = launched - n + 1;
-    CY_HIP(ctx, hipMemcpyAsync(&ctx->h_iter[lo], &ctx->d_iter[lo], sizeof(IterStats) * n,
-                               hipMemcpyDeviceToHost, st));
-    CY_HIP(ctx, hipStreamSynchronize(st));
-    it = launched;
-    for (uint32_t k = lo; k <= launched; ++k) {
-      if (ctx->h_iter[k].count == 0) {
-        converged = true;
-        it = k;
-        break;
-      }
+    if (prof) (void)hipEventRecord(P.kev[2 * i + 1], st);
+    const int nx = P.cur_act ^ 1;
+    IterStats* nit = P.d_iter + i + 1;
+    if (prd_like(P.kind)) {
+      k_apply_prd<<<ab, APPLY_THREADS, 0, st>>>(
+          ctx->V, ctx->log2vpt, P.dctl, ctx->delta, ldexp(1.0, -ctx->prd_shift), P.rank, P.delta,
+          P.dsum, P.contrib, P.changed, P.act[P.cur_act], P.act[nx], P.gact[nx], ctx->outdeg, nit);
+    } else if (P.kind == CYTOM_BFS) {
+      k_apply_min<CYTOM_BFS><<<ab, APPLY_THREADS, 0, st>>>(
+          ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
+          P.gact[nx], ctx->outdeg, nit);
+    } else if (P.kind == CYTOM_CC) {
+      k_apply_min<CYTOM_CC><<<ab, APPLY_THREADS, 0, st>>>(
+          ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
+          P.gact[nx], ctx->outdeg, nit);
+    } else if (P.kind == KIND_SSSP_PAIR) {
+      k_apply_pair<<<ab, APPLY_THREADS, 0, st>>>(
+          ctx->V, ctx->log2vpt, reinterpret_cast<uint2*>(P.cur),
+          reinterpret_cast<const uint2*>(P.nxt), reinterpret_cast<uint2*>(P.fval), P.changed,
+          P.act[P.cur_act], P.act[nx], P.gact[nx], ctx->outdeg, nit);
+    } else {
+      k_apply_min<CYTOM_SSSP><<<ab, APPLY_THREADS, 0, st>>>(
+          ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
+          P.gact[nx], ctx->outdeg, nit);
     }
-    group = next;
-    next = std::min<uint32_t>(next * 2, 16);
+    P.cur_act = nx;
   }
+}
+
+void run_readback(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t hi) {
+  (void)hipMemcpyAsync(&P.h_iter[lo], &P.d_iter[lo], sizeof(IterStats) * (hi - lo + 1),
+                       hipMemcpyDeviceToHost, ctx->stream);
+}
+
+int run_converged_at(const Program& P, uint32_t lo, uint32_t hi) {
+  for (uint32_t k = lo; k <= hi; ++k)
+    if (P.h_iter[k].count == 0) return (int)k;
+  return -1;
+}
+
+int run_finish(cytom_ctx* ctx, Program& P, uint32_t it, bool converged, cytom_run_stats* out) {
   P.last_iters = it;
-  CY_HIP(ctx, hipEventRecord(ctx->ev[3], st));
-  CY_HIP(ctx, hipMemcpyAsync(ctx->h_iter, ctx->d_iter, sizeof(IterStats) * (it + 1),
-                             hipMemcpyDeviceToHost, st));
-  CY_HIP(ctx, hipStreamSynchronize(st));
-  CY_HIP(ctx, hipGetLastError());
   P.fixpoint = converged;
-  if (prof) {
+  const uint32_t maxit = ctx->max_iter;
+  if (ctx->profile && P.kev.size() >= 2 * (size_t)maxit && P.kind <= CYTOM_PPR) {
     // Algorithmic bytes of one gather launch: 8 B per traversed packed edge
     // (+4 B weight for SSSP), the frontier's vertex state (4 B, or the 12 B
     // PageRank-delta vertex), a read-modify-write of every updated target and
@@ -1186,8 +1264,8 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
     const double sv = prd_like(P.kind) ? 12.0 : 4.0;
     for (uint32_t i = 0; i < it; ++i) {
       float ms = 0.f;
-      CY_HIP(ctx, hipEventElapsedTime(&ms, ctx->kev[2 * i], ctx->kev[2 * i + 1]));
-      const IterStats& s = ctx->h_iter[i];
+      CY_HIP(ctx, hipEventElapsedTime(&ms, P.kev[2 * i], P.kev[2 * i + 1]));
+      const IterStats& s = P.h_iter[i];
       ctx->gather_ms[P.kind] += ms;
       ctx->gather_launches[P.kind] += 1;
       ctx->gather_alg_bytes[P.kind] += (double)s.te * eb + (double)s.count * sv +
@@ -1197,12 +1275,12 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
   if (out) {
     unsigned long long te = 0, act = 0, sc = 0;
     for (uint32_t i = 0; i < it; ++i) {
-      te += ctx->h_iter[i].te;
-      act += ctx->h_iter[i].count;
-      sc += ctx->h_iter[i].scanned;
+      te += P.h_iter[i].te;
+      act += P.h_iter[i].count;
+      sc += P.h_iter[i].scanned;
     }
     float ms = 0.f;
-    CY_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+    CY_HIP(ctx, hipEventElapsedTime(&ms, P.ev[0], P.ev[1]));
     out->iterations = it;
     out->converged = converged ? 1 : 0;
     out->reexecuted = (uint32_t)P.last_reexec;
@@ -1211,6 +1289,62 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
     out->active_vertices = act;
     out->slots_scanned = sc;
     out->t_compute_ms = ms;
+  }
+  return 0;
+}
+
+// Runs the program to convergence or max_iterations with the iterations
+// already enqueued (launched of them, run_begin included) and read back to
+// h_iter[0..launched]: more groups (4, 8, then 16 iterations, one host check
+// each) until the frontier is empty.  Launches past convergence are no-ops:
+// the gather returns at once and the apply only clears the next bitmap.
+int run_complete(cytom_ctx* ctx, Program& P, uint32_t launched, cytom_run_stats* out) {
+  hipStream_t st = ctx->stream;
+  const uint32_t maxit = ctx->max_iter;
+  int conv = run_converged_at(P, 1, launched);
+  uint32_t group = 4;
+  while (conv < 0 && launched < maxit) {
+    const uint32_t n = std::min(group, maxit - launched);
+    run_iters(ctx, P, launched, n);
+    run_readback(ctx, P, launched + 1, launched + n);
+    CY_HIP(ctx, hipStreamSynchronize(st));
+    conv = run_converged_at(P, launched + 1, launched + n);
+    launched += n;
+    group = std::min<uint32_t>(group * 2, 16);
+  }
+  CY_HIP(ctx, hipEventRecord(P.ev[1], st));
+  CY_HIP(ctx, hipStreamSynchronize(st));
+  CY_HIP(ctx, hipGetLastError());
+  const uint32_t it = conv >= 0 ? (uint32_t)conv : launched;
+  return run_finish(ctx, P, it, conv >= 0, out);
+}
+
+// The first group of a run: as long as the program's previous run
+// (PageRank-delta runs take a similar number of iterations batch after batch).
+uint32_t first_group(const cytom_ctx* ctx, const Program& P) {
+  return std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(P.last_iters, 2), 64),
+                            ctx->max_iter);
+}
+
+int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
+  int rc;
+  if (ctx->jobs_dirty && (rc = build_jobs(ctx))) return rc;
+  if ((rc = program_events(ctx, P))) return rc;
+  const uint32_t n = first_group(ctx, P);
+  run_begin(ctx, P);
+  run_iters(ctx, P, 0, n);
+  run_readback(ctx, P, 0, n);
+  CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return run_complete(ctx, P, n, out);
+}
+
+// Per-iteration gather timing events of a program, created on demand.
+int program_events(cytom_ctx* ctx, Program& P) {
+  if (!ctx->profile) return 0;
+  while (P.kev.size() < 2 * (size_t)ctx->max_iter) {
+    hipEvent_t e;
+    CY_HIP(ctx, hipEventCreate(&e));
+    P.kev.push_back(e);
   }
   return 0;
 }

@@ -76,6 +76,18 @@ __host__ __device__ inline bool prd_like(int kind) { return kind == CYTOM_PRD ||
 // context's program list, so the update hooks never see it.
 constexpr int KIND_SSSP_PAIR = 5;
 
+// Per-iteration device counters.
+struct IterStats {
+  unsigned long long te;       // live out-edges of the frontier
+  unsigned long long count;    // frontier size
+  unsigned long long scanned;  // edge slots read by the gather
+  unsigned long long updated;  // targets whose value the gather changed
+  unsigned long long job_cursor;
+  unsigned long long pad[3];
+  unsigned long long band_cursor[16];  // next job of each source band's queue
+};
+constexpr uint32_t MAX_SOURCE_BANDS = 16;
+
 // Per-program device state.
 struct Program {
   int kind = 0;
@@ -93,25 +105,21 @@ struct Program {
   uint64_t* act[2] = {nullptr, nullptr};  // active bitmaps (wave-sized words)
   uint8_t* gact[2] = {nullptr, nullptr};  // per source row: 64 group-activity bytes
   uint8_t* changed = nullptr;               // per-vertex changed flags
+  // Per-run iteration counters ([max_iterations + 2], device and pinned host
+  // copy) and device control words: [0] = the PageRank-delta first-iteration
+  // rule applies (set by init, cleared by the first apply), [1] = apply blocks
+  // done (the last one clears [0]).
+  IterStats* d_iter = nullptr;
+  IterStats* h_iter = nullptr;
+  uint32_t* dctl = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // around a run (compute time)
+  std::vector<hipEvent_t> kev;            // around every gather launch (profiling)
   int cur_act = 0;
-  bool first_iter = true;
   bool fixpoint = false;
-  bool frontier_stale = true;
   int last_reexec = 0;
   uint32_t last_iters = 0;  // iterations of the previous run (first launch group)
 };
 
-// Per-iteration device counters.
-struct IterStats {
-  unsigned long long te;       // live out-edges of the frontier
-  unsigned long long count;    // frontier size
-  unsigned long long scanned;  // edge slots read by the gather
-  unsigned long long updated;  // targets whose value the gather changed
-  unsigned long long job_cursor;
-  unsigned long long pad[3];
-  unsigned long long band_cursor[16];  // next job of each source band's queue
-};
-constexpr uint32_t MAX_SOURCE_BANDS = 16;
 
 struct GrowBuf {
   void* p = nullptr;
@@ -146,13 +154,15 @@ struct cytom_ctx {
   uint64_t live_edges = 0;
   // device scalars: [0] pool_top, [1..] scratch counters
   unsigned long long* d_misc = nullptr;
-  unsigned long long* h_misc = nullptr;  // pinned mirror
+  unsigned long long* h_misc = nullptr;  // pinned: [0, 64) the control words, [64, 112) decisions
+  uint32_t* h_pflags = nullptr;  // pinned per-program batch rules: [p] force, [64 + p] found_forces
   unsigned long long* h_scalar = nullptr;  // pinned landing slots of small reads (8)
   // ingest scratch
   cytom::GrowBuf g_keys, g_vals, g_keys2, g_vals2, g_flags, g_scan, g_seg_start, g_seg_tile,
       g_reloc_cap, g_reloc_list, g_ins_slot, g_scan_tmp, g_hist, g_in_src, g_in_dst, g_in_w,
       g_del_src, g_del_dst, g_del_w, g_tab_key, g_tab_head, g_node_next, g_claimed,
-      g_tile_flag, g_del_tiles, g_found, g_tile_scan, g_rs_e, g_rs_w, g_rs_off, g_rec, g_del_sig;
+      g_tile_flag, g_del_tiles, g_found, g_tile_scan, g_rs_e, g_rs_w, g_rs_off, g_rec, g_del_sig,
+      g_pflags;  // per program: tight-deletion flags, decisions, rules
   // pinned staging of host records (two buffers used in turn)
   void* h_stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
@@ -179,8 +189,6 @@ struct cytom_ctx {
   // bits: 60 - ceil(log2(2V)) keeps any vertex's sum (bounded by 2V) in range.
   int prd_shift = 36;
   int num_cus = 256;
-  cytom::IterStats* d_iter = nullptr;  // [max_iter + 2]
-  cytom::IterStats* h_iter = nullptr;  // pinned
   std::vector<cytom::Program> progs;
   cytom::QueryEngine* qe = nullptr;
   std::string err;
@@ -188,7 +196,6 @@ struct cytom_ctx {
   // Optional per-kernel timing of the gather launches (HIP events on the
   // context stream), accumulated per program kind.
   bool profile = false;
-  std::vector<hipEvent_t> kev;
   double gather_ms[5] = {0, 0, 0, 0, 0};
   uint64_t gather_launches[5] = {0, 0, 0, 0, 0};
   double gather_alg_bytes[5] = {0, 0, 0, 0, 0};
@@ -239,6 +246,10 @@ void scan_exclusive_u64(const uint64_t* in, uint64_t* out, uint64_t n, void* tmp
 // the scan runs; the launch and tmp are sized for n_max.
 void scan_exclusive_u64_devn(const uint64_t* in, uint64_t* out, const uint32_t* n_dev,
                              uint64_t n_max, void* tmp, hipStream_t st);
+// Fills n words with v (a kernel: usable inside captured graphs, where
+// memset nodes are avoided).
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+void fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st);
 // Stable LSD radix sort of (key, value) pairs on the low key_bits bits.
 // Result lands in (k_out, v_out); (k_tmp, v_tmp) are scratch of n entries;
 // hist must hold radix_hist_bytes(n) bytes.
@@ -246,6 +257,14 @@ size_t radix_hist_bytes(uint64_t n);
 void radix_sort_pairs(const uint32_t* k_in, const uint32_t* v_in, uint32_t* k_out,
                       uint32_t* v_out, uint32_t* k_tmp, uint32_t* v_tmp, uint64_t n,
                       uint32_t key_bits, void* hist, void* scan_tmp, hipStream_t st);
+// The same with the length read from device memory (*n_dev <= n_max) when
+// the sort runs; launches, hist and scan_tmp are sized for n_max.
+void radix_sort_pairs_devn(const uint32_t* k_in, const uint32_t* v_in, uint32_t* k_out,
+                           uint32_t* v_out, uint32_t* k_tmp, uint32_t* v_tmp,
+                           const uint32_t* n_dev, uint64_t n_max, uint32_t key_bits, void* hist,
+                           void* scan_tmp, hipStream_t st);
+void scan_exclusive_u32_devn(const uint32_t* in, uint32_t* out, const uint32_t* n_dev,
+                             uint64_t n_max, void* tmp, hipStream_t st);
 
 // ------------------------------------------------------------ stream gen
 void launch_rmat(uint32_t scale, uint64_t seed, uint64_t first, const uint64_t* index,
@@ -261,15 +280,34 @@ int64_t export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_t* w, 
 // ------------------------------------------------------------ executors
 int program_alloc(cytom_ctx* ctx, Program& P);
 void program_free(Program& P);
-int program_init(cytom_ctx* ctx, Program& P);
+// Re-runs init_vertices; with a gate, only when the batch decided so on the
+// device (*gate == 1; see k_init_min).
+int program_init(cytom_ctx* ctx, Program& P, const unsigned long long* gate = nullptr);
 // Replaces a BFS/SSSP source or the PPR teleport set (host ids, validated).
 int program_set_sources(cytom_ctx* ctx, Program& P, const uint32_t* sources, uint32_t n);
 int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* st);
+// The stages of a run, for callers that enqueue several programs before one
+// host check: run_begin (counters, frontier), run_iters (iterations
+// [lo, lo + n)), run_readback (counters [lo, hi] to the host mirror); after a
+// synchronisation run_complete launches more groups until convergence or
+// the cap and fills the stats.
+int program_events(cytom_ctx* ctx, Program& P);
+void run_begin(cytom_ctx* ctx, Program& P);
+void run_iters(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t n);
+void run_readback(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t hi);
+int run_complete(cytom_ctx* ctx, Program& P, uint32_t launched, cytom_run_stats* out);
+uint32_t first_group(const cytom_ctx* ctx, const Program& P);
 // Sets the gather kernels' LDS limit and persistent grid for the context.
 int configure_gather(cytom_ctx* ctx);
 // Rebuilds the work lists of the current store (chunks of chunk_slots slots
-// per tile in g_chunks, n_chunks of them, and the column jobs).
+// per tile in g_chunks, the column jobs, the row source groups):
+// prepare_jobs sizes the buffers by upper bounds (host only), enqueue_jobs
+// launches the device-driven build, build_jobs does both.
 int build_jobs(cytom_ctx* ctx);
+int prepare_jobs(cytom_ctx* ctx);
+void enqueue_jobs(cytom_ctx* ctx);
+// The current chunk count (one host round trip).
+int64_t jobs_chunk_count(cytom_ctx* ctx);
 
 // ------------------------------------------------------------ batched queries
 int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run_stats* stats);
@@ -286,14 +324,18 @@ int ingest_records(cytom_ctx* ctx, const void* rec, uint32_t rec_bytes, uint64_t
 // out == nullptr returns the byte count only.
 int64_t read_results(cytom_ctx* ctx, const Program& P, void* out, size_t bytes);
 void io_free(cytom_ctx* ctx);
-// Re-execution tests of the found deletions against the program's values;
-// sets *flag (device) to 1 when a re-initialisation is needed.
-void launch_reexec_check(cytom_ctx* ctx, Program& P, const uint32_t* found, uint64_t n_found,
-                         unsigned long long* flag);
+// Re-execution tests of the found deletions (*n_dev of at most n_max)
+// against the program's values; sets *flag (device) to 1 when a
+// re-initialisation is needed.
+void launch_reexec_check(cytom_ctx* ctx, Program& P, const uint32_t* found, uint64_t n_max,
+                         const unsigned long long* n_dev, unsigned long long* flag);
 // edgeChanged(INSERT) hooks of the batch's surviving inserts and the
-// PRD edgeChanged(DELETE) side effect.
+// PRD edgeChanged(DELETE) side effect of the found deletions (*n_found_dev
+// of at most n_found_max); with a gate, only when the batch decided on
+// hooks (*gate == 0).
 void launch_hooks(cytom_ctx* ctx, Program& P, const uint32_t* d_src, const uint32_t* d_dst,
                   const uint32_t* d_w, const uint64_t* ins_slot, uint64_t n_ins,
-                  const uint32_t* found, uint64_t n_found);
+                  const uint32_t* found, uint64_t n_found_max,
+                  const unsigned long long* n_found_dev, const unsigned long long* gate);
 
 }  // namespace cytom

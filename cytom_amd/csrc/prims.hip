@@ -9,6 +9,32 @@
 namespace cytom {
 namespace {
 
+// n_dev (nullable): the length, read from device memory, when it is known
+// only on the device (n is then the launch bound).
+__device__ inline uint64_t dev_len(uint64_t n, const uint32_t* n_dev) {
+  return n_dev ? min(n, (uint64_t)*n_dev) : n;
+}
+
+
+__global__ void k_copy2_u32(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                            uint64_t n, const uint32_t* __restrict__ n_dev,
+                            uint32_t* __restrict__ ao, uint32_t* __restrict__ bo) {
+  n = dev_len(n, n_dev);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    ao[i] = a[i];
+    bo[i] = b[i];
+  }
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, uint64_t n, T v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 8;
 constexpr uint64_t SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 2048
@@ -68,7 +94,7 @@ template <typename T>
 void scan_impl(const T* in, T* out, uint64_t n, void* tmp, hipStream_t st,
                const uint32_t* n_dev = nullptr) {
   if (n == 0) {
-    hipMemsetAsync(out, 0, sizeof(T), st);
+    k_fill<T><<<1, 64, 0, st>>>(out, 1, T(0));
     return;
   }
   if (n <= SCAN_TILE) {
@@ -93,7 +119,9 @@ constexpr uint64_t RS_TILE = RS_THREADS * RS_ROUNDS;  // 4096 keys per block
 __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t* __restrict__ keys,
                                                            uint64_t n, uint32_t shift,
                                                            uint32_t* __restrict__ hist,
-                                                           uint32_t nb) {
+                                                           uint32_t nb,
+                                                           const uint32_t* __restrict__ n_dev) {
+  n = dev_len(n, n_dev);
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -111,7 +139,8 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t* __res
 __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t n,
     uint32_t shift, const uint32_t* __restrict__ hscan, uint32_t nb,
-    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev) {
+  n = dev_len(n, n_dev);
   __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[RS_THREADS / 64][256];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -153,6 +182,35 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(
   }
 }
 
+void radix_sort_impl(const uint32_t* k_in, const uint32_t* v_in, uint32_t* k_out,
+                     uint32_t* v_out, uint32_t* k_tmp, uint32_t* v_tmp, uint64_t n,
+                     const uint32_t* n_dev, uint32_t key_bits, void* hist, void* scan_tmp,
+                     hipStream_t st) {
+  if (n == 0) return;
+  const uint32_t passes = (key_bits + 7) / 8;
+  if (passes == 0) {
+    const uint32_t g = (uint32_t)std::min<uint64_t>((n + 255) / 256, 16384);
+    k_copy2_u32<<<g, 256, 0, st>>>(k_in, v_in, n, n_dev, k_out, v_out);
+    return;
+  }
+  const uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+  uint32_t* h = static_cast<uint32_t*>(hist);
+  uint32_t* hs = reinterpret_cast<uint32_t*>(static_cast<char*>(hist) + align256(256ull * nb * 4));
+  const uint32_t* ks = k_in;
+  const uint32_t* vs = v_in;
+  for (uint32_t p = 0; p < passes; ++p) {
+    const bool last_to_out = ((passes - p) & 1u) == 1u;
+    uint32_t* kd = last_to_out ? k_out : k_tmp;
+    uint32_t* vd = last_to_out ? v_out : v_tmp;
+    k_radix_hist<<<nb, RS_THREADS, 0, st>>>(ks, n, 8 * p, h, nb, n_dev);
+    scan_exclusive_u32(h, hs, 256ull * nb, scan_tmp, st);
+    k_radix_scatter<<<nb, RS_THREADS, 0, st>>>(ks, vs, n, 8 * p, hs, nb, kd, vd, n_dev);
+    ks = kd;
+    vs = vd;
+  }
+}
+
+
 }  // namespace
 
 size_t scan_tmp_bytes(uint64_t n) {
@@ -183,31 +241,36 @@ size_t radix_hist_bytes(uint64_t n) {
   return align256(256 * nb * 4) + align256((256 * nb + 1) * 4);
 }
 
+
 void radix_sort_pairs(const uint32_t* k_in, const uint32_t* v_in, uint32_t* k_out,
                       uint32_t* v_out, uint32_t* k_tmp, uint32_t* v_tmp, uint64_t n,
                       uint32_t key_bits, void* hist, void* scan_tmp, hipStream_t st) {
-  if (n == 0) return;
-  const uint32_t passes = (key_bits + 7) / 8;
-  if (passes == 0) {
-    hipMemcpyAsync(k_out, k_in, n * 4, hipMemcpyDeviceToDevice, st);
-    hipMemcpyAsync(v_out, v_in, n * 4, hipMemcpyDeviceToDevice, st);
-    return;
-  }
-  const uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
-  uint32_t* h = static_cast<uint32_t*>(hist);
-  uint32_t* hs = reinterpret_cast<uint32_t*>(static_cast<char*>(hist) + align256(256ull * nb * 4));
-  const uint32_t* ks = k_in;
-  const uint32_t* vs = v_in;
-  for (uint32_t p = 0; p < passes; ++p) {
-    const bool last_to_out = ((passes - p) & 1u) == 1u;
-    uint32_t* kd = last_to_out ? k_out : k_tmp;
-    uint32_t* vd = last_to_out ? v_out : v_tmp;
-    k_radix_hist<<<nb, RS_THREADS, 0, st>>>(ks, n, 8 * p, h, nb);
-    scan_exclusive_u32(h, hs, 256ull * nb, scan_tmp, st);
-    k_radix_scatter<<<nb, RS_THREADS, 0, st>>>(ks, vs, n, 8 * p, hs, nb, kd, vd);
-    ks = kd;
-    vs = vd;
-  }
+  radix_sort_impl(k_in, v_in, k_out, v_out, k_tmp, v_tmp, n, nullptr, key_bits, hist, scan_tmp,
+                  st);
 }
+
+void radix_sort_pairs_devn(const uint32_t* k_in, const uint32_t* v_in, uint32_t* k_out,
+                           uint32_t* v_out, uint32_t* k_tmp, uint32_t* v_tmp,
+                           const uint32_t* n_dev, uint64_t n_max, uint32_t key_bits, void* hist,
+                           void* scan_tmp, hipStream_t st) {
+  radix_sort_impl(k_in, v_in, k_out, v_out, k_tmp, v_tmp, std::max<uint64_t>(n_max, 1), n_dev,
+                  key_bits, hist, scan_tmp, st);
+}
+
+void scan_exclusive_u32_devn(const uint32_t* in, uint32_t* out, const uint32_t* n_dev,
+                             uint64_t n_max, void* tmp, hipStream_t st) {
+  scan_impl<uint32_t>(in, out, std::max<uint64_t>(n_max, 1), tmp, st, n_dev);
+}
+
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st) {
+  if (!n) return;
+  k_fill<uint32_t><<<(uint32_t)std::min<uint64_t>((n + 255) / 256, 16384), 256, 0, st>>>(p, n, v);
+}
+
+void fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st) {
+  if (!n) return;
+  k_fill<uint64_t><<<(uint32_t)std::min<uint64_t>((n + 255) / 256, 16384), 256, 0, st>>>(p, n, v);
+}
+
 
 }  // namespace cytom

@@ -769,7 +769,9 @@ int fill_rows(cytom_ctx* ctx, QueryEngine* qe, uint64_t E) {
   int rc;
   if (ctx->jobs_dirty && (rc = build_jobs(ctx))) return rc;
   if (E > 0xFFFFFFFFull) return set_err(ctx, CYTOM_ERANGE, "snapshot above 2^32 edges");
-  const uint64_t nc = ctx->n_chunks;
+  const int64_t ncr = jobs_chunk_count(ctx);
+  if (ncr < 0) return (int)ncr;
+  const uint64_t nc = (uint64_t)ncr;
   uint32_t bits = 0;
   while ((1ull << bits) < ctx->V) ++bits;
   uint64_t *live = nullptr, *base = nullptr;

@@ -30,17 +30,29 @@ namespace {
 constexpr uint64_t EMPTY64 = ~0ull;
 constexpr uint32_t NONE32 = 0xFFFFFFFFu;
 
+// Control words of a batch in d_misc (all on the device: the batch runs
+// without host round trips and the host reads them once at the end).
 enum Misc : int {
   M_POOL_TOP = 0,
-  M_ERR = 1,
-  M_NEED = 2,
+  M_ERR = 1,       // bit 0: invalid insertion, bit 1: invalid deletion
+  M_NEED = 2,      // slots the relocated tiles need
   M_NRELOC = 3,
   M_RELOC_CUR = 4,
-  M_NFOUND = 5,
+  M_NFOUND = 5,    // deletions that matched a live edge
   M_NCOMPACT = 6,
-  M_RS_CUT = 7,  // first selected tile (by rank) left for a later sort-compaction
-  M_REEXEC = 8,  // one flag per program from here
+  M_RS_CUT = 7,    // first selected tile (by rank) left for a later sort-compaction
+  M_OVERFLOW = 8,  // the relocations do not fit the pool: the batch is not applied
+  M_RS_NPROC = 9,  // tiles sort-compacted by this batch
+  M_RS_NSLOTS = 10,
+  M_POOL_SIZE = 11,
 };
+
+// A batch with an invalid update, or one whose relocations overflow the
+// pool, modifies nothing: every kernel that writes the store, the degrees or
+// a program checks this first (uniform per launch).
+__device__ inline bool batch_dead(const unsigned long long* misc) {
+  return (misc[M_ERR] | misc[M_OVERFLOW]) != 0;
+}
 
 __host__ __device__ inline uint32_t grow_cap(uint64_t n) {
   uint64_t c = n + n / 2 + 64;
@@ -118,11 +130,18 @@ __global__ void k_plan(const uint32_t* __restrict__ nseg_p, const uint32_t* __re
   }
 }
 
+// misc[M_OVERFLOW] := the relocations of this batch do not fit the pool.
+__global__ void k_check_pool(unsigned long long* misc) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && !misc[M_ERR])
+    misc[M_OVERFLOW] = misc[M_POOL_TOP] + misc[M_NEED] > misc[M_POOL_SIZE] ? 1ull : 0ull;
+}
+
 __global__ void k_reloc_commit(const uint32_t* __restrict__ nseg_p,
                                const uint32_t* __restrict__ seg_tile,
                                const uint32_t* __restrict__ reloc_cap,
                                unsigned long long* __restrict__ reloc_list,
                                unsigned long long* misc) {
+  if (batch_dead(misc)) return;
   const uint32_t nseg = *nseg_p;
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x) {
     const uint32_t nc = reloc_cap[s];
@@ -176,7 +195,10 @@ __device__ inline uint32_t copy_live(const uint32_t* __restrict__ pool_in,
 }
 
 __global__ void __launch_bounds__(CP_THREADS) k_relocate(const unsigned long long* __restrict__ rl,
-                                                         uint32_t nreloc, StoreView S) {
+                                                         const unsigned long long* misc,
+                                                         StoreView S) {
+  if (batch_dead(misc)) return;
+  const uint32_t nreloc = (uint32_t)misc[M_RELOC_CUR];
   for (uint32_t b = blockIdx.x; b < nreloc; b += gridDim.x) {
     const uint32_t t = (uint32_t)rl[3 * b];
     const uint64_t noff = rl[3 * b + 1];
@@ -228,7 +250,8 @@ __global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __r
                           const uint32_t* __restrict__ sx, const uint32_t* __restrict__ seg_start,
                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                           const uint32_t* __restrict__ w, StoreView S,
-                          uint64_t* __restrict__ ins_slot) {
+                          uint64_t* __restrict__ ins_slot, const unsigned long long* misc) {
+  if (batch_dead(misc)) return;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t seg = sx[i] + f[i] - 1u;
@@ -247,7 +270,9 @@ __global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __r
 
 __global__ void k_seg_commit(const uint32_t* __restrict__ nseg_p,
                              const uint32_t* __restrict__ seg_start,
-                             const uint32_t* __restrict__ seg_tile, StoreView S) {
+                             const uint32_t* __restrict__ seg_tile, StoreView S,
+                             const unsigned long long* misc) {
+  if (batch_dead(misc)) return;
   const uint32_t nseg = *nseg_p;
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x) {
     const uint32_t len = seg_start[s + 1] - seg_start[s];
@@ -334,7 +359,9 @@ __global__ void __launch_bounds__(256) k_del_match(const uint64_t* __restrict__ 
                                                    const uint32_t* __restrict__ node_next,
                                                    const uint32_t* __restrict__ dw,
                                                    const unsigned long long* __restrict__ tile_sig,
-                                                   unsigned long long* __restrict__ claimed) {
+                                                   unsigned long long* __restrict__ claimed,
+                                                   const unsigned long long* misc) {
+  if (batch_dead(misc)) return;
   const uint64_t hmask = (1ull << log_h) - 1;
   const uint32_t nchunks = *nchunks_p;
   const int lane = threadIdx.x & 63;
@@ -391,6 +418,7 @@ __global__ void __launch_bounds__(256) k_del_match(const uint64_t* __restrict__ 
 __global__ void k_del_finish(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                              uint64_t m, const unsigned long long* __restrict__ claimed,
                              StoreView S, uint32_t* __restrict__ found, unsigned long long* misc) {
+  if (batch_dead(misc)) return;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const unsigned long long c = claimed[i];
@@ -411,7 +439,10 @@ __global__ void k_del_finish(const uint32_t* __restrict__ src, const uint32_t* _
 // meta-tile-manager.h:200-228, leave the degrees alone, tile-manager.h:175-189),
 // so rank / out_degree never divides by zero there.
 __global__ void k_del_degrees(const uint32_t* __restrict__ found, uint64_t n,
-                              uint32_t* __restrict__ outdeg, uint32_t* __restrict__ indeg) {
+                              uint32_t* __restrict__ outdeg, uint32_t* __restrict__ indeg,
+                              const unsigned long long* misc) {
+  if (batch_dead(misc)) return;
+  n = min(n, (uint64_t)misc[M_NFOUND]);
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (uint64_t)gridDim.x * blockDim.x) {
     atomicSub(&outdeg[found[3 * k + 0]], 1u);
@@ -424,6 +455,7 @@ __global__ void k_del_degrees(const uint32_t* __restrict__ found, uint64_t n,
 __global__ void __launch_bounds__(CP_THREADS) k_compact(const uint32_t* __restrict__ tiles,
                                                         const uint32_t* __restrict__ ntiles_p,
                                                         StoreView S, unsigned long long* misc) {
+  if (batch_dead(misc)) return;
   const uint32_t ntiles = *ntiles_p;
   for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
     const uint32_t t = tiles[b];
@@ -450,26 +482,44 @@ __global__ void __launch_bounds__(CP_THREADS) k_compact(const uint32_t* __restri
 // enough, which bounds the rewrite work per inserted edge by a geometric sum.
 constexpr uint32_t RESORT_MIN_TAIL = 256;
 
-__global__ void k_resort_flag(StoreView S, uint32_t pct, uint32_t* __restrict__ flag,
+// A tile larger than the slot budget cap is never selected (it would stage
+// past the buffers and block every later pass).
+__global__ void k_resort_flag(StoreView S, uint32_t pct, uint64_t cap, uint32_t* __restrict__ flag,
                               uint64_t* __restrict__ size) {
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < S.n_tiles;
        t += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t cnt = S.tile_cnt[t], s = S.tile_sorted[t], tail = cnt - s;
-    const bool f = tail >= RESORT_MIN_TAIL && (uint64_t)tail * 100 >= (uint64_t)s * pct;
+    const bool f = tail >= RESORT_MIN_TAIL && (uint64_t)tail * 100 >= (uint64_t)s * pct &&
+                   cnt <= cap;
     flag[t] = f ? 1u : 0u;
     size[t] = f ? cnt : 0u;
   }
 }
 
-// Selected tiles past the rank or slot budget of this pass: the smallest
-// such rank is the cut (ranks and offsets grow together).
+// Selected tiles past the rank budget of this pass, or whose slots would end
+// past the slot budget: the smallest such rank is the cut (ranks and offsets
+// grow together).
 __global__ void k_resort_cut(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rank,
-                             const uint64_t* __restrict__ off, uint64_t n_tiles, uint32_t rmax,
-                             uint64_t cap, unsigned long long* misc) {
+                             const uint64_t* __restrict__ off, const uint64_t* __restrict__ size,
+                             uint64_t n_tiles, uint32_t rmax, uint64_t cap,
+                             unsigned long long* misc) {
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles;
        t += (uint64_t)gridDim.x * blockDim.x)
-    if (flag[t] && (rank[t] >= rmax || off[t] >= cap))
+    if (flag[t] && (rank[t] >= rmax || off[t] + size[t] > cap))
       atomicMin(&misc[M_RS_CUT], (unsigned long long)rank[t]);
+}
+
+// Tiles and slots of this pass: the selected tiles before the cut.
+__global__ void k_resort_count(const uint32_t* __restrict__ n_sel_p,
+                               const uint64_t* __restrict__ off,
+                               const uint32_t* __restrict__ list, uint64_t n_tiles,
+                               unsigned long long* misc) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const uint32_t n_sel = *n_sel_p;
+  const uint64_t cut = misc[M_RS_CUT];
+  const uint32_t nproc = cut < n_sel ? (uint32_t)cut : n_sel;
+  misc[M_RS_NPROC] = nproc;
+  misc[M_RS_NSLOTS] = nproc < n_sel ? off[list[nproc]] : off[n_tiles];
 }
 
 // The staging copy of the selected tiles is cut into RS_BLOCK-slot pieces,
@@ -479,10 +529,12 @@ __global__ void k_resort_cut(const uint32_t* __restrict__ flag, const uint32_t* 
 constexpr uint32_t RS_BLOCK = 1024;
 constexpr uint32_t RS_TILES = RS_BLOCK / RESORT_MIN_TAIL + 2;  // tiles one piece can touch
 
-__global__ void k_resort_pieces(const uint32_t* __restrict__ list, uint32_t nproc,
+__global__ void k_resort_pieces(const uint32_t* __restrict__ list,
+                                const unsigned long long* __restrict__ misc,
                                 const uint64_t* __restrict__ off,
                                 const uint32_t* __restrict__ tile_cnt,
                                 uint32_t* __restrict__ bfirst) {
+  const uint32_t nproc = (uint32_t)misc[M_RS_NPROC];
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nproc;
        r += gridDim.x * blockDim.x) {
     const uint32_t t = list[r];
@@ -494,9 +546,9 @@ __global__ void k_resort_pieces(const uint32_t* __restrict__ list, uint32_t npro
 // Keys (rank, tombstone bit, source) and staging copies of one piece of
 // selected slots per block.
 __global__ void __launch_bounds__(256) k_resort_gather(StoreView S, const uint32_t* __restrict__ list,
-                                                       uint32_t nproc, const uint64_t* __restrict__ off,
+                                                       const unsigned long long* __restrict__ misc,
+                                                       const uint64_t* __restrict__ off,
                                                        const uint32_t* __restrict__ bfirst,
-                                                       uint32_t nslots,
                                                        uint32_t* __restrict__ keys,
                                                        uint32_t* __restrict__ vals,
                                                        uint32_t* __restrict__ e_st,
@@ -505,6 +557,8 @@ __global__ void __launch_bounds__(256) k_resort_gather(StoreView S, const uint32
   __shared__ uint64_t s_src[RS_TILES];
   __shared__ uint32_t s_n;
   const uint32_t lv = S.log2vpt;
+  const uint32_t nproc = (uint32_t)misc[M_RS_NPROC];
+  const uint32_t nslots = (uint32_t)misc[M_RS_NSLOTS];
   for (uint32_t b = blockIdx.x; (uint64_t)b * RS_BLOCK < nslots; b += gridDim.x) {
     const uint32_t k0 = b * RS_BLOCK, k1 = min(k0 + RS_BLOCK, nslots);
     if (threadIdx.x < RS_TILES) {
@@ -552,10 +606,13 @@ __global__ void __launch_bounds__(256) k_resort_gather(StoreView S, const uint32
 __global__ void k_resort_scatter(StoreView S, const uint32_t* __restrict__ list,
                                  const uint64_t* __restrict__ off,
                                  const uint32_t* __restrict__ skeys,
-                                 const uint32_t* __restrict__ svals, uint32_t n,
+                                 const uint32_t* __restrict__ svals,
+                                 const unsigned long long* __restrict__ misc,
                                  const uint32_t* __restrict__ e_st,
                                  const uint32_t* __restrict__ w_st) {
+  if (batch_dead(misc)) return;
   const uint32_t lv = S.log2vpt;
+  const uint32_t n = (uint32_t)misc[M_RS_NSLOTS];
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const uint32_t key = skeys[k];
     if (key & (1u << lv)) continue;
@@ -567,7 +624,10 @@ __global__ void k_resort_scatter(StoreView S, const uint32_t* __restrict__ list,
   }
 }
 
-__global__ void k_resort_commit(StoreView S, const uint32_t* __restrict__ list, uint32_t nproc) {
+__global__ void k_resort_commit(StoreView S, const uint32_t* __restrict__ list,
+                                const unsigned long long* __restrict__ misc) {
+  if (batch_dead(misc)) return;
+  const uint32_t nproc = (uint32_t)misc[M_RS_NPROC];
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nproc; r += gridDim.x * blockDim.x) {
     const uint32_t t = list[r];
     const uint32_t live = S.tile_live[t];
@@ -581,8 +641,6 @@ inline uint32_t grid_for(uint64_t n, uint32_t threads, uint32_t cap = 65536) {
   if (b == 0) b = 1;
   return (uint32_t)std::min<uint64_t>(b, cap);
 }
-
-inline size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
 
 // Repack every tile into a new pool of at least min_slots slots.
 int repack(cytom_ctx* ctx, uint64_t min_slots) {
@@ -620,117 +678,113 @@ int repack(cytom_ctx* ctx, uint64_t min_slots) {
   return 0;
 }
 
-// Sort-compaction of the tiles k_resort_flag selects, at most
-// resort_max_slots staged slots (plus one tile) per pass; tiles past the
-// budget wait for the next batch.
+// Slots staged per sort-compaction pass: the budget, bounded by the pool.
+uint64_t resort_cap(const cytom_ctx* ctx) {
+  return std::min<uint64_t>(ctx->resort_max_slots, ctx->pool_size);
+}
 
-int resort_tiles(cytom_ctx* ctx, uint32_t* n_sorted) {
-  *n_sorted = 0;
+int resort_prepare(cytom_ctx* ctx) {
   if (ctx->resort_pct == CYTOM_RESORT_OFF) return 0;
-  hipStream_t st = ctx->stream;
-  const uint64_t nt = ctx->n_tiles;
-  const uint32_t lv = ctx->log2vpt;
+  const uint64_t nt = ctx->n_tiles, cap = std::max<uint64_t>(resort_cap(ctx), 1);
   int rc;
   if ((rc = ensure(ctx, ctx->g_tile_flag, nt * 4)) ||
       (rc = ensure(ctx, ctx->g_tile_scan, (nt + 1) * 16 + 256)) ||
       (rc = ensure(ctx, ctx->g_rs_off, (2 * nt + 1) * 8)) ||
-      (rc = ensure(ctx, ctx->g_del_tiles, nt * 4)) ||
-      (rc = ensure(ctx, ctx->g_scan_tmp, scan_tmp_bytes(nt + 1) + 256)))
+      (rc = ensure(ctx, ctx->g_del_tiles, nt * 4)) || (rc = ensure(ctx, ctx->g_keys, cap * 4)) ||
+      (rc = ensure(ctx, ctx->g_vals, cap * 4)) || (rc = ensure(ctx, ctx->g_keys2, cap * 4)) ||
+      (rc = ensure(ctx, ctx->g_vals2, cap * 4)) || (rc = ensure(ctx, ctx->g_flags, cap * 4)) ||
+      (rc = ensure(ctx, ctx->g_scan, cap * 4)) || (rc = ensure(ctx, ctx->g_rs_e, cap * 4)) ||
+      (rc = ensure(ctx, ctx->g_rs_w, ctx->wpool ? cap * 4 : 4)) ||
+      (rc = ensure(ctx, ctx->g_hist, radix_hist_bytes(cap))) ||
+      (rc = ensure(ctx, ctx->g_scan_tmp,
+                   scan_tmp_bytes(std::max<uint64_t>({cap, 256 * ((cap + 4095) / 4096) + 1,
+                                                      nt + 1})) +
+                       256)))
     return rc;
+  return 0;
+}
+
+// Sort-compaction of the tiles k_resort_flag selects, at most the slot
+// budget per pass (tiles past it wait for a later batch), device-driven.
+void resort_enqueue(cytom_ctx* ctx) {
+  if (ctx->resort_pct == CYTOM_RESORT_OFF) return;
+  hipStream_t st = ctx->stream;
+  const uint64_t nt = ctx->n_tiles, cap = std::max<uint64_t>(resort_cap(ctx), 1);
+  const uint32_t lv = ctx->log2vpt;
   auto* flag = static_cast<uint32_t*>(ctx->g_tile_flag.p);
   auto* rank = static_cast<uint32_t*>(ctx->g_tile_scan.p);
   auto* size = static_cast<uint64_t*>(ctx->g_rs_off.p);
   uint64_t* off = size + nt;
   auto* list = static_cast<uint32_t*>(ctx->g_del_tiles.p);
-  k_resort_flag<<<grid_for(nt, 256), 256, 0, st>>>(ctx->view(), ctx->resort_pct, flag, size);
+  unsigned long long* misc = ctx->d_misc;
+  k_resort_flag<<<grid_for(nt, 256), 256, 0, st>>>(ctx->view(), ctx->resort_pct, cap, flag,
+                                                   size);
   scan_exclusive_u32(flag, rank, nt, ctx->g_scan_tmp.p, st);
   scan_exclusive_u64(size, off, nt, ctx->g_scan_tmp.p, st);
   k_flag_emit<<<grid_for(nt, 256), 256, 0, st>>>(flag, rank, nt, list);
-  ctx->h_scalar[0] = 0;
-  CY_HIP(ctx, hipMemcpyAsync(ctx->h_scalar, rank + nt, 4, hipMemcpyDeviceToHost, st));
-  CY_HIP(ctx, hipMemcpyAsync(ctx->h_scalar + 1, off + nt, 8, hipMemcpyDeviceToHost, st));
-  CY_HIP(ctx, hipStreamSynchronize(st));
-  const uint32_t n_sel = (uint32_t)ctx->h_scalar[0];
-  const uint64_t total = ctx->h_scalar[1];
-  if (n_sel == 0) return 0;
   // Keys are (rank, tombstone bit, lv source bits) in 32 bits.
   const uint32_t rmax = 1u << (31 - lv);
-  uint32_t nproc = n_sel;
-  uint64_t nslots = total;
-  if (n_sel > rmax || total > ctx->resort_max_slots) {
-    CY_HIP(ctx, hipMemsetAsync(ctx->d_misc + M_RS_CUT, 0xFF, 8, st));
-    k_resort_cut<<<grid_for(nt, 256), 256, 0, st>>>(flag, rank, off, nt, rmax,
-                                                    ctx->resort_max_slots, ctx->d_misc);
-    unsigned long long cut = 0;
-    CY_HIP(ctx, hipMemcpyAsync(&cut, ctx->d_misc + M_RS_CUT, 8, hipMemcpyDeviceToHost, st));
-    CY_HIP(ctx, hipStreamSynchronize(st));
-    nproc = (uint32_t)std::min<unsigned long long>(cut, n_sel);
-    if (nproc < n_sel) {
-      uint32_t t_cut = 0;
-      CY_HIP(ctx, hipMemcpy(&t_cut, list + nproc, 4, hipMemcpyDeviceToHost));
-      CY_HIP(ctx, hipMemcpy(&nslots, off + t_cut, 8, hipMemcpyDeviceToHost));
-    }
-  }
-  if (nslots > 0xFFFFFFFFull) return set_err(ctx, CYTOM_ERANGE, "sort-compaction pass too large");
-  if ((rc = ensure(ctx, ctx->g_keys, nslots * 4)) || (rc = ensure(ctx, ctx->g_vals, nslots * 4)) ||
-      (rc = ensure(ctx, ctx->g_keys2, nslots * 4)) ||
-      (rc = ensure(ctx, ctx->g_vals2, nslots * 4)) ||
-      (rc = ensure(ctx, ctx->g_flags, nslots * 4)) || (rc = ensure(ctx, ctx->g_scan, nslots * 4)) ||
-      (rc = ensure(ctx, ctx->g_rs_e, nslots * 4)) ||
-      (rc = ensure(ctx, ctx->g_rs_w, ctx->wpool ? nslots * 4 : 4)) ||
-      (rc = ensure(ctx, ctx->g_hist, radix_hist_bytes(nslots))) ||
-      (rc = ensure(ctx, ctx->g_scan_tmp,
-                   scan_tmp_bytes(std::max<uint64_t>({nslots, 256 * ((nslots + 4095) / 4096) + 1,
-                                                      nt + 1})) +
-                       256)))
-    return rc;
+  fill_u64(reinterpret_cast<uint64_t*>(misc + M_RS_CUT), 1, ~0ull, st);
+  k_resort_cut<<<grid_for(nt, 256), 256, 0, st>>>(flag, rank, off, size, nt, rmax, cap, misc);
+  k_resort_count<<<1, 64, 0, st>>>(rank + nt, off, list, nt, misc);
+  const uint64_t max_proc = std::min<uint64_t>(nt, rmax);
+  uint32_t rank_bits = 0;
+  while ((1ull << rank_bits) < max_proc) ++rank_bits;
+  const uint64_t pieces = (cap + RS_BLOCK - 1) / RS_BLOCK;
   auto* keys = static_cast<uint32_t*>(ctx->g_keys.p);
   auto* vals = static_cast<uint32_t*>(ctx->g_vals.p);
   auto* skeys = static_cast<uint32_t*>(ctx->g_keys2.p);
   auto* svals = static_cast<uint32_t*>(ctx->g_vals2.p);
   auto* e_st = static_cast<uint32_t*>(ctx->g_rs_e.p);
   auto* w_st = static_cast<uint32_t*>(ctx->g_rs_w.p);
-  const uint32_t pieces = (uint32_t)((nslots + RS_BLOCK - 1) / RS_BLOCK);
   auto* bfirst = static_cast<uint32_t*>(ctx->g_flags.p);  // free until the sort below
-  k_resort_pieces<<<grid_for(nproc, 256), 256, 0, st>>>(list, nproc, off, ctx->tile_cnt, bfirst);
-  k_resort_gather<<<std::min<uint32_t>(pieces, 16384), 256, 0, st>>>(
-      ctx->view(), list, nproc, off, bfirst, (uint32_t)nslots, keys, vals, e_st, w_st);
-  uint32_t rank_bits = 0;
-  while ((1ull << rank_bits) < nproc) ++rank_bits;
-  radix_sort_pairs(keys, vals, skeys, svals, static_cast<uint32_t*>(ctx->g_flags.p),
-                   static_cast<uint32_t*>(ctx->g_scan.p), nslots, rank_bits + lv + 1,
-                   ctx->g_hist.p, ctx->g_scan_tmp.p, st);
-  k_resort_scatter<<<grid_for(nslots, 256), 256, 0, st>>>(ctx->view(), list, off, skeys, svals,
-                                                          (uint32_t)nslots, e_st, w_st);
-  k_resort_commit<<<grid_for(nproc, 256), 256, 0, st>>>(ctx->view(), list, nproc);
-  ctx->tiles_resorted += nproc;
-  ctx->slots_resorted += nslots;
-  *n_sorted = nproc;
-  return 0;
+  // The pass's counts are 32-bit values in the low half of their control
+  // words: the sort reads them through 32-bit pointers.
+  const auto* nslots_dev = reinterpret_cast<const uint32_t*>(misc + M_RS_NSLOTS);
+  k_resort_pieces<<<grid_for(max_proc, 256), 256, 0, st>>>(list, misc, off, ctx->tile_cnt, bfirst);
+  k_resort_gather<<<(uint32_t)std::min<uint64_t>(pieces, 16384), 256, 0, st>>>(
+      ctx->view(), list, misc, off, bfirst, keys, vals, e_st, w_st);
+  radix_sort_pairs_devn(keys, vals, skeys, svals, static_cast<uint32_t*>(ctx->g_flags.p),
+                        static_cast<uint32_t*>(ctx->g_scan.p), nslots_dev, cap,
+                        rank_bits + lv + 1, ctx->g_hist.p, ctx->g_scan_tmp.p, st);
+  k_resort_scatter<<<grid_for(cap, 256), 256, 0, st>>>(ctx->view(), list, off, skeys, svals, misc,
+                                                       e_st, w_st);
+  k_resort_commit<<<grid_for(max_proc, 256), 256, 0, st>>>(ctx->view(), list, misc);
+}
+
+// Per program: the batch's decision, written to gate[p] = 1 (re-initialise),
+// 0 (insert hooks) or 2 (the batch was refused: neither).  force: the host's
+// rule alone decides re-initialisation (PPR with inserts); found_forces: any
+// found deletion re-initialises (BFS/CC/SSSP whose values are not a
+// converged fixpoint, PPR); otherwise a tight deletion (flag[p]) does.
+__global__ void k_decide(const unsigned long long* __restrict__ misc,
+                         const unsigned long long* __restrict__ flag,
+                         const uint32_t* __restrict__ force,
+                         const uint32_t* __restrict__ found_forces, uint32_t np,
+                         unsigned long long* __restrict__ gate) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  if (batch_dead(misc)) {
+    gate[p] = 2;
+    return;
+  }
+  const bool found = misc[M_NFOUND] != 0;
+  gate[p] = (force[p] || (found && (flag[p] || found_forces[p]))) ? 1ull : 0ull;
 }
 
 }  // namespace
 
-int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
-                 const uint32_t* d_w, uint64_t n_ins, const uint32_t* d_dsrc,
-                 const uint32_t* d_ddst, const uint32_t* d_dw, uint64_t n_del,
-                 cytom_batch_stats* bst) {
-  hipStream_t st = ctx->stream;
+// Sizes every scratch buffer of a batch of n_ins insertions and n_del
+// deletions (host only: may allocate, so it runs before any capture).
+int ingest_prepare(cytom_ctx* ctx, uint64_t n_ins, uint64_t n_del) {
   int rc;
   if (n_ins > 0xFFFFFFFEull || n_del > 0xFFFFFFFEull)
     return set_err(ctx, CYTOM_ERANGE, "batch larger than 2^32-2 updates");
-  if (ctx->weighted && n_ins && !d_w)
-    return set_err(ctx, CYTOM_EINVAL, "weighted graph needs insertion weights");
-  const uint32_t lv = ctx->log2vpt, T = ctx->T;
-  const uint32_t tile_bits = [&] {
-    uint32_t b = 0;
-    while ((1ull << b) < ctx->n_tiles) ++b;
-    return b;
-  }();
-  // Source bits that fit beside the tile id in a 32-bit key.
-  const uint32_t sb = std::min<uint32_t>(lv, 32 - tile_bits);
-  const uint32_t key_bits = tile_bits + sb;
-  const uint64_t nmax = std::max<uint64_t>(n_ins, 1);
+  const uint64_t nmax = std::max<uint64_t>(n_ins, 1), dmax = std::max<uint64_t>(n_del, 1);
+  uint64_t H = 64;
+  while (H < 2 * dmax) H <<= 1;
+  const uint64_t max_del_chunks = ctx->pool_size / DEL_CH + ctx->n_tiles + 1;
+  const uint64_t np = std::max<uint64_t>(ctx->progs.size(), 1);
   if ((rc = ensure(ctx, ctx->g_keys, nmax * 4)) || (rc = ensure(ctx, ctx->g_vals, nmax * 4)) ||
       (rc = ensure(ctx, ctx->g_keys2, nmax * 4)) || (rc = ensure(ctx, ctx->g_vals2, nmax * 4)) ||
       (rc = ensure(ctx, ctx->g_flags, nmax * 4)) ||
@@ -743,9 +797,47 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
       (rc = ensure(ctx, ctx->g_hist, radix_hist_bytes(nmax))) ||
       (rc = ensure(ctx, ctx->g_scan_tmp,
                    scan_tmp_bytes(std::max<uint64_t>({nmax, 256 * ((nmax + 4095) / 4096) + 1,
-                                                      ctx->n_tiles})) +
-                       256)))
+                                                      ctx->n_tiles + 1})) +
+                       256)) ||
+      (rc = ensure(ctx, ctx->g_tab_key, H * 8)) || (rc = ensure(ctx, ctx->g_tab_head, H * 4)) ||
+      (rc = ensure(ctx, ctx->g_node_next, dmax * 4)) ||
+      (rc = ensure(ctx, ctx->g_claimed, dmax * 8)) ||
+      (rc = ensure(ctx, ctx->g_tile_flag, ctx->n_tiles * 4)) ||
+      (rc = ensure(ctx, ctx->g_del_sig, ctx->n_tiles * 32)) ||
+      (rc = ensure(ctx, ctx->g_tile_scan, (ctx->n_tiles + 1) * 16 + 256)) ||
+      (rc = ensure(ctx, ctx->g_del_tiles, ctx->n_tiles * 4)) ||
+      (rc = ensure(ctx, ctx->g_found, dmax * 12)) ||
+      (rc = ensure(ctx, ctx->g_nch, ctx->n_tiles * 4)) ||
+      (rc = ensure(ctx, ctx->g_cbase, (ctx->n_tiles + 1) * 4)) ||
+      (rc = ensure(ctx, ctx->g_chunks, max_del_chunks * 8)) ||
+      (rc = ensure(ctx, ctx->g_pflags, np * 24)) || (rc = resort_prepare(ctx)))
     return rc;
+  // The programs' host-side rules of this batch.
+  auto* hp = reinterpret_cast<uint32_t*>(ctx->h_pflags);
+  for (size_t p = 0; p < ctx->progs.size(); ++p) {
+    const Program& P = ctx->progs[p];
+    hp[p] = (P.kind == CYTOM_PPR && n_ins) ? 1u : 0u;                        // force
+    hp[64 + p] = ((!prd_like(P.kind) && !P.fixpoint) || P.kind == CYTOM_PPR) ? 1u : 0u;
+  }
+  return 0;
+}
+
+// Launches one batch (no host reads; the counts the host used to read stay
+// on the device).  The pinned rules written by ingest_prepare are copied in
+// first, so a captured graph picks up each replay's rules.
+void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
+                    const uint32_t* d_w, uint64_t n_ins, const uint32_t* d_dsrc,
+                    const uint32_t* d_ddst, const uint32_t* d_dw, uint64_t n_del) {
+  hipStream_t st = ctx->stream;
+  const uint32_t lv = ctx->log2vpt, T = ctx->T;
+  const uint32_t tile_bits = [&] {
+    uint32_t b = 0;
+    while ((1ull << b) < ctx->n_tiles) ++b;
+    return b;
+  }();
+  // Source bits that fit beside the tile id in a 32-bit key.
+  const uint32_t sb = std::min<uint32_t>(lv, 32 - tile_bits);
+  const uint32_t key_bits = tile_bits + sb;
   uint32_t* keys = static_cast<uint32_t*>(ctx->g_keys.p);
   uint32_t* vals = static_cast<uint32_t*>(ctx->g_vals.p);
   uint32_t* skeys = static_cast<uint32_t*>(ctx->g_keys2.p);
@@ -758,24 +850,27 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
   unsigned long long* reloc_list = static_cast<unsigned long long*>(ctx->g_reloc_list.p);
   uint64_t* ins_slot = static_cast<uint64_t*>(ctx->g_ins_slot.p);
   unsigned long long* misc = ctx->d_misc;
+  const size_t np = ctx->progs.size();
+  auto* pf = static_cast<unsigned long long*>(ctx->g_pflags.p);  // [np] tight flags
+  unsigned long long* gate = pf + np;                             // [np] decisions
+  auto* rules = reinterpret_cast<uint32_t*>(gate + np);           // [np] force, [np] found_forces
 
-  CY_HIP(ctx, hipEventRecord(ctx->ev[0], st));
-  // misc[1..63] := 0 (the pool top in misc[0] persists)
-  CY_HIP(ctx, hipMemsetAsync(misc + 1, 0, 63 * 8, st));
-  // 1. validation (nothing is modified before it passes)
+  (void)hipEventRecord(ctx->ev[0], st);
+  // misc[1..] := 0 (the pool top in misc[0] persists), the pool size, rules
+  fill_u64(reinterpret_cast<uint64_t*>(misc + 1), 63, 0, st);
+  fill_u64(reinterpret_cast<uint64_t*>(misc + M_POOL_SIZE), 1, ctx->pool_size, st);
+  if (np) {
+    (void)hipMemcpyAsync(rules, ctx->h_pflags, np * 4, hipMemcpyHostToDevice, st);
+    (void)hipMemcpyAsync(rules + np, reinterpret_cast<uint32_t*>(ctx->h_pflags) + 64, np * 4,
+                         hipMemcpyHostToDevice, st);
+    fill_u64(reinterpret_cast<uint64_t*>(pf), np, 0, st);
+  }
+  // 1. validation (a refused batch modifies nothing)
   if (n_ins)
     k_keys<<<grid_for(n_ins, 256), 256, 0, st>>>(d_src, d_dst, d_w, n_ins, ctx->V, lv, T, sb,
                                                  ctx->weighted ? 1 : 0, keys, vals, misc);
   if (n_del)
     k_del_validate<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, ctx->V, misc);
-  CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st));
-  CY_HIP(ctx, hipStreamSynchronize(st));
-  if (ctx->h_misc[M_ERR]) {
-    return set_err(ctx, CYTOM_EINVAL,
-                   (ctx->h_misc[M_ERR] & 1) ? "insertion with out-of-range vertex or zero weight"
-                                            : "deletion with out-of-range vertex");
-  }
-  uint32_t nreloc = 0;
   // 2. insertions
   if (n_ins) {
     // flags / sx are free until the segment pass: they serve as the sort's
@@ -787,34 +882,22 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
     k_seg_emit<<<grid_for(n_ins, 256), 256, 0, st>>>(skeys, flags, sx, n_ins, sb, seg_start,
                                                      seg_tile);
     const uint32_t* nseg_p = sx + n_ins;
-    for (int attempt = 0;; ++attempt) {
-      CY_HIP(ctx, hipMemsetAsync(misc + M_NEED, 0, 16, st));
-      k_plan<<<grid_for(n_ins, 256), 256, 0, st>>>(nseg_p, seg_start, seg_tile, ctx->view(),
-                                                   reloc_cap, misc);
-      CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st));
-      CY_HIP(ctx, hipStreamSynchronize(st));
-      const uint64_t need = ctx->h_misc[M_NEED], top = ctx->h_misc[M_POOL_TOP];
-      nreloc = (uint32_t)ctx->h_misc[M_NRELOC];
-      if (top + need <= ctx->pool_size) break;
-      if (attempt > 0) return set_err(ctx, CYTOM_ENOMEM, "edge pool exhausted after repack");
-      if ((rc = repack(ctx, std::max<uint64_t>(2 * ctx->pool_size,
-                                               2 * (ctx->live_edges + n_ins) + need))))
-        return rc;
-    }
-    if (nreloc) {
-      k_reloc_commit<<<grid_for(n_ins, 256), 256, 0, st>>>(nseg_p, seg_tile, reloc_cap,
-                                                           reloc_list, misc);
-      k_relocate<<<std::min<uint32_t>(nreloc, 8192), CP_THREADS, 0, st>>>(reloc_list, nreloc,
-                                                                         ctx->view());
-    }
+    k_plan<<<grid_for(n_ins, 256), 256, 0, st>>>(nseg_p, seg_start, seg_tile, ctx->view(),
+                                                 reloc_cap, misc);
+    k_check_pool<<<1, 64, 0, st>>>(misc);
+    k_reloc_commit<<<grid_for(n_ins, 256), 256, 0, st>>>(nseg_p, seg_tile, reloc_cap,
+                                                         reloc_list, misc);
+    k_relocate<<<(uint32_t)std::min<uint64_t>(n_ins, 8192), CP_THREADS, 0, st>>>(
+        reloc_list, misc, ctx->view());
     k_scatter<<<grid_for(n_ins, 256), 256, 0, st>>>(skeys, svals, n_ins, sb, flags, sx, seg_start,
-                                                    d_src, d_dst, d_w, ctx->view(), ins_slot);
-    k_seg_commit<<<grid_for(n_ins, 256), 256, 0, st>>>(nseg_p, seg_start, seg_tile, ctx->view());
+                                                    d_src, d_dst, d_w, ctx->view(), ins_slot,
+                                                    misc);
+    k_seg_commit<<<grid_for(n_ins, 256), 256, 0, st>>>(nseg_p, seg_start, seg_tile, ctx->view(),
+                                                       misc);
   }
   // 3. deletions
-  uint64_t n_found = 0;
-  uint32_t* found = nullptr;
-  uint32_t* del_tiles = nullptr;
+  uint32_t* found = static_cast<uint32_t*>(ctx->g_found.p);
+  uint32_t* del_tiles = static_cast<uint32_t*>(ctx->g_del_tiles.p);
   uint32_t* n_del_tiles_p = nullptr;
   if (n_del) {
     uint64_t H = 64;
@@ -823,20 +906,6 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
       H <<= 1;
       ++log_h;
     }
-    const uint64_t max_chunks = ctx->pool_size / DEL_CH + ctx->n_tiles + 1;
-    if ((rc = ensure(ctx, ctx->g_tab_key, H * 8)) || (rc = ensure(ctx, ctx->g_tab_head, H * 4)) ||
-        (rc = ensure(ctx, ctx->g_node_next, n_del * 4)) ||
-        (rc = ensure(ctx, ctx->g_claimed, n_del * 8)) ||
-        (rc = ensure(ctx, ctx->g_tile_flag, ctx->n_tiles * 4)) ||
-        (rc = ensure(ctx, ctx->g_del_sig, ctx->n_tiles * 32)) ||
-        (rc = ensure(ctx, ctx->g_tile_scan, (ctx->n_tiles + 1) * 16 + 256)) ||
-        (rc = ensure(ctx, ctx->g_del_tiles, ctx->n_tiles * 4)) ||
-        (rc = ensure(ctx, ctx->g_found, n_del * 12)) ||
-        (rc = ensure(ctx, ctx->g_nch, ctx->n_tiles * 4)) ||
-        (rc = ensure(ctx, ctx->g_cbase, (ctx->n_tiles + 1) * 4)) ||
-        (rc = ensure(ctx, ctx->g_chunks, max_chunks * 8)) ||
-        (rc = ensure(ctx, ctx->g_scan_tmp, scan_tmp_bytes(ctx->n_tiles + 1) + 256)))
-      return rc;
     auto* tab_key = static_cast<unsigned long long*>(ctx->g_tab_key.p);
     auto* tab_head = static_cast<uint32_t*>(ctx->g_tab_head.p);
     auto* node_next = static_cast<uint32_t*>(ctx->g_node_next.p);
@@ -846,15 +915,13 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
     auto* dnch = static_cast<uint32_t*>(ctx->g_nch.p);
     auto* dcbase = static_cast<uint32_t*>(ctx->g_cbase.p);
     auto* dchunks = static_cast<uint64_t*>(ctx->g_chunks.p);
-    del_tiles = static_cast<uint32_t*>(ctx->g_del_tiles.p);
-    found = static_cast<uint32_t*>(ctx->g_found.p);
-    ctx->jobs_dirty = true;  // the gather work lists share these buffers
-    CY_HIP(ctx, hipMemsetAsync(tab_key, 0xFF, H * 8, st));
-    CY_HIP(ctx, hipMemsetAsync(tab_head, 0xFF, H * 4, st));
-    CY_HIP(ctx, hipMemsetAsync(claimed, 0xFF, n_del * 8, st));
-    CY_HIP(ctx, hipMemsetAsync(tile_flag, 0, ctx->n_tiles * 4, st));
     auto* tile_sig = static_cast<unsigned long long*>(ctx->g_del_sig.p);
-    CY_HIP(ctx, hipMemsetAsync(tile_sig, 0, ctx->n_tiles * 32, st));
+    ctx->jobs_dirty = true;  // the gather work lists share these buffers
+    fill_u64(reinterpret_cast<uint64_t*>(tab_key), H, ~0ull, st);
+    fill_u32(tab_head, H, 0xFFFFFFFFu, st);
+    fill_u64(reinterpret_cast<uint64_t*>(claimed), n_del, ~0ull, st);
+    fill_u32(tile_flag, ctx->n_tiles, 0, st);
+    fill_u64(reinterpret_cast<uint64_t*>(tile_sig), ctx->n_tiles * 4, 0, st);
     k_del_build<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, tab_key, tab_head,
                                                       log_h, node_next, tile_flag, tile_sig,
                                                       lv, T);
@@ -868,52 +935,57 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
     k_del_chunks<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(dnch, dcbase, ctx->n_tiles, dchunks);
     k_del_match<<<(uint32_t)ctx->num_cus * 8, 256, 0, st>>>(
         dchunks, dcbase + ctx->n_tiles, ctx->view(), tab_key, tab_head, log_h, node_next,
-        ctx->weighted ? d_dw : nullptr, tile_sig, claimed);
+        ctx->weighted ? d_dw : nullptr, tile_sig, claimed, misc);
     k_del_finish<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, claimed,
                                                        ctx->view(), found, misc);
-    CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st));
-    CY_HIP(ctx, hipStreamSynchronize(st));
-    n_found = ctx->h_misc[M_NFOUND];
   }
-  // 4. per program: re-execution test, then init or hooks
-  const size_t np = ctx->progs.size();
-  if (n_found && np) {
-    CY_HIP(ctx, hipMemsetAsync(misc + M_REEXEC, 0, np * 8, st));
-    for (size_t p = 0; p < np; ++p)
-      launch_reexec_check(ctx, ctx->progs[p], found, n_found, misc + M_REEXEC + p);
-    CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st));
-    CY_HIP(ctx, hipStreamSynchronize(st));
-  }
+  // 4. per program: re-execution test, the decision, then init or hooks
+  const unsigned long long* n_found_dev = misc + M_NFOUND;
+  for (size_t p = 0; p < np; ++p)
+    if (n_del) launch_reexec_check(ctx, ctx->progs[p], found, n_del, n_found_dev, pf + p);
+  if (np)
+    k_decide<<<grid_for(np, 64), 64, 0, st>>>(misc, pf, rules, rules + np, (uint32_t)np, gate);
   for (size_t p = 0; p < np; ++p) {
     Program& P = ctx->progs[p];
-    bool reexec = false;
-    if (n_found) {
-      reexec = ctx->h_misc[M_REEXEC + p] != 0;
-      if (!prd_like(P.kind) && !P.fixpoint) reexec = true;
-    }
-    if (P.kind == CYTOM_PPR && (n_ins || n_found)) reexec = true;  // from scratch per batch
-    P.last_reexec = reexec ? 1 : 0;
-    if (reexec) {
-      if ((rc = program_init(ctx, P))) return rc;
-    } else if (n_ins || n_found) {
-      launch_hooks(ctx, P, d_src, d_dst, d_w, ins_slot, n_ins, found, n_found);
-    }
+    program_init(ctx, P, gate + p);
+    if (n_ins || n_del)
+      launch_hooks(ctx, P, d_src, d_dst, d_w, ins_slot, n_ins, found, n_del, n_found_dev,
+                   gate + p);
   }
-  if (n_found)
-    k_del_degrees<<<grid_for(n_found, 256), 256, 0, st>>>(found, n_found, ctx->outdeg,
-                                                           ctx->indeg);
+  if (n_del)
+    k_del_degrees<<<grid_for(n_del, 256), 256, 0, st>>>(found, n_del, ctx->outdeg, ctx->indeg,
+                                                        misc);
   // 5. compaction of tiles with many tombstones, then sort-compaction of
   // tiles with long unsorted tails
-  if (n_del) {
-    k_compact<<<4096, CP_THREADS, 0, st>>>(del_tiles, n_del_tiles_p, ctx->view(), misc);
+  if (n_del) k_compact<<<4096, CP_THREADS, 0, st>>>(del_tiles, n_del_tiles_p, ctx->view(), misc);
+  resort_enqueue(ctx);
+  (void)hipEventRecord(ctx->ev[1], st);
+  // the control words and decisions, for ingest_finish
+  (void)hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st);
+  if (np) (void)hipMemcpyAsync(ctx->h_misc + 64, gate, np * 8, hipMemcpyDeviceToHost, st);
+}
+
+// After the batch's synchronisation: 0, CYTOM_EINVAL (refused, nothing
+// changed) or 1 (the pool is too small: nothing changed; repack and run the
+// batch again).
+int ingest_finish(cytom_ctx* ctx, uint64_t n_ins, uint64_t n_del, cytom_batch_stats* bst) {
+  const unsigned long long* hm = ctx->h_misc;
+  if (hm[M_ERR])
+    return set_err(ctx, CYTOM_EINVAL,
+                   (hm[M_ERR] & 1) ? "insertion with out-of-range vertex or zero weight"
+                                   : "deletion with out-of-range vertex");
+  if (hm[M_OVERFLOW]) return 1;
+  const uint64_t n_found = hm[M_NFOUND];
+  for (size_t p = 0; p < ctx->progs.size(); ++p) {
+    Program& P = ctx->progs[p];
+    const bool reexec = hm[64 + p] == 1;
+    P.last_reexec = reexec ? 1 : 0;
+    if (reexec || n_ins || n_found) P.fixpoint = false;
   }
-  uint32_t n_sorted = 0;
-  if ((rc = resort_tiles(ctx, &n_sorted))) return rc;
-  CY_HIP(ctx, hipEventRecord(ctx->ev[1], st));
-  CY_HIP(ctx, hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st));
-  CY_HIP(ctx, hipStreamSynchronize(st));
   ctx->live_edges += n_ins;
   ctx->live_edges -= n_found;
+  ctx->tiles_resorted += hm[M_RS_NPROC];
+  ctx->slots_resorted += hm[M_RS_NSLOTS];
   ctx->jobs_dirty = true;
   ctx->batches += 1;
   if (bst) {
@@ -923,13 +995,37 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
     bst->deleted = n_found;
     bst->missing = n_del - n_found;
     bst->live_edges = ctx->live_edges;
-    bst->tiles_relocated = nreloc;
-    bst->tiles_compacted = (uint32_t)ctx->h_misc[M_NCOMPACT];
-    bst->tiles_sorted = n_sorted;
+    bst->tiles_relocated = (uint32_t)hm[M_NRELOC];
+    bst->tiles_compacted = (uint32_t)hm[M_NCOMPACT];
+    bst->tiles_sorted = (uint32_t)hm[M_RS_NPROC];
     bst->pad = 0;
     bst->t_ingest_ms = ms;
   }
   return 0;
+}
+
+// The slot budget a repack must leave for this batch's relocations.
+uint64_t ingest_need(const cytom_ctx* ctx) { return ctx->h_misc[M_NEED]; }
+
+int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
+                 const uint32_t* d_w, uint64_t n_ins, const uint32_t* d_dsrc,
+                 const uint32_t* d_ddst, const uint32_t* d_dw, uint64_t n_del,
+                 cytom_batch_stats* bst) {
+  if (ctx->weighted && n_ins && !d_w)
+    return set_err(ctx, CYTOM_EINVAL, "weighted graph needs insertion weights");
+  for (int attempt = 0;; ++attempt) {
+    int rc = ingest_prepare(ctx, n_ins, n_del);
+    if (rc) return rc;
+    ingest_enqueue(ctx, d_src, d_dst, d_w, n_ins, d_dsrc, d_ddst, d_dw, n_del);
+    CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CY_HIP(ctx, hipGetLastError());
+    rc = ingest_finish(ctx, n_ins, n_del, bst);
+    if (rc <= 0) return rc;
+    if (attempt > 0) return set_err(ctx, CYTOM_ENOMEM, "edge pool exhausted after repack");
+    if ((rc = repack(ctx, std::max<uint64_t>(2 * ctx->pool_size,
+                                             2 * (ctx->live_edges + n_ins) + ingest_need(ctx)))))
+      return rc;
+  }
 }
 
 int64_t export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_t* w, uint64_t cap) {
