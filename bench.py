@@ -136,6 +136,9 @@ def parse_args(argv):
     ap.add_argument("--q-graphs", type=int, default=128)
     ap.add_argument("--ppr-grad", choices=["on", "off"], default="on",
                     help="time the differentiable PPR forward/adjoint on RMAT-20")
+    ap.add_argument("--api", choices=["step", "calls"], default="step",
+                    help="step: one cytom_step per batch (batch and first iteration groups "
+                         "enqueued together); calls: update_device then execute per program")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--cpu-batches", type=int, default=2,
@@ -214,14 +217,21 @@ def gpu_run(args):
 
     def step(b, acc, run=names):
         s, d, w, ds, dd, dw, nd = batches[b]
-        bs = g.update_device(s, d, w, n_ins, ds, dd, dw, nd)
+        run_names = [n for n in pids if n in run]
+        if args.api == "step":
+            bs, rs = g.step([pids[n] for n in run_names], s, d, w, n_ins, ds, dd, dw, nd)
+            stats = dict(zip(run_names, rs))
+            syncs = bs.host_syncs
+        else:
+            bs = g.update_device(s, d, w, n_ins, ds, dd, dw, nd)
+            stats = {n: g.execute(pids[n]) for n in run_names}
+            syncs = bs.host_syncs + sum(1 for _ in run_names)  # at least one per run
         acc["ingest_ms"] += bs.t_ingest_ms
         acc["deleted"] += bs.deleted
+        acc["syncs"] += syncs
         te = 0
-        for n, p in pids.items():
-            if n not in run:
-                continue
-            st = g.execute(p)
+        for n in run_names:
+            st = stats[n]
             acc["te"][n] += st.traversed_edges
             acc["ms"][n] += st.t_compute_ms
             acc["iters"][n] += st.iterations
@@ -231,7 +241,8 @@ def gpu_run(args):
         return te
 
     def new_acc():
-        return {"ingest_ms": 0.0, "deleted": 0, "live": 0, "te": {n: 0 for n in names},
+        return {"ingest_ms": 0.0, "deleted": 0, "live": 0, "syncs": 0,
+                "te": {n: 0 for n in names},
                 "ms": {n: 0.0 for n in names}, "iters": {n: 0 for n in names},
                 "scanned": {n: 0 for n in names}}
 
@@ -276,6 +287,14 @@ def gpu_run(args):
                                      if acc["te"][n] else None for n in names},
         "compute_ms_by_program": {n: round(acc["ms"][n], 3) for n in names},
         "ingest_ms_total": round(acc["ingest_ms"], 3),
+        "api": args.api,
+        "host_syncs_per_batch": round(acc["syncs"] / steps, 2) if steps and args.api == "step"
+        else None,
+        # Wall time not covered by the batch's and the runs' device events:
+        # host work and launch gaps (work lists are built between them).
+        "host_overhead_ms_per_batch": round(
+            (wall * 1e3 - acc["ingest_ms"] - sum(acc["ms"].values())) / steps, 3)
+        if steps else None,
         "update_batches_per_s": steps / wall if wall > 0 else 0.0,
         "live_edges_final": acc["live"],
         # The GPU rate over exactly the batches the CPU baseline times.
@@ -322,7 +341,63 @@ def gpu_run(args):
     if args.queries == "on":
         res["detail"]["batched_queries"] = queries_run(args, g, V, source)
     g.close()
+    res["detail"]["host_paths"] = api_comparison(args, scale=max(args.scale - 2, 10))
     return res
+
+
+def api_comparison(args, scale=20, batches=16):
+    """The same RMAT stream through both host paths (a graph each): per batch,
+    the wall time, the device time of the batch and the runs (HIP events),
+    their difference, and the host synchronisations of the step path."""
+    import cytom_amd as C
+
+    V = 1 << scale
+    E = args.edge_factor << scale
+    n_del = int(E // batches * args.delete_fraction)
+    n_ins = E // batches - n_del
+    names = [p.strip() for p in args.programs.split(",") if p.strip()]
+    weighted = "sssp" in names
+    stream = C.Stream(args.seed + 1)
+    data = []
+    for b in range(batches):
+        first, didx = stream.next(n_ins, n_del)
+        s, d, w = C.rmat_generate(scale, args.seed + 1, first, n_ins, weights=weighted)
+        di = C.DeviceArray.from_host(didx)
+        ds, dd, dw = C.rmat_gather(scale, args.seed + 1, di, len(didx), weights=weighted)
+        data.append((s, d, w, ds, dd, dw, len(didx)))
+    out = {"workload": "RMAT-%d, %d batches x %d updates, %s; batches 5-%d timed"
+                       % (scale, batches, n_ins + n_del, ", ".join(names), batches)}
+    finals = {}
+    for api in ("calls", "step"):
+        g = C.Graph(V, vertices_per_tile=args.vertices_per_tile, weighted=weighted,
+                    max_iterations=100, delta=1e-7, edge_capacity=int(3 * E))
+        ids = [g.register(n, 0) for n in names]
+        wall = dev = 0.0
+        syncs = 0
+        for b, (s, d, w, ds, dd, dw, nd) in enumerate(data):
+            C.synchronize()
+            t0 = time.perf_counter()
+            if api == "step":
+                bs, rs = g.step(ids, s, d, w, n_ins, ds, dd, dw, nd)
+            else:
+                bs = g.update_device(s, d, w, n_ins, ds, dd, dw, nd)
+                rs = [g.execute(p) for p in ids]
+            t = time.perf_counter() - t0
+            if b >= 4:
+                wall += t * 1e3
+                dev += bs.t_ingest_ms + sum(r.t_compute_ms for r in rs)
+                syncs += bs.host_syncs
+        finals[api] = [g.read(p) for p in ids]
+        g.close()
+        n = batches - 4
+        out[api] = {"wall_ms_per_batch": round(wall / n, 3), "device_ms_per_batch": round(dev / n, 3),
+                    "host_overhead_ms_per_batch": round((wall - dev) / n, 3)}
+        if api == "step":
+            out[api]["host_syncs_per_batch"] = round(syncs / n, 2)
+    out["values_equal"] = all(np.array_equal(a, b) for a, b in zip(finals["calls"], finals["step"]))
+    out["host_overhead_saved_ms_per_batch"] = round(
+        out["calls"]["host_overhead_ms_per_batch"] - out["step"]["host_overhead_ms_per_batch"], 3)
+    return out
 
 
 def _timed(fn):

@@ -87,7 +87,7 @@ class BatchStats(ctypes.Structure):
         ("tiles_relocated", ctypes.c_uint32),
         ("tiles_compacted", ctypes.c_uint32),
         ("tiles_sorted", ctypes.c_uint32),
-        ("pad", ctypes.c_uint32),
+        ("host_syncs", ctypes.c_uint32),
         ("t_ingest_ms", ctypes.c_double),
     ]
 
@@ -132,6 +132,8 @@ SIGNATURES = {
                                                   ctypes.POINTER(BatchStats)]),
     "cytom_read_results": (ctypes.c_int64, [_VP, ctypes.c_int, _VP, ctypes.c_size_t]),
     "cytom_execute": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(RunStats)]),
+    "cytom_step": (ctypes.c_int, [_VP, _VP, _VP, _VP, _U64, _VP, _VP, _VP, _U64, _VP, _U32,
+                                  ctypes.POINTER(BatchStats), _VP]),
     "cytom_read_values": (ctypes.c_int, [_VP, ctypes.c_int, _VP, ctypes.c_size_t]),
     "cytom_read_degrees": (ctypes.c_int, [_VP, _VP, _VP]),
     "cytom_export_edges": (ctypes.c_int64, [_VP, _VP, _VP, _VP, _U64]),
@@ -471,6 +473,24 @@ class Graph:
         st = RunStats()
         self._check(lib().cytom_execute(self._h, pid, ctypes.byref(st)), "execute")
         return st
+
+    def step(self, pids, src, dst, w, n_ins, dsrc=None, ddst=None, dw=None, n_del=0):
+        """One update batch from device arrays (as update_device) followed by
+        a run of every program in pids, with one host synchronisation when
+        the runs converge within their first iteration group.  Returns
+        (BatchStats, [RunStats per program])."""
+        for name, a, n in (("src", src, n_ins), ("dst", dst, n_ins), ("w", w, n_ins),
+                           ("dsrc", dsrc, n_del), ("ddst", ddst, n_del), ("dw", dw, n_del)):
+            if isinstance(a, DeviceArray) and a.n < n:
+                raise ValueError("%s holds %d elements, %d needed" % (name, a.n, n))
+        ids = (ctypes.c_int * max(1, len(pids)))(*pids)
+        rs = (RunStats * max(1, len(pids)))()
+        st = BatchStats()
+        self._check(lib().cytom_step(
+            self._h, _p(src), _p(dst), _p(w), n_ins, _p(dsrc), _p(ddst), _p(dw), n_del,
+            ctypes.cast(ids, ctypes.c_void_p), len(pids), ctypes.byref(st),
+            ctypes.cast(rs, ctypes.c_void_p)), "step")
+        return st, [rs[k] for k in range(len(pids))]
 
     def batched_execute(self, queries):
         """queries: list of (kind, sources); kind BFS/SSSP/PPR or its name.

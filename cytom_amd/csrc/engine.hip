@@ -315,6 +315,84 @@ int cytom_execute(cytom_ctx* ctx, int prog_id, cytom_run_stats* stats) {
   return run_program(ctx, ctx->progs[prog_id], stats);
 }
 
+// edge-inserter.h:129-184: one batch, then (re-initialisation where the
+// batch requires it and) every registered algorithm's run.  The batch, the
+// work lists and the first iteration group of every program are enqueued
+// before the host checks anything; programs that need more iterations get
+// one further group each per synchronisation.
+int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, const uint32_t* d_w,
+               uint64_t n_ins, const uint32_t* d_del_src, const uint32_t* d_del_dst,
+               const uint32_t* d_del_w, uint64_t n_del, const int* prog_ids, uint32_t n_progs,
+               cytom_batch_stats* batch_stats, cytom_run_stats* run_stats) {
+  CY_CHECK_CTX(ctx);
+  if ((n_ins && (!d_src || !d_dst)) || (n_del && (!d_del_src || !d_del_dst)))
+    return set_err(ctx, CYTOM_EINVAL, "null update arrays");
+  if (n_progs && !prog_ids) return set_err(ctx, CYTOM_EINVAL, "null program list");
+  if (ctx->weighted && n_ins && !d_w)
+    return set_err(ctx, CYTOM_EINVAL, "weighted graph needs insertion weights");
+  std::vector<Program*> ps;
+  for (uint32_t k = 0; k < n_progs; ++k) {
+    const int id = prog_ids[k];
+    if (id < 0 || id >= (int)ctx->progs.size()) return set_err(ctx, CYTOM_EINVAL, "bad program id");
+    for (uint32_t j = 0; j < k; ++j)
+      if (prog_ids[j] == id) return set_err(ctx, CYTOM_EINVAL, "program listed twice");
+    ps.push_back(&ctx->progs[id]);
+  }
+  int rc;
+  if ((rc = ingest_prepare(ctx, n_ins, n_del)) || (rc = prepare_jobs(ctx))) return rc;
+  for (Program* P : ps)
+    if ((rc = program_events(ctx, *P))) return rc;
+  std::vector<int> parity(ps.size());
+  ingest_enqueue(ctx, d_src, d_dst, d_w, n_ins, d_del_src, d_del_dst, d_del_w, n_del);
+  enqueue_jobs(ctx);
+  for (size_t k = 0; k < ps.size(); ++k) {
+    Program& P = *ps[k];
+    parity[k] = P.cur_act;
+    P.refuse = ctx->d_misc;
+    run_open(P);
+    run_group(ctx, P);
+    P.refuse = nullptr;
+  }
+  uint32_t syncs = 1;
+  CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  CY_HIP(ctx, hipGetLastError());
+  rc = ingest_finish(ctx, n_ins, n_del, batch_stats);
+  if (rc != 0) {
+    // The batch was refused or needs a larger pool: the runs did nothing but
+    // flip the host's buffer parity.
+    for (size_t k = 0; k < ps.size(); ++k) ps[k]->cur_act = parity[k];
+    if (rc < 0) return rc;
+    cytom_batch_stats bs{};
+    if ((rc = ingest_batch(ctx, d_src, d_dst, d_w, n_ins, d_del_src, d_del_dst, d_del_w, n_del,
+                           &bs)))
+      return rc;
+    syncs += bs.host_syncs;
+    for (size_t k = 0; k < ps.size(); ++k) {
+      if ((rc = run_program(ctx, *ps[k], run_stats ? &run_stats[k] : nullptr))) return rc;
+      syncs += ps[k]->groups;  // one synchronisation per group
+    }
+    bs.host_syncs = syncs;
+    if (batch_stats) *batch_stats = bs;
+    return 0;
+  }
+  std::vector<char> more(ps.size());
+  bool any = false;
+  for (size_t k = 0; k < ps.size(); ++k) any |= (more[k] = run_needs_more(ctx, *ps[k])) != 0;
+  while (any) {
+    for (size_t k = 0; k < ps.size(); ++k)
+      if (more[k]) run_group(ctx, *ps[k]);
+    ++syncs;
+    CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    any = false;
+    for (size_t k = 0; k < ps.size(); ++k)
+      if (more[k]) any |= (more[k] = run_needs_more(ctx, *ps[k])) != 0;
+  }
+  for (size_t k = 0; k < ps.size(); ++k)
+    if ((rc = run_finish(ctx, *ps[k], run_stats ? &run_stats[k] : nullptr))) return rc;
+  if (batch_stats) batch_stats->host_syncs = syncs;
+  return 0;
+}
+
 int cytom_batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq,
                           cytom_run_stats* stats) {
   CY_CHECK_CTX(ctx);

@@ -178,7 +178,9 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_frontier(
     int kind, uint64_t V, uint32_t log2vpt, const uint64_t* __restrict__ act,
     const uint32_t* __restrict__ outdeg, uint8_t* __restrict__ gact,
     const float* __restrict__ delta, float* __restrict__ contrib,
-    const uint32_t* __restrict__ cur, uint32_t* __restrict__ fval, IterStats* it) {
+    const uint32_t* __restrict__ cur, uint32_t* __restrict__ fval, IterStats* it,
+    const unsigned long long* refuse) {
+  if (batch_refused(refuse)) return;  // the run follows a refused batch: it does nothing
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
   const uint64_t nw_total = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -647,7 +649,9 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_min(
     uint64_t V, uint32_t log2vpt, uint32_t* __restrict__ cur, const uint32_t* __restrict__ nxt,
     uint32_t* __restrict__ fval, uint8_t* __restrict__ changed,
     const uint64_t* __restrict__ act_cur, uint64_t* __restrict__ act_next,
-    uint8_t* __restrict__ gact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
+    uint8_t* __restrict__ gact_next, const uint32_t* __restrict__ outdeg, IterStats* nit,
+    const unsigned long long* refuse) {
+  if (batch_refused(refuse)) return;
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
   if (empty_frontier(nit, act_next, nwords)) return;
@@ -709,7 +713,9 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_pair(
     uint64_t V, uint32_t log2vpt, uint2* __restrict__ cur, const uint2* __restrict__ nxt,
     uint2* __restrict__ fval, uint8_t* __restrict__ changed,
     const uint64_t* __restrict__ act_cur, uint64_t* __restrict__ act_next,
-    uint8_t* __restrict__ gact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
+    uint8_t* __restrict__ gact_next, const uint32_t* __restrict__ outdeg, IterStats* nit,
+    const unsigned long long* refuse) {
+  if (batch_refused(refuse)) return;
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
   if (empty_frontier(nit, act_next, nwords)) return;
@@ -751,7 +757,8 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_prd(
     float* __restrict__ delta, long long* __restrict__ dsum, float* __restrict__ contrib,
     uint8_t* __restrict__ changed, const uint64_t* __restrict__ act_cur,
     uint64_t* __restrict__ act_next, uint8_t* __restrict__ gact_next,
-    const uint32_t* __restrict__ outdeg, IterStats* nit) {
+    const uint32_t* __restrict__ outdeg, IterStats* nit, const unsigned long long* refuse) {
+  if (batch_refused(refuse)) return;
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
   // The first-iteration rule holds for this launch when init set dctl[0];
@@ -1059,8 +1066,8 @@ int program_alloc(cytom_ctx* ctx, Program& P) {
   const size_t nit = sizeof(IterStats) * (ctx->max_iter + 2);
   ok = ok && alloc((void**)&P.d_iter, nit) && alloc((void**)&P.dctl, 64) &&
        hipHostMalloc((void**)&P.h_iter, nit, hipHostMallocDefault) == hipSuccess;
-  for (int i = 0; i < 2 && ok; ++i) ok = hipEventCreate(&P.ev[i]) == hipSuccess;
-  ok = ok && hipMemsetAsync(P.dctl, 0, 64, ctx->stream) == hipSuccess;
+  ok = ok && hipMemsetAsync(P.dctl, 0, 64, ctx->stream) == hipSuccess &&
+       program_events(ctx, P) == 0;
   if (!ok) {
     program_free(P);
     return set_err(ctx, CYTOM_ENOMEM, "program state allocation failed");
@@ -1077,13 +1084,12 @@ void program_free(Program& P) {
   if (P.d_iter) (void)hipFree(P.d_iter);
   if (P.dctl) (void)hipFree(P.dctl);
   if (P.h_iter) (void)hipHostFree(P.h_iter);
-  for (auto& e : P.ev)
-    if (e) (void)hipEventDestroy(e);
+  for (auto& e : P.gev) (void)hipEventDestroy(e);
   for (auto& e : P.kev) (void)hipEventDestroy(e);
   P.d_iter = nullptr;
   P.dctl = nullptr;
   P.h_iter = nullptr;
-  P.ev[0] = P.ev[1] = nullptr;
+  P.gev.clear();
   P.kev.clear();
   P.cur = P.nxt = P.fval = nullptr;
   P.rank = P.delta = P.contrib = nullptr;
@@ -1187,17 +1193,6 @@ inline uint32_t apply_blocks_for(const cytom_ctx* ctx) {
 
 }  // namespace
 
-void run_begin(cytom_ctx* ctx, Program& P) {
-  hipStream_t st = ctx->stream;
-  fill_u64(reinterpret_cast<uint64_t*>(P.d_iter),
-           sizeof(IterStats) / 8 * (uint64_t)(ctx->max_iter + 2), 0, st);
-  fill_u64(reinterpret_cast<uint64_t*>(P.gact[P.cur_act]), (uint64_t)ctx->T * 8, 0, st);
-  (void)hipEventRecord(P.ev[0], st);
-  k_frontier<<<apply_blocks_for(ctx), APPLY_THREADS, 0, st>>>(
-      P.kind, ctx->V, ctx->log2vpt, P.act[P.cur_act], ctx->outdeg, P.gact[P.cur_act], P.delta,
-      P.contrib, P.cur, P.fval, P.d_iter);
-}
-
 void run_iters(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t n) {
   hipStream_t st = ctx->stream;
   const uint32_t ab = apply_blocks_for(ctx);
@@ -1217,41 +1212,92 @@ void run_iters(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t n) {
     if (prd_like(P.kind)) {
       k_apply_prd<<<ab, APPLY_THREADS, 0, st>>>(
           ctx->V, ctx->log2vpt, P.dctl, ctx->delta, ldexp(1.0, -ctx->prd_shift), P.rank, P.delta,
-          P.dsum, P.contrib, P.changed, P.act[P.cur_act], P.act[nx], P.gact[nx], ctx->outdeg, nit);
+          P.dsum, P.contrib, P.changed, P.act[P.cur_act], P.act[nx], P.gact[nx], ctx->outdeg, nit, P.refuse);
     } else if (P.kind == CYTOM_BFS) {
       k_apply_min<CYTOM_BFS><<<ab, APPLY_THREADS, 0, st>>>(
           ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
-          P.gact[nx], ctx->outdeg, nit);
+          P.gact[nx], ctx->outdeg, nit, P.refuse);
     } else if (P.kind == CYTOM_CC) {
       k_apply_min<CYTOM_CC><<<ab, APPLY_THREADS, 0, st>>>(
           ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
-          P.gact[nx], ctx->outdeg, nit);
+          P.gact[nx], ctx->outdeg, nit, P.refuse);
     } else if (P.kind == KIND_SSSP_PAIR) {
       k_apply_pair<<<ab, APPLY_THREADS, 0, st>>>(
           ctx->V, ctx->log2vpt, reinterpret_cast<uint2*>(P.cur),
           reinterpret_cast<const uint2*>(P.nxt), reinterpret_cast<uint2*>(P.fval), P.changed,
-          P.act[P.cur_act], P.act[nx], P.gact[nx], ctx->outdeg, nit);
+          P.act[P.cur_act], P.act[nx], P.gact[nx], ctx->outdeg, nit, P.refuse);
     } else {
       k_apply_min<CYTOM_SSSP><<<ab, APPLY_THREADS, 0, st>>>(
           ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
-          P.gact[nx], ctx->outdeg, nit);
+          P.gact[nx], ctx->outdeg, nit, P.refuse);
     }
     P.cur_act = nx;
   }
 }
 
-void run_readback(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t hi) {
-  (void)hipMemcpyAsync(&P.h_iter[lo], &P.d_iter[lo], sizeof(IterStats) * (hi - lo + 1),
-                       hipMemcpyDeviceToHost, ctx->stream);
+void run_open(Program& P) {
+  P.launched = 0;
+  P.groups = 0;
+  P.checked = 0;
+  P.conv = -1;
 }
 
-int run_converged_at(const Program& P, uint32_t lo, uint32_t hi) {
-  for (uint32_t k = lo; k <= hi; ++k)
-    if (P.h_iter[k].count == 0) return (int)k;
-  return -1;
+namespace {
+
+// Iterations of the next group of a run: the first is as long as the
+// program's previous run plus 1/16 (PageRank-delta runs take a similar number
+// of iterations batch after batch, and launches past convergence are no-ops:
+// the gather returns at once and the apply only clears the next bitmap),
+// later ones 4, 8, then 16.
+uint32_t group_length(const cytom_ctx* ctx, const Program& P) {
+  uint32_t n;
+  if (P.groups == 0)
+    n = std::max<uint32_t>(P.last_iters + std::max<uint32_t>(P.last_iters / 16, 2), 2);
+  else
+    n = std::min<uint32_t>(4u << std::min<uint32_t>(P.groups - 1, 2), 16);
+  return std::min(n, ctx->max_iter - P.launched);
 }
 
-int run_finish(cytom_ctx* ctx, Program& P, uint32_t it, bool converged, cytom_run_stats* out) {
+// Upper bound of the groups of one run (first group of at least one
+// iteration, then 4, 8 and 16 at a time).
+size_t max_groups(const cytom_ctx* ctx) { return ctx->max_iter / 16 + 6; }
+
+}  // namespace
+
+void run_group(cytom_ctx* ctx, Program& P) {
+  hipStream_t st = ctx->stream;
+  const uint32_t n = group_length(ctx, P);
+  const uint32_t g = P.groups++;
+  if (g == 0) {
+    fill_u64(reinterpret_cast<uint64_t*>(P.d_iter),
+             sizeof(IterStats) / 8 * (uint64_t)(ctx->max_iter + 2), 0, st);
+    fill_u64(reinterpret_cast<uint64_t*>(P.gact[P.cur_act]), (uint64_t)ctx->T * 8, 0, st);
+  }
+  (void)hipEventRecord(P.gev[2 * g], st);
+  if (g == 0)
+    k_frontier<<<apply_blocks_for(ctx), APPLY_THREADS, 0, st>>>(
+        P.kind, ctx->V, ctx->log2vpt, P.act[P.cur_act], ctx->outdeg, P.gact[P.cur_act], P.delta,
+        P.contrib, P.cur, P.fval, P.d_iter, P.refuse);
+  run_iters(ctx, P, P.launched, n);
+  (void)hipEventRecord(P.gev[2 * g + 1], st);
+  // counters of the frontier (first group) and of every iteration launched
+  const uint32_t lo = g == 0 ? 0 : P.launched + 1;
+  (void)hipMemcpyAsync(&P.h_iter[lo], &P.d_iter[lo], sizeof(IterStats) * (P.launched + n + 1 - lo),
+                       hipMemcpyDeviceToHost, st);
+  P.launched += n;
+}
+
+bool run_needs_more(const cytom_ctx* ctx, Program& P) {
+  for (uint32_t k = P.checked + 1; P.conv < 0 && k <= P.launched; ++k)
+    if (P.h_iter[k].count == 0) P.conv = (int)k;
+  P.checked = P.launched;
+  return P.conv < 0 && P.launched < ctx->max_iter;
+}
+
+int run_finish(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
+  CY_HIP(ctx, hipGetLastError());
+  const bool converged = P.conv >= 0;
+  const uint32_t it = converged ? (uint32_t)P.conv : P.launched;
   P.last_iters = it;
   P.fixpoint = converged;
   const uint32_t maxit = ctx->max_iter;
@@ -1279,8 +1325,14 @@ int run_finish(cytom_ctx* ctx, Program& P, uint32_t it, bool converged, cytom_ru
       act += P.h_iter[i].count;
       sc += P.h_iter[i].scanned;
     }
-    float ms = 0.f;
-    CY_HIP(ctx, hipEventElapsedTime(&ms, P.ev[0], P.ev[1]));
+    // device time of the run: its groups (other programs' groups may lie
+    // between them in a step)
+    double ms = 0.0;
+    for (uint32_t g = 0; g < P.groups; ++g) {
+      float gm = 0.f;
+      CY_HIP(ctx, hipEventElapsedTime(&gm, P.gev[2 * g], P.gev[2 * g + 1]));
+      ms += gm;
+    }
     out->iterations = it;
     out->converged = converged ? 1 : 0;
     out->reexecuted = (uint32_t)P.last_reexec;
@@ -1293,53 +1345,26 @@ int run_finish(cytom_ctx* ctx, Program& P, uint32_t it, bool converged, cytom_ru
   return 0;
 }
 
-// Runs the program to convergence or max_iterations with the iterations
-// already enqueued (launched of them, run_begin included) and read back to
-// h_iter[0..launched]: more groups (4, 8, then 16 iterations, one host check
-// each) until the frontier is empty.  Launches past convergence are no-ops:
-// the gather returns at once and the apply only clears the next bitmap.
-int run_complete(cytom_ctx* ctx, Program& P, uint32_t launched, cytom_run_stats* out) {
-  hipStream_t st = ctx->stream;
-  const uint32_t maxit = ctx->max_iter;
-  int conv = run_converged_at(P, 1, launched);
-  uint32_t group = 4;
-  while (conv < 0 && launched < maxit) {
-    const uint32_t n = std::min(group, maxit - launched);
-    run_iters(ctx, P, launched, n);
-    run_readback(ctx, P, launched + 1, launched + n);
-    CY_HIP(ctx, hipStreamSynchronize(st));
-    conv = run_converged_at(P, launched + 1, launched + n);
-    launched += n;
-    group = std::min<uint32_t>(group * 2, 16);
-  }
-  CY_HIP(ctx, hipEventRecord(P.ev[1], st));
-  CY_HIP(ctx, hipStreamSynchronize(st));
-  CY_HIP(ctx, hipGetLastError());
-  const uint32_t it = conv >= 0 ? (uint32_t)conv : launched;
-  return run_finish(ctx, P, it, conv >= 0, out);
-}
-
-// The first group of a run: as long as the program's previous run
-// (PageRank-delta runs take a similar number of iterations batch after batch).
-uint32_t first_group(const cytom_ctx* ctx, const Program& P) {
-  return std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(P.last_iters, 2), 64),
-                            ctx->max_iter);
-}
-
 int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
   int rc;
   if (ctx->jobs_dirty && (rc = build_jobs(ctx))) return rc;
   if ((rc = program_events(ctx, P))) return rc;
-  const uint32_t n = first_group(ctx, P);
-  run_begin(ctx, P);
-  run_iters(ctx, P, 0, n);
-  run_readback(ctx, P, 0, n);
-  CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return run_complete(ctx, P, n, out);
+  run_open(P);
+  do {
+    run_group(ctx, P);
+    CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  } while (run_needs_more(ctx, P));
+  return run_finish(ctx, P, out);
 }
 
-// Per-iteration gather timing events of a program, created on demand.
+// Events of a program's runs: the iteration groups' bounds, and with
+// profiling one pair around every gather launch.
 int program_events(cytom_ctx* ctx, Program& P) {
+  while (P.gev.size() < 2 * max_groups(ctx)) {
+    hipEvent_t e;
+    CY_HIP(ctx, hipEventCreate(&e));
+    P.gev.push_back(e);
+  }
   if (!ctx->profile) return 0;
   while (P.kev.size() < 2 * (size_t)ctx->max_iter) {
     hipEvent_t e;

@@ -70,6 +70,16 @@ struct StoreView {
 // PageRank-delta and its personalized form share the gather and apply.
 __host__ __device__ inline bool prd_like(int kind) { return kind == CYTOM_PRD || kind == CYTOM_PPR; }
 
+// Control words of an update batch in cytom_ctx::d_misc that refuse it: an
+// invalid update (MISC_ERR) or relocations past the pool (MISC_OVERFLOW).  A
+// refused batch changes nothing, and program runs enqueued behind it in the
+// same step (Program::refuse) do nothing either.
+constexpr int MISC_ERR = 1;
+constexpr int MISC_OVERFLOW = 8;
+__device__ inline bool batch_refused(const unsigned long long* misc) {
+  return misc != nullptr && (misc[MISC_ERR] | misc[MISC_OVERFLOW]) != 0;
+}
+
 // Internal program kind of the batched queries: two SSSP queries in one run,
 // with a (u32, u32) pair of distances per vertex in cur / nxt / fval (each
 // array V pairs) and a pair window in the gather.  Never registered in a
@@ -112,8 +122,14 @@ struct Program {
   IterStats* d_iter = nullptr;
   IterStats* h_iter = nullptr;
   uint32_t* dctl = nullptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};  // around a run (compute time)
-  std::vector<hipEvent_t> kev;            // around every gather launch (profiling)
+  std::vector<hipEvent_t> gev;  // start and end of every iteration group of a run
+  std::vector<hipEvent_t> kev;  // around every gather launch (profiling)
+  // The run in flight: iterations launched, groups enqueued, counters
+  // examined, the converged iteration once seen (-1 before), and the refusal
+  // words of the batch it follows (nullptr outside a step).
+  uint32_t launched = 0, groups = 0, checked = 0;
+  int conv = -1;
+  const unsigned long long* refuse = nullptr;
   int cur_act = 0;
   bool fixpoint = false;
   int last_reexec = 0;
@@ -275,6 +291,16 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
                  const uint32_t* d_w, uint64_t n_ins, const uint32_t* d_dsrc,
                  const uint32_t* d_ddst, const uint32_t* d_dw, uint64_t n_del,
                  cytom_batch_stats* st);
+// The stages of ingest_batch: ingest_prepare sizes the scratch (host only),
+// ingest_enqueue launches the batch with no host reads, and after one
+// synchronisation ingest_finish returns 0, CYTOM_EINVAL (refused) or 1 (the
+// pool must grow by ingest_need() slots plus the live edges: nothing was
+// applied).
+int ingest_prepare(cytom_ctx* ctx, uint64_t n_ins, uint64_t n_del);
+void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
+                    const uint32_t* d_w, uint64_t n_ins, const uint32_t* d_dsrc,
+                    const uint32_t* d_ddst, const uint32_t* d_dw, uint64_t n_del);
+int ingest_finish(cytom_ctx* ctx, uint64_t n_ins, uint64_t n_del, cytom_batch_stats* bst);
 int64_t export_edges(cytom_ctx* ctx, uint32_t* src, uint32_t* dst, uint32_t* w, uint64_t cap);
 
 // ------------------------------------------------------------ executors
@@ -287,16 +313,15 @@ int program_init(cytom_ctx* ctx, Program& P, const unsigned long long* gate = nu
 int program_set_sources(cytom_ctx* ctx, Program& P, const uint32_t* sources, uint32_t n);
 int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* st);
 // The stages of a run, for callers that enqueue several programs before one
-// host check: run_begin (counters, frontier), run_iters (iterations
-// [lo, lo + n)), run_readback (counters [lo, hi] to the host mirror); after a
-// synchronisation run_complete launches more groups until convergence or
-// the cap and fills the stats.
+// host check: run_open resets the run state, run_group enqueues the next
+// iteration group (the first one also computes the frontier) and its
+// counter readback, run_needs_more (after a synchronisation) tells whether
+// another group is needed, and run_finish fills the stats.
 int program_events(cytom_ctx* ctx, Program& P);
-void run_begin(cytom_ctx* ctx, Program& P);
-void run_iters(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t n);
-void run_readback(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t hi);
-int run_complete(cytom_ctx* ctx, Program& P, uint32_t launched, cytom_run_stats* out);
-uint32_t first_group(const cytom_ctx* ctx, const Program& P);
+void run_open(Program& P);
+void run_group(cytom_ctx* ctx, Program& P);
+bool run_needs_more(const cytom_ctx* ctx, Program& P);
+int run_finish(cytom_ctx* ctx, Program& P, cytom_run_stats* out);
 // Sets the gather kernels' LDS limit and persistent grid for the context.
 int configure_gather(cytom_ctx* ctx);
 // Rebuilds the work lists of the current store (chunks of chunk_slots slots

@@ -34,14 +34,14 @@ constexpr uint32_t NONE32 = 0xFFFFFFFFu;
 // without host round trips and the host reads them once at the end).
 enum Misc : int {
   M_POOL_TOP = 0,
-  M_ERR = 1,       // bit 0: invalid insertion, bit 1: invalid deletion
+  M_ERR = MISC_ERR,  // bit 0: invalid insertion, bit 1: invalid deletion
   M_NEED = 2,      // slots the relocated tiles need
   M_NRELOC = 3,
   M_RELOC_CUR = 4,
   M_NFOUND = 5,    // deletions that matched a live edge
   M_NCOMPACT = 6,
   M_RS_CUT = 7,    // first selected tile (by rank) left for a later sort-compaction
-  M_OVERFLOW = 8,  // the relocations do not fit the pool: the batch is not applied
+  M_OVERFLOW = MISC_OVERFLOW,  // the relocations do not fit the pool: the batch is not applied
   M_RS_NPROC = 9,  // tiles sort-compacted by this batch
   M_RS_NSLOTS = 10,
   M_POOL_SIZE = 11,
@@ -50,9 +50,7 @@ enum Misc : int {
 // A batch with an invalid update, or one whose relocations overflow the
 // pool, modifies nothing: every kernel that writes the store, the degrees or
 // a program checks this first (uniform per launch).
-__device__ inline bool batch_dead(const unsigned long long* misc) {
-  return (misc[M_ERR] | misc[M_OVERFLOW]) != 0;
-}
+__device__ inline bool batch_dead(const unsigned long long* misc) { return batch_refused(misc); }
 
 __host__ __device__ inline uint32_t grow_cap(uint64_t n) {
   uint64_t c = n + n / 2 + 64;
@@ -675,6 +673,7 @@ int repack(cytom_ctx* ctx, uint64_t min_slots) {
   ctx->pool = npool;
   ctx->wpool = nw;
   ctx->pool_size = new_size;
+  ctx->jobs_dirty = true;
   return 0;
 }
 
@@ -855,6 +854,7 @@ void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst
   unsigned long long* gate = pf + np;                             // [np] decisions
   auto* rules = reinterpret_cast<uint32_t*>(gate + np);           // [np] force, [np] found_forces
 
+  ctx->jobs_dirty = true;  // the store changes (and the deletion pass reuses the list buffers)
   (void)hipEventRecord(ctx->ev[0], st);
   // misc[1..] := 0 (the pool top in misc[0] persists), the pool size, rules
   fill_u64(reinterpret_cast<uint64_t*>(misc + 1), 63, 0, st);
@@ -916,7 +916,6 @@ void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst
     auto* dcbase = static_cast<uint32_t*>(ctx->g_cbase.p);
     auto* dchunks = static_cast<uint64_t*>(ctx->g_chunks.p);
     auto* tile_sig = static_cast<unsigned long long*>(ctx->g_del_sig.p);
-    ctx->jobs_dirty = true;  // the gather work lists share these buffers
     fill_u64(reinterpret_cast<uint64_t*>(tab_key), H, ~0ull, st);
     fill_u32(tab_head, H, 0xFFFFFFFFu, st);
     fill_u64(reinterpret_cast<uint64_t*>(claimed), n_del, ~0ull, st);
@@ -986,7 +985,6 @@ int ingest_finish(cytom_ctx* ctx, uint64_t n_ins, uint64_t n_del, cytom_batch_st
   ctx->live_edges -= n_found;
   ctx->tiles_resorted += hm[M_RS_NPROC];
   ctx->slots_resorted += hm[M_RS_NSLOTS];
-  ctx->jobs_dirty = true;
   ctx->batches += 1;
   if (bst) {
     float ms = 0.f;
@@ -998,7 +996,7 @@ int ingest_finish(cytom_ctx* ctx, uint64_t n_ins, uint64_t n_del, cytom_batch_st
     bst->tiles_relocated = (uint32_t)hm[M_NRELOC];
     bst->tiles_compacted = (uint32_t)hm[M_NCOMPACT];
     bst->tiles_sorted = (uint32_t)hm[M_RS_NPROC];
-    bst->pad = 0;
+    bst->host_syncs = 1;
     bst->t_ingest_ms = ms;
   }
   return 0;
@@ -1020,6 +1018,7 @@ int ingest_batch(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
     CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     CY_HIP(ctx, hipGetLastError());
     rc = ingest_finish(ctx, n_ins, n_del, bst);
+    if (bst) bst->host_syncs = (uint32_t)attempt + 1;
     if (rc <= 0) return rc;
     if (attempt > 0) return set_err(ctx, CYTOM_ENOMEM, "edge pool exhausted after repack");
     if ((rc = repack(ctx, std::max<uint64_t>(2 * ctx->pool_size,

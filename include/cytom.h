@@ -86,7 +86,7 @@ typedef struct {
   uint32_t tiles_relocated;
   uint32_t tiles_compacted;
   uint32_t tiles_sorted;     /* tiles rewritten in source order (sort-compaction) */
-  uint32_t pad;
+  uint32_t host_syncs;       /* host synchronisations the call made */
   double t_ingest_ms;        /* device time of the batch (HIP events) */
 } cytom_batch_stats;
 
@@ -152,6 +152,20 @@ int cytom_update_batch_records(cytom_ctx* ctx, const void* records, uint32_t rec
  * (continuous-algorithm-executor.h:49-249): runs the program until
  * convergence or max_iterations. */
 int cytom_execute(cytom_ctx* ctx, int prog_id, cytom_run_stats* stats);
+/* One step of the continuous loop, edge-inserter.h:129-184: the update batch
+ * (device pointers, as cytom_update_batch_device), then a run of each of the
+ * n_progs listed programs (distinct ids), with the re-initialisations the
+ * batch requires.  The values equal those of cytom_update_batch_device
+ * followed by cytom_execute of each program in turn; the batch and every program's first iteration group are enqueued before the
+ * host reads anything, so a step whose programs converge within the
+ * previous run's iteration count plus 1/16 synchronises once, and otherwise
+ * once more per further group (batch_stats->host_syncs).  A refused batch
+ * (CYTOM_EINVAL) changes nothing and runs no program.  run_stats: n_progs
+ * entries or NULL. */
+int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, const uint32_t* d_w,
+               uint64_t n_ins, const uint32_t* d_del_src, const uint32_t* d_del_dst,
+               const uint32_t* d_del_w, uint64_t n_del, const int* prog_ids, uint32_t n_progs,
+               cytom_batch_stats* batch_stats, cytom_run_stats* run_stats);
 /* Copies the program's values (the `next` array written by ResultsWriter,
  * results-writer.h:54-67): BFS/CC/SSSP uint32[V], PRD/PPR float rank[V]. */
 int cytom_read_values(cytom_ctx* ctx, int prog_id, void* out, size_t bytes);
