@@ -41,8 +41,19 @@ class Config(ctypes.Structure):
     ]
 
 
+STATICS_GIVEN = 1 << 0
+NEEDS_SOURCES_ACTIVE = 1 << 1
+NEEDS_TARGETS_ACTIVE = 1 << 2
+ASYNC = 1 << 3
+USE_CHANGED = 1 << 4
+FIRST_ITERATION_NONE, FIRST_ITERATION_PRD_ALL = 0, 1
+
+
 class ProgramDesc(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_uint32), ("source", ctypes.c_uint32)]
+    _fields_ = [("kind", ctypes.c_uint32), ("source", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32), ("vertex_bytes", ctypes.c_uint32),
+                ("first_iteration", ctypes.c_uint32), ("n_fields", ctypes.c_uint32),
+                ("neutral_element", ctypes.c_double * 3)]
 
 
 class RunStats(ctypes.Structure):
@@ -105,6 +116,7 @@ SIGNATURES = {
     "cytom_program_register": (ctypes.c_int, [_VP, ctypes.POINTER(ProgramDesc),
                                               ctypes.POINTER(ctypes.c_int)]),
     "cytom_program_init": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cytom_program_statics": (ctypes.c_int, [_U32, ctypes.POINTER(ProgramDesc)]),
     "cytom_program_set_sources": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _U32]),
     "cytom_batched_execute": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
     "cytom_ppr_weighted": (ctypes.c_int, [_U64, _U64, _VP, _VP, _VP, _VP, ctypes.c_float,
@@ -350,6 +362,20 @@ class Stream:
             pass
 
 
+def program_statics(kind):
+    """The Algorithm concept's statics of a kind, as a dict (the fields of
+    cytom_program_desc after the kind and source)."""
+    if isinstance(kind, str):
+        kind = PROGRAMS[kind]
+    d = ProgramDesc()
+    rc = lib().cytom_program_statics(kind, ctypes.byref(d))
+    if rc < 0:
+        raise CytomError(rc, "program_statics: unknown kind %r" % (kind,))
+    return {"flags": d.flags, "vertex_bytes": d.vertex_bytes,
+            "first_iteration": d.first_iteration, "n_fields": d.n_fields,
+            "neutral_element": list(d.neutral_element)}
+
+
 class Graph:
     """One evolving graph in HBM plus the vertex programs registered on it."""
 
@@ -387,10 +413,22 @@ class Graph:
         except Exception:
             pass
 
-    def register(self, kind, source=0):
+    def register(self, kind, source=0, statics=None):
+        """Registers a program of kind (name or number) with its source.
+        statics: optional dict of the Algorithm concept's statics
+        (program_statics(kind) gives the kind's own); registration refuses
+        statics the kind's kernels do not implement."""
         if isinstance(kind, str):
             kind = PROGRAMS[kind]
         desc = ProgramDesc(kind, source)
+        if statics is not None:
+            for k, v in statics.items():
+                if k == "neutral_element":
+                    for i, x in enumerate(v):
+                        desc.neutral_element[i] = x
+                else:
+                    setattr(desc, k, v)
+            desc.flags |= STATICS_GIVEN
         pid = ctypes.c_int()
         self._check(lib().cytom_program_register(self._h, ctypes.byref(desc), ctypes.byref(pid)),
                     "register")

@@ -207,10 +207,48 @@ int cytom_geometry(const cytom_ctx* ctx, uint64_t* n_vertices, uint32_t* vertice
   return 0;
 }
 
+int cytom_program_statics(uint32_t kind, cytom_program_desc* out) {
+  if (!out || kind > CYTOM_PPR) return CYTOM_EINVAL;
+  *out = cytom_program_desc{};
+  out->kind = kind;
+  out->flags = CYTOM_STATICS_GIVEN | CYTOM_NEEDS_SOURCES_ACTIVE;
+  if (kind != CYTOM_SSSP) out->flags |= CYTOM_USE_CHANGED;  // sssp.cc:30 states false
+  if (prd_like((int)kind)) {
+    // pagerank-delta.cc:54-57: {delta_sum, rank, delta} = {0, 0, 0}
+    out->vertex_bytes = 12;
+    out->n_fields = 3;
+    out->first_iteration =
+        kind == CYTOM_PRD ? CYTOM_FIRST_ITERATION_PRD_ALL : CYTOM_FIRST_ITERATION_NONE;
+  } else {
+    out->vertex_bytes = 4;
+    out->n_fields = 1;
+    out->first_iteration = CYTOM_FIRST_ITERATION_NONE;
+    // bfs.cc:22, cc.cc:22: UINT32_MAX; sssp.cc:22: FLT_MAX
+    out->neutral_element[0] = kind == CYTOM_SSSP ? 3.4028234663852886e38 : 4294967295.0;
+  }
+  return 0;
+}
+
 int cytom_program_register(cytom_ctx* ctx, const cytom_program_desc* desc, int* prog_id) {
   CY_CHECK_CTX(ctx);
   if (!desc || !prog_id) return set_err(ctx, CYTOM_EINVAL, "null descriptor");
   if (desc->kind > CYTOM_PPR) return set_err(ctx, CYTOM_EINVAL, "unknown program kind");
+  if (desc->flags & CYTOM_STATICS_GIVEN) {
+    cytom_program_desc k;
+    cytom_program_statics(desc->kind, &k);
+    const uint32_t must = CYTOM_NEEDS_SOURCES_ACTIVE | CYTOM_NEEDS_TARGETS_ACTIVE | CYTOM_ASYNC;
+    if ((desc->flags & must) != (k.flags & must))
+      return set_err(ctx, CYTOM_EINVAL,
+                     "statics: the engine's kernels need NEEDS_SOURCES_ACTIVE set and "
+                     "NEEDS_TARGETS_ACTIVE and ASYNC clear");
+    if (desc->vertex_bytes != k.vertex_bytes || desc->n_fields != k.n_fields)
+      return set_err(ctx, CYTOM_EINVAL, "statics: VertexType differs from the kind's");
+    if (desc->first_iteration != k.first_iteration)
+      return set_err(ctx, CYTOM_EINVAL, "statics: first-iteration rule differs from the kind's");
+    for (int f = 0; f < 3; ++f)
+      if (desc->neutral_element[f] != k.neutral_element[f])
+        return set_err(ctx, CYTOM_EINVAL, "statics: neutral_element differs from the kind's");
+  }
   if (desc->kind == CYTOM_SSSP && !ctx->weighted)
     return set_err(ctx, CYTOM_EINVAL, "SSSP needs a weighted graph");
   if ((desc->kind == CYTOM_BFS || desc->kind == CYTOM_SSSP || desc->kind == CYTOM_PPR) &&

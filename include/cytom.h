@@ -57,11 +57,42 @@ typedef struct {
   uint32_t device;            /* HIP device ordinal */
 } cytom_config;
 
-/* Program descriptor: mirrors the statics and constructor arguments of the
- * Algorithm concept (examples/bfs.cc:20-34). */
+/* Boolean statics of the Algorithm concept (examples/bfs.cc:24-30) in
+ * cytom_program_desc.flags, plus CYTOM_STATICS_GIVEN when the caller states
+ * the statics for cytom_program_register to check. */
+#define CYTOM_STATICS_GIVEN (1u << 0)
+#define CYTOM_NEEDS_SOURCES_ACTIVE (1u << 1) /* tile-manager.h:405-427 */
+#define CYTOM_NEEDS_TARGETS_ACTIVE (1u << 2) /* tile-manager.h:429-451 */
+#define CYTOM_ASYNC (1u << 3)
+#define CYTOM_USE_CHANGED (1u << 4)          /* apply-executor.h:65, algorithm-executor.h:496 */
+
+/* First-iteration rule of a program's apply. */
+enum cytom_first_iteration {
+  CYTOM_FIRST_ITERATION_NONE = 0,
+  /* GLOBAL_FIRST_ITERATION (pagerank-delta.cc:146-152): in the first
+   * iteration after init_vertices every vertex applies
+   * delta = (1 - ALPHA) + ALPHA * delta_sum, rank += delta, delta -= 1. */
+  CYTOM_FIRST_ITERATION_PRD_ALL = 1,
+};
+
+/* Program descriptor: the constructor argument and the statics of the
+ * Algorithm concept (examples/bfs.cc:20-34).  kind and source are required;
+ * the statics are optional (flags = 0 skips them): cytom_program_statics
+ * fills them for a kind, and with CYTOM_STATICS_GIVEN registration checks
+ * them against the kind.  neutral_element, vertex_bytes, n_fields,
+ * first_iteration and the NEEDS_SOURCES_ACTIVE, NEEDS_TARGETS_ACTIVE and
+ * ASYNC bits must equal the kind's, since they change the values a run
+ * produces; USE_CHANGED may take either value (the engine's apply visits the
+ * vertices a gather or hook changed, which gives the values of a sweep over
+ * all vertices: only those can have next != current). */
 typedef struct {
-  uint32_t kind;   /* enum cytom_program_kind */
-  uint32_t source; /* algo_start_id for BFS/SSSP (execute.h:312) */
+  uint32_t kind;             /* enum cytom_program_kind */
+  uint32_t source;           /* algo_start_id for BFS/SSSP (execute.h:312) */
+  uint32_t flags;            /* CYTOM_STATICS_GIVEN | CYTOM_NEEDS_SOURCES_ACTIVE | ... */
+  uint32_t vertex_bytes;     /* sizeof(VertexType) in the results layout: 4, or 12 */
+  uint32_t first_iteration;  /* enum cytom_first_iteration */
+  uint32_t n_fields;         /* fields of VertexType: 1, or 3 (delta_sum, rank, delta) */
+  double neutral_element[3]; /* neutral_element, field by field (unused fields 0) */
 } cytom_program_desc;
 
 /* Per-run statistics of cytom_execute
@@ -107,6 +138,10 @@ int cytom_geometry(const cytom_ctx* ctx, uint64_t* n_vertices, uint32_t* vertice
 /* Registers a vertex program on the graph and runs its init_vertices
  * (the main_<name> export + execute<> instantiation, examples/bfs.cc:235-238;
  * ContinuousAlgorithmExecutor::initAlgorithm, continuous-algorithm-executor.h:342). */
+/* The statics of kind, as the reference's algorithm states them
+ * (examples/bfs.cc:20-32, cc.cc:20-32, sssp.cc:20-32,
+ * pagerank-delta.cc:47-65), with CYTOM_STATICS_GIVEN set and source 0. */
+int cytom_program_statics(uint32_t kind, cytom_program_desc* out);
 int cytom_program_register(cytom_ctx* ctx, const cytom_program_desc* desc, int* prog_id);
 /* Re-runs init_vertices (EdgeInserter re-execution, edge-inserter.h:164-167). */
 int cytom_program_init(cytom_ctx* ctx, int prog_id);
