@@ -338,8 +338,9 @@ def gpu_run(args):
         "detail": detail,
         "final": final,
     }
+    res["grad_arrays"] = None
     if args.queries == "on":
-        res["detail"]["batched_queries"] = queries_run(args, g, V, source)
+        res["detail"]["batched_queries"], res["grad_arrays"] = queries_run(args, g, V, source)
     g.close()
     res["detail"]["host_paths"] = api_comparison(args, scale=max(args.scale - 2, 10))
     return res
@@ -462,7 +463,7 @@ def queries_run(args, g, V, source):
                                                                           tb * 1e3, tp * 1e3))
     if args.q_graphs:
         out["graphs"] = tenants_run(args)
-    grad = ppr_grad_run(args) if args.ppr_grad == "on" else None
+    grad, grad_arrays = ppr_grad_run(args) if args.ppr_grad == "on" else (None, None)
     tq = sum(v["queries"] for v in out.values())
     tb = sum(v["batched_ms"] for v in out.values())
     tp = sum(v["plain_ms"] for v in out.values())
@@ -472,14 +473,16 @@ def queries_run(args, g, V, source):
                     "speedup": round(tp / tb, 2) if tb else 0.0}
     if grad:
         out["ppr_grad"] = grad
-    return out
+    return out, grad_arrays
 
 
 def ppr_grad_run(args):
     """Differentiable weighted PPR on RMAT-20 (2^24 edges, U(0,1] weights,
     256 teleport vertices, loss = sum of r_i^2 over 1,024 targets): time of
     the operator build (edge upload, CSR/CSC rows, transition probabilities),
-    then of one forward and one adjoint on it."""
+    then of one forward and one adjoint on it, and central differences
+    (h = 1e-3) on 512 edges.  Returns the record and the arrays the CPU leg
+    checks against the FP64 restatement."""
     import cytom_amd as C
     from cytom_amd import autograd as A
 
@@ -502,21 +505,21 @@ def ppr_grad_run(args):
     dw = C.DeviceArray(E, np.float32)
     ds = C.DeviceArray(V, np.float32)
     ta, sweeps_a = _timed(lambda: G.adjoint_device(r_dev, g_dev, dw, ds, tol=1e-10))
-    # Central differences (h = 1e-3) of the loss on edges drawn with
-    # probability proportional to |dL/dw| (most edges' gradients are below the
-    # FP32 resolution of the loss), against the adjoint's gradient.
+    # Central differences of the loss on 512 edges drawn with probability
+    # proportional to |dL/dw| (most edges' gradients are below the FP32
+    # resolution of the loss), against the adjoint's gradient; a fixed sweep
+    # count, so both sides of a difference run the same iteration.
     grad = dw.to_host()
-    n_fd = 64
+    n_fd, sweeps_fd, h = 512, 160, 1e-3
     pw = np.abs(grad).astype(np.float64)
     picks = rng.choice(E, n_fd, replace=False, p=pw / pw.sum())
 
     def loss(wv):
         G.set_weights(wv)
-        G.forward_device(t_dev, r_dev, tol=1e-10)
+        G.forward_device(t_dev, r_dev, tol=0.0, max_iter=sweeps_fd)
         rv = r_dev.to_host().astype(np.float64)
         return float(np.sum(rv[targets] ** 2))
 
-    h = 1e-3
     t_fd = time.perf_counter()
     fd = []
     for e in picks:
@@ -531,11 +534,15 @@ def ppr_grad_run(args):
     log("bench: PPR gradient on RMAT-20: build %.1f ms, forward %.1f ms (%d sweeps), adjoint %.1f ms "
         "(%d sweeps); central differences on %d edges: rel. error %.2e"
         % (tb * 1e3, tf * 1e3, sweeps_f, ta * 1e3, sweeps_a, n_fd, rel_fd))
-    return {"edges": E, "build_ms": round(tb * 1e3, 2), "forward_ms": round(tf * 1e3, 2),
-            "forward_sweeps": sweeps_f, "adjoint_ms": round(ta * 1e3, 2),
-            "adjoint_sweeps": sweeps_a,
-            "fd_check": {"edges": n_fd, "h": h, "rel_error_vs_central_differences": rel_fd,
-                         "seconds": round(t_fd, 2)}}
+    rec = {"edges": E, "build_ms": round(tb * 1e3, 2), "forward_ms": round(tf * 1e3, 2),
+           "forward_sweeps": sweeps_f, "adjoint_ms": round(ta * 1e3, 2),
+           "adjoint_sweeps": sweeps_a,
+           "fd_check": {"edges": n_fd, "h": h, "sweeps": sweeps_fd,
+                        "rel_error_vs_central_differences": rel_fd,
+                        "seconds": round(t_fd, 2)}}
+    arrays = {"V": V, "src": sh, "dst": dh, "w": w, "t": t, "targets": targets, "r": r,
+              "dw": grad, "ds": ds.to_host(), "picks": picks, "fd": fd}
+    return rec, arrays
 
 
 def tenants_run(args):
@@ -710,6 +717,36 @@ def cpu_baseline(args, source, gpu_final):
     return out
 
 
+def grad_fp64_check(a):
+    """The CPU leg's check of the PPR-gradient record: the FP64 restatement's
+    forward and adjoint on the same RMAT-20 operator, the relative errors of
+    the GPU's r, dL/dw and dL/ds against it, and of the FP64 dL/dw against
+    the same central differences."""
+    from oracle import ppr_grad as PG
+
+    t0 = time.perf_counter()
+    V, s, d = a["V"], a["src"], a["dst"]
+    w64 = a["w"].astype(np.float64)
+    r64 = PG.forward(V, s, d, w64, a["t"].astype(np.float64), tol=1e-13)
+    _, g64 = PG.loss_sq(r64, a["targets"])
+    dw64, ds64, _ = PG.adjoint(V, s, d, w64, r64, g64, tol=1e-13)
+
+    def rel(x, y):
+        x = np.asarray(x, np.float64)
+        y = np.asarray(y, np.float64)
+        return float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+
+    out = {"rel_error_r_vs_fp64": rel(a["r"], r64),
+           "rel_error_dw_vs_fp64_adjoint": rel(a["dw"], dw64),
+           "rel_error_ds_vs_fp64_adjoint": rel(a["ds"], ds64),
+           "fp64_adjoint_rel_error_vs_central_differences": rel(dw64[a["picks"]], a["fd"]),
+           "seconds": round(time.perf_counter() - t0, 1)}
+    log("bench: PPR gradient vs FP64: r %.2e, dL/dw %.2e, dL/ds %.2e (%.1fs)"
+        % (out["rel_error_r_vs_fp64"], out["rel_error_dw_vs_fp64_adjoint"],
+           out["rel_error_ds_vs_fp64_adjoint"], out["seconds"]))
+    return out
+
+
 def l1n(a, b):
     """L1 distance of two rank vectors after L1 normalisation."""
     a = np.asarray(a, np.float64)
@@ -731,6 +768,9 @@ def main():
     if args.cpu_baseline == "auto":
         cpu = cpu_baseline(args, res["source"], res["final"])
         res["detail"]["final_parity"] = cpu.pop("final_parity")
+        if res["grad_arrays"] is not None:
+            res["detail"]["batched_queries"]["ppr_grad"]["fp64_check"] = \
+                grad_fp64_check(res["grad_arrays"])
     names = [p.strip() for p in args.programs.split(",") if p.strip()]
     line = {
         "metric": METRIC,

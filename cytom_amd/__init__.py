@@ -144,6 +144,9 @@ SIGNATURES = {
                                                   ctypes.POINTER(BatchStats)]),
     "cytom_read_results": (ctypes.c_int64, [_VP, ctypes.c_int, _VP, ctypes.c_size_t]),
     "cytom_execute": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(RunStats)]),
+    "cytom_ppr_program_adjoint": (ctypes.c_int64, [_VP, ctypes.c_int, _VP, ctypes.c_float, _U32,
+                                                 _VP, _VP, _VP, _VP, _U64,
+                                                 ctypes.POINTER(ctypes.c_uint32)]),
     "cytom_step": (ctypes.c_int, [_VP, _VP, _VP, _VP, _U64, _VP, _VP, _VP, _U64, _VP, _U32,
                                   ctypes.POINTER(BatchStats), _VP]),
     "cytom_read_values": (ctypes.c_int, [_VP, ctypes.c_int, _VP, ctypes.c_size_t]),
@@ -511,6 +514,23 @@ class Graph:
         st = RunStats()
         self._check(lib().cytom_execute(self._h, pid, ctypes.byref(st)), "execute")
         return st
+
+    def ppr_adjoint(self, pid, dL_dr, tol=0.0, max_iterations=400):
+        """Reverse mode of a registered PPR program on the current graph:
+        returns (dL/dt float32[V], src, dst, dL/dw float32[E] per live edge in
+        export_edges order, sweeps).  dL_dr: float32[V] (host or DeviceArray).
+        The defaults run 400 sweeps (0.85^400: FP32 convergence)."""
+        g = dL_dr if isinstance(dL_dr, DeviceArray) else \
+            DeviceArray.from_host(np.ascontiguousarray(dL_dr, np.float32))
+        E = self.live_edge_count()
+        dt = DeviceArray(self.V, np.float32)
+        s, d, dw = (DeviceArray(max(E, 1), np.uint32), DeviceArray(max(E, 1), np.uint32),
+                    DeviceArray(max(E, 1), np.float32))
+        it = ctypes.c_uint32()
+        n = self._check(lib().cytom_ppr_program_adjoint(
+            self._h, pid, _p(g), tol, max_iterations, _p(dt), _p(s), _p(d), _p(dw), max(E, 1),
+            ctypes.byref(it)), "ppr_adjoint")
+        return dt.to_host(), s.to_host()[:n], d.to_host()[:n], dw.to_host()[:n], it.value
 
     def step(self, pids, src, dst, w, n_ins, dsrc=None, ddst=None, dw=None, n_del=0):
         """One update batch from device arrays (as update_device) followed by

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libcytom_hip.so")
 SOURCES = ["prims.hip", "rmat.hip", "store.hip", "exec.hip", "query.hip", "ppr_grad.hip", "io.hip",
-           "engine.hip"]
+           "program_grad.hip", "engine.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CYTOM_OFFLOAD_ARCH", "gfx950")

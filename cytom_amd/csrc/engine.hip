@@ -431,6 +431,17 @@ int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, con
   return 0;
 }
 
+int64_t cytom_ppr_program_adjoint(cytom_ctx* ctx, int prog_id, const float* d_dL_dr, float tol,
+                                  uint32_t max_iterations, float* d_dL_dt, uint32_t* d_src,
+                                  uint32_t* d_dst, float* d_dL_dw, uint64_t cap,
+                                  uint32_t* iterations) {
+  CY_CHECK_CTX(ctx);
+  if (prog_id < 0 || prog_id >= (int)ctx->progs.size())
+    return set_err(ctx, CYTOM_EINVAL, "bad program id");
+  return ppr_program_adjoint(ctx, ctx->progs[prog_id], d_dL_dr, tol, max_iterations, d_dL_dt, d_src,
+                             d_dst, d_dL_dw, cap, iterations);
+}
+
 int cytom_batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq,
                           cytom_run_stats* stats) {
   CY_CHECK_CTX(ctx);

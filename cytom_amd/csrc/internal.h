@@ -334,6 +334,13 @@ void enqueue_jobs(cytom_ctx* ctx);
 // The current chunk count (one host round trip).
 int64_t jobs_chunk_count(cytom_ctx* ctx);
 
+// ------------------------------------------------------------ program adjoint
+// Reverse mode of a registered CYTOM_PPR program on the current graph; the
+// live-edge count, or a negative error code.
+int64_t ppr_program_adjoint(cytom_ctx* ctx, Program& P, const float* d_g, float tol,
+                            uint32_t max_iterations, float* d_ds, uint32_t* d_src,
+                            uint32_t* d_dst, float* d_dw, uint64_t cap, uint32_t* iterations);
+
 // ------------------------------------------------------------ batched queries
 int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run_stats* stats);
 int read_query(cytom_ctx* ctx, uint32_t query, void* out, size_t bytes);

@@ -290,6 +290,26 @@ int cytom_ppr_weighted_noalloc(uint64_t n_vertices, uint64_t n_edges, const uint
                                const uint32_t* d_dst, const float* d_w, const float* d_s,
                                float alpha, float tol, uint32_t max_iterations, float* d_r,
                                uint32_t* iterations);
+/* Reverse mode of a registered personalized-PageRank program (CYTOM_PPR)
+ * on the context's current graph (no reference counterpart).  The
+ * program's ranks r solve r = (1-alpha) t + alpha P^T r over the live
+ * multigraph with P_uv = 1/out_degree(u) per live edge, i.e. every edge
+ * weight 1 in the operator of cytom_ppr_weighted.  For d_dL_dr (float[V],
+ * device) this solves lambda = dL/dr + alpha P lambda by FP32 Jacobi sweeps
+ * over the tile store's live slots in the reverse orientation of the
+ * program's gather, from lambda = dL/dr until the largest change is <= tol
+ * or max_iterations sweeps (0: 1000), and writes dL/dt = (1-alpha) lambda
+ * (d_dL_dt, float[V], may be NULL) and, for every live edge in the order of
+ * cytom_export_edges, its endpoints and dL/dw at w = 1,
+ * alpha r_u (lambda_v - mu_u) / out_degree(u) with mu_u the mean lambda over
+ * u's live out-edges (d_src, d_dst, d_dL_dw: cap entries each, any may be
+ * NULL).  r is the program's rank from its last run.  Returns the number of
+ * live edges; CYTOM_ERANGE when an edge output is given and cap is below it,
+ * CYTOM_EINVAL for a program of another kind. */
+int64_t cytom_ppr_program_adjoint(cytom_ctx* ctx, int prog_id, const float* d_dL_dr, float tol,
+                                  uint32_t max_iterations, float* d_dL_dt, uint32_t* d_src,
+                                  uint32_t* d_dst, float* d_dL_dw, uint64_t cap,
+                                  uint32_t* iterations);
 /* Message of the last failed cytom_ppr_* call on this thread. */
 const char* cytom_ppr_last_error(void);
 
