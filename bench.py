@@ -504,7 +504,9 @@ def ppr_grad_run(args):
     g_dev = C.DeviceArray.from_host(g)
     dw = C.DeviceArray(E, np.float32)
     ds = C.DeviceArray(V, np.float32)
-    ta, sweeps_a = _timed(lambda: G.adjoint_device(r_dev, g_dev, dw, ds, tol=1e-10))
+    # lambda is of the order of the ranks (~1e-6): 1e-14 keeps its relative
+    # error near 1e-8
+    ta, sweeps_a = _timed(lambda: G.adjoint_device(r_dev, g_dev, dw, ds, tol=1e-14))
     # Central differences of the loss on 512 edges drawn with probability
     # proportional to |dL/dw| (most edges' gradients are below the FP32
     # resolution of the loss), against the adjoint's gradient; a fixed sweep

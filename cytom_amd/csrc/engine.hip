@@ -11,6 +11,7 @@
 #include <new>
 
 #include "internal.h"
+#include "tuning_table.h"
 
 namespace cytom {
 
@@ -114,6 +115,31 @@ struct cytom_stream {
 
 extern "C" {
 
+namespace {
+
+// The autotuned row for a context of V vertices (see tuning_table.h).
+const TuneRow* tune_row(uint64_t V, uint64_t edge_capacity) {
+  uint32_t lv = 0;
+  while ((1ull << lv) < V) ++lv;
+  const uint64_t ef = edge_capacity ? std::max<uint64_t>(edge_capacity / V, 1) : 16;
+  uint64_t best_gap = ~0ull;
+  for (const TuneRow& r : TUNE_TABLE) {
+    const uint64_t gap = r.edge_factor > ef ? r.edge_factor - ef : ef - r.edge_factor;
+    best_gap = std::min(best_gap, gap);
+  }
+  const TuneRow* below = nullptr;
+  const TuneRow* above = nullptr;
+  for (const TuneRow& r : TUNE_TABLE) {
+    const uint64_t gap = r.edge_factor > ef ? r.edge_factor - ef : ef - r.edge_factor;
+    if (gap != best_gap) continue;
+    if (r.log2v <= lv && (!below || r.log2v > below->log2v)) below = &r;
+    if (r.log2v > lv && (!above || r.log2v < above->log2v)) above = &r;
+  }
+  return below ? below : above;
+}
+
+}  // namespace
+
 int cytom_create(const cytom_config* cfg, cytom_ctx** out) {
   if (!cfg || !out) return CYTOM_EINVAL;
   *out = nullptr;
@@ -121,7 +147,14 @@ int cytom_create(const cytom_config* cfg, cytom_ctx** out) {
   uint64_t V = 64;
   while (V < cfg->n_vertices) V <<= 1;
   uint32_t vpt = cfg->vertices_per_tile;
-  if (vpt == 0) vpt = 16384;
+  const TuneRow* tune = nullptr;
+  if (vpt == 0) {
+    tune = tune_row(V, cfg->edge_capacity);
+    vpt = tune->vpt;
+    // A row measured on a smaller graph: wider tiles until the tile count
+    // stays within 2^26 (tile metadata and work lists grow with it).
+    while (vpt < 16384 && vpt < V && (V / vpt) * (V / vpt) > (1ull << 26)) vpt <<= 1;
+  }
   // 16-bit local ids in the edge slots and an 8-byte LDS column window.
   if (vpt < 64 || vpt > 16384 || (vpt & (vpt - 1))) return CYTOM_EINVAL;
   if (vpt > V) vpt = (uint32_t)V;
@@ -140,6 +173,10 @@ int cytom_create(const cytom_config* cfg, cytom_ctx** out) {
   c->n_tiles = (uint64_t)c->T * c->T;
   c->prd_shift = max_prd_shift(V);
   c->max_iter = cfg->max_iterations ? cfg->max_iterations : 100;
+  if (tune) {
+    c->chunk_slots = tune->chunk_slots;
+    c->source_bands = tune->source_bands;
+  }
   c->delta = cfg->delta;
   c->weighted = cfg->weighted != 0;
   auto fail = [&](int code) {
