@@ -357,6 +357,7 @@ struct GatherArgs {
   const uint8_t* gact;         // active groups, 64 bytes per source row
   uint64_t* gact_clear;        // the next iteration's group bytes, cleared here (T * 8 words)
   const uint64_t* grp;         // source groups of every 64-slot pool row
+  const uint64_t* act;         // this iteration's active bitmap
   const uint32_t* fval;
   uint32_t* nxt;
   const float* contrib;
@@ -414,7 +415,8 @@ __device__ inline void load_rows(const GatherArgs<KIND>& A, const uint64_t* rows
 // active-bitmap test costs no extra load.
 template <int KIND>
 __device__ inline void gather_rows(const GatherArgs<KIND>& A, uint32_t* win,
-                                   unsigned long long* winq, const uint64_t* rows, uint32_t R) {
+                                   unsigned long long* winq, const uint64_t* rows, uint32_t R,
+                                   bool sparse) {
   const uint32_t step = (GATHER_THREADS / 64) * GATHER_ROWS;
   uint32_t r0 = (threadIdx.x >> 6) * GATHER_ROWS;
   if (r0 >= R) return;
@@ -433,15 +435,35 @@ __device__ inline void gather_rows(const GatherArgs<KIND>& A, uint32_t* win,
     float c[GATHER_ROWS];
     uint32_t val[GATHER_ROWS];
     uint2 val2[GATHER_ROWS];
+    if (KIND != CYTOM_PRD && sparse) {
+      // Sparse frontier: the active bitmap (V/8 bytes, held by the L2) says
+      // which sources are active, so inactive sources' values, scattered over
+      // 4 B per vertex, are not fetched.
+#pragma unroll
+      for (int u = 0; u < GATHER_ROWS; ++u)
+        ok[u] = ok[u] && ((A.act[s[u] >> 6] >> (s[u] & 63u)) & 1ull);
+    }
     if (KIND == CYTOM_PRD) {
 #pragma unroll
       for (int u = 0; u < GATHER_ROWS; ++u) c[u] = A.contrib[s[u]];
     } else if (KIND == KIND_SSSP_PAIR) {
+      if (sparse) {
 #pragma unroll
-      for (int u = 0; u < GATHER_ROWS; ++u) val2[u] = reinterpret_cast<const uint2*>(A.fval)[s[u]];
+        for (int u = 0; u < GATHER_ROWS; ++u)
+          val2[u] = ok[u] ? reinterpret_cast<const uint2*>(A.fval)[s[u]] : make_uint2(INF, INF);
+      } else {
+#pragma unroll
+        for (int u = 0; u < GATHER_ROWS; ++u)
+          val2[u] = reinterpret_cast<const uint2*>(A.fval)[s[u]];
+      }
     } else {
+      if (sparse) {
 #pragma unroll
-      for (int u = 0; u < GATHER_ROWS; ++u) val[u] = A.fval[s[u]];
+        for (int u = 0; u < GATHER_ROWS; ++u) val[u] = ok[u] ? A.fval[s[u]] : INF;
+      } else {
+#pragma unroll
+        for (int u = 0; u < GATHER_ROWS; ++u) val[u] = A.fval[s[u]];
+      }
     }
     // Issued unconditionally (past the end the clamped descriptors point at
     // the last row, a valid address): a conditional block here makes the
@@ -497,6 +519,10 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 8 * T; i += gridDim.x * blockDim.x)
     A.gact_clear[i] = 0;
   if (A.it->count == 0) return;  // empty frontier (uniform)
+  // A frontier whose live out-edges are below V/64 tests the active bitmap
+  // before fetching a source value (measured: only when active slots are this
+  // rare does skipping the value loads outweigh the extra bitmap load).
+  const bool sparse = A.it->te * 64 < A.S.V;
   const uint32_t NB = A.nbands;
   const uint32_t home = blockIdx.x % NB;
   // A job's chunks are taken in passes of at most PASS chunks (one thread
@@ -573,7 +599,7 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
         cleared = true;
       }
       __syncthreads();
-      gather_rows<KIND>(A, win, winq, s_rows, R);
+      gather_rows<KIND>(A, win, winq, s_rows, R, sparse);
       __syncthreads();
     }
     if (!cleared) continue;  // no active row in the job
@@ -955,6 +981,7 @@ void launch_gather(cytom_ctx* ctx, Program& P, int it) {
   A.chunks = static_cast<const uint64_t*>(ctx->g_chunks.p);
   A.S = ctx->view();
   A.gact = P.gact[P.cur_act];
+  A.act = P.act[P.cur_act];
   A.gact_clear = reinterpret_cast<uint64_t*>(P.gact[P.cur_act ^ 1]);
   A.grp = static_cast<const uint64_t*>(ctx->g_grp.p);
   A.fval = P.fval;
