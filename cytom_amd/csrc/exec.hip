@@ -33,6 +33,10 @@ namespace {
 constexpr int GATHER_THREADS = 1024;
 constexpr int APPLY_THREADS = 1024;
 constexpr int GATHER_ROWS = 8;  // 64-slot rows per wavefront step
+// Rows per step of a kind: the eight-query SSSP kind holds 8 distances per
+// row in flight, so it takes 4 rows per step to stay within its registers.
+template <int KIND>
+constexpr int RPS = KIND == KIND_SSSP_OCT ? 4 : GATHER_ROWS;
 
 inline uint32_t grid_for(uint64_t n, uint32_t threads, uint32_t cap = 65536) {
   uint64_t b = (n + threads - 1) / threads;
@@ -170,6 +174,36 @@ __global__ void k_init_ppr_set(const uint32_t* __restrict__ set, uint32_t n, flo
   }
 }
 
+// W SSSP queries per vertex (the batched queries' eight-query kind): every
+// distance INF, nothing active.
+template <uint32_t W>
+__global__ void k_init_vec(uint64_t V, uint32_t* __restrict__ cur, uint32_t* __restrict__ nxt,
+                           uint64_t* __restrict__ act, uint8_t* __restrict__ changed) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (uint32_t q = 0; q < W; ++q) {
+      cur[v * W + q] = INF;
+      nxt[v * W + q] = INF;
+    }
+    changed[v] = 0;
+    if ((v & 63) == 0) act[v >> 6] = 0;
+  }
+}
+
+// Sources: seed[i] is the source of query seed[n + i] (distance 0, active).
+template <uint32_t W>
+__global__ void k_seed_vec(const uint32_t* __restrict__ seed, uint32_t n,
+                           uint32_t* __restrict__ cur, uint32_t* __restrict__ nxt,
+                           uint64_t* __restrict__ act) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t v = seed[i], q = seed[n + i];
+    cur[(uint64_t)v * W + q] = 0;
+    nxt[(uint64_t)v * W + q] = 0;
+    atomicOr(reinterpret_cast<unsigned long long*>(&act[v >> 6]), 1ull << (v & 63));
+  }
+}
+
 // ------------------------------------------------------------- frontier
 // Frontier size, live out-edges and row activity of the active bitmap
 // (TileManager::isActive over full words); PRD contributions of active
@@ -197,7 +231,9 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_frontier(
       ++cnt;
     }
     // Frontier values: what the gather reads per edge instead of the bitmap.
-    if (prd_like(kind)) {
+    if (kind == KIND_SSSP_OCT) {
+      for (uint32_t q = 0; q < 8; ++q) fval[v * 8 + q] = a ? cur[v * 8 + q] : INF;
+    } else if (prd_like(kind)) {
       contrib[v] = (a && od) ? delta[v] / (float)od : 0.f;
     } else if (kind == KIND_SSSP_PAIR) {
       reinterpret_cast<uint2*>(fval)[v] =
@@ -386,22 +422,22 @@ __device__ inline uint64_t row_desc(uint64_t off, uint32_t x, uint32_t nvalid) {
 
 template <int KIND>
 __device__ inline void load_rows(const GatherArgs<KIND>& A, const uint64_t* rows, uint32_t R,
-                                 uint32_t r0, uint32_t (&e)[GATHER_ROWS],
-                                 uint32_t (&wt)[GATHER_ROWS], uint32_t (&xb)[GATHER_ROWS],
+                                 uint32_t r0, uint32_t (&e)[RPS<KIND>],
+                                 uint32_t (&wt)[RPS<KIND>], uint32_t (&xb)[RPS<KIND>],
                                  uint32_t& vmask) {
   const uint32_t lane = threadIdx.x & 63;
-  uint64_t d[GATHER_ROWS];
+  uint64_t d[RPS<KIND>];
 #pragma unroll
-  for (int u = 0; u < GATHER_ROWS; ++u) d[u] = rows[min(r0 + u, R - 1)];
+  for (int u = 0; u < RPS<KIND>; ++u) d[u] = rows[min(r0 + u, R - 1)];
   vmask = 0;
 #pragma unroll
-  for (int u = 0; u < GATHER_ROWS; ++u) {
+  for (int u = 0; u < RPS<KIND>; ++u) {
     const bool v = r0 + u < R && lane <= (uint32_t)(d[u] & 63u);
     const uint64_t idx = (d[u] >> 22) + (v ? lane : 0u);
     // Slots are read once per launch: non-temporal loads keep them from
     // evicting the source values the L2 should hold.
     e[u] = __builtin_nontemporal_load(&A.S.pool[idx]);
-    if (KIND == CYTOM_SSSP || KIND == KIND_SSSP_PAIR)
+    if (KIND == CYTOM_SSSP || KIND == KIND_SSSP_PAIR || KIND == KIND_SSSP_OCT)
       wt[u] = __builtin_nontemporal_load(&A.S.wpool[idx]);
     xb[u] = ((uint32_t)(d[u] >> 6) & 0xFFFFu) << A.S.log2vpt;
     vmask |= (v ? 1u : 0u) << u;
@@ -417,74 +453,96 @@ template <int KIND>
 __device__ inline void gather_rows(const GatherArgs<KIND>& A, uint32_t* win,
                                    unsigned long long* winq, const uint64_t* rows, uint32_t R,
                                    bool sparse) {
-  const uint32_t step = (GATHER_THREADS / 64) * GATHER_ROWS;
-  uint32_t r0 = (threadIdx.x >> 6) * GATHER_ROWS;
+  const uint32_t step = (GATHER_THREADS / 64) * RPS<KIND>;
+  uint32_t r0 = (threadIdx.x >> 6) * RPS<KIND>;
   if (r0 >= R) return;
-  uint32_t e[GATHER_ROWS], wt[GATHER_ROWS], xb[GATHER_ROWS], vm;
+  uint32_t e[RPS<KIND>], wt[RPS<KIND>], xb[RPS<KIND>], vm;
   load_rows<KIND>(A, rows, R, r0, e, wt, xb, vm);
   for (;;) {
     const uint32_t rn = r0 + step;
     const bool more = rn < R;
-    uint32_t s[GATHER_ROWS];
-    bool ok[GATHER_ROWS];
+    uint32_t s[RPS<KIND>];
+    bool ok[RPS<KIND>];
 #pragma unroll
-    for (int u = 0; u < GATHER_ROWS; ++u) {
+    for (int u = 0; u < RPS<KIND>; ++u) {
       ok[u] = ((vm >> u) & 1u) && e[u] != TOMB;
       s[u] = ok[u] ? xb[u] + (e[u] & 0xFFFFu) : 0u;
     }
-    float c[GATHER_ROWS];
-    uint32_t val[GATHER_ROWS];
-    uint2 val2[GATHER_ROWS];
+    float c[RPS<KIND>];
+    uint32_t val[RPS<KIND>];
+    uint2 val2[RPS<KIND>];
+    uint4 v8[RPS<KIND>][2];
     if (KIND != CYTOM_PRD && sparse) {
       // Sparse frontier: the active bitmap (V/8 bytes, held by the L2) says
       // which sources are active, so inactive sources' values, scattered over
       // 4 B per vertex, are not fetched.
 #pragma unroll
-      for (int u = 0; u < GATHER_ROWS; ++u)
+      for (int u = 0; u < RPS<KIND>; ++u)
         ok[u] = ok[u] && ((A.act[s[u] >> 6] >> (s[u] & 63u)) & 1ull);
     }
     if (KIND == CYTOM_PRD) {
 #pragma unroll
-      for (int u = 0; u < GATHER_ROWS; ++u) c[u] = A.contrib[s[u]];
+      for (int u = 0; u < RPS<KIND>; ++u) c[u] = A.contrib[s[u]];
+    } else if (KIND == KIND_SSSP_OCT) {
+#pragma unroll
+      for (int u = 0; u < RPS<KIND>; ++u) {
+        if (!sparse || ok[u]) {
+          v8[u][0] = reinterpret_cast<const uint4*>(A.fval)[2 * (uint64_t)s[u]];
+          v8[u][1] = reinterpret_cast<const uint4*>(A.fval)[2 * (uint64_t)s[u] + 1];
+        } else {
+          v8[u][0] = v8[u][1] = make_uint4(INF, INF, INF, INF);
+        }
+      }
     } else if (KIND == KIND_SSSP_PAIR) {
       if (sparse) {
 #pragma unroll
-        for (int u = 0; u < GATHER_ROWS; ++u)
+        for (int u = 0; u < RPS<KIND>; ++u)
           val2[u] = ok[u] ? reinterpret_cast<const uint2*>(A.fval)[s[u]] : make_uint2(INF, INF);
       } else {
 #pragma unroll
-        for (int u = 0; u < GATHER_ROWS; ++u)
+        for (int u = 0; u < RPS<KIND>; ++u)
           val2[u] = reinterpret_cast<const uint2*>(A.fval)[s[u]];
       }
     } else {
       if (sparse) {
 #pragma unroll
-        for (int u = 0; u < GATHER_ROWS; ++u) val[u] = ok[u] ? A.fval[s[u]] : INF;
+        for (int u = 0; u < RPS<KIND>; ++u) val[u] = ok[u] ? A.fval[s[u]] : INF;
       } else {
 #pragma unroll
-        for (int u = 0; u < GATHER_ROWS; ++u) val[u] = A.fval[s[u]];
+        for (int u = 0; u < RPS<KIND>; ++u) val[u] = A.fval[s[u]];
       }
     }
     // Issued unconditionally (past the end the clamped descriptors point at
     // the last row, a valid address): a conditional block here makes the
     // compiler wait for these loads before the atomics below.
-    uint32_t en[GATHER_ROWS], wtn[GATHER_ROWS], xbn[GATHER_ROWS], vmn = 0;
+    uint32_t en[RPS<KIND>], wtn[RPS<KIND>], xbn[RPS<KIND>], vmn = 0;
     load_rows<KIND>(A, rows, R, rn, en, wtn, xbn, vmn);
     if (KIND == CYTOM_PRD) {
 #pragma unroll
-      for (int u = 0; u < GATHER_ROWS; ++u)
+      for (int u = 0; u < RPS<KIND>; ++u)
         if (ok[u] && c[u] != 0.f)  // pagerank-delta.cc:127-132, as exact fixed point
           atomicAdd(&winq[e[u] >> 16], (unsigned long long)__float2ll_rn(c[u] * A.qscale));
+    } else if (KIND == KIND_SSSP_OCT) {
+#pragma unroll
+      for (int u = 0; u < RPS<KIND>; ++u) {  // sssp.cc:90-99 per query
+        const uint32_t dq[8] = {v8[u][0].x, v8[u][0].y, v8[u][0].z, v8[u][0].w,
+                                v8[u][1].x, v8[u][1].y, v8[u][1].z, v8[u][1].w};
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const uint32_t cand = sat_add(dq[q], wt[u]);
+          if (ok[u] && cand != INF) atomicMin(&win[8 * (e[u] >> 16) + q], cand);
+        }
+      }
     } else if (KIND == KIND_SSSP_PAIR) {
 #pragma unroll
-      for (int u = 0; u < GATHER_ROWS; ++u) {  // sssp.cc:90-99 for each query of the pair
+      for (int u = 0; u < RPS<KIND>; ++u) {  // sssp.cc:90-99 for each query of the pair
         const uint32_t cx = sat_add(val2[u].x, wt[u]), cy = sat_add(val2[u].y, wt[u]);
         if (ok[u] && cx != INF) atomicMin(&win[2 * (e[u] >> 16)], cx);
         if (ok[u] && cy != INF) atomicMin(&win[2 * (e[u] >> 16) + 1], cy);
       }
     } else {
 #pragma unroll
-      for (int u = 0; u < GATHER_ROWS; ++u) {
+      for (int u = 0; u < RPS<KIND>; ++u) {
         uint32_t cand;
         if (KIND == CYTOM_BFS) cand = val[u] == INF ? INF : val[u] + 1;  // bfs.cc:83-91
         else if (KIND == CYTOM_CC) cand = val[u];                         // cc.cc:81-91
@@ -494,7 +552,7 @@ __device__ inline void gather_rows(const GatherArgs<KIND>& A, uint32_t* win,
     }
     if (!more) break;
 #pragma unroll
-    for (int u = 0; u < GATHER_ROWS; ++u) {
+    for (int u = 0; u < RPS<KIND>; ++u) {
       e[u] = en[u];
       wt[u] = wtn[u];
       xb[u] = xbn[u];
@@ -591,6 +649,8 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
       if (!cleared) {
         if (KIND == CYTOM_PRD) {
           for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) winq[i] = 0ull;
+        } else if (KIND == KIND_SSSP_OCT) {
+          for (uint32_t i = threadIdx.x; i < 8 * vpt; i += GATHER_THREADS) win[i] = INF;
         } else if (KIND == KIND_SSSP_PAIR) {
           for (uint32_t i = threadIdx.x; i < 2 * vpt; i += GATHER_THREADS) win[i] = INF;
         } else {
@@ -611,6 +671,15 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
           atomicAdd(reinterpret_cast<unsigned long long*>(&A.dsum[base + i]), q);
           A.changed[base + i] = 1;
         }
+      } else if (KIND == KIND_SSSP_OCT) {
+        bool ch = false;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+          const uint32_t val = win[8 * i + k];
+          uint32_t* dst = &A.nxt[8 * (base + i) + k];
+          if (val != INF && val < *dst) ch |= val < atomicMin(dst, val);
+        }
+        if (ch) A.changed[base + i] = 1;
       } else if (KIND == KIND_SSSP_PAIR) {
         bool ch = false;
 #pragma unroll
@@ -765,6 +834,54 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_apply_pair(
       }
     }
     if (was && !a) fval[v] = make_uint2(INF, INF);
+    const uint64_t word = __ballot(a);
+    if (lane == 0) {
+      act_next[w] = word;
+      mark_groups(gact_next, word, v, log2vpt);
+    }
+  }
+  apply_tail(te, cnt, upd, nit);
+}
+
+// Apply of a W-query SSSP run (the pair rule, W wide): a vertex is active
+// when any of its distances dropped; its frontier values carry the dropped
+// distances and INF for the others.
+template <uint32_t W>
+__global__ void __launch_bounds__(APPLY_THREADS) k_apply_vec_min(
+    uint64_t V, uint32_t log2vpt, uint32_t* __restrict__ cur, const uint32_t* __restrict__ nxt,
+    uint32_t* __restrict__ fval, uint8_t* __restrict__ changed,
+    const uint64_t* __restrict__ act_cur, uint64_t* __restrict__ act_next,
+    uint8_t* __restrict__ gact_next, const uint32_t* __restrict__ outdeg, IterStats* nit) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nwords = V >> 6;
+  if (empty_frontier(nit, act_next, nwords)) return;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long te = 0, cnt = 0, upd = 0;
+  for (uint64_t w = wave; w < nwords; w += stride) {
+    const uint64_t v = (w << 6) | lane;
+    const bool was = (act_cur[w] >> lane) & 1ull;
+    bool a = false;
+    if (changed[v]) {
+      changed[v] = 0;
+      ++upd;
+#pragma unroll
+      for (uint32_t q = 0; q < W; ++q) {
+        const uint64_t k = v * W + q;
+        const uint32_t n = nxt[k], c = cur[k];
+        const bool dq = n != c;
+        if (dq) cur[k] = n;
+        fval[k] = dq ? n : INF;
+        a = a || dq;
+      }
+      if (a) {
+        te += outdeg[v];
+        ++cnt;
+      }
+    } else if (was) {
+#pragma unroll
+      for (uint32_t q = 0; q < W; ++q) fval[v * W + q] = INF;
+    }
     const uint64_t word = __ballot(a);
     if (lane == 0) {
       act_next[w] = word;
@@ -963,6 +1080,13 @@ __global__ void k_hooks_prd(const uint32_t* __restrict__ src, const uint32_t* __
   }
 }
 
+// The LDS column window of a kind: one u32 (BFS/CC/SSSP) or int64 sum
+// (PageRank-delta) per target and query.
+size_t gather_lds_bytes(int kind, uint32_t vpt) {
+  const size_t per = kind == CYTOM_PRD ? 8 : 4;
+  return (size_t)vpt * per * kind_width(kind);
+}
+
 // Persistent grid: as many blocks per CU as the LDS window allows (at most
 // two), unless tuned.
 uint32_t gather_grid(const cytom_ctx* ctx, size_t lds) {
@@ -992,7 +1116,7 @@ void launch_gather(cytom_ctx* ctx, Program& P, int it) {
   A.changed = P.changed;
   A.it = P.d_iter + it;
   A.CH = ctx->chunk_slots;
-  const size_t lds = (size_t)ctx->vpt * (KIND == CYTOM_PRD || KIND == KIND_SSSP_PAIR ? 8 : 4);
+  const size_t lds = gather_lds_bytes(KIND, ctx->vpt);
   k_gather<KIND><<<gather_grid(ctx, lds), GATHER_THREADS, lds, ctx->stream>>>(A);
 }
 
@@ -1079,11 +1203,12 @@ int program_alloc(cytom_ctx* ctx, Program& P) {
   const uint64_t V = ctx->V;
   auto alloc = [&](void** p, size_t b) -> bool { return hipMalloc(p, b) == hipSuccess; };
   bool ok = true;
+  const size_t W = kind_width(P.kind);  // queries per vertex
   if (prd_like(P.kind)) {
-    ok = ok && alloc((void**)&P.rank, V * 4) && alloc((void**)&P.delta, V * 4) &&
-         alloc((void**)&P.dsum, V * 8) && alloc((void**)&P.contrib, V * 4);
+    ok = ok && alloc((void**)&P.rank, V * 4 * W) && alloc((void**)&P.delta, V * 4 * W) &&
+         alloc((void**)&P.dsum, V * 8 * W) && alloc((void**)&P.contrib, V * 4 * W);
   } else {
-    const size_t vb = P.kind == KIND_SSSP_PAIR ? 8 : 4;  // one u32, or a pair, per vertex
+    const size_t vb = 4 * W;  // one u32 per vertex and query
     ok = ok && alloc((void**)&P.cur, V * vb) && alloc((void**)&P.nxt, V * vb) &&
          alloc((void**)&P.fval, V * vb);
   }
@@ -1105,7 +1230,7 @@ int program_alloc(cytom_ctx* ctx, Program& P) {
 void program_free(Program& P) {
   void* ps[] = {P.cur,     P.nxt,     P.fval,      P.rank,      P.delta,   P.dsum,
                 P.contrib, P.act[0],  P.act[1],    P.gact[0], P.gact[1], P.changed,
-                P.teleport};
+                P.teleport, P.qseed};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (P.d_iter) (void)hipFree(P.d_iter);
@@ -1126,6 +1251,8 @@ void program_free(Program& P) {
   P.changed = nullptr;
   P.teleport = nullptr;
   P.n_teleport = 0;
+  P.qseed = nullptr;
+  P.n_qseed = 0;
 }
 
 int program_init(cytom_ctx* ctx, Program& P, const unsigned long long* gate) {
@@ -1141,6 +1268,11 @@ int program_init(cytom_ctx* ctx, Program& P, const unsigned long long* gate) {
     if (ppr && P.n_teleport)
       k_init_ppr_set<<<grid_for(P.n_teleport, 256), 256, 0, st>>>(P.teleport, P.n_teleport,
                                                                   P.rank, P.delta, act, gate);
+  } else if (P.kind == KIND_SSSP_OCT) {
+    k_init_vec<8><<<grid_for(ctx->V, 256), 256, 0, st>>>(ctx->V, P.cur, P.nxt, act, P.changed);
+    if (P.n_qseed)
+      k_seed_vec<8><<<grid_for(P.n_qseed, 256), 256, 0, st>>>(P.qseed, P.n_qseed, P.cur, P.nxt,
+                                                            act);
   } else if (P.kind == KIND_SSSP_PAIR) {
     k_init_pair<<<grid_for(ctx->V, 256), 256, 0, st>>>(
         ctx->V, P.source, P.source2, reinterpret_cast<uint2*>(P.cur),
@@ -1231,6 +1363,7 @@ void run_iters(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t n) {
       case CYTOM_CC: launch_gather<CYTOM_CC>(ctx, P, i); break;
       case CYTOM_SSSP: launch_gather<CYTOM_SSSP>(ctx, P, i); break;
       case KIND_SSSP_PAIR: launch_gather<KIND_SSSP_PAIR>(ctx, P, i); break;
+      case KIND_SSSP_OCT: launch_gather<KIND_SSSP_OCT>(ctx, P, i); break;
       default: launch_gather<CYTOM_PRD>(ctx, P, i); break;
     }
     if (prof) (void)hipEventRecord(P.kev[2 * i + 1], st);
@@ -1248,6 +1381,10 @@ void run_iters(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t n) {
       k_apply_min<CYTOM_CC><<<ab, APPLY_THREADS, 0, st>>>(
           ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval, P.changed, P.act[P.cur_act], P.act[nx],
           P.gact[nx], ctx->outdeg, nit, P.refuse);
+    } else if (P.kind == KIND_SSSP_OCT) {
+      k_apply_vec_min<8><<<ab, APPLY_THREADS, 0, st>>>(ctx->V, ctx->log2vpt, P.cur, P.nxt, P.fval,
+                                                       P.changed, P.act[P.cur_act], P.act[nx],
+                                                       P.gact[nx], ctx->outdeg, nit);
     } else if (P.kind == KIND_SSSP_PAIR) {
       k_apply_pair<<<ab, APPLY_THREADS, 0, st>>>(
           ctx->V, ctx->log2vpt, reinterpret_cast<uint2*>(P.cur),
@@ -1402,17 +1539,23 @@ int program_events(cytom_ctx* ctx, Program& P) {
 }
 
 int configure_gather(cytom_ctx* ctx) {
-  const int l4 = (int)ctx->vpt * 4, l8 = (int)ctx->vpt * 8;
-  CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_BFS>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, l4));
-  CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_CC>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, l4));
-  CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_SSSP>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, l4));
-  CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<CYTOM_PRD>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, l8));
-  CY_HIP(ctx, hipFuncSetAttribute((const void*)k_gather<KIND_SSSP_PAIR>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, l8));
+  // The limits are per kernel, not per context: each is set to the largest
+  // window any context launches it with (tiles of up to 16,384 vertices; the
+  // multi-query kinds run on 4,096-vertex query tiles), so contexts of
+  // different tile widths coexist.
+  struct {
+    const void* fn;
+    int kind;
+    uint32_t vpt;
+  } ks[] = {{(const void*)k_gather<CYTOM_BFS>, CYTOM_BFS, 16384},
+            {(const void*)k_gather<CYTOM_CC>, CYTOM_CC, 16384},
+            {(const void*)k_gather<CYTOM_SSSP>, CYTOM_SSSP, 16384},
+            {(const void*)k_gather<CYTOM_PRD>, CYTOM_PRD, 16384},
+            {(const void*)k_gather<KIND_SSSP_PAIR>, KIND_SSSP_PAIR, 16384},
+            {(const void*)k_gather<KIND_SSSP_OCT>, KIND_SSSP_OCT, QUERY_TILE_VPT}};
+  for (const auto& k : ks)
+    CY_HIP(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)gather_lds_bytes(k.kind, k.vpt)));
   ctx->gather_blocks = 0;
   return 0;
 }

@@ -85,6 +85,18 @@ __device__ inline bool batch_refused(const unsigned long long* misc) {
 // array V pairs) and a pair window in the gather.  Never registered in a
 // context's program list, so the update hooks never see it.
 constexpr int KIND_SSSP_PAIR = 5;
+// Internal kind of the batched queries on the 4,096-vertex query tiles: eight
+// SSSP queries per vertex (u32 distances, vertex-major).
+constexpr int KIND_SSSP_OCT = 6;
+
+// Tile width of the query tiles the eight-query kind runs on (its LDS window
+// holds 4,096 targets x 8 u32 = 128 KB).
+constexpr uint32_t QUERY_TILE_VPT = 4096;
+
+// Queries held per vertex by a program kind.
+__host__ __device__ constexpr uint32_t kind_width(int kind) {
+  return kind == KIND_SSSP_PAIR ? 2u : kind == KIND_SSSP_OCT ? 8u : 1u;
+}
 
 // Per-iteration device counters.
 struct IterStats {
@@ -105,6 +117,10 @@ struct Program {
   uint32_t source2 = 0;          // second query's source (KIND_SSSP_PAIR)
   uint32_t* teleport = nullptr;  // PPR teleport set (device, distinct ids)
   uint32_t n_teleport = 0;
+  // Sources of the eight-query kind (device): n_qseed vertices, then each
+  // one's query.
+  uint32_t* qseed = nullptr;
+  uint32_t n_qseed = 0;
   uint32_t* cur = nullptr;  // BFS/CC/SSSP current values
   uint32_t* nxt = nullptr;  // BFS/CC/SSSP next values
   uint32_t* fval = nullptr; // BFS/CC/SSSP: cur[v] while v is active, else INF

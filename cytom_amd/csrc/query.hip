@@ -20,8 +20,9 @@
 // Sparse frontiers push along out-edges (CSR); once a frontier's out-edges
 // exceed E / PULL_DIV the targets pull along in-edges (CSC) instead, which
 // needs no atomics for BFS and SSSP and keeps PPR sums in registers.
-// SSSP and PPR groups on graphs above VECTOR_MAX_EDGES run the tile-engine
-// program once per query instead (see run_seq_group).
+// SSSP and PPR groups on graphs above VECTOR_MAX_EDGES run on the tile engine
+// instead: SSSP eight queries per run on a 4,096-vertex-tile copy of the
+// snapshot (run_tile_sssp), PPR one query per run (run_seq_group).
 // The frontier's out-edges are split evenly over the wavefronts (a lane
 // finds its first edge by binary search over the frontier's degree prefix
 // sum), so hub vertices do not serialise the expansion.
@@ -49,6 +50,10 @@ struct BfsCtl {
 struct QueryEngine {
   // CSR snapshot
   uint64_t snap_batch = ~0ull;  // ctx->batches when the snapshot was taken
+  // The snapshot's edges in a context of 4,096-vertex query tiles, for the
+  // multi-query SSSP and PPR kinds (built on first use per snapshot).
+  cytom_ctx* qt = nullptr;
+  uint64_t qt_batch = ~0ull;
   uint64_t E = 0;
   uint64_t* row_off = nullptr;  // V + 1
   uint32_t* col = nullptr;      // E
@@ -738,7 +743,14 @@ void free_results(QueryEngine* qe) {
   qe->queries.clear();
 }
 
+void free_query_tiles(QueryEngine* qe) {
+  if (qe->qt) (void)cytom_destroy(qe->qt);
+  qe->qt = nullptr;
+  qe->qt_batch = ~0ull;
+}
+
 void free_snapshot(QueryEngine* qe) {
+  free_query_tiles(qe);
   void* ps[] = {qe->row_off, qe->col,     qe->wt,   qe->in_off,  qe->in_src,  qe->in_w,
                 qe->heavy,   qe->flist,   qe->nlist, qe->glist, qe->fdeg,   qe->fpre,    qe->mark,
                 qe->counter, qe->scan_tmp, qe->seg_v, qe->seg_b, qe->bvis, qe->bfront,
@@ -1126,12 +1138,129 @@ int run_ppr_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<std::vector
   return done(0);
 }
 
+// Sources of the CSR rows: one wavefront per row writes its row id over the
+// row's entries.
+__global__ void k_row_sources(const uint64_t* __restrict__ row_off, uint64_t V,
+                              uint32_t* __restrict__ src) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t u = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; u < V;
+       u += ((uint64_t)gridDim.x * blockDim.x) >> 6)
+    for (uint64_t e = row_off[u] + lane; e < row_off[u + 1]; e += 64) src[e] = (uint32_t)u;
+}
+
+// Column q of a vertex-major [V][W] array into a V-entry result row.
+template <uint32_t W, typename T>
+__global__ void k_split_vec(const T* __restrict__ x, uint64_t V, uint32_t m,
+                            T* __restrict__ out) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < V * m;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t q = t / V, v = t - q * V;
+    out[q * V + v] = x[v * W + q];
+  }
+}
+
+// The query-tile context of the current snapshot: the snapshot's edges
+// ingested as one batch into 4,096-vertex tiles (the eight-query kind's LDS
+// window holds 4,096 targets x 8 distances), with the main context's
+// iteration cap so every query runs as its single run.
+int query_tiles(cytom_ctx* ctx, QueryEngine* qe) {
+  if (qe->qt && qe->qt_batch == ctx->batches) return 0;
+  free_query_tiles(qe);
+  const uint64_t V = ctx->V, E = qe->E;
+  uint32_t* src = nullptr;
+  int rc;
+  if ((rc = dev_alloc(ctx, &src, E * 4))) return rc;
+  k_row_sources<<<blocks_for(V * 64, 256), 256, 0, ctx->stream>>>(qe->row_off, V, src);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    (void)hipFree(src);
+    return set_err(ctx, CYTOM_EDEVICE, "query tile sources failed");
+  }
+  cytom_config cfg{};
+  cfg.n_vertices = V;
+  cfg.vertices_per_tile = QUERY_TILE_VPT;
+  cfg.max_iterations = ctx->max_iter;
+  cfg.delta = ctx->delta;
+  const uint64_t T = std::max<uint64_t>(V / QUERY_TILE_VPT, 1);
+  // room for every tile's first block (1.5 x its edges + 64 slots)
+  cfg.edge_capacity = 2 * E + 64 * std::min<uint64_t>(T * T, E) + 4096;
+  cfg.weighted = ctx->weighted ? 1 : 0;
+  cfg.device = (uint32_t)ctx->device;
+  cytom_ctx* qt = nullptr;
+  if ((rc = cytom_create(&cfg, &qt))) {
+    (void)hipFree(src);
+    return set_err(ctx, rc, "query tile context creation failed");
+  }
+  qt->prd_shift = ctx->prd_shift;
+  qt->resort_pct = CYTOM_RESORT_OFF;  // the batch sort already orders each tile by source
+  if (E) rc = ingest_batch(qt, src, qe->col, ctx->weighted ? qe->wt : nullptr, E, nullptr,
+                          nullptr, nullptr, 0, nullptr);
+  (void)hipFree(src);
+  if (rc) {
+    const std::string msg = "query tile ingest failed: " + qt->err;
+    (void)cytom_destroy(qt);
+    return set_err(ctx, rc, msg);
+  }
+  qe->qt = qt;
+  qe->qt_batch = ctx->batches;
+  return 0;
+}
+
+void accumulate(cytom_run_stats* acc, const cytom_run_stats& rs) {
+  acc->iterations += rs.iterations;
+  acc->traversed_edges += rs.traversed_edges;
+  acc->active_vertices += rs.active_vertices;
+  acc->slots_scanned += rs.slots_scanned;
+  if (!rs.converged) acc->converged = 0;
+}
+
+// SSSP queries on the query tiles, eight per run (KIND_SSSP_OCT): one pass
+// over the slots and weights per iteration serves all eight, and each query
+// reaches its own fixpoint (the single run's distances).
+int run_tile_sssp(cytom_ctx* ctx, QueryEngine* qe, const cytom_query* q, const uint32_t* idx,
+                  uint32_t n, uint32_t* buf, cytom_run_stats* acc) {
+  cytom_ctx* qt = qe->qt;
+  Program P;
+  P.kind = KIND_SSSP_OCT;
+  int rc = program_alloc(qt, P);
+  if (rc) return set_err(ctx, rc, qt->err);
+  if (hipMalloc((void**)&P.qseed, 3 * 8 * 4) != hipSuccess) {
+    program_free(P);
+    return set_err(ctx, CYTOM_ENOMEM, "query seed allocation failed");
+  }
+  for (uint32_t k = 0; k < n && !rc; k += 8) {
+    const uint32_t m = std::min<uint32_t>(8, n - k);
+    std::vector<uint32_t> seed(2 * m + 8, 0u);
+    for (uint32_t j = 0; j < m; ++j) {
+      seed[j] = q[idx[k + j]].sources[0];
+      seed[m + j] = j;
+    }
+    P.n_qseed = m;
+    cytom_run_stats rs{};
+    if (hipMemcpy(P.qseed, seed.data(), seed.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+      rc = set_err(ctx, CYTOM_EDEVICE, "query seed copy failed");
+    } else if ((rc = program_init(qt, P)) || (rc = run_program(qt, P, &rs))) {
+      rc = set_err(ctx, rc, qt->err);
+    } else {
+      accumulate(acc, rs);
+      k_split_vec<8, uint32_t><<<blocks_for(ctx->V * m, 256, 16384), 256, 0, qt->stream>>>(
+          P.cur, ctx->V, m, buf + (uint64_t)k * ctx->V);
+    }
+  }
+  (void)hipStreamSynchronize(qt->stream);
+  program_free(P);
+  return rc;
+}
+
 // SSSP and PPR on large graphs: the vector groups trade their shared edge
 // reads for 16x wider random vertex-state reads, which loses to the tile
 // engine's LDS-windowed gather once the graph outgrows the caches.  Such
 // groups run the tile-engine program once per query instead (same values by
 // construction), each result copied into the group's buffer.
 constexpr uint64_t VECTOR_MAX_EDGES = 1ull << 24;
+// Up to this many edges SSSP groups run on the query tiles, eight queries per
+// run (the one-batch ingest of the snapshot needs ~40 B of scratch per edge);
+// beyond, two per run on the context's own tiles.
+constexpr uint64_t QUERY_TILES_MAX_EDGES = 1ull << 30;
 constexpr int SEQ_ROUTE = 8;  // group_kind flag: one V-sized result per query
 
 int run_seq_group(cytom_ctx* ctx, int kind, const cytom_query* q, const uint32_t* idx, uint32_t n,
@@ -1257,7 +1386,12 @@ int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run
       qe->group_buf.push_back(buf);
       qe->group_kind.push_back(seq ? kind | SEQ_ROUTE : kind);
       for (size_t k = g0; k < g1; ++k) qe->queries[idx[k]] = QueryEngine::Q{kind, gid, (uint32_t)(k - g0)};
-      if (seq && kind == CYTOM_SSSP) {
+      const bool tiles = seq && kind == CYTOM_SSSP && qe->E <= QUERY_TILES_MAX_EDGES;
+      if (tiles && (rc = query_tiles(ctx, qe))) return rc;
+      if (tiles) {
+        rc = run_tile_sssp(ctx, qe, q, idx.data() + g0, (uint32_t)(g1 - g0),
+                           static_cast<uint32_t*>(buf), &acc);
+      } else if (seq && kind == CYTOM_SSSP) {
         rc = run_seq_pairs(ctx, q, idx.data() + g0, (uint32_t)(g1 - g0),
                            static_cast<uint32_t*>(buf), &acc);
       } else if (seq) {

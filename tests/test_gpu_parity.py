@@ -459,9 +459,9 @@ def test_batched_bfs_deep_graph_rounds():
 
 
 def test_batched_queries_large_graph_route():
-    # Above 2^24 live edges SSSP and PPR groups run the tile-engine program
-    # per query; BFS keeps the bit-parallel groups.  Values still equal the
-    # single-query runs.
+    # Above 2^24 live edges SSSP groups run on the query tiles, eight queries
+    # per tile-engine run, and PPR queries run one per tile-engine run; BFS
+    # keeps the bit-parallel groups.  Values still equal the single-query runs.
     scale, V = 20, 1 << 20
     g = C.Graph(V, weighted=True)
     n = 1 << 22
@@ -471,12 +471,14 @@ def test_batched_queries_large_graph_route():
     assert g.live_edge_count() > (1 << 24)
     rng = np.random.default_rng(2)
     src = rng.integers(0, 64, 6)  # low ids: high degree under RMAT
-    # SSSP runs two queries per tile-engine pass: five queries give two
-    # pairs (one repeating a source, one with a high-id vertex that reaches
-    # little) and a single run.
-    sssp = [int(x) for x in src[:3]] + [int(src[0]), V - 1]
+    # Eleven SSSP queries: one run of eight and one of three (a repeated
+    # source, a high-id vertex that reaches little); five PPR queries (sets
+    # with a repeated vertex, a large set, a sink).
+    sssp = [int(x) for x in src[:3]] + [int(src[0]), V - 1] + [int(x) for x in
+                                                                 rng.integers(0, V, 6)]
     queries = ([("sssp", x) for x in sssp] + [("bfs", x) for x in src[3:]] +
-               [("ppr", src), ("ppr", [int(src[0])])])
+               [("ppr", src), ("ppr", [int(src[0])]), ("ppr", [int(src[1]), int(src[1]), 7]),
+                ("ppr", rng.integers(0, V, 300)), ("ppr", [V - 1])])
     g.batched_execute(queries)
     pid = {k: g.register(k, 0) for k in ("bfs", "sssp", "ppr")}
     for i, (kind, sv) in enumerate(queries):
