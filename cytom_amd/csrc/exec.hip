@@ -320,10 +320,15 @@ __device__ inline void cell_range(const uint32_t* cbase, const uint32_t* xb, uin
 // Slots per source row (summed over the columns).
 __global__ void k_row_slots(const uint32_t* __restrict__ cnt, uint32_t T,
                             unsigned long long* __restrict__ rs) {
-  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < T; x += gridDim.x * blockDim.x) {
+  // one wavefront per source row, its lanes over the row's tiles
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t x = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; x < T;
+       x += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
     unsigned long long s = 0;
-    for (uint32_t y = 0; y < T; ++y) s += cnt[(uint64_t)y * T + x];
-    rs[x] = s;
+    for (uint32_t y = lane; y < T; y += 64) s += cnt[(uint64_t)y * T + x];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) rs[x] = s;
   }
 }
 
@@ -1171,8 +1176,13 @@ void enqueue_jobs(cytom_ctx* ctx) {
   uint32_t* jc = xb + 32;  // the job length chosen on the device
   auto* rs = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->g_bands.p) + 256);
   auto* chunks = static_cast<uint64_t*>(ctx->g_chunks.p);
-  k_row_slots<<<grid_for(ctx->T, 256), 256, 0, st>>>(ctx->tile_cnt, ctx->T, rs);
-  k_band_bounds<<<1, 64, 0, st>>>(rs, ctx->T, NB, xb);
+  if (NB > 1) {
+    k_row_slots<<<grid_for((uint64_t)ctx->T * 64, 256), 256, 0, st>>>(ctx->tile_cnt, ctx->T, rs);
+    k_band_bounds<<<1, 64, 0, st>>>(rs, ctx->T, NB, xb);
+  } else {
+    fill_u32(xb, 1, 0, st);  // one band: all source rows
+    fill_u32(xb + 1, 1, ctx->T, st);
+  }
   k_nch<<<grid_for(nt, 256), 256, 0, st>>>(ctx->tile_cnt, nt, ctx->chunk_slots, nch);
   scan_exclusive_u32(nch, cbase, nt, ctx->g_scan_tmp.p, st);
   k_emit_chunks<<<grid_for(nt, 256), 256, 0, st>>>(nch, cbase, nt, ctx->chunk_slots, chunks);
