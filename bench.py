@@ -84,6 +84,61 @@ def measured_traffic(args, kernel):
     return rec["hbm_bytes_per_launch"]
 
 
+def traffic_summary(args):
+    """Writes the traffic record of the PageRank-delta gather from two
+    rocprofv3 counter runs of this script (FETCH_SIZE, then WRITE_SIZE, each
+    with --calibrate-copy-mib 256): per-launch means over the last
+    --traffic-launches dispatches of the gather, each counter scaled by what
+    it reads for the calibration copy (a device-to-device copy of known
+    size).  The record is keyed on this invocation's workload settings."""
+    import csv
+
+    def load(d, counter):
+        path = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)[0]
+        gather, copy = [], []
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            v = float(r["Counter_Value"])
+            if "k_gather<3>" in r["Kernel_Name"]:
+                gather.append((int(r["Dispatch_Id"]), v))
+            elif "copyBuffer" in r["Kernel_Name"]:
+                copy.append(v)
+        gather.sort()
+        last = [v for _, v in gather[-args.traffic_launches:]]
+        return sum(last) / len(last), max(copy), len(last)
+
+    copy_kib = args.calibrate_copy_mib * 1024.0
+    f, fc, n = load(args.traffic_summary[0], "FETCH_SIZE")
+    w, wc, _ = load(args.traffic_summary[1], "WRITE_SIZE")
+    fs, ws = copy_kib / fc, copy_kib / wc
+    rec = {
+        "kernel": "k_gather<pagerank-delta>",
+        "workload_key": {k: getattr(args, k) for k in WORKLOAD_KEYS},
+        "source_digest": source_digest(),
+        "launches": n,
+        "fetch_kib_per_launch": round(f, 1),
+        "write_kib_per_launch": round(w, 1),
+        "fetch_scale": fs,
+        "write_scale": ws,
+        "hbm_bytes_per_launch": round((f * fs + w * ws) * 1024),
+        "commands": [
+            "rocprofv3 --pmc %s --kernel-include-regex 'k_gather|copyBuffer' -d gpurun_out/traffic/%s"
+            " -o run --output-format csv -- python3 bench.py --cpu-baseline off --queries off"
+            " --ppr-grad off --host-paths off --calibrate-copy-mib %d"
+            % (c, c, args.calibrate_copy_mib)
+            for c in ("FETCH_SIZE", "WRITE_SIZE")] + [
+            "python3 bench.py --traffic-summary gpurun_out/traffic/FETCH_SIZE "
+            "gpurun_out/traffic/WRITE_SIZE --traffic-launches %d --calibrate-copy-mib %d"
+            % (args.traffic_launches, args.calibrate_copy_mib)],
+    }
+    with open(TRAFFIC_FILE, "w") as fh:
+        json.dump(rec, fh, indent=1)
+        fh.write("\n")
+    print(json.dumps(rec, indent=1))
+    return 0
+
+
 def parse(argv=None):
     args = parse_args(sys.argv[1:] if argv is None else argv)
     if args.config == "rmat27":
@@ -139,12 +194,20 @@ def parse_args(argv):
     ap.add_argument("--api", choices=["step", "calls"], default="step",
                     help="step: one cytom_step per batch (batch and first iteration groups "
                          "enqueued together); calls: update_device then execute per program")
+    ap.add_argument("--host-paths", choices=["on", "off"], default="on",
+                    help="also stream an RMAT-(scale-2) graph through both host paths "
+                         "(step vs separate calls) after the timed stream")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--cpu-batches", type=int, default=2,
                     help="timed batches of the CPU sample (the last ones of the stream, "
                          "inside the GPU's timed window)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--traffic-summary", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/gather_prd_rmat22_traffic.json from two rocprofv3 "
+                         "counter runs (no GPU work)")
+    ap.add_argument("--traffic-launches", type=int, default=4026,
+                    help="gather dispatches averaged by --traffic-summary (the timed ones)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (the reference "
                          "kernel for HBM traffic counters)")
@@ -342,7 +405,8 @@ def gpu_run(args):
     if args.queries == "on":
         res["detail"]["batched_queries"], res["grad_arrays"] = queries_run(args, g, V, source)
     g.close()
-    res["detail"]["host_paths"] = api_comparison(args, scale=max(args.scale - 2, 10))
+    if args.host_paths == "on":
+        res["detail"]["host_paths"] = api_comparison(args, scale=max(args.scale - 2, 10))
     return res
 
 
@@ -758,6 +822,8 @@ def l1n(a, b):
 
 def main():
     args = parse()
+    if args.traffic_summary:
+        return traffic_summary(args)
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)

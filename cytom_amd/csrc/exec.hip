@@ -24,6 +24,8 @@
 //    reduced on the fly.
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 #include "devutil.h"
 #include "internal.h"
 
@@ -1102,7 +1104,7 @@ uint32_t gather_grid(const cytom_ctx* ctx, size_t lds) {
 }
 
 template <int KIND>
-void launch_gather(cytom_ctx* ctx, Program& P, int it) {
+void launch_gather(cytom_ctx* ctx, Program& P, int it, hipEvent_t ev0, hipEvent_t ev1) {
   GatherArgs<KIND> A;
   A.jobs = static_cast<const uint4*>(ctx->g_jobs.p);
   A.jbase = static_cast<const uint32_t*>(ctx->g_jbase.p);
@@ -1122,7 +1124,14 @@ void launch_gather(cytom_ctx* ctx, Program& P, int it) {
   A.it = P.d_iter + it;
   A.CH = ctx->chunk_slots;
   const size_t lds = gather_lds_bytes(KIND, ctx->vpt);
-  k_gather<KIND><<<gather_grid(ctx, lds), GATHER_THREADS, lds, ctx->stream>>>(A);
+  if (ev0) {
+    // Profiling: the events are part of the dispatch, so they time the
+    // kernel itself (separate event records would add the dispatch gaps).
+    hipExtLaunchKernelGGL(k_gather<KIND>, dim3(gather_grid(ctx, lds)), dim3(GATHER_THREADS),
+                          (uint32_t)lds, ctx->stream, ev0, ev1, 0u, A);
+  } else {
+    k_gather<KIND><<<gather_grid(ctx, lds), GATHER_THREADS, lds, ctx->stream>>>(A);
+  }
 }
 
 }  // namespace
@@ -1367,16 +1376,15 @@ void run_iters(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t n) {
   const uint32_t ab = apply_blocks_for(ctx);
   const bool prof = ctx->profile && P.kev.size() >= 2 * (size_t)ctx->max_iter;
   for (uint32_t i = lo; i < lo + n; ++i) {
-    if (prof) (void)hipEventRecord(P.kev[2 * i], st);
+    hipEvent_t e0 = prof ? P.kev[2 * i] : nullptr, e1 = prof ? P.kev[2 * i + 1] : nullptr;
     switch (P.kind) {
-      case CYTOM_BFS: launch_gather<CYTOM_BFS>(ctx, P, i); break;
-      case CYTOM_CC: launch_gather<CYTOM_CC>(ctx, P, i); break;
-      case CYTOM_SSSP: launch_gather<CYTOM_SSSP>(ctx, P, i); break;
-      case KIND_SSSP_PAIR: launch_gather<KIND_SSSP_PAIR>(ctx, P, i); break;
-      case KIND_SSSP_OCT: launch_gather<KIND_SSSP_OCT>(ctx, P, i); break;
-      default: launch_gather<CYTOM_PRD>(ctx, P, i); break;
+      case CYTOM_BFS: launch_gather<CYTOM_BFS>(ctx, P, i, e0, e1); break;
+      case CYTOM_CC: launch_gather<CYTOM_CC>(ctx, P, i, e0, e1); break;
+      case CYTOM_SSSP: launch_gather<CYTOM_SSSP>(ctx, P, i, e0, e1); break;
+      case KIND_SSSP_PAIR: launch_gather<KIND_SSSP_PAIR>(ctx, P, i, e0, e1); break;
+      case KIND_SSSP_OCT: launch_gather<KIND_SSSP_OCT>(ctx, P, i, e0, e1); break;
+      default: launch_gather<CYTOM_PRD>(ctx, P, i, e0, e1); break;
     }
-    if (prof) (void)hipEventRecord(P.kev[2 * i + 1], st);
     const int nx = P.cur_act ^ 1;
     IterStats* nit = P.d_iter + i + 1;
     if (prd_like(P.kind)) {
