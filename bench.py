@@ -150,6 +150,7 @@ def parse(argv=None):
                   "--prefix-programs": ("prefix_programs", "cc"),
                   "--cpu-baseline": ("cpu_baseline", "off"), "--queries": ("queries", "off"),
                   "--ppr-grad": ("ppr_grad", "off"),
+                  "--host-paths": ("host_paths", "off"),
                   # 67M tiles of ~29 edges: small chunks put more tiles in
                   # each row-table pass (136.9 vs 131.9 GTEPS at 512 slots)
                   "--chunk-slots": ("chunk_slots", 64),

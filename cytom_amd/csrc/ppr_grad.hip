@@ -139,7 +139,10 @@ __device__ inline void block_max_to(float d, unsigned int* out) {
 __global__ void k_sweep(const uint64_t* __restrict__ off, uint64_t R, const uint32_t* __restrict__ nb,
                         const float* __restrict__ p, const float* __restrict__ x,
                         const float* __restrict__ base, float base_scale, float a,
-                        float* __restrict__ y, unsigned int* __restrict__ diff) {
+                        float* __restrict__ y, unsigned int* __restrict__ diff,
+                        unsigned int* __restrict__ next_diff) {
+  // the next sweep's change accumulator (the previous sweep's, already read)
+  if (blockIdx.x == 0 && threadIdx.x == 0) *next_diff = 0u;
   float dmax = 0.f;
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R;
        r += (uint64_t)gridDim.x * blockDim.x) {
@@ -379,7 +382,7 @@ int build_rows(Op& op, Rows& R, const uint32_t* key, const uint32_t* other, hipS
   k_row_bounds<<<gblocks(E + 1, GT, 16384), GT, 0, st>>>(k_out, E, V, R.off);
   k_gather_u32<<<gblocks(E, GT, 16384), GT, 0, st>>>(R.perm, other, E, R.nb);
   // long rows, summed by segments of SEGL entries in the sweeps
-  if (hipMemsetAsync(op.scratch + 1, 0, 4, st) != hipSuccess) return CYTOM_EDEVICE;
+  fill_u32(op.scratch + 1, 1, 0, st);
   k_long_rows<<<gblocks(V, GT, 16384), GT, 0, st>>>(R.off, V, ROW_LIGHT, R.longr, op.scratch + 1);
   if (hipMemcpyAsync(&R.nlong, op.scratch + 1, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
@@ -424,7 +427,7 @@ int build_structure(Op& op, uint64_t V, uint64_t E, const uint32_t* src, const u
   op.W = op.alloc<float>(V * 4);
   if (op.err) return op.err;
   if (E) {
-    if (hipMemsetAsync(op.scratch + 2, 0, 4, st) != hipSuccess) return CYTOM_EDEVICE;
+    fill_u32(op.scratch + 2, 1, 0, st);
     k_max_id<<<gblocks(E, GT, 4096), GT, 0, st>>>(src, dst, E, op.scratch + 2);
     unsigned int mx = 0;
     if (hipMemcpyAsync(&mx, op.scratch + 2, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -450,7 +453,7 @@ int set_weights(Op& op, const uint32_t* src, const float* w, hipStream_t st) {
   op.msg.clear();
   if (E) {
     unsigned int bad = 0;
-    if (hipMemsetAsync(op.scratch + 3, 0, 4, st) != hipSuccess) return CYTOM_EDEVICE;
+    fill_u32(op.scratch + 3, 1, 0, st);
     k_check_weights<<<gblocks(E, GT, 4096), GT, 0, st>>>(w, E, op.scratch + 3);
     if (hipMemcpyAsync(&bad, op.scratch + 3, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -460,7 +463,7 @@ int set_weights(Op& op, const uint32_t* src, const float* w, hipStream_t st) {
       return CYTOM_EINVAL;
     }
   }
-  if (hipMemsetAsync(op.W, 0, V * 4, st) != hipSuccess) return CYTOM_EDEVICE;
+  fill_u32(reinterpret_cast<uint32_t*>(op.W), V, 0, st);
   k_row_sum<<<gblocks(V, GT, 16384), GT, 0, st>>>(op.out.off, V, op.out.perm, w, op.W);
   if (op.out.nlong) {
     k_seg_wsum<<<gblocks((uint64_t)op.out.nseg * 64, GT, 16384), GT, 0, st>>>(
@@ -482,21 +485,25 @@ int solve(Op& op, const Rows& R, const float* base, float base_scale, float a, f
   const uint64_t V = op.V;
   op.msg.clear();
   uint32_t it = 0;
+  // Sweep k accumulates its largest change in scratch[4 + (k & 1)] and
+  // clears the other word for sweep k + 1 (no memset between sweeps).
+  fill_u32(op.scratch + 4, 1, 0, st);
   for (; it < max_iter; ++it) {
-    if (hipMemsetAsync(op.scratch, 0, 4, st) != hipSuccess) return CYTOM_EDEVICE;
+    unsigned int* dcur = op.scratch + 4 + (it & 1);
+    unsigned int* dnext = op.scratch + 4 + ((it + 1) & 1);
     k_sweep<<<gblocks(V, GT, 16384), GT, 0, st>>>(R.off, V, R.nb, R.p, x, base, base_scale, a, y,
-                                                  op.scratch);
+                                                  dcur, dnext);
     if (R.nlong) {
       k_seg_partial<<<gblocks((uint64_t)R.nseg * 64, GT, 16384), GT, 0, st>>>(
           R.seg_b, R.nseg, R.nb, R.p, x, R.part);
       k_sweep_rows<<<gblocks(R.nlong, GT, 16384), GT, 0, st>>>(
-          R.longr, R.nlong, R.sbase, R.part, x, base, base_scale, a, y, op.scratch);
+          R.longr, R.nlong, R.sbase, R.part, x, base, base_scale, a, y, dcur);
     }
     std::swap(x, y);
     // Check every 8 sweeps (and at the cap) to keep the host out of the loop.
     if ((it & 7) == 7 || it + 1 == max_iter) {
       unsigned int bits = 0;
-      if (hipMemcpyAsync(&bits, op.scratch, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      if (hipMemcpyAsync(&bits, dcur, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
           hipStreamSynchronize(st) != hipSuccess)
         return CYTOM_EDEVICE;
       float d;
