@@ -155,6 +155,9 @@ SIGNATURES = {
     "cytom_set_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
     "cytom_get_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
     "cytom_set_profiling": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cytom_set_step_graphs": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cytom_step_graph_stats": (ctypes.c_int, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U64),
+                                              ctypes.POINTER(ctypes.c_uint32)]),
     "cytom_kernel_profile": (ctypes.c_int, [_VP, _U32, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(_U64),
                                             ctypes.POINTER(ctypes.c_double)]),
@@ -291,6 +294,17 @@ class DeviceArray:
             if rc:
                 raise CytomError(rc, "H2D copy failed")
         return d
+
+    def copy_from(self, src, n=None):
+        """Copies the first n elements (all of src by default) of the device
+        array src into this one."""
+        n = src.n if n is None else int(n)
+        if n > self.n or n > src.n:
+            raise ValueError("copy of %d elements between arrays of %d and %d" % (n, src.n, self.n))
+        if n:
+            rc = lib().cytom_memcpy_d2d(self.ptr, src.ptr, n * self.dtype.itemsize)
+            if rc:
+                raise CytomError(rc, "D2D copy failed")
 
     def to_host(self, n=None):
         n = self.n if n is None else int(n)
@@ -603,6 +617,20 @@ class Graph:
 
     def set_profiling(self, enable=True):
         self._check(lib().cytom_set_profiling(self._h, 1 if enable else 0), "profiling")
+
+    def set_step_graphs(self, enable=True):
+        """Steps record their enqueue phase into HIP graphs and replay them
+        while the launch arguments repeat (see cytom_set_step_graphs)."""
+        self._check(lib().cytom_set_step_graphs(self._h, 1 if enable else 0), "step_graphs")
+
+    def step_graph_stats(self):
+        """(graphs recorded, graphs replayed, graphs kept) by this context's steps."""
+        c = ctypes.c_uint64()
+        r = ctypes.c_uint64()
+        k = ctypes.c_uint32()
+        self._check(lib().cytom_step_graph_stats(self._h, ctypes.byref(c), ctypes.byref(r),
+                                                 ctypes.byref(k)), "step_graph_stats")
+        return c.value, r.value, k.value
 
     def kernel_profile(self, kind):
         """(gather ms, launches, algorithmic bytes) accumulated for a program kind."""

@@ -8,6 +8,7 @@
 // bitcode surgery.
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <new>
 
 #include "internal.h"
@@ -57,10 +58,30 @@ void free_buf(GrowBuf& b) {
   b.bytes = 0;
 }
 
+// Every grow-only scratch buffer of a context.
+std::vector<GrowBuf*> grow_bufs(cytom_ctx* c) {
+  return {&c->g_keys,      &c->g_vals,       &c->g_keys2,    &c->g_vals2,     &c->g_flags,
+          &c->g_scan,      &c->g_seg_start,  &c->g_seg_tile, &c->g_reloc_cap, &c->g_reloc_list,
+          &c->g_ins_slot,  &c->g_scan_tmp,   &c->g_hist,     &c->g_in_src,    &c->g_in_dst,
+          &c->g_in_w,      &c->g_del_src,    &c->g_del_dst,  &c->g_del_w,     &c->g_tab_key,
+          &c->g_tab_head,  &c->g_node_next,  &c->g_claimed,  &c->g_tile_flag, &c->g_del_tiles,
+          &c->g_found,     &c->g_tile_scan,  &c->g_nch,      &c->g_cbase,     &c->g_chunks,
+          &c->g_colj,      &c->g_jbase,      &c->g_jobs,     &c->g_bands,     &c->g_grp,
+          &c->g_rs_e,      &c->g_rs_w,       &c->g_rs_off,   &c->g_pflags,    &c->g_rec,
+          &c->g_del_sig};
+}
+
+void free_graphs(cytom_ctx* c) {
+  for (auto& g : c->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+}
+
 void free_ctx(cytom_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_graphs(c);
   for (auto& P : c->progs) program_free(P);
   query_free(c);
   io_free(c);
@@ -71,18 +92,7 @@ void free_ctx(cytom_ctx* c) {
   if (c->h_misc) (void)hipHostFree(c->h_misc);
   if (c->h_scalar) (void)hipHostFree(c->h_scalar);
   if (c->h_pflags) (void)hipHostFree(c->h_pflags);
-  GrowBuf* bufs[] = {&c->g_keys,     &c->g_vals,      &c->g_keys2,     &c->g_vals2,
-                     &c->g_flags,    &c->g_scan,      &c->g_seg_start, &c->g_seg_tile,
-                     &c->g_reloc_cap, &c->g_reloc_list, &c->g_ins_slot, &c->g_scan_tmp,
-                     &c->g_hist,     &c->g_in_src,    &c->g_in_dst,    &c->g_in_w,
-                     &c->g_del_src,  &c->g_del_dst,   &c->g_del_w,     &c->g_tab_key,
-                     &c->g_tab_head, &c->g_node_next, &c->g_claimed,   &c->g_tile_flag,
-                     &c->g_del_tiles, &c->g_found,    &c->g_tile_scan, &c->g_nch,
-                     &c->g_cbase,    &c->g_chunks,    &c->g_colj,      &c->g_jbase,
-                     &c->g_jobs,     &c->g_bands,     &c->g_grp,       &c->g_rs_e,      &c->g_rs_w,      &c->g_rs_off,
-                     &c->g_pflags,
-                     &c->g_rec,      &c->g_del_sig};
-  for (GrowBuf* b : bufs) free_buf(*b);
+  for (GrowBuf* b : grow_bufs(c)) free_buf(*b);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -390,6 +400,160 @@ int cytom_execute(cytom_ctx* ctx, int prog_id, cytom_run_stats* stats) {
   return run_program(ctx, ctx->progs[prog_id], stats);
 }
 
+namespace {
+
+// Keys of step graphs: every host value an enqueue phase passes to a launch
+// or a copy.  Equal keys record identical graphs, so a graph is replayed only
+// while its key holds.  key_ctx covers the context (geometry, tuning, the
+// store's and the scratch buffers' addresses and sizes), key_prog one
+// program (its buffers, sources and buffer parity).
+void key_ctx(cytom_ctx* c, std::vector<uint64_t>& k) {
+  uint64_t dbits;
+  memcpy(&dbits, &c->delta, 8);
+  for (uint64_t v : {c->V, (uint64_t)c->vpt, (uint64_t)c->T, (uint64_t)c->log2vpt, c->n_tiles,
+                     (uint64_t)c->max_iter, dbits, (uint64_t)c->weighted, c->pool_size,
+                     c->n_chunks, c->n_cells, c->n_jobs, (uint64_t)c->n_bands,
+                     (uint64_t)c->chunk_slots, (uint64_t)c->job_chunks, (uint64_t)c->source_bands,
+                     (uint64_t)c->resort_pct, (uint64_t)c->resort_max_slots,
+                     (uint64_t)c->gather_threads, (uint64_t)c->gather_blocks,
+                     (uint64_t)(int64_t)c->prd_shift, (uint64_t)c->num_cus,
+                     (uint64_t)c->progs.size()})
+    k.push_back(v);
+  for (const void* p : {(const void*)c->stream, (const void*)c->tile_off,
+                        (const void*)c->tile_cap, (const void*)c->tile_cnt,
+                        (const void*)c->tile_live, (const void*)c->tile_sorted,
+                        (const void*)c->pool, (const void*)c->wpool, (const void*)c->outdeg,
+                        (const void*)c->indeg, (const void*)c->d_misc, (const void*)c->h_misc,
+                        (const void*)c->h_pflags})
+    k.push_back((uint64_t)(uintptr_t)p);
+  for (GrowBuf* b : grow_bufs(c)) {
+    k.push_back((uint64_t)(uintptr_t)b->p);
+    k.push_back(b->bytes);
+  }
+}
+
+void key_prog(const Program& P, std::vector<uint64_t>& k) {
+  for (uint64_t v : {(uint64_t)P.kind, (uint64_t)P.source, (uint64_t)P.source2,
+                     (uint64_t)P.n_teleport, (uint64_t)P.n_qseed, (uint64_t)P.cur_act})
+    k.push_back(v);
+  for (const void* p :
+       {(const void*)P.teleport, (const void*)P.qseed, (const void*)P.cur, (const void*)P.nxt,
+        (const void*)P.fval, (const void*)P.rank, (const void*)P.delta, (const void*)P.dsum,
+        (const void*)P.contrib, (const void*)P.act[0], (const void*)P.act[1],
+        (const void*)P.gact[0], (const void*)P.gact[1], (const void*)P.changed,
+        (const void*)P.d_iter, (const void*)P.h_iter, (const void*)P.dctl})
+    k.push_back((uint64_t)(uintptr_t)p);
+}
+
+constexpr size_t MAX_STEP_GRAPHS = 32;
+
+// Launches one enqueue phase through a step graph: replays the graph of an
+// equal key and applies the host-side effects the recorded phase had
+// (replayed), or records the phase (enqueue) into a new graph and launches
+// it.  A failed recording restores the host state the phase changed
+// (save / restore) and launches the phase directly.
+int launch_recorded(cytom_ctx* ctx, const std::vector<uint64_t>& key,
+                    const std::function<void()>& enqueue, const std::function<void()>& replayed,
+                    const std::function<void()>& save, const std::function<void()>& restore) {
+  for (auto& g : ctx->graphs) {
+    if (g.key != key) continue;
+    CY_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
+    replayed();
+    g.used = ++ctx->graph_clock;
+    ++ctx->graph_replays;
+    return 0;
+  }
+  save();
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ok = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  if (ok) {
+    ctx->capturing = true;
+    enqueue();
+    ctx->capturing = false;
+    ok = hipStreamEndCapture(ctx->stream, &graph) == hipSuccess && graph != nullptr;
+    ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  (void)hipGetLastError();
+  if (!ok) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    restore();
+    enqueue();
+    return 0;
+  }
+  if (ctx->graphs.size() >= MAX_STEP_GRAPHS) {
+    auto lru = std::min_element(ctx->graphs.begin(), ctx->graphs.end(),
+                                [](const cytom_ctx::StepGraph& a, const cytom_ctx::StepGraph& b) {
+                                  return a.used < b.used;
+                                });
+    (void)hipGraphExecDestroy(lru->exec);
+    ctx->graphs.erase(lru);
+  }
+  cytom_ctx::StepGraph g;
+  g.key = key;
+  g.exec = exec;
+  g.used = ++ctx->graph_clock;
+  ctx->graphs.push_back(std::move(g));
+  ++ctx->graph_captures;
+  CY_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+  return 0;
+}
+
+// The enqueue phase of a step through step graphs: one graph for the batch
+// and the work lists, one per listed program for its first iteration group,
+// with the timing events recorded between the graph launches.
+int step_graphs(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
+                const uint32_t* d_w, uint64_t n_ins, const uint32_t* d_del_src,
+                const uint32_t* d_del_dst, const uint32_t* d_del_w, uint64_t n_del,
+                const int* prog_ids, const std::vector<Program*>& ps) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  std::vector<uint64_t> key = {1};  // the batch phase
+  for (const void* a : {(const void*)d_src, (const void*)d_dst, (const void*)d_w,
+                        (const void*)d_del_src, (const void*)d_del_dst, (const void*)d_del_w})
+    key.push_back((uint64_t)(uintptr_t)a);
+  key.push_back(n_ins);
+  key.push_back(n_del);
+  key_ctx(ctx, key);
+  for (const Program& P : ctx->progs) key_prog(P, key);
+  bool dirty = false;
+  CY_HIP(ctx, hipEventRecord(ctx->ev[0], st));
+  rc = launch_recorded(
+      ctx, key,
+      [&] {
+        ingest_enqueue(ctx, d_src, d_dst, d_w, n_ins, d_del_src, d_del_dst, d_del_w, n_del);
+        enqueue_jobs(ctx);
+      },
+      [&] { ctx->jobs_dirty = false; }, [&] { dirty = ctx->jobs_dirty; },
+      [&] { ctx->jobs_dirty = dirty; });
+  if (rc) return rc;
+  CY_HIP(ctx, hipEventRecord(ctx->ev[1], st));
+  for (size_t i = 0; i < ps.size(); ++i) {
+    Program& P = *ps[i];
+    P.refuse = ctx->d_misc;
+    run_open(P);
+    key.assign({2, (uint64_t)prog_ids[i], run_group_length(ctx, P)});  // a first group
+    key_ctx(ctx, key);
+    key_prog(P, key);
+    int cur_act = P.cur_act;
+    CY_HIP(ctx, hipEventRecord(P.gev[0], st));
+    rc = launch_recorded(
+        ctx, key, [&] { run_group(ctx, P); }, [&] { run_group_replayed(ctx, P); },
+        [&] { cur_act = P.cur_act; },
+        [&] {
+          P.cur_act = cur_act;
+          run_open(P);
+        });
+    P.refuse = nullptr;
+    if (rc) return rc;
+    CY_HIP(ctx, hipEventRecord(P.gev[1], st));
+  }
+  return 0;
+}
+
+}  // namespace
+
 // edge-inserter.h:129-184: one batch, then (re-initialisation where the
 // batch requires it and) every registered algorithm's run.  The batch, the
 // work lists and the first iteration group of every program are enqueued
@@ -418,15 +582,27 @@ int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, con
   for (Program* P : ps)
     if ((rc = program_events(ctx, *P))) return rc;
   std::vector<int> parity(ps.size());
-  ingest_enqueue(ctx, d_src, d_dst, d_w, n_ins, d_del_src, d_del_dst, d_del_w, n_del);
-  enqueue_jobs(ctx);
   for (size_t k = 0; k < ps.size(); ++k) {
-    Program& P = *ps[k];
-    parity[k] = P.cur_act;
-    P.refuse = ctx->d_misc;
-    run_open(P);
-    run_group(ctx, P);
-    P.refuse = nullptr;
+    parity[k] = ps[k]->cur_act;
+    run_open(*ps[k]);
+  }
+  // The enqueue phase: no host reads, no allocation (so it can be recorded).
+  auto enqueue = [&]() {
+    ingest_enqueue(ctx, d_src, d_dst, d_w, n_ins, d_del_src, d_del_dst, d_del_w, n_del);
+    enqueue_jobs(ctx);
+    for (Program* P : ps) {
+      P->refuse = ctx->d_misc;
+      run_open(*P);
+      run_group(ctx, *P);
+      P->refuse = nullptr;
+    }
+  };
+  if (ctx->step_graphs && !ctx->profile) {
+    if ((rc = step_graphs(ctx, d_src, d_dst, d_w, n_ins, d_del_src, d_del_dst, d_del_w, n_del,
+                          prog_ids, ps)))
+      return rc;
+  } else {
+    enqueue();
   }
   uint32_t syncs = 1;
   CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -573,6 +749,25 @@ int cytom_set_profiling(cytom_ctx* ctx, int enable) {
     ctx->gather_launches[k] = 0;
     ctx->gather_alg_bytes[k] = 0;
   }
+  return 0;
+}
+
+int cytom_set_step_graphs(cytom_ctx* ctx, int enable) {
+  CY_CHECK_CTX(ctx);
+  ctx->step_graphs = enable != 0;
+  if (!ctx->step_graphs) {
+    CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    free_graphs(ctx);
+  }
+  return 0;
+}
+
+int cytom_step_graph_stats(cytom_ctx* ctx, uint64_t* captures, uint64_t* replays,
+                           uint32_t* cached) {
+  CY_CHECK_CTX(ctx);
+  if (captures) *captures = ctx->graph_captures;
+  if (replays) *replays = ctx->graph_replays;
+  if (cached) *cached = (uint32_t)ctx->graphs.size();
   return 0;
 }
 

@@ -1437,6 +1437,10 @@ uint32_t group_length(const cytom_ctx* ctx, const Program& P) {
     n = std::max<uint32_t>(P.last_iters + std::max<uint32_t>(P.last_iters / 16, 2), 2);
   else
     n = std::min<uint32_t>(4u << std::min<uint32_t>(P.groups - 1, 2), 16);
+  // With step graphs the first group has an even length, so the buffer
+  // parity it leaves (every later group is even too) repeats from batch to
+  // batch, and so does the recorded graph of each length.
+  if (ctx->step_graphs) n = (n + 1) & ~1u;
   return std::min(n, ctx->max_iter - P.launched);
 }
 
@@ -1455,17 +1459,28 @@ void run_group(cytom_ctx* ctx, Program& P) {
              sizeof(IterStats) / 8 * (uint64_t)(ctx->max_iter + 2), 0, st);
     fill_u64(reinterpret_cast<uint64_t*>(P.gact[P.cur_act]), (uint64_t)ctx->T * 8, 0, st);
   }
-  (void)hipEventRecord(P.gev[2 * g], st);
+  if (!ctx->capturing) (void)hipEventRecord(P.gev[2 * g], st);
   if (g == 0)
     k_frontier<<<apply_blocks_for(ctx), APPLY_THREADS, 0, st>>>(
         P.kind, ctx->V, ctx->log2vpt, P.act[P.cur_act], ctx->outdeg, P.gact[P.cur_act], P.delta,
         P.contrib, P.cur, P.fval, P.d_iter, P.refuse);
   run_iters(ctx, P, P.launched, n);
-  (void)hipEventRecord(P.gev[2 * g + 1], st);
+  if (!ctx->capturing) (void)hipEventRecord(P.gev[2 * g + 1], st);
   // counters of the frontier (first group) and of every iteration launched
   const uint32_t lo = g == 0 ? 0 : P.launched + 1;
   (void)hipMemcpyAsync(&P.h_iter[lo], &P.d_iter[lo], sizeof(IterStats) * (P.launched + n + 1 - lo),
                        hipMemcpyDeviceToHost, st);
+  P.launched += n;
+}
+
+uint32_t run_group_length(const cytom_ctx* ctx, const Program& P) {
+  return group_length(ctx, P);
+}
+
+void run_group_replayed(const cytom_ctx* ctx, Program& P) {
+  const uint32_t n = group_length(ctx, P);
+  ++P.groups;
+  P.cur_act ^= (int)(n & 1u);  // run_iters flips the parity once per iteration
   P.launched += n;
 }
 

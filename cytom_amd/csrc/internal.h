@@ -231,6 +231,21 @@ struct cytom_ctx {
   double gather_ms[5] = {0, 0, 0, 0, 0};
   uint64_t gather_launches[5] = {0, 0, 0, 0, 0};
   double gather_alg_bytes[5] = {0, 0, 0, 0, 0};
+  // Step graphs (cytom_set_step_graphs): the enqueue phase of cytom_step
+  // recorded as HIP graphs, each keyed by every host value its launches
+  // read, replayed while the key holds (least recently used evicted).
+  struct StepGraph {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec = nullptr;
+    uint64_t used = 0;
+  };
+  bool step_graphs = false;
+  // While a step graph is being recorded the enqueue functions leave their
+  // timing events out (events recorded inside a graph carry no timestamps);
+  // the step records them around the graph launches instead.
+  bool capturing = false;
+  std::vector<StepGraph> graphs;
+  uint64_t graph_captures = 0, graph_replays = 0, graph_clock = 0;
 
   cytom::StoreView view() const {
     cytom::StoreView s;
@@ -336,6 +351,10 @@ int run_program(cytom_ctx* ctx, Program& P, cytom_run_stats* st);
 int program_events(cytom_ctx* ctx, Program& P);
 void run_open(Program& P);
 void run_group(cytom_ctx* ctx, Program& P);
+// The host-side effects of run_group without its launches (a replayed step
+// graph holds them), and the length of the group run_group would enqueue.
+void run_group_replayed(const cytom_ctx* ctx, Program& P);
+uint32_t run_group_length(const cytom_ctx* ctx, const Program& P);
 bool run_needs_more(const cytom_ctx* ctx, Program& P);
 int run_finish(cytom_ctx* ctx, Program& P, cytom_run_stats* out);
 // Sets the gather kernels' LDS limit and persistent grid for the context.

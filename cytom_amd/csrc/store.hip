@@ -855,7 +855,7 @@ void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst
   auto* rules = reinterpret_cast<uint32_t*>(gate + np);           // [np] force, [np] found_forces
 
   ctx->jobs_dirty = true;  // the store changes (and the deletion pass reuses the list buffers)
-  (void)hipEventRecord(ctx->ev[0], st);
+  if (!ctx->capturing) (void)hipEventRecord(ctx->ev[0], st);
   // misc[1..] := 0 (the pool top in misc[0] persists), the pool size, rules
   fill_u64(reinterpret_cast<uint64_t*>(misc + 1), 63, 0, st);
   fill_u64(reinterpret_cast<uint64_t*>(misc + M_POOL_SIZE), 1, ctx->pool_size, st);
@@ -958,7 +958,7 @@ void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst
   // tiles with long unsorted tails
   if (n_del) k_compact<<<4096, CP_THREADS, 0, st>>>(del_tiles, n_del_tiles_p, ctx->view(), misc);
   resort_enqueue(ctx);
-  (void)hipEventRecord(ctx->ev[1], st);
+  if (!ctx->capturing) (void)hipEventRecord(ctx->ev[1], st);
   // the control words and decisions, for ingest_finish
   (void)hipMemcpyAsync(ctx->h_misc, misc, 64 * 8, hipMemcpyDeviceToHost, st);
   if (np) (void)hipMemcpyAsync(ctx->h_misc + 64, gate, np * 8, hipMemcpyDeviceToHost, st);

@@ -201,6 +201,22 @@ int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, con
                uint64_t n_ins, const uint32_t* d_del_src, const uint32_t* d_del_dst,
                const uint32_t* d_del_w, uint64_t n_del, const int* prog_ids, uint32_t n_progs,
                cytom_batch_stats* batch_stats, cytom_run_stats* run_stats);
+/* Step graphs (no reference counterpart: the reference's EdgeInserter
+ * launches nothing).  With enable != 0, cytom_step records its enqueue phase
+ * into HIP graphs (one for the update batch and the work lists, one for each
+ * listed program's first iteration group) and replays them on later steps
+ * whose arguments, programs, device buffers, geometry and tuning match; a
+ * phase that differs in any of them (another batch size, a grown scratch
+ * buffer, a repacked pool) records another graph, and at most 32 are kept
+ * (least recently used evicted).  First iteration groups get an even length
+ * so that the buffer parity they leave repeats from batch to batch (a
+ * launch past convergence changes nothing).  Values and statistics
+ * equal those of direct launches; with profiling on, steps launch directly.
+ * 0 disables graphs and frees them. */
+int cytom_set_step_graphs(cytom_ctx* ctx, int enable);
+/* Graphs recorded and replayed by the context's steps, and graphs kept. */
+int cytom_step_graph_stats(cytom_ctx* ctx, uint64_t* captures, uint64_t* replays,
+                           uint32_t* cached);
 /* Copies the program's values (the `next` array written by ResultsWriter,
  * results-writer.h:54-67): BFS/CC/SSSP uint32[V], PRD/PPR float rank[V]. */
 int cytom_read_values(cytom_ctx* ctx, int prog_id, void* out, size_t bytes);

@@ -139,3 +139,50 @@ def test_step_refused_batch_changes_nothing():
         h.step(hid, s, d, w, 6000, ds, dd, dw, nd)
         for p, q in zip(ids, hid):
             assert np.array_equal(g.read(p), h.read(q))
+
+
+@pytest.mark.parametrize("cap", [0, 4096])
+def test_step_graphs_equal_direct_steps(cap):
+    # Steps through recorded graphs against direct steps on the same stream.
+    # The updates are staged in fixed device buffers (a step graph is keyed
+    # by its argument pointers), batch sizes change once mid-stream (a new
+    # graph) and, with cap 4096, the pool overflows and is repacked (new
+    # graphs after each repack).
+    scale, vpt, seed = 14, 1024, 4
+    V = 1 << scale
+    st = O.Stream(seed)
+    ga, ia = make(V, vpt, cap)
+    gb, ib = make(V, vpt, cap)
+    gb.set_step_graphs(True)
+    n_max = 20000
+    bufs = [C.DeviceArray(n_max, np.uint32) for _ in range(6)]
+    sizes = [(20000, 2000)] * 6 + [(12000, 1500)] * 6
+    for b, (n_ins, n_del) in enumerate(sizes):
+        first, didx = st.next(n_ins, n_del if b else 0)
+        arrs = device_batch(scale, seed, first, n_ins, didx)
+        for buf, a in zip(bufs, arrs):
+            buf.copy_from(a)
+        s, d, w, ds, dd, dw = bufs
+        ba, ra = ga.step(ia, s, d, w, n_ins, ds, dd, dw, len(didx))
+        bb, rb = gb.step(ib, s, d, w, n_ins, ds, dd, dw, len(didx))
+        assert (bb.inserted, bb.deleted, bb.missing, bb.live_edges) == \
+            (ba.inserted, ba.deleted, ba.missing, ba.live_edges), b
+        for k, name in enumerate(NAMES):
+            assert np.array_equal(ga.read(ia[k]), gb.read(ib[k])), (b, name)
+            assert (rb[k].converged, rb[k].reexecuted) == (ra[k].converged, ra[k].reexecuted), \
+                (b, name)
+            if name == "pagerank-delta":
+                # Its trajectory is exact (fixed-point sums); the min kinds'
+                # insert hooks race on purpose (any order reaches the same
+                # fixpoint), so only their values are compared.
+                assert (rb[k].iterations, rb[k].traversed_edges) == \
+                    (ra[k].iterations, ra[k].traversed_edges), b
+    captures, replays, kept = gb.step_graph_stats()
+    assert captures >= 2 and kept <= 32, (captures, replays, kept)
+    if cap == 0:
+        # Equal batches with the same buffers replay after the first few
+        # batches (scratch buffers stop growing, parities repeat).
+        assert replays >= 4, (captures, replays)
+    assert ga.step_graph_stats() == (0, 0, 0)
+    gb.set_step_graphs(False)
+    assert gb.step_graph_stats()[2] == 0
