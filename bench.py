@@ -195,6 +195,9 @@ def parse_args(argv):
     ap.add_argument("--api", choices=["step", "calls"], default="step",
                     help="step: one cytom_step per batch (batch and first iteration groups "
                          "enqueued together); calls: update_device then execute per program")
+    ap.add_argument("--concurrent-runs", choices=["on", "off"], default="on",
+                    help="on: a step runs its programs concurrently, one stream each; off: one "
+                         "after the other on the context's stream")
     ap.add_argument("--host-paths", choices=["on", "off"], default="on",
                     help="also stream an RMAT-(scale-2) graph through both host paths "
                          "(step vs separate calls) after the timed stream")
@@ -270,6 +273,7 @@ def gpu_run(args):
                 max_iterations=100, delta=1e-7, edge_capacity=int(3 * E))
     g.set_tuning(args.chunk_slots, args.job_chunks, args.blocks_per_cu, args.source_bands,
                  C.RESORT_OFF if args.resort_pct < 0 else args.resort_pct)
+    g.set_concurrent_runs(args.concurrent_runs == "on")
     pids = {}
     for n in names:
         pids[n] = g.register(n, source if n in ("bfs", "sssp") else 0)

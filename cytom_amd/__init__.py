@@ -156,6 +156,7 @@ SIGNATURES = {
     "cytom_get_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
     "cytom_set_profiling": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cytom_set_step_graphs": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cytom_set_concurrent_runs": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cytom_step_graph_stats": (ctypes.c_int, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U64),
                                               ctypes.POINTER(ctypes.c_uint32)]),
     "cytom_kernel_profile": (ctypes.c_int, [_VP, _U32, ctypes.POINTER(ctypes.c_double),
@@ -622,6 +623,11 @@ class Graph:
         """Steps record their enqueue phase into HIP graphs and replay them
         while the launch arguments repeat (see cytom_set_step_graphs)."""
         self._check(lib().cytom_set_step_graphs(self._h, 1 if enable else 0), "step_graphs")
+
+    def set_concurrent_runs(self, enable=True):
+        """Steps run their programs concurrently, one stream each (default)."""
+        self._check(lib().cytom_set_concurrent_runs(self._h, 1 if enable else 0),
+                    "concurrent_runs")
 
     def step_graph_stats(self):
         """(graphs recorded, graphs replayed, graphs kept) by this context's steps."""

@@ -82,6 +82,8 @@ void free_ctx(cytom_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_graphs(c);
+  for (hipStream_t st : c->run_streams) (void)hipStreamDestroy(st);
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
   for (auto& P : c->progs) program_free(P);
   query_free(c);
   io_free(c);
@@ -452,12 +454,12 @@ constexpr size_t MAX_STEP_GRAPHS = 32;
 // (replayed), or records the phase (enqueue) into a new graph and launches
 // it.  A failed recording restores the host state the phase changed
 // (save / restore) and launches the phase directly.
-int launch_recorded(cytom_ctx* ctx, const std::vector<uint64_t>& key,
+int launch_recorded(cytom_ctx* ctx, hipStream_t st, const std::vector<uint64_t>& key,
                     const std::function<void()>& enqueue, const std::function<void()>& replayed,
                     const std::function<void()>& save, const std::function<void()>& restore) {
   for (auto& g : ctx->graphs) {
     if (g.key != key) continue;
-    CY_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
+    CY_HIP(ctx, hipGraphLaunch(g.exec, st));
     replayed();
     g.used = ++ctx->graph_clock;
     ++ctx->graph_replays;
@@ -466,12 +468,12 @@ int launch_recorded(cytom_ctx* ctx, const std::vector<uint64_t>& key,
   save();
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  bool ok = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
   if (ok) {
     ctx->capturing = true;
     enqueue();
     ctx->capturing = false;
-    ok = hipStreamEndCapture(ctx->stream, &graph) == hipSuccess && graph != nullptr;
+    ok = hipStreamEndCapture(st, &graph) == hipSuccess && graph != nullptr;
     ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
     if (graph) (void)hipGraphDestroy(graph);
   }
@@ -496,7 +498,7 @@ int launch_recorded(cytom_ctx* ctx, const std::vector<uint64_t>& key,
   g.used = ++ctx->graph_clock;
   ctx->graphs.push_back(std::move(g));
   ++ctx->graph_captures;
-  CY_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+  CY_HIP(ctx, hipGraphLaunch(exec, st));
   return 0;
 }
 
@@ -506,7 +508,8 @@ int launch_recorded(cytom_ctx* ctx, const std::vector<uint64_t>& key,
 int step_graphs(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
                 const uint32_t* d_w, uint64_t n_ins, const uint32_t* d_del_src,
                 const uint32_t* d_del_dst, const uint32_t* d_del_w, uint64_t n_del,
-                const int* prog_ids, const std::vector<Program*>& ps) {
+                const int* prog_ids, const std::vector<Program*>& ps,
+                const std::function<void()>& fork) {
   hipStream_t st = ctx->stream;
   int rc;
   std::vector<uint64_t> key = {1};  // the batch phase
@@ -520,7 +523,7 @@ int step_graphs(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
   bool dirty = false;
   CY_HIP(ctx, hipEventRecord(ctx->ev[0], st));
   rc = launch_recorded(
-      ctx, key,
+      ctx, st, key,
       [&] {
         ingest_enqueue(ctx, d_src, d_dst, d_w, n_ins, d_del_src, d_del_dst, d_del_w, n_del);
         enqueue_jobs(ctx);
@@ -529,6 +532,7 @@ int step_graphs(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
       [&] { ctx->jobs_dirty = dirty; });
   if (rc) return rc;
   CY_HIP(ctx, hipEventRecord(ctx->ev[1], st));
+  fork();
   for (size_t i = 0; i < ps.size(); ++i) {
     Program& P = *ps[i];
     P.refuse = ctx->d_misc;
@@ -537,9 +541,10 @@ int step_graphs(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
     key_ctx(ctx, key);
     key_prog(P, key);
     int cur_act = P.cur_act;
-    CY_HIP(ctx, hipEventRecord(P.gev[0], st));
+    const hipStream_t ps_st = prog_stream(ctx, P);
+    CY_HIP(ctx, hipEventRecord(P.gev[0], ps_st));
     rc = launch_recorded(
-        ctx, key, [&] { run_group(ctx, P); }, [&] { run_group_replayed(ctx, P); },
+        ctx, ps_st, key, [&] { run_group(ctx, P); }, [&] { run_group_replayed(ctx, P); },
         [&] { cur_act = P.cur_act; },
         [&] {
           P.cur_act = cur_act;
@@ -547,8 +552,21 @@ int step_graphs(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
         });
     P.refuse = nullptr;
     if (rc) return rc;
-    CY_HIP(ctx, hipEventRecord(P.gev[1], st));
+    CY_HIP(ctx, hipEventRecord(P.gev[1], ps_st));
   }
+  return 0;
+}
+
+constexpr size_t MAX_RUN_STREAMS = 4;
+
+// The streams of a concurrent step (created once per context).
+int run_streams(cytom_ctx* ctx, size_t n) {
+  while (ctx->run_streams.size() < n) {
+    hipStream_t s = nullptr;
+    CY_HIP(ctx, hipStreamCreate(&s));
+    ctx->run_streams.push_back(s);
+  }
+  if (!ctx->fork_ev) CY_HIP(ctx, hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
   return 0;
 }
 
@@ -581,6 +599,31 @@ int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, con
   if ((rc = ingest_prepare(ctx, n_ins, n_del)) || (rc = prepare_jobs(ctx))) return rc;
   for (Program* P : ps)
     if ((rc = program_events(ctx, *P))) return rc;
+  // Programs run concurrently after the batch, each on a stream of its own
+  // (their state is disjoint; the store and the work lists are read-only
+  // during runs).
+  const size_t ns = ctx->concurrent_runs && ps.size() > 1 ? std::min(ps.size(), MAX_RUN_STREAMS)
+                                                          : 0;
+  if (ns && (rc = run_streams(ctx, ns))) return rc;
+  struct StreamReset {
+    std::vector<Program*>& ps;
+    ~StreamReset() {
+      for (Program* P : ps) P->stream = nullptr;
+    }
+  } reset{ps};
+  auto fork = [&]() {
+    if (!ns) return;
+    (void)hipEventRecord(ctx->fork_ev, ctx->stream);
+    for (size_t k = 0; k < ps.size(); ++k) {
+      ps[k]->stream = ctx->run_streams[k % ns];
+      (void)hipStreamWaitEvent(ps[k]->stream, ctx->fork_ev, 0);
+    }
+  };
+  auto sync_all = [&]() -> int {
+    CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < ns; ++i) CY_HIP(ctx, hipStreamSynchronize(ctx->run_streams[i]));
+    return 0;
+  };
   std::vector<int> parity(ps.size());
   for (size_t k = 0; k < ps.size(); ++k) {
     parity[k] = ps[k]->cur_act;
@@ -590,6 +633,7 @@ int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, con
   auto enqueue = [&]() {
     ingest_enqueue(ctx, d_src, d_dst, d_w, n_ins, d_del_src, d_del_dst, d_del_w, n_del);
     enqueue_jobs(ctx);
+    fork();
     for (Program* P : ps) {
       P->refuse = ctx->d_misc;
       run_open(*P);
@@ -599,19 +643,22 @@ int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, con
   };
   if (ctx->step_graphs && !ctx->profile) {
     if ((rc = step_graphs(ctx, d_src, d_dst, d_w, n_ins, d_del_src, d_del_dst, d_del_w, n_del,
-                          prog_ids, ps)))
+                          prog_ids, ps, fork)))
       return rc;
   } else {
     enqueue();
   }
   uint32_t syncs = 1;
-  CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if ((rc = sync_all())) return rc;
   CY_HIP(ctx, hipGetLastError());
   rc = ingest_finish(ctx, n_ins, n_del, batch_stats);
   if (rc != 0) {
     // The batch was refused or needs a larger pool: the runs did nothing but
     // flip the host's buffer parity.
-    for (size_t k = 0; k < ps.size(); ++k) ps[k]->cur_act = parity[k];
+    for (size_t k = 0; k < ps.size(); ++k) {
+      ps[k]->cur_act = parity[k];
+      ps[k]->stream = nullptr;
+    }
     if (rc < 0) return rc;
     cytom_batch_stats bs{};
     if ((rc = ingest_batch(ctx, d_src, d_dst, d_w, n_ins, d_del_src, d_del_dst, d_del_w, n_del,
@@ -633,7 +680,7 @@ int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, con
     for (size_t k = 0; k < ps.size(); ++k)
       if (more[k]) run_group(ctx, *ps[k]);
     ++syncs;
-    CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = sync_all())) return rc;
     any = false;
     for (size_t k = 0; k < ps.size(); ++k)
       if (more[k]) any |= (more[k] = run_needs_more(ctx, *ps[k])) != 0;
@@ -759,6 +806,12 @@ int cytom_set_step_graphs(cytom_ctx* ctx, int enable) {
     CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     free_graphs(ctx);
   }
+  return 0;
+}
+
+int cytom_set_concurrent_runs(cytom_ctx* ctx, int enable) {
+  CY_CHECK_CTX(ctx);
+  ctx->concurrent_runs = enable != 0;
   return 0;
 }
 

@@ -1128,9 +1128,9 @@ void launch_gather(cytom_ctx* ctx, Program& P, int it, hipEvent_t ev0, hipEvent_
     // Profiling: the events are part of the dispatch, so they time the
     // kernel itself (separate event records would add the dispatch gaps).
     hipExtLaunchKernelGGL(k_gather<KIND>, dim3(gather_grid(ctx, lds)), dim3(GATHER_THREADS),
-                          (uint32_t)lds, ctx->stream, ev0, ev1, 0u, A);
+                          (uint32_t)lds, prog_stream(ctx, P), ev0, ev1, 0u, A);
   } else {
-    k_gather<KIND><<<gather_grid(ctx, lds), GATHER_THREADS, lds, ctx->stream>>>(A);
+    k_gather<KIND><<<gather_grid(ctx, lds), GATHER_THREADS, lds, prog_stream(ctx, P)>>>(A);
   }
 }
 
@@ -1372,7 +1372,7 @@ inline uint32_t apply_blocks_for(const cytom_ctx* ctx) {
 }  // namespace
 
 void run_iters(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t n) {
-  hipStream_t st = ctx->stream;
+  hipStream_t st = prog_stream(ctx, P);
   const uint32_t ab = apply_blocks_for(ctx);
   const bool prof = ctx->profile && P.kev.size() >= 2 * (size_t)ctx->max_iter;
   for (uint32_t i = lo; i < lo + n; ++i) {
@@ -1451,7 +1451,7 @@ size_t max_groups(const cytom_ctx* ctx) { return ctx->max_iter / 16 + 6; }
 }  // namespace
 
 void run_group(cytom_ctx* ctx, Program& P) {
-  hipStream_t st = ctx->stream;
+  hipStream_t st = prog_stream(ctx, P);
   const uint32_t n = group_length(ctx, P);
   const uint32_t g = P.groups++;
   if (g == 0) {

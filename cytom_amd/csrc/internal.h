@@ -146,6 +146,8 @@ struct Program {
   uint32_t launched = 0, groups = 0, checked = 0;
   int conv = -1;
   const unsigned long long* refuse = nullptr;
+  // The stream a step runs the program on (nullptr: the context's stream).
+  hipStream_t stream = nullptr;
   int cur_act = 0;
   bool fixpoint = false;
   int last_reexec = 0;
@@ -240,6 +242,11 @@ struct cytom_ctx {
     uint64_t used = 0;
   };
   bool step_graphs = false;
+  // Steps run their programs concurrently, each on a stream of its own
+  // (forked from the context's stream after the update batch).
+  bool concurrent_runs = true;
+  std::vector<hipStream_t> run_streams;
+  hipEvent_t fork_ev = nullptr;
   // While a step graph is being recorded the enqueue functions leave their
   // timing events out (events recorded inside a graph carry no timestamps);
   // the step records them around the graph launches instead.
@@ -269,6 +276,11 @@ struct cytom_ctx {
 };
 
 namespace cytom {
+
+// The stream a program's iterations are launched on.
+inline hipStream_t prog_stream(const cytom_ctx* c, const Program& P) {
+  return P.stream ? P.stream : c->stream;
+}
 
 int set_err(cytom_ctx* ctx, int code, const std::string& msg);
 int hip_fail(cytom_ctx* ctx, hipError_t e, const char* what);

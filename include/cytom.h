@@ -214,6 +214,11 @@ int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, con
  * equal those of direct launches; with profiling on, steps launch directly.
  * 0 disables graphs and frees them. */
 int cytom_set_step_graphs(cytom_ctx* ctx, int enable);
+/* With enable != 0 (the default) cytom_step runs its programs concurrently
+ * after the update batch, each on a stream of its own (at most 4), so one
+ * program's launches fill the gaps and tails of another's; 0 runs them one
+ * after the other on the context's stream.  Values are the same either way. */
+int cytom_set_concurrent_runs(cytom_ctx* ctx, int enable);
 /* Graphs recorded and replayed by the context's steps, and graphs kept. */
 int cytom_step_graph_stats(cytom_ctx* ctx, uint64_t* captures, uint64_t* replays,
                            uint32_t* cached);
