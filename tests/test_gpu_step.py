@@ -186,3 +186,28 @@ def test_step_graphs_equal_direct_steps(cap):
     assert ga.step_graph_stats() == (0, 0, 0)
     gb.set_step_graphs(False)
     assert gb.step_graph_stats()[2] == 0
+
+
+def test_concurrent_runs_equal_serial_runs():
+    # The same stream with the programs run concurrently (one stream each)
+    # and one after the other: identical values, and for PageRank-delta an
+    # identical trajectory.
+    scale, vpt, seed, n_ins, n_del = 14, 512, 6, 16000, 1600
+    V = 1 << scale
+    st = O.Stream(seed)
+    ga, ia = make(V, vpt)
+    gb, ib = make(V, vpt)
+    ga.set_concurrent_runs(False)
+    gb.set_concurrent_runs(True)
+    for b in range(6):
+        first, didx = st.next(n_ins, n_del if b else 0)
+        s, d, w, ds, dd, dw = device_batch(scale, seed, first, n_ins, didx)
+        ba, ra = ga.step(ia, s, d, w, n_ins, ds, dd, dw, len(didx))
+        bb, rb = gb.step(ib, s, d, w, n_ins, ds, dd, dw, len(didx))
+        assert (bb.deleted, bb.live_edges) == (ba.deleted, ba.live_edges), b
+        for k, name in enumerate(NAMES):
+            assert np.array_equal(ga.read(ia[k]), gb.read(ib[k])), (b, name)
+            assert rb[k].converged == ra[k].converged == 1, (b, name)
+        k = NAMES.index("pagerank-delta")
+        assert (rb[k].iterations, rb[k].traversed_edges) == \
+            (ra[k].iterations, ra[k].traversed_edges), b
