@@ -866,6 +866,23 @@ int cytom_stream_destroy(cytom_stream* s) {
 // Inserts take the next n_ins stream indices; then n_del live indices are
 // removed by sequential swap-remove sampling: draw j picks position
 // hash(hd ^ (batch << 32 | j)) mod |live|.
+namespace {
+
+// Position of a deletion draw among n live indices: SplitMix64's output
+// function (Steele, Lea and Flood, OOPSLA 2014) as a full-avalanche mix of
+// the draw's key, scaled to [0, n) by the high half of a 64 x 64-bit product.
+// (Ligra's hash64 taken modulo n correlated the positions of different
+// seeds: the drawn indices were visibly non-uniform.)
+uint64_t mix_position(uint64_t z, uint64_t n) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return (uint64_t)(((unsigned __int128)z * n) >> 64);
+}
+
+}  // namespace
+
 int64_t cytom_stream_next(cytom_stream* s, uint64_t n_ins, uint64_t n_del, uint64_t* ins_first,
                           uint64_t* del_index) {
   if (!s || !ins_first || (n_del && !del_index)) return CYTOM_EINVAL;
@@ -883,7 +900,7 @@ int64_t cytom_stream_next(cytom_stream* s, uint64_t n_ins, uint64_t n_del, uint6
   const uint64_t m = std::min<uint64_t>(n_del, s->live.size());
   for (uint64_t j = 0; j < m; ++j) {
     const uint64_t n_live = s->live.size();
-    const uint64_t pos = hash64(s->hash_del ^ ((s->batch << 32) | j)) % n_live;
+    const uint64_t pos = mix_position(s->hash_del ^ ((s->batch << 32) | j), n_live);
     del_index[j] = s->live[pos];
     s->live[pos] = s->live.back();
     s->live.pop_back();

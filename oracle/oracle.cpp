@@ -625,8 +625,18 @@ orc_stream* orc_stream_create(uint64_t seed) {
 void orc_stream_destroy(orc_stream* s) { delete s; }
 uint64_t orc_stream_live_count(const orc_stream* s) { return s->live.size(); }
 
+// SplitMix64's output function (Steele, Lea and Flood, "Fast splittable
+// pseudorandom number generators", OOPSLA 2014): a full-avalanche 64-bit mix.
+static uint64_t splitmix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
 // Deletions: sequential swap-remove sampling, uniform without replacement
-// over the live stream indices after this batch's inserts.
+// over the live stream indices after this batch's inserts.  Draw j of batch b
+// takes position floor(splitmix64(h ^ (b << 32 | j)) * n_live / 2^64).
 int64_t orc_stream_next(orc_stream* s, uint64_t n_ins, uint64_t n_del, uint64_t* ins_first,
                         uint64_t* del_idx) {
   *ins_first = s->next_index;
@@ -635,7 +645,8 @@ int64_t orc_stream_next(orc_stream* s, uint64_t n_ins, uint64_t n_del, uint64_t*
   uint64_t m = std::min<uint64_t>(n_del, s->live.size());
   for (uint64_t j = 0; j < m; ++j) {
     uint64_t n_live = s->live.size();
-    uint64_t r = hash64(s->hash_del ^ ((s->batch << 32) | j)) % n_live;
+    uint64_t r = (uint64_t)(((unsigned __int128)splitmix64(s->hash_del ^ ((s->batch << 32) | j)) *
+                             n_live) >> 64);
     del_idx[j] = s->live[r];
     s->live[r] = s->live[n_live - 1];
     s->live.pop_back();

@@ -104,6 +104,56 @@ def test_stream_deletions_are_live_and_distinct():
     assert len(dels) == sum(live.values()) and st.live == 0
 
 
+def test_stream_weights_are_uniform_in_1_255():
+    # SSSP weights of the stream: uniform over [1, 255], consecutive weights
+    # uncorrelated.
+    from scipy import stats
+    w = O.weights_u32(2, 0, 1 << 20)
+    assert w.min() == 1 and w.max() == 255
+    c = np.bincount(w, minlength=256)[1:]
+    e = len(w) / 255
+    chi = float(((c - e) ** 2 / e).sum())
+    assert 1e-4 < stats.chi2.sf(chi, 254) < 1 - 1e-4, chi
+    r = np.corrcoef(w[:-1].astype(np.float64), w[1:].astype(np.float64))[0, 1]
+    assert abs(r) < 5 / np.sqrt(len(w)), r
+
+
+def test_stream_deletions_are_uniform_over_live_indices():
+    # The deletion draws of a batch are a uniform sample without replacement
+    # of the live stream indices: over many seeds every live index is drawn
+    # equally often (chi-square against the uniform expectation), also when
+    # an earlier batch already removed part of the live list.  The product's
+    # sampler equals this one draw for draw (test_abi).
+    from scipy import stats
+    n, k1, k2, seeds = 1000, 400, 100, 300
+    first_batch = np.zeros(n, np.int64)
+    second_batch = np.zeros(n, np.int64)
+    survivors = np.zeros(n, np.int64)
+    for seed in range(seeds):
+        st = O.Stream(1000 + seed)
+        _, d1 = st.next(n, k1)
+        first_batch[d1.astype(np.int64)] += 1
+        alive = np.ones(n, bool)
+        alive[d1.astype(np.int64)] = False
+        survivors += alive
+        _, d2 = st.next(0, k2)
+        assert alive[d2.astype(np.int64)].all()
+        second_batch[d2.astype(np.int64)] += 1
+    # Each index's count over the seeds is Binomial(seeds, k1 / n).
+    p1 = k1 / n
+    chi1 = float(((first_batch - seeds * p1) ** 2 / (seeds * p1 * (1 - p1))).sum())
+    assert 1e-4 < stats.chi2.sf(chi1, n - 1) < 1 - 1e-4, chi1
+    # Second batch: an index that survived the first batch is drawn with
+    # probability k2 / (n - k1) each time it survived.
+    p2 = k2 / (n - k1)
+    var2 = survivors * p2 * (1 - p2)
+    chi2 = float(((second_batch - survivors * p2) ** 2 / var2).sum())
+    assert 1e-4 < stats.chi2.sf(chi2, n - 1) < 1 - 1e-4, chi2
+    # No drift along the stream: the mean drawn index is the middle one.
+    idx = np.repeat(np.arange(n), first_batch)
+    assert abs(idx.mean() - (n - 1) / 2) < 5 * idx.std() / np.sqrt(len(idx))
+
+
 # ------------------------------------------------------------------ store
 def _multiset(s, d, w=None):
     if w is None:
