@@ -144,6 +144,7 @@ SIGNATURES = {
                                                   ctypes.POINTER(BatchStats)]),
     "cytom_read_results": (ctypes.c_int64, [_VP, ctypes.c_int, _VP, ctypes.c_size_t]),
     "cytom_execute": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(RunStats)]),
+    "cytom_execute_many": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
     "cytom_ppr_program_adjoint": (ctypes.c_int64, [_VP, ctypes.c_int, _VP, ctypes.c_float, _U32,
                                                  _VP, _VP, _VP, _VP, _U64,
                                                  ctypes.POINTER(ctypes.c_uint32)]),
@@ -546,6 +547,15 @@ class Graph:
             self._h, pid, _p(g), tol, max_iterations, _p(dt), _p(s), _p(d), _p(dw), max(E, 1),
             ctypes.byref(it)), "ppr_adjoint")
         return dt.to_host(), s.to_host()[:n], d.to_host()[:n], dw.to_host()[:n], it.value
+
+    def execute_many(self, pids):
+        """Runs every program in pids on the current graph, concurrently (one
+        stream each); returns [RunStats per program]."""
+        ids = (ctypes.c_int * max(1, len(pids)))(*pids)
+        rs = (RunStats * max(1, len(pids)))()
+        self._check(lib().cytom_execute_many(self._h, ctypes.cast(ids, ctypes.c_void_p), len(pids),
+                                             ctypes.cast(rs, ctypes.c_void_p)), "execute_many")
+        return [rs[k] for k in range(len(pids))]
 
     def step(self, pids, src, dst, w, n_ins, dsrc=None, ddst=None, dw=None, n_del=0):
         """One update batch from device arrays (as update_device) followed by

@@ -572,6 +572,20 @@ int run_streams(cytom_ctx* ctx, size_t n) {
 
 }  // namespace
 
+// The listed programs of a call (ids in range, each at most once).
+static int program_list(cytom_ctx* ctx, const int* prog_ids, uint32_t n_progs,
+                 std::vector<Program*>& ps) {
+  if (n_progs && !prog_ids) return set_err(ctx, CYTOM_EINVAL, "null program list");
+  for (uint32_t k = 0; k < n_progs; ++k) {
+    const int id = prog_ids[k];
+    if (id < 0 || id >= (int)ctx->progs.size()) return set_err(ctx, CYTOM_EINVAL, "bad program id");
+    for (uint32_t j = 0; j < k; ++j)
+      if (prog_ids[j] == id) return set_err(ctx, CYTOM_EINVAL, "program listed twice");
+    ps.push_back(&ctx->progs[id]);
+  }
+  return 0;
+}
+
 // edge-inserter.h:129-184: one batch, then (re-initialisation where the
 // batch requires it and) every registered algorithm's run.  The batch, the
 // work lists and the first iteration group of every program are enqueued
@@ -584,18 +598,11 @@ int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, con
   CY_CHECK_CTX(ctx);
   if ((n_ins && (!d_src || !d_dst)) || (n_del && (!d_del_src || !d_del_dst)))
     return set_err(ctx, CYTOM_EINVAL, "null update arrays");
-  if (n_progs && !prog_ids) return set_err(ctx, CYTOM_EINVAL, "null program list");
   if (ctx->weighted && n_ins && !d_w)
     return set_err(ctx, CYTOM_EINVAL, "weighted graph needs insertion weights");
   std::vector<Program*> ps;
-  for (uint32_t k = 0; k < n_progs; ++k) {
-    const int id = prog_ids[k];
-    if (id < 0 || id >= (int)ctx->progs.size()) return set_err(ctx, CYTOM_EINVAL, "bad program id");
-    for (uint32_t j = 0; j < k; ++j)
-      if (prog_ids[j] == id) return set_err(ctx, CYTOM_EINVAL, "program listed twice");
-    ps.push_back(&ctx->progs[id]);
-  }
   int rc;
+  if ((rc = program_list(ctx, prog_ids, n_progs, ps))) return rc;
   if ((rc = ingest_prepare(ctx, n_ins, n_del)) || (rc = prepare_jobs(ctx))) return rc;
   for (Program* P : ps)
     if ((rc = program_events(ctx, *P))) return rc;
@@ -806,6 +813,50 @@ int cytom_set_step_graphs(cytom_ctx* ctx, int enable) {
     CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     free_graphs(ctx);
   }
+  return 0;
+}
+
+// The runs of several programs on the current graph (no update batch): as
+// cytom_execute of each in turn, but run concurrently, one stream each, with
+// one host synchronisation per iteration group of all of them.
+int cytom_execute_many(cytom_ctx* ctx, const int* prog_ids, uint32_t n_progs,
+                       cytom_run_stats* run_stats) {
+  CY_CHECK_CTX(ctx);
+  std::vector<Program*> ps;
+  int rc;
+  if ((rc = program_list(ctx, prog_ids, n_progs, ps))) return rc;
+  if (ctx->jobs_dirty && (rc = build_jobs(ctx))) return rc;
+  for (Program* P : ps)
+    if ((rc = program_events(ctx, *P))) return rc;
+  const size_t ns = ctx->concurrent_runs && ps.size() > 1 ? std::min(ps.size(), MAX_RUN_STREAMS)
+                                                          : 0;
+  if (ns && (rc = run_streams(ctx, ns))) return rc;
+  struct StreamReset {
+    std::vector<Program*>& ps;
+    ~StreamReset() {
+      for (Program* P : ps) P->stream = nullptr;
+    }
+  } reset{ps};
+  if (ns) {
+    CY_HIP(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
+    for (size_t k = 0; k < ps.size(); ++k) {
+      ps[k]->stream = ctx->run_streams[k % ns];
+      CY_HIP(ctx, hipStreamWaitEvent(ps[k]->stream, ctx->fork_ev, 0));
+    }
+  }
+  std::vector<char> more(ps.size(), 1);
+  for (Program* P : ps) run_open(*P);
+  for (bool any = !ps.empty(); any;) {
+    for (size_t k = 0; k < ps.size(); ++k)
+      if (more[k]) run_group(ctx, *ps[k]);
+    CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < ns; ++i) CY_HIP(ctx, hipStreamSynchronize(ctx->run_streams[i]));
+    any = false;
+    for (size_t k = 0; k < ps.size(); ++k)
+      if (more[k]) any |= (more[k] = run_needs_more(ctx, *ps[k])) != 0;
+  }
+  for (size_t k = 0; k < ps.size(); ++k)
+    if ((rc = run_finish(ctx, *ps[k], run_stats ? &run_stats[k] : nullptr))) return rc;
   return 0;
 }
 

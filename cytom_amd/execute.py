@@ -204,9 +204,10 @@ def run_group(jobs, idxs, kinds, source, out, seed=RMAT_SEED):
         acc["ingest_ms"] += st.t_ingest_ms
         line = "batch %d: +%d -%d live %d" % (acc["batches"], n_ins, st.deleted, st.live_edges)
         if j0["enable-interactive-algorithm"] and acc["inserted"] >= j0["start-algorithm-count"]:
-            for r in runs:
-                rs = _execute(g, r, reinit=bool(r.job["enable-algorithm-reexecution"]
-                                                 or not r.job["enable-edge-apis"]))
+            # The jobs sharing this graph run together, concurrently on the device.
+            stats = _execute_all(g, runs, [bool(r.job["enable-algorithm-reexecution"]
+                                                or not r.job["enable-edge-apis"]) for r in runs])
+            for r, rs in zip(runs, stats):
                 line += " | job %d %s: %d it, %d edges, %.3f ms" % (
                     r.index, r.job["algorithm"], rs.iterations, rs.traversed_edges,
                     rs.t_compute_ms)
@@ -228,10 +229,25 @@ def run_group(jobs, idxs, kinds, source, out, seed=RMAT_SEED):
     return summaries
 
 
+def _execute_all(g, runs, reinit):
+    for r, ri in zip(runs, reinit):
+        if ri:
+            g.init(r.pid)
+    stats = g.execute_many([r.pid for r in runs])
+    for r, rs in zip(runs, stats):
+        _account(g, r, rs)
+    return stats
+
+
 def _execute(g, r, reinit):
     if reinit:
         g.init(r.pid)
     rs = g.execute(r.pid)
+    _account(g, r, rs)
+    return rs
+
+
+def _account(g, r, rs):
     r.executions += 1
     r.iterations += rs.iterations
     r.te += rs.traversed_edges
@@ -240,7 +256,6 @@ def _execute(g, r, reinit):
         vals = g.read_results(r.pid)[:r.job["max-vertices"]]
         formats.write_results(r.results_dir, r.results, vals)
         r.results += 1
-    return rs
 
 
 def run_jobs(jobs, store=None, out=print, seed=RMAT_SEED):

@@ -187,6 +187,14 @@ int cytom_update_batch_records(cytom_ctx* ctx, const void* records, uint32_t rec
  * (continuous-algorithm-executor.h:49-249): runs the program until
  * convergence or max_iterations. */
 int cytom_execute(cytom_ctx* ctx, int prog_id, cytom_run_stats* stats);
+/* Runs of several programs on the current graph, as cytom_execute of each in
+ * turn (the same values), but concurrently, each on a stream of its own
+ * (cytom_set_concurrent_runs), with one host synchronisation per iteration
+ * group of all of them: the jobs of one process that share a graph
+ * (jobs.h:33-140) run as one device-side mix.  run_stats: n_progs entries or
+ * NULL. */
+int cytom_execute_many(cytom_ctx* ctx, const int* prog_ids, uint32_t n_progs,
+                       cytom_run_stats* run_stats);
 /* One step of the continuous loop, edge-inserter.h:129-184: the update batch
  * (device pointers, as cytom_update_batch_device), then a run of each of the
  * n_progs listed programs (distinct ids), with the re-initialisations the

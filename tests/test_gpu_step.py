@@ -211,3 +211,34 @@ def test_concurrent_runs_equal_serial_runs():
         k = NAMES.index("pagerank-delta")
         assert (rb[k].iterations, rb[k].traversed_edges) == \
             (ra[k].iterations, ra[k].traversed_edges), b
+
+
+def test_execute_many_equals_separate_runs():
+    # Several programs run together (concurrently) on the same graph equal
+    # their separate runs, including after re-initialisation and with a
+    # program listed alone; argument errors are refused.
+    scale, vpt, seed, n_ins, n_del = 13, 512, 8, 12000, 1200
+    V = 1 << scale
+    st = O.Stream(seed)
+    ga, ia = make(V, vpt)
+    gb, ib = make(V, vpt)
+    for b in range(4):
+        first, didx = st.next(n_ins, n_del if b else 0)
+        s, d, w, ds, dd, dw = device_batch(scale, seed, first, n_ins, didx)
+        ga.update_device(s, d, w, n_ins, ds, dd, dw, len(didx))
+        gb.update_device(s, d, w, n_ins, ds, dd, dw, len(didx))
+        if b == 2:
+            for g, ids in ((ga, ia), (gb, ib)):
+                for p in ids:
+                    g.init(p)
+        ra = [ga.execute(p) for p in ia]
+        rb = gb.execute_many(ib)
+        for k, name in enumerate(NAMES):
+            assert np.array_equal(ga.read(ia[k]), gb.read(ib[k])), (b, name)
+            assert rb[k].converged == ra[k].converged == 1, (b, name)
+    [r] = gb.execute_many([ib[3]])
+    assert r.converged == 1
+    with pytest.raises(C.CytomError):
+        gb.execute_many([ib[0], ib[0]])
+    with pytest.raises(C.CytomError):
+        gb.execute_many([len(NAMES) + 5])
