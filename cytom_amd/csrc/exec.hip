@@ -300,6 +300,22 @@ __global__ void k_row_groups(const uint64_t* __restrict__ chunks,
   }
 }
 
+// Row descriptors of every chunk in chunk order (a chunk of n slots owns
+// ceil(n/64) rows; rowb, the exclusive scan of those counts, gives a job's
+// rows as rowb[first chunk] .. rowb[end chunk]).  A gather over a dense
+// frontier copies them as they are instead of rebuilding them per pass.
+__global__ void k_chunk_rows(const uint64_t* __restrict__ chunks,
+                             const uint32_t* __restrict__ n_chunks_dev,
+                             const uint32_t* __restrict__ cnt, uint32_t CH,
+                             uint32_t* __restrict__ nrow) {
+  const uint32_t n_chunks = *n_chunks_dev;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n_chunks;
+       c += gridDim.x * blockDim.x) {
+    const uint64_t ch = chunks[c];
+    nrow[c] = (min(CH, cnt[ch >> 32] - (uint32_t)ch) + 63) >> 6;
+  }
+}
+
 // Jobs are cut from cells: cell (b, y) holds the chunks of column y whose
 // source rows fall in band b (source rows [xb[b], xb[b+1])).  Cells are
 // numbered band-major, so the jobs of one band form one contiguous queue.
@@ -389,6 +405,31 @@ __global__ void k_emit_jobs(const uint32_t* __restrict__ cbase, const uint32_t* 
   }
 }
 
+// Largest jobs first within each band's queue (longest-processing-time
+// order): a block that takes a job late takes a short one, so the blocks of
+// a launch finish close together.  Sort key: the band (bits 24+), then the
+// job's row count, descending (bits 0-23); the sort is stable.
+__global__ void k_job_keys(const uint4* __restrict__ jobs, const uint32_t* __restrict__ jbase,
+                           uint32_t T, uint32_t NB, const uint32_t* __restrict__ rowb,
+                           uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
+  const uint32_t n = jbase[T * NB];
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    uint32_t b = 0;
+    while (b + 1 < NB && jbase[(b + 1) * T] <= j) ++b;
+    const uint4 jb = jobs[j];
+    const uint32_t rows = min(rowb[jb.y] - rowb[jb.x], 0xFFFFFFu);
+    key[j] = (b << 24) | (0xFFFFFFu - rows);
+    idx[j] = j;
+  }
+}
+
+__global__ void k_permute_jobs(const uint4* __restrict__ in, const uint32_t* __restrict__ idx,
+                               const uint32_t* __restrict__ n_dev, uint4* __restrict__ out) {
+  const uint32_t n = *n_dev;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    out[j] = in[idx[j]];
+}
+
 // --------------------------------------------------------------- gather
 template <int KIND>
 struct GatherArgs {
@@ -396,6 +437,8 @@ struct GatherArgs {
   const uint32_t* jbase;  // first job of every cell (cells band-major), cells + 1 entries
   uint32_t nbands;
   const uint64_t* chunks;
+  const uint32_t* rowb;        // first row of every chunk in rows (chunks + 1 entries)
+  const uint64_t* rows;        // row descriptors of every chunk in chunk order
   StoreView S;
   const uint8_t* gact;         // active groups, 64 bytes per source row
   uint64_t* gact_clear;        // the next iteration's group bytes, cleared here (T * 8 words)
@@ -425,6 +468,22 @@ constexpr uint32_t MAX_JOB_ROWS = 2048;
 // this lane).
 __device__ inline uint64_t row_desc(uint64_t off, uint32_t x, uint32_t nvalid) {
   return (off << 22) | ((uint64_t)x << 6) | (uint64_t)(nvalid - 1);
+}
+
+// The row descriptors of every chunk, at rowb[c] onwards (one thread per chunk).
+__global__ void k_emit_rows(const uint64_t* __restrict__ chunks,
+                            const uint32_t* __restrict__ n_chunks_dev, StoreView S, uint32_t CH,
+                            const uint32_t* __restrict__ rowb, uint64_t* __restrict__ rows) {
+  const uint32_t n_chunks = *n_chunks_dev;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n_chunks;
+       c += gridDim.x * blockDim.x) {
+    const uint64_t ch = chunks[c];
+    const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
+    const uint32_t n = min(CH, S.tile_cnt[t] - s0), x = t % S.T;
+    const uint64_t off = S.tile_off[t] + s0;
+    uint64_t* out = rows + rowb[c];
+    for (uint32_t k = 0; 64 * k < n; ++k) out[k] = row_desc(off + 64 * k, x, min(64u, n - 64 * k));
+  }
 }
 
 template <int KIND>
@@ -569,6 +628,52 @@ __device__ inline void gather_rows(const GatherArgs<KIND>& A, uint32_t* win,
   }
 }
 
+// A frontier of at least V / DENSE_DIV active vertices gathers whole jobs.
+constexpr uint64_t DENSE_DIV = 8;
+
+template <int KIND>
+__device__ inline void clear_window(uint32_t* win, unsigned long long* winq, uint32_t vpt) {
+  if (KIND == CYTOM_PRD) {
+    for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) winq[i] = 0ull;
+  } else {
+    const uint32_t n = (KIND == KIND_SSSP_OCT ? 8u : KIND == KIND_SSSP_PAIR ? 2u : 1u) * vpt;
+    for (uint32_t i = threadIdx.x; i < n; i += GATHER_THREADS) win[i] = INF;
+  }
+}
+
+// Folds a job's column window into the column's values (targets base ..
+// base + vpt): int64 sums into dsum, minima into nxt, flagging the changed
+// targets.  Other jobs of the same column flush concurrently, hence atomics.
+template <int KIND>
+__device__ inline void flush_window(const GatherArgs<KIND>& A, const uint32_t* win,
+                                    const unsigned long long* winq, uint64_t base, uint32_t vpt) {
+  for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) {
+    if (KIND == CYTOM_PRD) {
+      const unsigned long long q = winq[i];
+      if (q != 0ull) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&A.dsum[base + i]), q);
+        A.changed[base + i] = 1;
+      }
+    } else if (KIND == KIND_SSSP_OCT || KIND == KIND_SSSP_PAIR) {
+      constexpr uint32_t W = KIND == KIND_SSSP_OCT ? 8u : 2u;
+      bool ch = false;
+#pragma unroll
+      for (uint32_t k = 0; k < W; ++k) {
+        const uint32_t val = win[W * i + k];
+        uint32_t* dst = &A.nxt[W * (base + i) + k];
+        if (val != INF && val < *dst) ch |= val < atomicMin(dst, val);
+      }
+      if (ch) A.changed[base + i] = 1;
+    } else {
+      const uint32_t val = win[i];
+      if (val != INF && val < A.nxt[base + i]) {
+        const uint32_t old = atomicMin(&A.nxt[base + i], val);
+        if (val < old) A.changed[base + i] = 1;
+      }
+    }
+  }
+}
+
 template <int KIND>
 __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
   // The column window: u32 values for BFS/CC/SSSP, int64 fixed-point sums
@@ -594,8 +699,14 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
   // each, at most MAX_JOB_ROWS rows); the window lives across the passes and
   // is flushed once per job.
   const uint32_t PASS = min((uint32_t)GATHER_THREADS, (MAX_JOB_ROWS * 64u) / A.CH);
+  // A dense frontier (PageRank-delta's in almost every iteration) takes
+  // every row of its jobs: the rows are copied from the prebuilt table, with
+  // no per-pass chunk decode or scan.  Rows of inactive sources cost their
+  // slot loads only (their source values are 0 / INF).
+  const bool dense = A.it->count * DENSE_DIV >= A.S.V;
   unsigned long long scanned = 0;
   for (;;) {
+    __syncthreads();  // every thread has read the previous s_job
     if (threadIdx.x == 0) {
       // own band's queue first, then the others in turn
       uint32_t j = UINT32_MAX;
@@ -616,97 +727,73 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather(GatherArgs<KIND> A) {
     if (j == UINT32_MAX) break;  // uniform: every thread read s_job after the barrier
     const uint4 job = A.jobs[j];
     const uint32_t y = job.z;
-    bool cleared = false;  // uniform
-    for (uint32_t cs = job.x; cs < job.y; cs += PASS) {
-      const uint32_t nc = min(PASS, job.y - cs);
-      // Row table of the pass's rows whose sources meet an active group of
-      // their source row.  One scan packs the rows (bits 24+) and slots
-      // (bits 0-23) before each chunk.
-      uint32_t n = 0, x = 0, keep = 0, nrows = 0, nslots = 0;
-      uint64_t off = 0;
-      if (threadIdx.x < nc) {
-        const uint64_t ch = A.chunks[cs + threadIdx.x];
-        const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
-        x = t - y * T;
-        const uint64_t m = group_mask(A.gact, x);
-        if (m) {
-          off = A.S.tile_off[t] + s0;
-          n = min(A.CH, A.S.tile_cnt[t] - s0);
-          const uint32_t nr = (n + 63) >> 6;
-          // PageRank-delta frontiers are dense in almost every iteration: its
-          // rows are taken whole once any group of their source row is active.
-          const bool whole = KIND == CYTOM_PRD || m == ~0ull;
-          for (uint32_t k = 0; k < nr; ++k) {
-            if (whole || (m & A.grp[(off >> 6) + k])) {
-              keep |= 1u << k;
-              ++nrows;
-              nslots += min(64u, n - 64 * k);
+    bool filled = false;  // uniform: the window holds this job's values
+    if (dense) {
+      const uint32_t rb0 = A.rowb[job.x], rb1 = A.rowb[job.y];
+      filled = rb1 > rb0;
+      if (filled) clear_window<KIND>(win, winq, vpt);
+      for (uint32_t r = rb0; r < rb1; r += MAX_JOB_ROWS) {
+        const uint32_t R = min(MAX_JOB_ROWS, rb1 - r);
+        for (uint32_t i = threadIdx.x; i < R; i += GATHER_THREADS) {
+          const uint64_t d = A.rows[r + i];
+          s_rows[i] = d;
+          scanned += (d & 63u) + 1;
+        }
+        __syncthreads();
+        gather_rows<KIND>(A, win, winq, s_rows, R, false);
+        __syncthreads();
+      }
+    } else {
+      for (uint32_t cs = job.x; cs < job.y; cs += PASS) {
+        const uint32_t nc = min(PASS, job.y - cs);
+        // Row table of the pass's rows whose sources meet an active group of
+        // their source row.  One scan packs the rows (bits 24+) and slots
+        // (bits 0-23) before each chunk.
+        uint32_t n = 0, x = 0, keep = 0, nrows = 0, nslots = 0;
+        uint64_t off = 0;
+        if (threadIdx.x < nc) {
+          const uint64_t ch = A.chunks[cs + threadIdx.x];
+          const uint32_t t = (uint32_t)(ch >> 32), s0 = (uint32_t)ch;
+          x = t - y * T;
+          const uint64_t m = group_mask(A.gact, x);
+          if (m) {
+            off = A.S.tile_off[t] + s0;
+            n = min(A.CH, A.S.tile_cnt[t] - s0);
+            const uint32_t nr = (n + 63) >> 6;
+            // PageRank-delta frontiers are dense in almost every iteration: its
+            // rows are taken whole once any group of their source row is active.
+            const bool whole = KIND == CYTOM_PRD || m == ~0ull;
+            for (uint32_t k = 0; k < nr; ++k) {
+              if (whole || (m & A.grp[(off >> 6) + k])) {
+                keep |= 1u << k;
+                ++nrows;
+                nslots += min(64u, n - 64 * k);
+              }
             }
           }
         }
-      }
-      const uint64_t packed = ((uint64_t)nrows << 24) | (uint64_t)nslots;
-      uint64_t tot;
-      const uint64_t ex = block_scan_excl<GATHER_THREADS, uint64_t>(packed, &tot);
-      const uint32_t R = (uint32_t)(tot >> 24);
-      if (R == 0) continue;  // uniform; the scan's barriers ordered s_rows / s_job reuse
-      for (uint32_t k = 0, r = (uint32_t)(ex >> 24); keep; ++k, keep >>= 1)
-        if (keep & 1u) s_rows[r++] = row_desc(off + 64 * k, x, min(64u, n - 64 * k));
-      scanned += tot & 0xFFFFFFull;
-      if (!cleared) {
-        if (KIND == CYTOM_PRD) {
-          for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) winq[i] = 0ull;
-        } else if (KIND == KIND_SSSP_OCT) {
-          for (uint32_t i = threadIdx.x; i < 8 * vpt; i += GATHER_THREADS) win[i] = INF;
-        } else if (KIND == KIND_SSSP_PAIR) {
-          for (uint32_t i = threadIdx.x; i < 2 * vpt; i += GATHER_THREADS) win[i] = INF;
-        } else {
-          for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) win[i] = INF;
+        const uint64_t packed = ((uint64_t)nrows << 24) | (uint64_t)nslots;
+        uint64_t tot;
+        const uint64_t ex = block_scan_excl<GATHER_THREADS, uint64_t>(packed, &tot);
+        const uint32_t R = (uint32_t)(tot >> 24);
+        if (R == 0) continue;  // uniform; the scan's barriers ordered s_rows reuse
+        for (uint32_t k = 0, r = (uint32_t)(ex >> 24); keep; ++k, keep >>= 1)
+          if (keep & 1u) s_rows[r++] = row_desc(off + 64 * k, x, min(64u, n - 64 * k));
+        scanned += tot & 0xFFFFFFull;
+        if (!filled) {
+          clear_window<KIND>(win, winq, vpt);
+          filled = true;
         }
-        cleared = true;
-      }
-      __syncthreads();
-      gather_rows<KIND>(A, win, winq, s_rows, R, sparse);
-      __syncthreads();
-    }
-    if (!cleared) continue;  // no active row in the job
-    const uint64_t base = (uint64_t)y * vpt;
-    for (uint32_t i = threadIdx.x; i < vpt; i += GATHER_THREADS) {
-      if (KIND == CYTOM_PRD) {
-        const unsigned long long q = winq[i];
-        if (q != 0ull) {
-          atomicAdd(reinterpret_cast<unsigned long long*>(&A.dsum[base + i]), q);
-          A.changed[base + i] = 1;
-        }
-      } else if (KIND == KIND_SSSP_OCT) {
-        bool ch = false;
-#pragma unroll
-        for (uint32_t k = 0; k < 8; ++k) {
-          const uint32_t val = win[8 * i + k];
-          uint32_t* dst = &A.nxt[8 * (base + i) + k];
-          if (val != INF && val < *dst) ch |= val < atomicMin(dst, val);
-        }
-        if (ch) A.changed[base + i] = 1;
-      } else if (KIND == KIND_SSSP_PAIR) {
-        bool ch = false;
-#pragma unroll
-        for (uint32_t k = 0; k < 2; ++k) {
-          const uint32_t val = win[2 * i + k];
-          uint32_t* dst = &A.nxt[2 * (base + i) + k];
-          if (val != INF && val < *dst) ch |= val < atomicMin(dst, val);
-        }
-        if (ch) A.changed[base + i] = 1;
-      } else {
-        const uint32_t val = win[i];
-        if (val != INF && val < A.nxt[base + i]) {
-          const uint32_t old = atomicMin(&A.nxt[base + i], val);
-          if (val < old) A.changed[base + i] = 1;
-        }
+        __syncthreads();
+        gather_rows<KIND>(A, win, winq, s_rows, R, sparse);
+        __syncthreads();
       }
     }
-    __syncthreads();
+    if (filled) flush_window<KIND>(A, win, winq, (uint64_t)y * vpt, vpt);
   }
-  // Every thread holds the block's count.
+  // The sparse path's scans leave the block's count in every thread; the
+  // dense path counts per thread.
+  if (dense) scanned = block_sum<GATHER_THREADS, uint64_t>(scanned);
   if (threadIdx.x == 0 && scanned) atomicAdd(&A.it->scanned, scanned);
 }
 
@@ -1110,6 +1197,8 @@ void launch_gather(cytom_ctx* ctx, Program& P, int it, hipEvent_t ev0, hipEvent_
   A.jbase = static_cast<const uint32_t*>(ctx->g_jbase.p);
   A.nbands = ctx->n_bands;
   A.chunks = static_cast<const uint64_t*>(ctx->g_chunks.p);
+  A.rowb = static_cast<const uint32_t*>(ctx->g_rowb.p);
+  A.rows = static_cast<const uint64_t*>(ctx->g_rows.p);
   A.S = ctx->view();
   A.gact = P.gact[P.cur_act];
   A.act = P.act[P.cur_act];
@@ -1148,10 +1237,22 @@ int prepare_jobs(cytom_ctx* ctx) {
   const uint64_t max_chunks = ctx->pool_size / ctx->chunk_slots + nt + 1;
   const uint64_t jc_min = ctx->job_chunks ? ctx->job_chunks : (1u << 17) / ctx->chunk_slots;
   const uint64_t max_jobs = max_chunks / jc_min + cells + 1;
+  // Rows: every chunk's slots in 64-slot rows, one partial row at most each.
+  const uint64_t max_rows = ctx->pool_size / 64 + max_chunks;
+  const uint64_t scan_n = std::max<uint64_t>(std::max<uint64_t>(nt, cells), max_chunks) + 1;
+  // the job sort: keys and indices in, out and between passes, its histograms
+  const uint64_t jw = (max_jobs * 4 + 255) / 256 * 256;
+  const uint64_t rs_scan = (radix_hist_bytes(max_jobs) / 4) + 1;
   if ((rc = ensure(ctx, ctx->g_nch, nt * 4)) || (rc = ensure(ctx, ctx->g_cbase, (nt + 1) * 4)) ||
       (rc = ensure(ctx, ctx->g_colj, cells * 4)) ||
       (rc = ensure(ctx, ctx->g_jbase, (cells + 1) * 4)) ||
-      (rc = ensure(ctx, ctx->g_scan_tmp, scan_tmp_bytes(std::max<uint64_t>(nt, cells) + 1) + 256)) ||
+      (rc = ensure(ctx, ctx->g_nrow, max_chunks * 4)) ||
+      (rc = ensure(ctx, ctx->g_rowb, (max_chunks + 1) * 4)) ||
+      (rc = ensure(ctx, ctx->g_rows, max_rows * 8)) ||
+      (rc = ensure(ctx, ctx->g_scan_tmp,
+                   scan_tmp_bytes(std::max<uint64_t>(scan_n, rs_scan)) + 256)) ||
+      (rc = ensure(ctx, ctx->g_jobs_raw, max_jobs * 16)) ||
+      (rc = ensure(ctx, ctx->g_jsort, 6 * jw + radix_hist_bytes(max_jobs))) ||
       (rc = ensure(ctx, ctx->g_bands, 256 + (uint64_t)ctx->T * 8)) ||
       (rc = ensure(ctx, ctx->g_chunks, max_chunks * 8)) ||
       (rc = ensure(ctx, ctx->g_grp, (ctx->pool_size / 64 + 1) * 8)) ||
@@ -1199,13 +1300,35 @@ void enqueue_jobs(cytom_ctx* ctx) {
   // batch moved, added and tombstoned slots).
   k_row_groups<<<grid_for(ctx->n_chunks * 64, 256), 256, 0, st>>>(
       chunks, cbase + nt, ctx->view(), ctx->chunk_slots, static_cast<uint64_t*>(ctx->g_grp.p));
+  // The row table of the dense gathers.
+  auto* nrow = static_cast<uint32_t*>(ctx->g_nrow.p);
+  auto* rowb = static_cast<uint32_t*>(ctx->g_rowb.p);
+  k_chunk_rows<<<grid_for(ctx->n_chunks, 256), 256, 0, st>>>(chunks, cbase + nt, ctx->tile_cnt,
+                                                            ctx->chunk_slots, nrow);
+  scan_exclusive_u32_devn(nrow, rowb, cbase + nt, ctx->n_chunks, ctx->g_scan_tmp.p, st);
+  k_emit_rows<<<grid_for(ctx->n_chunks, 256), 256, 0, st>>>(
+      chunks, cbase + nt, ctx->view(), ctx->chunk_slots, rowb, static_cast<uint64_t*>(ctx->g_rows.p));
   // Jobs of job_chunks chunks, or sized from the store: at least 2^17 slots,
   // and about eight jobs per CU when the store is large (fewer column flushes).
   k_job_length<<<1, 64, 0, st>>>(cbase + nt, ctx->job_chunks, ctx->chunk_slots,
                                  (uint32_t)ctx->num_cus, jc);
   k_col_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, xb, ctx->T, NB, jc, colj);
   scan_exclusive_u32(colj, jbase, cells, ctx->g_scan_tmp.p, st);
-  k_emit_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, xb, colj, jbase, ctx->T, NB, jc,
+  auto* raw = static_cast<uint4*>(ctx->g_jobs_raw.p);
+  k_emit_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, xb, colj, jbase, ctx->T, NB, jc, raw);
+  // Largest first within each band.
+  const uint64_t mj = ctx->n_jobs;
+  const uint64_t jw = (mj * 4 + 255) / 256 * 256;
+  char* js = static_cast<char*>(ctx->g_jsort.p);
+  auto* key = reinterpret_cast<uint32_t*>(js);
+  auto* idx = reinterpret_cast<uint32_t*>(js + jw);
+  auto* key_out = reinterpret_cast<uint32_t*>(js + 2 * jw);
+  auto* idx_out = reinterpret_cast<uint32_t*>(js + 3 * jw);
+  k_job_keys<<<grid_for(mj, 256), 256, 0, st>>>(raw, jbase, ctx->T, NB, rowb, key, idx);
+  radix_sort_pairs_devn(key, idx, key_out, idx_out, reinterpret_cast<uint32_t*>(js + 4 * jw),
+                        reinterpret_cast<uint32_t*>(js + 5 * jw), jbase + cells, mj, 28,
+                        js + 6 * jw, ctx->g_scan_tmp.p, st);
+  k_permute_jobs<<<grid_for(mj, 256), 256, 0, st>>>(raw, idx_out, jbase + cells,
                                                     static_cast<uint4*>(ctx->g_jobs.p));
   ctx->jobs_dirty = false;
 }
