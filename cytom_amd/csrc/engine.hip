@@ -69,7 +69,7 @@ std::vector<GrowBuf*> grow_bufs(cytom_ctx* c) {
           &c->g_colj,      &c->g_jbase,      &c->g_jobs,     &c->g_bands,     &c->g_grp,
           &c->g_rs_e,      &c->g_rs_w,       &c->g_rs_off,   &c->g_pflags,    &c->g_rec,
           &c->g_del_sig,   &c->g_nrow,       &c->g_rowb,     &c->g_rows,      &c->g_jobs_raw,
-          &c->g_jsort};
+          &c->g_jsort,     &c->g_del_bloom};
 }
 
 void free_graphs(cytom_ctx* c) {

@@ -219,31 +219,47 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_frontier(
   if (batch_refused(refuse)) return;  // the run follows a refused batch: it does nothing
   const int lane = threadIdx.x & 63;
   const uint64_t nwords = V >> 6;
-  const uint64_t nw_total = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  // Each wavefront takes FW words per step and loads them all before using
+  // any (several memory round trips in flight per wavefront).
+  constexpr int FW = 4;
+  const uint64_t stride = (((uint64_t)gridDim.x * blockDim.x) >> 6) * FW;
   unsigned long long te = 0, cnt = 0;
-  for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwords;
-       w += nw_total) {
-    const uint64_t word = act[w];
-    const uint64_t v = (w << 6) | lane;
-    const bool a = (word >> lane) & 1ull;
-    uint32_t od = 0;
-    if (a) {
-      od = outdeg[v];
-      te += od;
-      ++cnt;
+  for (uint64_t w0 = (((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * FW; w0 < nwords;
+       w0 += stride) {
+    uint64_t word[FW];
+#pragma unroll
+    for (int k = 0; k < FW; ++k) word[k] = act[min(w0 + k, nwords - 1)];
+    uint32_t od[FW];
+    uint32_t cv[FW];
+#pragma unroll
+    for (int k = 0; k < FW; ++k) {
+      const uint64_t v = (min(w0 + k, nwords - 1) << 6) | lane;
+      od[k] = outdeg[v];
+      cv[k] = (kind == CYTOM_BFS || kind == CYTOM_CC || kind == CYTOM_SSSP) ? cur[v] : 0u;
     }
-    // Frontier values: what the gather reads per edge instead of the bitmap.
-    if (kind == KIND_SSSP_OCT) {
-      for (uint32_t q = 0; q < 8; ++q) fval[v * 8 + q] = a ? cur[v * 8 + q] : INF;
-    } else if (prd_like(kind)) {
-      contrib[v] = (a && od) ? delta[v] / (float)od : 0.f;
-    } else if (kind == KIND_SSSP_PAIR) {
-      reinterpret_cast<uint2*>(fval)[v] =
-          a ? reinterpret_cast<const uint2*>(cur)[v] : make_uint2(INF, INF);
-    } else {
-      fval[v] = a ? cur[v] : INF;
+#pragma unroll
+    for (int k = 0; k < FW; ++k) {
+      const uint64_t w = w0 + k;
+      if (w >= nwords) break;  // uniform
+      const uint64_t v = (w << 6) | lane;
+      const bool a = (word[k] >> lane) & 1ull;
+      if (a) {
+        te += od[k];
+        ++cnt;
+      }
+      // Frontier values: what the gather reads per edge instead of the bitmap.
+      if (kind == KIND_SSSP_OCT) {
+        for (uint32_t q = 0; q < 8; ++q) fval[v * 8 + q] = a ? cur[v * 8 + q] : INF;
+      } else if (prd_like(kind)) {
+        contrib[v] = (a && od[k]) ? delta[v] / (float)od[k] : 0.f;
+      } else if (kind == KIND_SSSP_PAIR) {
+        reinterpret_cast<uint2*>(fval)[v] =
+            a ? reinterpret_cast<const uint2*>(cur)[v] : make_uint2(INF, INF);
+      } else {
+        fval[v] = a ? cv[k] : INF;
+      }
+      if (lane == 0) mark_groups(gact, word[k], v, log2vpt);
     }
-    if (lane == 0) mark_groups(gact, word, v, log2vpt);
   }
   te = block_sum<APPLY_THREADS, unsigned long long>(te);
   cnt = block_sum<APPLY_THREADS, unsigned long long>(cnt);

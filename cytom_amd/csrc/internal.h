@@ -208,6 +208,7 @@ struct cytom_ctx {
   cytom::GrowBuf g_grp;    // source groups of every 64-slot pool row
   cytom::GrowBuf g_nrow, g_rowb, g_rows;  // rows per chunk, their scan, the row descriptors
   cytom::GrowBuf g_jobs_raw, g_jsort;     // jobs before the largest-first sort, its scratch
+  cytom::GrowBuf g_del_bloom;             // Bloom filter of a batch's deletion keys
   bool jobs_dirty = true;
   uint64_t n_chunks = 0, n_cells = 0, n_jobs = 0;
   uint32_t n_bands = 1;

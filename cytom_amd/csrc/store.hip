@@ -294,11 +294,26 @@ __device__ inline uint32_t sig_bit(uint64_t key) {
   return (uint32_t)((key * 0xC2B2AE3D27D4EB4Full) >> 56);
 }
 
+// Blocked Bloom filter of the batch's deletion keys: two bits of one 64-bit
+// word per key, about one key per word, so a slot of a heavy tile (whose
+// 256-bit signature is saturated) reads one word instead of walking a
+// linear-probe chain: only the slots whose two bits are both set (the
+// matches and ~0.2 % false positives) probe the table.
+__device__ inline uint64_t bloom_mask(uint64_t key) {
+  const uint64_t h = key * 0xD6E8FEB86659FD93ull;
+  return (1ull << (h >> 58)) | (1ull << ((h >> 52) & 63u));
+}
+__device__ inline uint64_t bloom_word(uint64_t key, uint32_t log_b) {
+  return (key * 0x9FB21C651E98DF25ull) >> (64 - log_b);
+}
+
 __global__ void k_del_build(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                             uint64_t m, unsigned long long* __restrict__ tab_key,
                             uint32_t* __restrict__ tab_head, uint32_t log_h,
                             uint32_t* __restrict__ node_next, uint32_t* __restrict__ tile_flag,
-                            unsigned long long* __restrict__ tile_sig, uint32_t lv, uint32_t T) {
+                            unsigned long long* __restrict__ tile_sig,
+                            unsigned long long* __restrict__ bloom, uint32_t log_b, uint32_t lv,
+                            uint32_t T) {
   const uint64_t hmask = (1ull << log_h) - 1;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * blockDim.x) {
@@ -314,6 +329,7 @@ __global__ void k_del_build(const uint32_t* __restrict__ src, const uint32_t* __
     const uint32_t t = tile_of(s, d, lv, T), b = sig_bit(key);
     tile_flag[t] = 1u;
     atomicOr(&tile_sig[4 * (uint64_t)t + (b >> 6)], 1ull << (b & 63u));
+    atomicOr(&bloom[bloom_word(key, log_b)], (unsigned long long)bloom_mask(key));
   }
 }
 
@@ -357,6 +373,8 @@ __global__ void __launch_bounds__(256) k_del_match(const uint64_t* __restrict__ 
                                                    const uint32_t* __restrict__ node_next,
                                                    const uint32_t* __restrict__ dw,
                                                    const unsigned long long* __restrict__ tile_sig,
+                                                   const unsigned long long* __restrict__ bloom,
+                                                   uint32_t log_b,
                                                    unsigned long long* __restrict__ claimed,
                                                    const unsigned long long* misc) {
   if (batch_dead(misc)) return;
@@ -380,12 +398,19 @@ __global__ void __launch_bounds__(256) k_del_match(const uint64_t* __restrict__ 
       for (int u = 0; u < DM_UNROLL; ++u) le[u] = S.pool[off + min(base + u * 64 + lane, n - 1)];
       uint64_t e[DM_UNROLL], h[DM_UNROLL];
       unsigned long long k[DM_UNROLL];
+      bool cand[DM_UNROLL];
 #pragma unroll
       for (int u = 0; u < DM_UNROLL; ++u) {
         e[u] = pack_edge(xb + (le[u] & 0xFFFFu), yb + (le[u] >> 16));
-        h[u] = tab_hash(e[u], log_h);
         const uint32_t b = sig_bit(e[u]);
-        k[u] = (sg[b >> 6] >> (b & 63u)) & 1ull ? tab_key[h[u]] : EMPTY64;
+        cand[u] = (sg[b >> 6] >> (b & 63u)) & 1ull;
+        const uint64_t bm = bloom_mask(e[u]);
+        cand[u] = cand[u] && (bloom[bloom_word(e[u], log_b)] & bm) == bm;
+      }
+#pragma unroll
+      for (int u = 0; u < DM_UNROLL; ++u) {
+        h[u] = tab_hash(e[u], log_h);
+        k[u] = cand[u] ? tab_key[h[u]] : EMPTY64;
       }
 #pragma unroll
       for (int u = 0; u < DM_UNROLL; ++u) {
@@ -775,6 +800,13 @@ __global__ void k_decide(const unsigned long long* __restrict__ misc,
 
 // Sizes every scratch buffer of a batch of n_ins insertions and n_del
 // deletions (host only: may allocate, so it runs before any capture).
+// Bytes of the deletion Bloom filter for n requests (2^ceil(log2 n) words, at least 64).
+uint64_t bloom_bytes(uint64_t n) {
+  uint64_t w = 64;
+  while (w < n) w <<= 1;
+  return w * 8;
+}
+
 int ingest_prepare(cytom_ctx* ctx, uint64_t n_ins, uint64_t n_del) {
   int rc;
   if (n_ins > 0xFFFFFFFEull || n_del > 0xFFFFFFFEull)
@@ -803,6 +835,7 @@ int ingest_prepare(cytom_ctx* ctx, uint64_t n_ins, uint64_t n_del) {
       (rc = ensure(ctx, ctx->g_claimed, dmax * 8)) ||
       (rc = ensure(ctx, ctx->g_tile_flag, ctx->n_tiles * 4)) ||
       (rc = ensure(ctx, ctx->g_del_sig, ctx->n_tiles * 32)) ||
+      (rc = ensure(ctx, ctx->g_del_bloom, bloom_bytes(dmax))) ||
       (rc = ensure(ctx, ctx->g_tile_scan, (ctx->n_tiles + 1) * 16 + 256)) ||
       (rc = ensure(ctx, ctx->g_del_tiles, ctx->n_tiles * 4)) ||
       (rc = ensure(ctx, ctx->g_found, dmax * 12)) ||
@@ -834,8 +867,12 @@ void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst
     while ((1ull << b) < ctx->n_tiles) ++b;
     return b;
   }();
-  // Source bits that fit beside the tile id in a 32-bit key.
-  const uint32_t sb = std::min<uint32_t>(lv, 32 - tile_bits);
+  // Source bits that fit beside the tile id in a 32-bit key, cut to a whole
+  // number of 8-bit radix passes while the source groups they order stay
+  // within 64 vertices (two lines of PageRank-delta source values).
+  uint32_t sb = std::min<uint32_t>(lv, 32 - tile_bits);
+  const uint32_t cut = (tile_bits + sb) % 8;
+  if (cut && sb >= cut && lv - (sb - cut) <= 6) sb -= cut;
   const uint32_t key_bits = tile_bits + sb;
   uint32_t* keys = static_cast<uint32_t*>(ctx->g_keys.p);
   uint32_t* vals = static_cast<uint32_t*>(ctx->g_vals.p);
@@ -916,6 +953,10 @@ void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst
     auto* dcbase = static_cast<uint32_t*>(ctx->g_cbase.p);
     auto* dchunks = static_cast<uint64_t*>(ctx->g_chunks.p);
     auto* tile_sig = static_cast<unsigned long long*>(ctx->g_del_sig.p);
+    auto* bloom = static_cast<unsigned long long*>(ctx->g_del_bloom.p);
+    uint32_t log_b = 6;  // about one key per filter word
+    while ((1ull << log_b) < n_del) ++log_b;
+    fill_u64(reinterpret_cast<uint64_t*>(bloom), 1ull << log_b, 0, st);
     fill_u64(reinterpret_cast<uint64_t*>(tab_key), H, ~0ull, st);
     fill_u32(tab_head, H, 0xFFFFFFFFu, st);
     fill_u64(reinterpret_cast<uint64_t*>(claimed), n_del, ~0ull, st);
@@ -923,7 +964,7 @@ void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst
     fill_u64(reinterpret_cast<uint64_t*>(tile_sig), ctx->n_tiles * 4, 0, st);
     k_del_build<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, tab_key, tab_head,
                                                       log_h, node_next, tile_flag, tile_sig,
-                                                      lv, T);
+                                                      bloom, log_b, lv, T);
     scan_exclusive_u32(tile_flag, tscan, ctx->n_tiles, ctx->g_scan_tmp.p, st);
     k_flag_emit<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(tile_flag, tscan, ctx->n_tiles,
                                                             del_tiles);
@@ -934,7 +975,7 @@ void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst
     k_del_chunks<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(dnch, dcbase, ctx->n_tiles, dchunks);
     k_del_match<<<(uint32_t)ctx->num_cus * 8, 256, 0, st>>>(
         dchunks, dcbase + ctx->n_tiles, ctx->view(), tab_key, tab_head, log_h, node_next,
-        ctx->weighted ? d_dw : nullptr, tile_sig, claimed, misc);
+        ctx->weighted ? d_dw : nullptr, tile_sig, bloom, log_b, claimed, misc);
     k_del_finish<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, claimed,
                                                        ctx->view(), found, misc);
   }
