@@ -277,12 +277,15 @@ __global__ void k_nch(const uint32_t* __restrict__ cnt, uint64_t n_tiles, uint32
     nch[t] = (cnt[t] + CH - 1) / CH;
 }
 
+// One wavefront per tile, its lanes over the tile's chunks (a heavy RMAT
+// tile holds thousands).
 __global__ void k_emit_chunks(const uint32_t* __restrict__ nch, const uint32_t* __restrict__ cbase,
                               uint64_t n_tiles, uint32_t CH, uint64_t* __restrict__ chunks) {
-  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles;
-       t += (uint64_t)gridDim.x * blockDim.x) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < n_tiles;
+       t += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
     const uint32_t n = nch[t], b = cbase[t];
-    for (uint32_t c = 0; c < n; ++c) chunks[b + c] = ((uint64_t)t << 32) | (uint64_t)(c * CH);
+    for (uint32_t c = lane; c < n; c += 64) chunks[b + c] = ((uint64_t)t << 32) | (uint64_t)(c * CH);
   }
 }
 
@@ -423,8 +426,9 @@ __global__ void k_emit_jobs(const uint32_t* __restrict__ cbase, const uint32_t* 
 
 // Largest jobs first within each band's queue (longest-processing-time
 // order): a block that takes a job late takes a short one, so the blocks of
-// a launch finish close together.  Sort key: the band (bits 24+), then the
-// job's row count, descending (bits 0-23); the sort is stable.
+// a launch finish close together.  Sort key: the band (bits 16+), then the
+// job's row count, descending, saturated at 2^16 - 1 rows (bits 0-15); the
+// sort is stable.
 __global__ void k_job_keys(const uint4* __restrict__ jobs, const uint32_t* __restrict__ jbase,
                            uint32_t T, uint32_t NB, const uint32_t* __restrict__ rowb,
                            uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
@@ -433,8 +437,8 @@ __global__ void k_job_keys(const uint4* __restrict__ jobs, const uint32_t* __res
     uint32_t b = 0;
     while (b + 1 < NB && jbase[(b + 1) * T] <= j) ++b;
     const uint4 jb = jobs[j];
-    const uint32_t rows = min(rowb[jb.y] - rowb[jb.x], 0xFFFFFFu);
-    key[j] = (b << 24) | (0xFFFFFFu - rows);
+    const uint32_t rows = min(rowb[jb.y] - rowb[jb.x], 0xFFFFu);
+    key[j] = (b << 16) | (0xFFFFu - rows);
     idx[j] = j;
   }
 }
@@ -1311,7 +1315,7 @@ void enqueue_jobs(cytom_ctx* ctx) {
   }
   k_nch<<<grid_for(nt, 256), 256, 0, st>>>(ctx->tile_cnt, nt, ctx->chunk_slots, nch);
   scan_exclusive_u32(nch, cbase, nt, ctx->g_scan_tmp.p, st);
-  k_emit_chunks<<<grid_for(nt, 256), 256, 0, st>>>(nch, cbase, nt, ctx->chunk_slots, chunks);
+  k_emit_chunks<<<grid_for(nt * 64, 256), 256, 0, st>>>(nch, cbase, nt, ctx->chunk_slots, chunks);
   // Source groups of every 64-slot row, recomputed after each batch (the
   // batch moved, added and tombstoned slots).
   k_row_groups<<<grid_for(ctx->n_chunks * 64, 256), 256, 0, st>>>(
@@ -1334,6 +1338,8 @@ void enqueue_jobs(cytom_ctx* ctx) {
   k_emit_jobs<<<grid_for(cells, 256), 256, 0, st>>>(cbase, xb, colj, jbase, ctx->T, NB, jc, raw);
   // Largest first within each band.
   const uint64_t mj = ctx->n_jobs;
+  uint32_t band_bits = 0;
+  while ((1u << band_bits) < NB) ++band_bits;
   const uint64_t jw = (mj * 4 + 255) / 256 * 256;
   char* js = static_cast<char*>(ctx->g_jsort.p);
   auto* key = reinterpret_cast<uint32_t*>(js);
@@ -1342,7 +1348,8 @@ void enqueue_jobs(cytom_ctx* ctx) {
   auto* idx_out = reinterpret_cast<uint32_t*>(js + 3 * jw);
   k_job_keys<<<grid_for(mj, 256), 256, 0, st>>>(raw, jbase, ctx->T, NB, rowb, key, idx);
   radix_sort_pairs_devn(key, idx, key_out, idx_out, reinterpret_cast<uint32_t*>(js + 4 * jw),
-                        reinterpret_cast<uint32_t*>(js + 5 * jw), jbase + cells, mj, 28,
+                        reinterpret_cast<uint32_t*>(js + 5 * jw), jbase + cells, mj,
+                        16 + band_bits,
                         js + 6 * jw, ctx->g_scan_tmp.p, st);
   k_permute_jobs<<<grid_for(mj, 256), 256, 0, st>>>(raw, idx_out, jbase + cells,
                                                     static_cast<uint4*>(ctx->g_jobs.p));

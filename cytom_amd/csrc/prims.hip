@@ -116,11 +116,22 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_ROUNDS = 16;
 constexpr uint64_t RS_TILE = RS_THREADS * RS_ROUNDS;  // 4096 keys per block
 
+// Blocks of a pass: with a device length, only the ones holding keys (the
+// histogram is laid out for that count, so its scan and the scatter read
+// 256 x that many entries; the launch is sized for the bound).
+__device__ inline uint32_t pass_blocks(uint64_t n, uint32_t nb, const uint32_t* n_dev) {
+  return n_dev ? (uint32_t)((dev_len(n, n_dev) + RS_TILE - 1) / RS_TILE) : nb;
+}
+
 __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t* __restrict__ keys,
                                                            uint64_t n, uint32_t shift,
                                                            uint32_t* __restrict__ hist,
                                                            uint32_t nb,
-                                                           const uint32_t* __restrict__ n_dev) {
+                                                           const uint32_t* __restrict__ n_dev,
+                                                           uint32_t* __restrict__ hist_len) {
+  nb = pass_blocks(n, nb, n_dev);
+  if (hist_len && blockIdx.x == 0 && threadIdx.x == 0) *hist_len = 256u * nb;
+  if (blockIdx.x >= nb) return;  // uniform
   n = dev_len(n, n_dev);
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
@@ -140,6 +151,8 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t n,
     uint32_t shift, const uint32_t* __restrict__ hscan, uint32_t nb,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev) {
+  nb = pass_blocks(n, nb, n_dev);
+  if (blockIdx.x >= nb) return;  // uniform
   n = dev_len(n, n_dev);
   __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[RS_THREADS / 64][256];
@@ -196,14 +209,21 @@ void radix_sort_impl(const uint32_t* k_in, const uint32_t* v_in, uint32_t* k_out
   const uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
   uint32_t* h = static_cast<uint32_t*>(hist);
   uint32_t* hs = reinterpret_cast<uint32_t*>(static_cast<char*>(hist) + align256(256ull * nb * 4));
+  // the histogram length of a device-length sort (256 x its blocks)
+  uint32_t* hlen = n_dev ? reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(hs) +
+                                                       align256((256ull * nb + 1) * 4))
+                         : nullptr;
   const uint32_t* ks = k_in;
   const uint32_t* vs = v_in;
   for (uint32_t p = 0; p < passes; ++p) {
     const bool last_to_out = ((passes - p) & 1u) == 1u;
     uint32_t* kd = last_to_out ? k_out : k_tmp;
     uint32_t* vd = last_to_out ? v_out : v_tmp;
-    k_radix_hist<<<nb, RS_THREADS, 0, st>>>(ks, n, 8 * p, h, nb, n_dev);
-    scan_exclusive_u32(h, hs, 256ull * nb, scan_tmp, st);
+    k_radix_hist<<<nb, RS_THREADS, 0, st>>>(ks, n, 8 * p, h, nb, n_dev, hlen);
+    if (n_dev)
+      scan_impl<uint32_t>(h, hs, 256ull * nb, scan_tmp, st, hlen);
+    else
+      scan_exclusive_u32(h, hs, 256ull * nb, scan_tmp, st);
     k_radix_scatter<<<nb, RS_THREADS, 0, st>>>(ks, vs, n, 8 * p, hs, nb, kd, vd, n_dev);
     ks = kd;
     vs = vd;
@@ -237,8 +257,8 @@ void scan_exclusive_u64_devn(const uint64_t* in, uint64_t* out, const uint32_t* 
 size_t radix_hist_bytes(uint64_t n) {
   uint64_t nb = (n + RS_TILE - 1) / RS_TILE;
   if (nb == 0) nb = 1;
-  // histogram + its scan (with the total slot)
-  return align256(256 * nb * 4) + align256((256 * nb + 1) * 4);
+  // histogram + its scan (with the total slot) + the device-length word
+  return align256(256 * nb * 4) + align256((256 * nb + 1) * 4) + 256;
 }
 
 

@@ -351,12 +351,15 @@ __global__ void k_del_nch(const uint32_t* __restrict__ flag, const uint32_t* __r
     nch[t] = flag[t] ? (cnt[t] + DEL_CH - 1) / DEL_CH : 0u;
 }
 
+// One wavefront per tile, its lanes over the tile's chunks.
 __global__ void k_del_chunks(const uint32_t* __restrict__ nch, const uint32_t* __restrict__ cbase,
                              uint64_t n_tiles, uint64_t* __restrict__ chunks) {
-  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles;
-       t += (uint64_t)gridDim.x * blockDim.x) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < n_tiles;
+       t += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
     const uint32_t n = nch[t], b = cbase[t];
-    for (uint32_t c = 0; c < n; ++c) chunks[b + c] = ((uint64_t)t << 32) | (uint64_t)(c * DEL_CH);
+    for (uint32_t c = lane; c < n; c += 64)
+      chunks[b + c] = ((uint64_t)t << 32) | (uint64_t)(c * DEL_CH);
   }
 }
 
@@ -972,7 +975,8 @@ void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst
     k_del_nch<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(tile_flag, ctx->tile_cnt, ctx->n_tiles,
                                                            dnch);
     scan_exclusive_u32(dnch, dcbase, ctx->n_tiles, ctx->g_scan_tmp.p, st);
-    k_del_chunks<<<grid_for(ctx->n_tiles, 256), 256, 0, st>>>(dnch, dcbase, ctx->n_tiles, dchunks);
+    k_del_chunks<<<grid_for(ctx->n_tiles * 64, 256), 256, 0, st>>>(dnch, dcbase, ctx->n_tiles,
+                                                                   dchunks);
     k_del_match<<<(uint32_t)ctx->num_cus * 8, 256, 0, st>>>(
         dchunks, dcbase + ctx->n_tiles, ctx->view(), tab_key, tab_head, log_h, node_next,
         ctx->weighted ? d_dw : nullptr, tile_sig, bloom, log_b, claimed, misc);
