@@ -1310,8 +1310,8 @@ void enqueue_jobs(cytom_ctx* ctx) {
     k_row_slots<<<grid_for((uint64_t)ctx->T * 64, 256), 256, 0, st>>>(ctx->tile_cnt, ctx->T, rs);
     k_band_bounds<<<1, 64, 0, st>>>(rs, ctx->T, NB, xb);
   } else {
-    fill_u32(xb, 1, 0, st);  // one band: all source rows
-    fill_u32(xb + 1, 1, ctx->T, st);
+    const FillOp fills[] = {{xb, 1, 0, 0}, {xb + 1, 1, ctx->T, 0}};  // one band: all source rows
+    fill_many(fills, 2, st);
   }
   k_nch<<<grid_for(nt, 256), 256, 0, st>>>(ctx->tile_cnt, nt, ctx->chunk_slots, nch);
   scan_exclusive_u32(nch, cbase, nt, ctx->g_scan_tmp.p, st);
@@ -1601,9 +1601,9 @@ void run_group(cytom_ctx* ctx, Program& P) {
   const uint32_t n = group_length(ctx, P);
   const uint32_t g = P.groups++;
   if (g == 0) {
-    fill_u64(reinterpret_cast<uint64_t*>(P.d_iter),
-             sizeof(IterStats) / 8 * (uint64_t)(ctx->max_iter + 2), 0, st);
-    fill_u64(reinterpret_cast<uint64_t*>(P.gact[P.cur_act]), (uint64_t)ctx->T * 8, 0, st);
+    const FillOp fills[] = {{P.d_iter, sizeof(IterStats) / 8 * (uint64_t)(ctx->max_iter + 2), 0, 1},
+                            {P.gact[P.cur_act], (uint64_t)ctx->T * 8, 0, 1}};
+    fill_many(fills, 2, st);
   }
   if (!ctx->capturing) (void)hipEventRecord(P.gev[2 * g], st);
   if (g == 0)

@@ -312,6 +312,18 @@ void scan_exclusive_u64_devn(const uint64_t* in, uint64_t* out, const uint32_t* 
 // memset nodes are avoided).
 void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
 void fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st);
+// Several fills in one launch (n words of v each, 8-byte words when wide).
+struct FillOp {
+  void* p;
+  uint64_t n;
+  uint64_t v;
+  uint32_t wide;
+};
+constexpr int MAX_FILLS = 8;
+struct FillSet {
+  FillOp op[MAX_FILLS];
+};
+void fill_many(const FillOp* ops, int n_ops, hipStream_t st);
 // Stable LSD radix sort of (key, value) pairs on the low key_bits bits.
 // Result lands in (k_out, v_out); (k_tmp, v_tmp) are scratch of n entries;
 // hist must hold radix_hist_bytes(n) bytes.

@@ -34,6 +34,18 @@ __global__ void k_fill(T* __restrict__ p, uint64_t n, T v) {
     p[i] = v;
 }
 
+// Several fills in one launch: blockIdx.y picks the fill.
+__global__ void k_fill_many(FillSet fs) {
+  const FillOp& f = fs.op[blockIdx.y];
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (f.wide)
+      static_cast<uint64_t*>(f.p)[i] = f.v;
+    else
+      static_cast<uint32_t*>(f.p)[i] = (uint32_t)f.v;
+  }
+}
+
 
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 8;
@@ -90,6 +102,40 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(const T* __restrict_
     out[n] = off + tot;
 }
 
+// Down-sweep that finds its block's offset itself: the sum of the block
+// sums before it (at most SCAN_FOLD_MAX of them, read by the whole block),
+// so a scan of up to SCAN_FOLD_MAX tiles takes two launches instead of three.
+constexpr uint64_t SCAN_FOLD_MAX = 8192;
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_down_fold(const T* __restrict__ in,
+                                                                 T* __restrict__ out, uint64_t n,
+                                                                 const T* __restrict__ bsum,
+                                                                 const uint32_t* __restrict__ n_dev) {
+  T part = 0;
+  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += SCAN_THREADS) part += bsum[b];
+  const T off = block_sum<SCAN_THREADS, T>(part);
+  if (n_dev) n = std::min<uint64_t>(n, *n_dev);
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  T v[SCAN_ITEMS];
+  T s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    uint64_t i = base + k;
+    v[k] = i < n ? in[i] : T(0);
+    s += v[k];
+  }
+  T tot;
+  T p = block_scan_excl<SCAN_THREADS, T>(s, &tot) + off;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    uint64_t i = base + k;
+    if (i < n) out[i] = p;
+    p += v[k];
+  }
+  if (blockIdx.x == std::min<uint64_t>(n / SCAN_TILE, gridDim.x - 1) && threadIdx.x == 0)
+    out[n] = off + tot;
+}
+
 template <typename T>
 void scan_impl(const T* in, T* out, uint64_t n, void* tmp, hipStream_t st,
                const uint32_t* n_dev = nullptr) {
@@ -107,6 +153,10 @@ void scan_impl(const T* in, T* out, uint64_t n, void* tmp, hipStream_t st,
   void* rest =
       static_cast<char*>(tmp) + align256(nb * sizeof(T)) + align256((nb + 1) * sizeof(T));
   k_scan_reduce<T><<<(uint32_t)nb, SCAN_THREADS, 0, st>>>(in, n, bsum, n_dev);
+  if (nb <= SCAN_FOLD_MAX) {
+    k_scan_down_fold<T><<<(uint32_t)nb, SCAN_THREADS, 0, st>>>(in, out, n, bsum, n_dev);
+    return;
+  }
   scan_impl<T>(bsum, boff, nb, rest, st);  // block sums past *n_dev are zero
   k_scan_down<T><<<(uint32_t)nb, SCAN_THREADS, 0, st>>>(in, out, n, boff, n_dev);
 }
@@ -285,6 +335,24 @@ void scan_exclusive_u32_devn(const uint32_t* in, uint32_t* out, const uint32_t* 
 void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st) {
   if (!n) return;
   k_fill<uint32_t><<<(uint32_t)std::min<uint64_t>((n + 255) / 256, 16384), 256, 0, st>>>(p, n, v);
+}
+
+void fill_many(const FillOp* ops, int n_ops, hipStream_t st) {
+  FillSet fs{};
+  uint64_t most = 1;
+  int k = 0;
+  for (int i = 0; i < n_ops; ++i) {
+    if (!ops[i].n) continue;
+    if (k == MAX_FILLS) {  // a full set: launch it, start the next
+      k_fill_many<<<dim3((uint32_t)std::min<uint64_t>((most + 255) / 256, 16384), k), 256, 0, st>>>(fs);
+      k = 0;
+      most = 1;
+    }
+    fs.op[k++] = ops[i];
+    most = std::max(most, ops[i].n);
+  }
+  if (k)
+    k_fill_many<<<dim3((uint32_t)std::min<uint64_t>((most + 255) / 256, 16384), k), 256, 0, st>>>(fs);
 }
 
 void fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st) {

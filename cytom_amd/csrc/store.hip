@@ -897,13 +897,17 @@ void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst
   ctx->jobs_dirty = true;  // the store changes (and the deletion pass reuses the list buffers)
   if (!ctx->capturing) (void)hipEventRecord(ctx->ev[0], st);
   // misc[1..] := 0 (the pool top in misc[0] persists), the pool size, rules
-  fill_u64(reinterpret_cast<uint64_t*>(misc + 1), 63, 0, st);
-  fill_u64(reinterpret_cast<uint64_t*>(misc + M_POOL_SIZE), 1, ctx->pool_size, st);
+  {
+    const FillOp fills[] = {{misc + 1, M_POOL_SIZE - 1, 0, 1},
+                            {misc + M_POOL_SIZE, 1, ctx->pool_size, 1},
+                            {misc + M_POOL_SIZE + 1, 63 - M_POOL_SIZE, 0, 1},
+                            {pf, np, 0, 1}};
+    fill_many(fills, 4, st);
+  }
   if (np) {
     (void)hipMemcpyAsync(rules, ctx->h_pflags, np * 4, hipMemcpyHostToDevice, st);
     (void)hipMemcpyAsync(rules + np, reinterpret_cast<uint32_t*>(ctx->h_pflags) + 64, np * 4,
                          hipMemcpyHostToDevice, st);
-    fill_u64(reinterpret_cast<uint64_t*>(pf), np, 0, st);
   }
   // 1. validation (a refused batch modifies nothing)
   if (n_ins)
@@ -959,12 +963,13 @@ void ingest_enqueue(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst
     auto* bloom = static_cast<unsigned long long*>(ctx->g_del_bloom.p);
     uint32_t log_b = 6;  // about one key per filter word
     while ((1ull << log_b) < n_del) ++log_b;
-    fill_u64(reinterpret_cast<uint64_t*>(bloom), 1ull << log_b, 0, st);
-    fill_u64(reinterpret_cast<uint64_t*>(tab_key), H, ~0ull, st);
-    fill_u32(tab_head, H, 0xFFFFFFFFu, st);
-    fill_u64(reinterpret_cast<uint64_t*>(claimed), n_del, ~0ull, st);
-    fill_u32(tile_flag, ctx->n_tiles, 0, st);
-    fill_u64(reinterpret_cast<uint64_t*>(tile_sig), ctx->n_tiles * 4, 0, st);
+    const FillOp fills[] = {{bloom, 1ull << log_b, 0, 1},
+                            {tab_key, H, ~0ull, 1},
+                            {tab_head, H, 0xFFFFFFFFu, 0},
+                            {claimed, n_del, ~0ull, 1},
+                            {tile_flag, ctx->n_tiles, 0, 0},
+                            {tile_sig, ctx->n_tiles * 4, 0, 1}};
+    fill_many(fills, 6, st);
     k_del_build<<<grid_for(n_del, 256), 256, 0, st>>>(d_dsrc, d_ddst, n_del, tab_key, tab_head,
                                                       log_h, node_next, tile_flag, tile_sig,
                                                       bloom, log_b, lv, T);
