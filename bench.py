@@ -198,6 +198,10 @@ def parse_args(argv):
     ap.add_argument("--concurrent-runs", choices=["on", "off"], default="on",
                     help="on: a step runs its programs concurrently, one stream each; off: one "
                          "after the other on the context's stream")
+    ap.add_argument("--profile-every", type=int, default=4,
+                    help="time every n-th gather launch of every run with HIP events, the "
+                         "phase rotating per run (1: every launch; each timed launch costs a "
+                         "few microseconds)")
     ap.add_argument("--host-paths", choices=["on", "off"], default="on",
                     help="also stream an RMAT-(scale-2) graph through both host paths "
                          "(step vs separate calls) after the timed stream")
@@ -323,7 +327,7 @@ def gpu_run(args):
     for b in range(prefix, prefix + warm):
         step(b, wacc)
     log("bench: %d untimed + %d warmup batches done" % (prefix, warm))
-    g.set_profiling(True)
+    g.set_profiling(True, every=args.profile_every)
     acc = new_acc()
     per_batch_te, per_batch_s = [], []
     C.synchronize()
@@ -386,6 +390,8 @@ def gpu_run(args):
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": measured_traffic(args, "k_gather<%s>" % dom),
         "avg_launch_us": round(ms * 1e3 / launches, 2) if launches else None,
+        "timed_launches": launches,
+        "timed_every": args.profile_every,
         "alg_bytes_per_launch": round(alg / launches) if launches else None,
     }
     # Final values of every program, for the comparison against the CPU

@@ -626,8 +626,10 @@ class Graph:
                 "resort_pct": t.resort_pct, "resort_max_slots": t.resort_max_slots,
                 "prd_frac_bits": t.prd_frac_bits}
 
-    def set_profiling(self, enable=True):
-        self._check(lib().cytom_set_profiling(self._h, 1 if enable else 0), "profiling")
+    def set_profiling(self, enable=True, every=1):
+        """HIP-event timing of gather launches: every launch (every=1) or
+        every every-th iteration of each run (the phase rotating per run)."""
+        self._check(lib().cytom_set_profiling(self._h, int(every) if enable else 0), "profiling")
 
     def set_step_graphs(self, enable=True):
         """Steps record their enqueue phase into HIP graphs and replay them

@@ -794,7 +794,9 @@ int cytom_get_tuning(const cytom_ctx* ctx, cytom_tuning* t) {
 
 int cytom_set_profiling(cytom_ctx* ctx, int enable) {
   CY_CHECK_CTX(ctx);
+  if (enable < 0) return set_err(ctx, CYTOM_EINVAL, "profiling takes 0 or a sampling period >= 1");
   ctx->profile = enable != 0;
+  ctx->profile_every = enable > 0 ? (uint32_t)enable : 1u;
   for (auto& P : ctx->progs) {
     int rc = program_events(ctx, P);
     if (rc) return rc;

@@ -1522,7 +1522,8 @@ void run_iters(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t n) {
   const uint32_t ab = apply_blocks_for(ctx);
   const bool prof = ctx->profile && P.kev.size() >= 2 * (size_t)ctx->max_iter;
   for (uint32_t i = lo; i < lo + n; ++i) {
-    hipEvent_t e0 = prof ? P.kev[2 * i] : nullptr, e1 = prof ? P.kev[2 * i + 1] : nullptr;
+    const bool timed = prof && i % ctx->profile_every == P.runs % ctx->profile_every;
+    hipEvent_t e0 = timed ? P.kev[2 * i] : nullptr, e1 = timed ? P.kev[2 * i + 1] : nullptr;
     switch (P.kind) {
       case CYTOM_BFS: launch_gather<CYTOM_BFS>(ctx, P, i, e0, e1); break;
       case CYTOM_CC: launch_gather<CYTOM_CC>(ctx, P, i, e0, e1); break;
@@ -1564,6 +1565,7 @@ void run_iters(cytom_ctx* ctx, Program& P, uint32_t lo, uint32_t n) {
 }
 
 void run_open(Program& P) {
+  ++P.runs;
   P.launched = 0;
   P.groups = 0;
   P.checked = 0;
@@ -1651,7 +1653,7 @@ int run_finish(cytom_ctx* ctx, Program& P, cytom_run_stats* out) {
     // one pass over the active bitmap.
     const double eb = P.kind == CYTOM_SSSP ? 12.0 : 8.0;
     const double sv = prd_like(P.kind) ? 12.0 : 4.0;
-    for (uint32_t i = 0; i < it; ++i) {
+    for (uint32_t i = P.runs % ctx->profile_every; i < it; i += ctx->profile_every) {
       float ms = 0.f;
       CY_HIP(ctx, hipEventElapsedTime(&ms, P.kev[2 * i], P.kev[2 * i + 1]));
       const IterStats& s = P.h_iter[i];

@@ -152,6 +152,9 @@ struct Program {
   bool fixpoint = false;
   int last_reexec = 0;
   uint32_t last_iters = 0;  // iterations of the previous run (first launch group)
+  // Runs opened so far: a sampled profile times iterations i with
+  // i % every == runs % every, so every iteration index is sampled alike.
+  uint32_t runs = 0;
 };
 
 
@@ -233,6 +236,7 @@ struct cytom_ctx {
   // Optional per-kernel timing of the gather launches (HIP events on the
   // context stream), accumulated per program kind.
   bool profile = false;
+  uint32_t profile_every = 1;  // time every n-th gather launch of a run (phase rotating per run)
   double gather_ms[5] = {0, 0, 0, 0, 0};
   uint64_t gather_launches[5] = {0, 0, 0, 0, 0};
   double gather_alg_bytes[5] = {0, 0, 0, 0, 0};

@@ -378,9 +378,13 @@ int cytom_get_tuning(const cytom_ctx* ctx, cytom_tuning* t);
 
 /* ---- measurement ------------------------------------------------------- */
 
-/* Enables (1) or disables (0) HIP-event timing of every gather launch and
+/* Enables (n >= 1) or disables (0) HIP-event timing of gather launches and
  * resets the accumulators (replaces the PerfEventScoped spans around the
- * pull phase, continuous-algorithm-executor.h:93-97). */
+ * pull phase, continuous-algorithm-executor.h:93-97): with n = 1 every launch
+ * is timed, with n > 1 every n-th iteration of every run, the first timed
+ * iteration rotating from run to run so that every iteration index is sampled
+ * alike (each timed launch costs a few microseconds of device time).
+ * The profile then accumulates the sampled launches only. */
 int cytom_set_profiling(cytom_ctx* ctx, int enable);
 /* Accumulated gather-kernel time, launch count and algorithmic bytes of one
  * program kind since profiling was enabled. */

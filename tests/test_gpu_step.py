@@ -213,6 +213,35 @@ def test_concurrent_runs_equal_serial_runs():
             (ra[k].iterations, ra[k].traversed_edges), b
 
 
+def test_sampled_profiling_times_every_nth_launch():
+    # Profiling every launch and every 4th launch of the same runs: the
+    # sampled profile holds every 4th iteration of each run, and a negative
+    # period is refused.
+    scale, vpt, seed = 13, 512, 10
+    V = 1 << scale
+    st = O.Stream(seed)
+    g, ids = make(V, vpt)
+    first, didx = st.next(12000, 0)
+    s, d, w, ds, dd, dw = device_batch(scale, seed, first, 12000, didx)
+    g.update_device(s, d, w, 12000, ds, dd, dw, len(didx))
+    prd = ids[NAMES.index("pagerank-delta")]
+    counts = {}
+    for every in (1, 4):
+        g.init(prd)
+        g.set_profiling(True, every=every)
+        r = g.execute(prd)
+        ms, launches, alg = g.kernel_profile(C.PRD)
+        counts[every] = (r.iterations, launches, ms, alg)
+        g.set_profiling(False)
+    it1, n1, ms1, alg1 = counts[1]
+    it4, n4, ms4, alg4 = counts[4]
+    assert it1 == it4 > 8
+    assert n1 == it1 and it4 // 4 <= n4 <= (it4 + 3) // 4
+    assert 0 < ms4 < ms1 and 0 < alg4 < alg1
+    with pytest.raises(C.CytomError):
+        g.set_profiling(True, every=-1)
+
+
 def test_execute_many_equals_separate_runs():
     # Several programs run together (concurrently) on the same graph equal
     # their separate runs, including after re-initialisation and with a
