@@ -85,6 +85,7 @@ void free_ctx(cytom_ctx* c) {
   free_graphs(c);
   for (hipStream_t st : c->run_streams) (void)hipStreamDestroy(st);
   if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->fork_ev2) (void)hipEventDestroy(c->fork_ev2);
   for (auto& P : c->progs) program_free(P);
   query_free(c);
   io_free(c);
@@ -568,6 +569,7 @@ int run_streams(cytom_ctx* ctx, size_t n) {
     ctx->run_streams.push_back(s);
   }
   if (!ctx->fork_ev) CY_HIP(ctx, hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+  if (!ctx->fork_ev2) CY_HIP(ctx, hipEventCreateWithFlags(&ctx->fork_ev2, hipEventDisableTiming));
   return 0;
 }
 
@@ -619,13 +621,19 @@ int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, con
       for (Program* P : ps) P->stream = nullptr;
     }
   } reset{ps};
-  auto fork = [&]() {
-    if (!ns) return;
-    (void)hipEventRecord(ctx->fork_ev, ctx->stream);
+  // Each program's stream waits for ev (recorded on the context's stream)
+  // when prd_like(kind) == prd.
+  auto fork_part = [&](hipEvent_t ev, bool prd) {
+    (void)hipEventRecord(ev, ctx->stream);
     for (size_t k = 0; k < ps.size(); ++k) {
       ps[k]->stream = ctx->run_streams[k % ns];
-      (void)hipStreamWaitEvent(ps[k]->stream, ctx->fork_ev, 0);
+      if (prd_like(ps[k]->kind) == prd) (void)hipStreamWaitEvent(ps[k]->stream, ev, 0);
     }
+  };
+  auto fork = [&]() {
+    if (!ns) return;
+    fork_part(ctx->fork_ev, true);
+    fork_part(ctx->fork_ev, false);
   };
   auto sync_all = [&]() -> int {
     CY_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -640,8 +648,16 @@ int cytom_step(cytom_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst, con
   // The enqueue phase: no host reads, no allocation (so it can be recorded).
   auto enqueue = [&]() {
     ingest_enqueue(ctx, d_src, d_dst, d_w, n_ins, d_del_src, d_del_dst, d_del_w, n_del);
-    enqueue_jobs(ctx);
-    fork();
+    if (ns) {
+      // PageRank-delta runs never read the row source groups: they start as
+      // soon as the job lists are built, the other runs after the groups.
+      enqueue_jobs(ctx, false);
+      fork_part(ctx->fork_ev, true);
+      enqueue_row_groups(ctx);
+      fork_part(ctx->fork_ev2, false);
+    } else {
+      enqueue_jobs(ctx);
+    }
     for (Program* P : ps) {
       P->refuse = ctx->d_misc;
       run_open(*P);

@@ -1293,7 +1293,14 @@ int build_jobs(cytom_ctx* ctx) {
   return 0;
 }
 
-void enqueue_jobs(cytom_ctx* ctx) {
+void enqueue_row_groups(cytom_ctx* ctx) {
+  k_row_groups<<<grid_for(ctx->n_chunks * 64, 256), 256, 0, ctx->stream>>>(
+      static_cast<const uint64_t*>(ctx->g_chunks.p),
+      static_cast<const uint32_t*>(ctx->g_cbase.p) + ctx->n_tiles, ctx->view(), ctx->chunk_slots,
+      static_cast<uint64_t*>(ctx->g_grp.p));
+}
+
+void enqueue_jobs(cytom_ctx* ctx, bool with_groups) {
   hipStream_t st = ctx->stream;
   const uint64_t nt = ctx->n_tiles;
   const uint32_t NB = ctx->n_bands;
@@ -1318,8 +1325,7 @@ void enqueue_jobs(cytom_ctx* ctx) {
   k_emit_chunks<<<grid_for(nt * 64, 256), 256, 0, st>>>(nch, cbase, nt, ctx->chunk_slots, chunks);
   // Source groups of every 64-slot row, recomputed after each batch (the
   // batch moved, added and tombstoned slots).
-  k_row_groups<<<grid_for(ctx->n_chunks * 64, 256), 256, 0, st>>>(
-      chunks, cbase + nt, ctx->view(), ctx->chunk_slots, static_cast<uint64_t*>(ctx->g_grp.p));
+  if (with_groups) enqueue_row_groups(ctx);
   // The row table of the dense gathers.
   auto* nrow = static_cast<uint32_t*>(ctx->g_nrow.p);
   auto* rowb = static_cast<uint32_t*>(ctx->g_rowb.p);

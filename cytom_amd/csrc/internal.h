@@ -253,7 +253,8 @@ struct cytom_ctx {
   // (forked from the context's stream after the update batch).
   bool concurrent_runs = true;
   std::vector<hipStream_t> run_streams;
-  hipEvent_t fork_ev = nullptr;
+  hipEvent_t fork_ev = nullptr;   // the work lists are built (PageRank-delta runs may start)
+  hipEvent_t fork_ev2 = nullptr;  // the row source groups are built too (the other runs)
   // While a step graph is being recorded the enqueue functions leave their
   // timing events out (events recorded inside a graph carry no timestamps);
   // the step records them around the graph launches instead.
@@ -396,7 +397,10 @@ int configure_gather(cytom_ctx* ctx);
 // launches the device-driven build, build_jobs does both.
 int build_jobs(cytom_ctx* ctx);
 int prepare_jobs(cytom_ctx* ctx);
-void enqueue_jobs(cytom_ctx* ctx);
+// with_groups = false leaves out the row source groups (enqueue_row_groups),
+// which only the BFS / CC / SSSP gathers read.
+void enqueue_jobs(cytom_ctx* ctx, bool with_groups = true);
+void enqueue_row_groups(cytom_ctx* ctx);
 // The current chunk count (one host round trip).
 int64_t jobs_chunk_count(cytom_ctx* ctx);
 
