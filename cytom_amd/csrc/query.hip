@@ -76,10 +76,21 @@ struct QueryEngine {
   uint32_t* mark = nullptr;   // V
   unsigned int* counter = nullptr;  // [0] next-list length
   void* scan_tmp = nullptr;
-  // BFS group masks (allocated by the first BFS group)
-  uint64_t *bvis = nullptr, *bfront = nullptr, *bnxt = nullptr;
-  uint32_t* bsrc = nullptr;
-  BfsCtl* bctl = nullptr;
+  // BFS groups run concurrently, each on a lane of its own: the group's
+  // visited / frontier / next masks, frontier lists, degree prefix, control
+  // block (device, and its pinned host copy) and stream (allocated by the
+  // first BFS groups).
+  struct BfsLane {
+    uint64_t *vis = nullptr, *front = nullptr, *nxt = nullptr;
+    uint32_t *src = nullptr, *flist = nullptr, *nlist = nullptr;
+    uint64_t *fdeg = nullptr, *fpre = nullptr;
+    void* scan_tmp = nullptr;
+    BfsCtl* c = nullptr;
+    BfsCtl* h = nullptr;
+    hipStream_t st = nullptr;
+  };
+  std::vector<BfsLane> blanes;
+  hipEvent_t bfork = nullptr;  // the snapshot is built (the lanes start after it)
   // results: per query (kind, group, lane)
   struct Q {
     int kind;
@@ -753,10 +764,19 @@ void free_snapshot(QueryEngine* qe) {
   free_query_tiles(qe);
   void* ps[] = {qe->row_off, qe->col,     qe->wt,   qe->in_off,  qe->in_src,  qe->in_w,
                 qe->heavy,   qe->flist,   qe->nlist, qe->glist, qe->fdeg,   qe->fpre,    qe->mark,
-                qe->counter, qe->scan_tmp, qe->seg_v, qe->seg_b, qe->bvis, qe->bfront,
-                qe->bnxt,    qe->bsrc,    qe->bctl};
+                qe->counter, qe->scan_tmp, qe->seg_v, qe->seg_b};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+  for (auto& b : qe->blanes) {
+    for (void* p : {(void*)b.vis, (void*)b.front, (void*)b.nxt, (void*)b.src, (void*)b.flist,
+                    (void*)b.nlist, (void*)b.fdeg, (void*)b.fpre, b.scan_tmp, (void*)b.c})
+      if (p) (void)hipFree(p);
+    if (b.h) (void)hipHostFree(b.h);
+    if (b.st) (void)hipStreamDestroy(b.st);
+  }
+  qe->blanes.clear();
+  if (qe->bfork) (void)hipEventDestroy(qe->bfork);
+  qe->bfork = nullptr;
   qe->row_off = qe->in_off = nullptr;
   qe->col = qe->wt = qe->in_src = qe->in_w = qe->heavy = qe->flist = qe->nlist = nullptr;
   qe->glist = nullptr;
@@ -768,9 +788,6 @@ void free_snapshot(QueryEngine* qe) {
   qe->mark = nullptr;
   qe->counter = nullptr;
   qe->scan_tmp = nullptr;
-  qe->bvis = qe->bfront = qe->bnxt = nullptr;
-  qe->bsrc = nullptr;
-  qe->bctl = nullptr;
   qe->snap_batch = ~0ull;
 }
 
@@ -925,72 +942,149 @@ int seed_frontier(cytom_ctx* ctx, QueryEngine* qe, std::vector<uint32_t> v, uint
 // the frontier size and the push/pull choice from the control block, and a
 // level whose frontier is empty does nothing.  The host reads the control
 // block after each round (8 levels, then 4) to see whether the group is done.
-int run_bfs_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<uint32_t>& src,
-                  uint16_t* lev, cytom_run_stats* acc) {
-  hipStream_t st = ctx->stream;
+constexpr size_t BFS_LANES = 4;
+
+int bfs_lanes(cytom_ctx* ctx, QueryEngine* qe, size_t n) {
   const uint64_t V = ctx->V;
   int rc;
-  // visited / frontier / next masks live with the snapshot (one group at a time)
-  if (!qe->bvis && ((rc = dev_alloc(ctx, &qe->bvis, V * 8)) ||
-                    (rc = dev_alloc(ctx, &qe->bfront, V * 8)) ||
-                    (rc = dev_alloc(ctx, &qe->bnxt, V * 8)) ||
-                    (rc = dev_alloc(ctx, &qe->bsrc, BFS_W * 4)) ||
-                    (rc = dev_alloc(ctx, &qe->bctl, sizeof(BfsCtl)))))
-    return rc;
-  uint64_t *vis = qe->bvis, *front = qe->bfront, *nxt = qe->bnxt;
-  uint32_t* dsrc = qe->bsrc;
-  BfsCtl* c = qe->bctl;
-  auto done = [&](int r) {
-    (void)hipStreamSynchronize(st);
+  while (qe->blanes.size() < n) {
+    qe->blanes.emplace_back();  // listed first: free_snapshot frees a partial lane
+    QueryEngine::BfsLane& b = qe->blanes.back();
+    if ((rc = dev_alloc(ctx, &b.vis, V * 8)) || (rc = dev_alloc(ctx, &b.front, V * 8)) ||
+        (rc = dev_alloc(ctx, &b.nxt, V * 8)) || (rc = dev_alloc(ctx, &b.src, BFS_W * 4)) ||
+        (rc = dev_alloc(ctx, &b.flist, V * 4)) || (rc = dev_alloc(ctx, &b.nlist, V * 4)) ||
+        (rc = dev_alloc(ctx, &b.fdeg, (V + 1) * 8)) || (rc = dev_alloc(ctx, &b.fpre, (V + 1) * 8)) ||
+        (rc = dev_alloc(ctx, (char**)&b.scan_tmp, scan_tmp_bytes(V + 1) + 256)) ||
+        (rc = dev_alloc(ctx, &b.c, sizeof(BfsCtl))))
+      return rc;
+    if (hipHostMalloc((void**)&b.h, sizeof(BfsCtl), hipHostMallocDefault) != hipSuccess ||
+        hipStreamCreate(&b.st) != hipSuccess)
+      return set_err(ctx, CYTOM_ENOMEM, "BFS lane allocation failed");
+  }
+  if (!qe->bfork && hipEventCreateWithFlags(&qe->bfork, hipEventDisableTiming) != hipSuccess)
+    return set_err(ctx, CYTOM_EDEVICE, "BFS lane event creation failed");
+  return 0;
+}
+
+// One BFS group (up to 64 queries as the bits of one u64 per vertex) in
+// flight on a lane.
+struct BfsRun {
+  QueryEngine::BfsLane* L = nullptr;
+  uint16_t* lev = nullptr;
+  uint64_t qmask = 0;
+  uint32_t enqueued = 0;  // levels enqueued
+  std::vector<uint32_t> src, seeds;
+};
+
+int bfs_start(cytom_ctx* ctx, BfsRun& r) {
+  QueryEngine::BfsLane& L = *r.L;
+  const hipStream_t st = L.st;
+  const uint64_t V = ctx->V;
+  r.seeds = r.src;
+  std::sort(r.seeds.begin(), r.seeds.end());
+  r.seeds.erase(std::unique(r.seeds.begin(), r.seeds.end()), r.seeds.end());
+  L.h[0] = BfsCtl{(uint32_t)r.seeds.size(), 0, 0, 0, 0, 0, 0};
+  if (hipMemsetAsync(L.vis, 0, V * 8, st) || hipMemsetAsync(L.front, 0, V * 8, st) ||
+      hipMemsetAsync(L.nxt, 0, V * 8, st) || hipMemsetAsync(r.lev, 0xFF, V * BFS_W * 2, st) ||
+      hipMemcpyAsync(L.src, r.src.data(), r.src.size() * 4, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(L.flist, r.seeds.data(), r.seeds.size() * 4, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(L.c, L.h, sizeof(BfsCtl), hipMemcpyHostToDevice, st))
+    return set_err(ctx, CYTOM_EDEVICE, "BFS group set-up failed");
+  k_bfs_seed<<<1, 64, 0, st>>>(L.src, (uint32_t)r.src.size(), L.vis, L.front, r.lev);
+  r.qmask = r.src.size() >= 64 ? ~0ull : ((1ull << r.src.size()) - 1);
+  r.enqueued = 0;
+  return 0;
+}
+
+// Enqueues the next round of levels (8 first, then 4) and the copy of the
+// control block the host reads after the round.  Levels past the last one
+// find an empty frontier and do nothing.
+int bfs_round(cytom_ctx* ctx, QueryEngine* qe, BfsRun& r) {
+  QueryEngine::BfsLane& L = *r.L;
+  const hipStream_t st = L.st;
+  const uint64_t V = ctx->V;
+  const uint32_t round = r.enqueued ? 4 : 8;
+  if (r.enqueued + round + 1 >= LEVEL_NONE)
+    return set_err(ctx, CYTOM_ERANGE, "BFS depth above 65534");
+  const uint32_t fixed = (uint32_t)ctx->num_cus * 4;  // grid of the per-frontier kernels
+  uint32_t* lists[2] = {L.flist, L.nlist};
+  BfsCtl* c = L.c;
+  for (uint32_t k = 0; k < round; ++k, ++r.enqueued) {
+    uint32_t* fl = lists[r.enqueued & 1];
+    uint32_t* nl = lists[(r.enqueued + 1) & 1];
+    k_bfs_deg<<<fixed, 256, 0, st>>>(fl, c, ctx->outdeg, L.fdeg);
+    scan_exclusive_u64_devn(L.fdeg, L.fpre, &c->n, V, L.scan_tmp, st);
+    k_bfs_level<<<1, 1, 0, st>>>(c, L.fpre, qe->E);
+    k_bfs_pull<<<blocks_for(V, 256, 16384), 256, 0, st>>>(qe->in_off, qe->in_src, V, r.qmask,
+                                                           L.front, L.vis, L.nxt, nl, c);
+    if (qe->n_seg)
+      k_bfs_pull_seg<<<std::min<uint32_t>(qe->n_seg, 16384), QT, 0, st>>>(
+          qe->seg_v, qe->seg_b, qe->n_seg, qe->in_src, r.qmask, L.front, L.vis,
+          reinterpret_cast<unsigned long long*>(L.nxt), nl, c);
+    k_bfs_expand<<<(uint32_t)ctx->num_cus * 16, QT, 0, st>>>(
+        fl, L.fpre, c, qe->row_off, qe->col, L.front, L.vis,
+        reinterpret_cast<unsigned long long*>(L.nxt), nl);
+    k_clear_front<<<fixed, 256, 0, st>>>(fl, c, L.front);
+    k_bfs_apply<<<fixed, 256, 0, st>>>(nl, c, L.nxt, L.vis, L.front, r.lev);
+    k_bfs_level_end<<<1, 1, 0, st>>>(c);
+  }
+  CY_HIP(ctx, hipMemcpyAsync(L.h, c, sizeof(BfsCtl), hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+// BFS groups, up to BFS_LANES at a time, each on its own stream: every
+// round enqueues the next levels of every group in flight, then waits for
+// all of them and reads their control blocks; a lane whose group is done
+// starts the next group.
+int run_bfs_groups(cytom_ctx* ctx, QueryEngine* qe, std::vector<std::vector<uint32_t>>& srcs,
+                   const std::vector<uint16_t*>& levs, cytom_run_stats* acc) {
+  const size_t n = srcs.size();
+  if (!n) return 0;
+  const size_t nl = std::min(n, BFS_LANES);
+  int rc;
+  if ((rc = bfs_lanes(ctx, qe, nl))) return rc;
+  CY_HIP(ctx, hipEventRecord(qe->bfork, ctx->stream));
+  for (size_t l = 0; l < nl; ++l) CY_HIP(ctx, hipStreamWaitEvent(qe->blanes[l].st, qe->bfork, 0));
+  std::vector<BfsRun> runs(nl);
+  std::vector<char> busy(nl, 0);
+  auto sync_lanes = [&]() -> int {
+    for (size_t l = 0; l < nl; ++l) CY_HIP(ctx, hipStreamSynchronize(qe->blanes[l].st));
+    return 0;
+  };
+  auto fail = [&](int r) {
+    (void)sync_lanes();
     return r;
   };
-  std::vector<uint32_t> seeds(src);
-  std::sort(seeds.begin(), seeds.end());
-  seeds.erase(std::unique(seeds.begin(), seeds.end()), seeds.end());
-  const BfsCtl init{(uint32_t)seeds.size(), 0, 0, 0, 0, 0, 0};
-  if (hipMemsetAsync(vis, 0, V * 8, st) || hipMemsetAsync(front, 0, V * 8, st) ||
-      hipMemsetAsync(nxt, 0, V * 8, st) || hipMemsetAsync(lev, 0xFF, V * BFS_W * 2, st) ||
-      hipMemcpyAsync(dsrc, src.data(), src.size() * 4, hipMemcpyHostToDevice, st) ||
-      hipMemcpyAsync(qe->flist, seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice, st) ||
-      hipMemcpyAsync(c, &init, sizeof(BfsCtl), hipMemcpyHostToDevice, st))
-    return done(set_err(ctx, CYTOM_EDEVICE, "BFS group set-up failed"));
-  k_bfs_seed<<<1, 64, 0, st>>>(dsrc, (uint32_t)src.size(), vis, front, lev);
-  const uint64_t qmask = src.size() >= 64 ? ~0ull : ((1ull << src.size()) - 1);
-  const uint32_t fixed = (uint32_t)ctx->num_cus * 4;  // grid of the per-frontier kernels
-  uint32_t* lists[2] = {qe->flist, qe->nlist};
-  uint32_t enqueued = 0;
-  BfsCtl h{};
-  for (uint32_t round = 8;; round = 4) {
-    if (enqueued + round + 1 >= LEVEL_NONE)
-      return done(set_err(ctx, CYTOM_ERANGE, "BFS depth above 65534"));
-    for (uint32_t k = 0; k < round; ++k, ++enqueued) {
-      uint32_t* fl = lists[enqueued & 1];
-      uint32_t* nl = lists[(enqueued + 1) & 1];
-      k_bfs_deg<<<fixed, 256, 0, st>>>(fl, c, ctx->outdeg, qe->fdeg);
-      scan_exclusive_u64_devn(qe->fdeg, qe->fpre, &c->n, V, qe->scan_tmp, st);
-      k_bfs_level<<<1, 1, 0, st>>>(c, qe->fpre, qe->E);
-      k_bfs_pull<<<blocks_for(V, 256, 16384), 256, 0, st>>>(qe->in_off, qe->in_src, V, qmask,
-                                                             front, vis, nxt, nl, c);
-      if (qe->n_seg)
-        k_bfs_pull_seg<<<std::min<uint32_t>(qe->n_seg, 16384), QT, 0, st>>>(
-            qe->seg_v, qe->seg_b, qe->n_seg, qe->in_src, qmask, front, vis,
-            reinterpret_cast<unsigned long long*>(nxt), nl, c);
-      k_bfs_expand<<<(uint32_t)ctx->num_cus * 16, QT, 0, st>>>(
-          fl, qe->fpre, c, qe->row_off, qe->col, front, vis,
-          reinterpret_cast<unsigned long long*>(nxt), nl);
-      k_clear_front<<<fixed, 256, 0, st>>>(fl, c, front);
-      k_bfs_apply<<<fixed, 256, 0, st>>>(nl, c, nxt, vis, front, lev);
-      k_bfs_level_end<<<1, 1, 0, st>>>(c);
+  size_t next = 0;
+  auto start = [&](size_t l) -> int {
+    runs[l].L = &qe->blanes[l];
+    runs[l].lev = levs[next];
+    runs[l].src = std::move(srcs[next]);
+    ++next;
+    busy[l] = 1;
+    return bfs_start(ctx, runs[l]);
+  };
+  for (size_t l = 0; l < nl; ++l)
+    if ((rc = start(l))) return fail(rc);
+  for (bool any = true; any;) {
+    for (size_t l = 0; l < nl; ++l)
+      if (busy[l] && (rc = bfs_round(ctx, qe, runs[l]))) return fail(rc);
+    if ((rc = sync_lanes())) return rc;
+    any = false;
+    for (size_t l = 0; l < nl; ++l) {
+      if (!busy[l]) continue;
+      const BfsCtl h = *runs[l].L->h;
+      if (h.n == 0) {
+        acc->traversed_edges += h.te;
+        acc->active_vertices += h.active;
+        acc->iterations += h.iters;
+        busy[l] = 0;
+        if (next < n && (rc = start(l))) return fail(rc);
+      }
+      any |= busy[l] != 0;
     }
-    CY_HIP(ctx, hipMemcpyAsync(ctx->h_scalar, c, sizeof(BfsCtl), hipMemcpyDeviceToHost, st));
-    CY_HIP(ctx, hipStreamSynchronize(st));
-    std::memcpy(&h, ctx->h_scalar, sizeof(BfsCtl));
-    if (h.n == 0) break;
   }
-  acc->traversed_edges += h.te;
-  acc->active_vertices += h.active;
-  acc->iterations += h.iters;
-  return done(0);
+  return 0;
 }
 
 int run_sssp_group(cytom_ctx* ctx, QueryEngine* qe, const std::vector<uint32_t>& src,
@@ -1376,6 +1470,8 @@ int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run
     for (uint32_t i = 0; i < nq; ++i)
       if ((int)q[i].kind == kind) idx.push_back(i);
     const bool seq = kind != CYTOM_BFS && qe->E > VECTOR_MAX_EDGES;
+    std::vector<std::vector<uint32_t>> bfs_src;  // BFS groups, run together below
+    std::vector<uint16_t*> bfs_lev;
     for (size_t g0 = 0; g0 < idx.size(); g0 += width) {
       const size_t g1 = std::min(idx.size(), g0 + width);
       const uint32_t gid = (uint32_t)qe->group_buf.size();
@@ -1398,9 +1494,9 @@ int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run
         rc = run_seq_group(ctx, kind, q, idx.data() + g0, (uint32_t)(g1 - g0),
                            static_cast<uint32_t*>(buf), &acc);
       } else if (kind == CYTOM_BFS) {
-        std::vector<uint32_t> src;
-        for (size_t k = g0; k < g1; ++k) src.push_back(q[idx[k]].sources[0]);
-        rc = run_bfs_group(ctx, qe, src, static_cast<uint16_t*>(buf), &acc);
+        bfs_src.emplace_back();
+        for (size_t k = g0; k < g1; ++k) bfs_src.back().push_back(q[idx[k]].sources[0]);
+        bfs_lev.push_back(static_cast<uint16_t*>(buf));
       } else if (kind == CYTOM_SSSP) {
         std::vector<uint32_t> src;
         for (size_t k = g0; k < g1; ++k) src.push_back(q[idx[k]].sources[0]);
@@ -1413,6 +1509,7 @@ int batched_execute(cytom_ctx* ctx, const cytom_query* q, uint32_t nq, cytom_run
       }
       if (rc) return rc;
     }
+    if (!bfs_src.empty() && (rc = run_bfs_groups(ctx, qe, bfs_src, bfs_lev, &acc))) return rc;
   }
   CY_HIP(ctx, hipEventRecord(ctx->ev[1], st));
   CY_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
