@@ -569,8 +569,14 @@ __global__ void k_resort_pieces(const uint32_t* __restrict__ list,
   }
 }
 
-// Keys (rank, tombstone bit, source) and staging copies of one piece of
-// selected slots per block.
+// Source bits of a sort-compaction key: the tile-local source's top 8 bits,
+// so a tile's prefix ends up ordered by 64-vertex source groups (the lines of
+// source values a row reads are the same as with exact order), and with the
+// pass's tile rank and the tombstone bit the key fits 24 bits: 3 radix passes.
+__host__ __device__ inline uint32_t rs_sbits(uint32_t lv) { return lv < 8 ? lv : 8u; }
+
+// Keys (rank, tombstone bit, source group) and staging copies of one piece
+// of selected slots per block.
 __global__ void __launch_bounds__(256) k_resort_gather(StoreView S, const uint32_t* __restrict__ list,
                                                        const unsigned long long* __restrict__ misc,
                                                        const uint64_t* __restrict__ off,
@@ -582,7 +588,7 @@ __global__ void __launch_bounds__(256) k_resort_gather(StoreView S, const uint32
   __shared__ uint32_t s_rank[RS_TILES], s_base[RS_TILES];
   __shared__ uint64_t s_src[RS_TILES];
   __shared__ uint32_t s_n;
-  const uint32_t lv = S.log2vpt;
+  const uint32_t lv = S.log2vpt, sb = rs_sbits(lv);
   const uint32_t nproc = (uint32_t)misc[M_RS_NPROC];
   const uint32_t nslots = (uint32_t)misc[M_RS_NSLOTS];
   for (uint32_t b = blockIdx.x; (uint64_t)b * RS_BLOCK < nslots; b += gridDim.x) {
@@ -617,8 +623,8 @@ __global__ void __launch_bounds__(256) k_resort_gather(StoreView S, const uint32
     for (int u = 0; u < (int)(RS_BLOCK / 256); ++u) {
       const uint32_t k = k0 + u * 256 + threadIdx.x;
       if (k >= k1) continue;
-      const uint32_t low = e[u] == TOMB ? (1u << lv) : (e[u] & 0xFFFFu);
-      keys[k] = (s_rank[j[u]] << (lv + 1)) | low;
+      const uint32_t low = e[u] == TOMB ? (1u << sb) : ((e[u] & 0xFFFFu) >> (lv - sb));
+      keys[k] = (s_rank[j[u]] << (sb + 1)) | low;
       vals[k] = k;
       e_st[k] = e[u];
       if (S.wpool) w_st[k] = wv[u];
@@ -637,12 +643,12 @@ __global__ void k_resort_scatter(StoreView S, const uint32_t* __restrict__ list,
                                  const uint32_t* __restrict__ e_st,
                                  const uint32_t* __restrict__ w_st) {
   if (batch_dead(misc)) return;
-  const uint32_t lv = S.log2vpt;
+  const uint32_t sb = rs_sbits(S.log2vpt);
   const uint32_t n = (uint32_t)misc[M_RS_NSLOTS];
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const uint32_t key = skeys[k];
-    if (key & (1u << lv)) continue;
-    const uint32_t t = list[key >> (lv + 1)];
+    if (key & (1u << sb)) continue;
+    const uint32_t t = list[key >> (sb + 1)];
     const uint64_t slot = S.tile_off[t] + (k - (uint32_t)off[t]);
     const uint32_t j = svals[k];
     S.pool[slot] = e_st[j];
@@ -749,8 +755,9 @@ void resort_enqueue(cytom_ctx* ctx) {
   scan_exclusive_u32(flag, rank, nt, ctx->g_scan_tmp.p, st);
   scan_exclusive_u64(size, off, nt, ctx->g_scan_tmp.p, st);
   k_flag_emit<<<grid_for(nt, 256), 256, 0, st>>>(flag, rank, nt, list);
-  // Keys are (rank, tombstone bit, lv source bits) in 32 bits.
-  const uint32_t rmax = 1u << (31 - lv);
+  // Keys are (rank, tombstone bit, source group bits) in 24 bits.
+  const uint32_t sb = rs_sbits(lv);
+  const uint32_t rmax = 1u << (23 - sb);
   fill_u64(reinterpret_cast<uint64_t*>(misc + M_RS_CUT), 1, ~0ull, st);
   k_resort_cut<<<grid_for(nt, 256), 256, 0, st>>>(flag, rank, off, size, nt, rmax, cap, misc);
   k_resort_count<<<1, 64, 0, st>>>(rank + nt, off, list, nt, misc);
@@ -773,7 +780,7 @@ void resort_enqueue(cytom_ctx* ctx) {
       ctx->view(), list, misc, off, bfirst, keys, vals, e_st, w_st);
   radix_sort_pairs_devn(keys, vals, skeys, svals, static_cast<uint32_t*>(ctx->g_flags.p),
                         static_cast<uint32_t*>(ctx->g_scan.p), nslots_dev, cap,
-                        rank_bits + lv + 1, ctx->g_hist.p, ctx->g_scan_tmp.p, st);
+                        rank_bits + sb + 1, ctx->g_hist.p, ctx->g_scan_tmp.p, st);
   k_resort_scatter<<<grid_for(cap, 256), 256, 0, st>>>(ctx->view(), list, off, skeys, svals, misc,
                                                        e_st, w_st);
   k_resort_commit<<<grid_for(max_proc, 256), 256, 0, st>>>(ctx->view(), list, misc);
