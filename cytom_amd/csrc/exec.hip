@@ -1203,11 +1203,23 @@ size_t gather_lds_bytes(int kind, uint32_t vpt) {
 
 // Persistent grid: as many blocks per CU as the LDS window allows (at most
 // two), unless tuned.
-uint32_t gather_grid(const cytom_ctx* ctx, size_t lds) {
+template <int KIND>
+uint32_t gather_grid(cytom_ctx* ctx, size_t lds) {
   if (ctx->gather_blocks) return ctx->gather_blocks;
-  const size_t per_block = lds + 16 * 1024;  // window + chunk table and scan scratch
-  const uint32_t per_cu = (uint32_t)std::max<size_t>(1, (160 * 1024) / per_block);
-  return (uint32_t)ctx->num_cus * std::min<uint32_t>(per_cu, 2);
+  // The blocks one CU holds at once with this window (LDS and registers both
+  // limit it), at most two: a persistent grid larger than what is resident
+  // only adds blocks that start after the job queue has drained.  Asked once
+  // per context and kind (the window is fixed by the context's tile width).
+  uint32_t& per_cu = ctx->gather_per_cu[KIND];
+  if (!per_cu) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gather<KIND>, GATHER_THREADS, lds) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 1;
+    per_cu = (uint32_t)std::min(nb, 2);
+  }
+  return (uint32_t)ctx->num_cus * per_cu;
 }
 
 template <int KIND>
@@ -1236,10 +1248,10 @@ void launch_gather(cytom_ctx* ctx, Program& P, int it, hipEvent_t ev0, hipEvent_
   if (ev0) {
     // Profiling: the events are part of the dispatch, so they time the
     // kernel itself (separate event records would add the dispatch gaps).
-    hipExtLaunchKernelGGL(k_gather<KIND>, dim3(gather_grid(ctx, lds)), dim3(GATHER_THREADS),
+    hipExtLaunchKernelGGL(k_gather<KIND>, dim3(gather_grid<KIND>(ctx, lds)), dim3(GATHER_THREADS),
                           (uint32_t)lds, prog_stream(ctx, P), ev0, ev1, 0u, A);
   } else {
-    k_gather<KIND><<<gather_grid(ctx, lds), GATHER_THREADS, lds, prog_stream(ctx, P)>>>(A);
+    k_gather<KIND><<<gather_grid<KIND>(ctx, lds), GATHER_THREADS, lds, prog_stream(ctx, P)>>>(A);
   }
 }
 

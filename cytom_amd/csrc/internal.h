@@ -224,7 +224,8 @@ struct cytom_ctx {
   uint32_t resort_max_slots = 1u << 26;  // staged slots per pass (plus one tile)
   uint64_t tiles_resorted = 0, slots_resorted = 0;
   uint32_t gather_threads = 512;
-  uint32_t gather_blocks = 0;   // persistent grid (0 = from the LDS budget per program)
+  uint32_t gather_blocks = 0;   // persistent grid (0 = the resident blocks per CU, at most 2)
+  uint32_t gather_per_cu[8] = {};  // resident gather blocks per CU by kind (0 = not asked yet)
   // PageRank-delta sums are exact int64 fixed point with this many fraction
   // bits: 60 - ceil(log2(2V)) keeps any vertex's sum (bounded by 2V) in range.
   int prd_shift = 36;
